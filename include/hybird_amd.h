@@ -1,0 +1,154 @@
+/*
+ * hybird_amd: MI355X-native coupled lattice-Boltzmann / discrete-element step.
+ *
+ * C ABI of libhybird_amd.so. Plain pointers, sizes and POD structs only; every
+ * function returns 0 on success or a negative HYB_E* code, with a message
+ * available from hyb_last_error(). All work for one handle is serialised on
+ * that handle's HIP stream; host synchronisation happens only in the *_get_*
+ * calls, the timing queries and hyb_sync().
+ *
+ * The entry points replace the methods of the reference's solver classes
+ * (reference file:line of the interface each one stands in for):
+ *   hyb_lb_create        LB::LB + LB::latticeBoltzmannGet + LB::latticeDefinition  (LB.h:104-155, LB.cpp:42-188)
+ *   hyb_dem_create       DEM::DEM + DEM::discreteElementGet                         (DEM.h:98-138, DEM.cpp:13-183)
+ *   hyb_dem_init         DEM::discreteElementInit                                   (DEM.h:139, DEM.cpp:185-297)
+ *   hyb_lb_init          LB::latticeBolzmannInit                                    (LB.h:158, LB.cpp:190-219)
+ *   hyb_dem_step         DEM::discreteElementStep                                   (DEM.h:137, DEM.cpp:331-376)
+ *   hyb_lb_fs_step       LB::latticeBoltzmannFreeSurfaceStep                        (LB.h:161, LB.cpp:235-245)
+ *   hyb_lb_couple        LB::latticeBoltzmannCouplingStep                           (LB.h:160, LB.cpp:247-280)
+ *   hyb_lb_step          LB::latticeBolzmannStep                                    (LB.h:159, LB.cpp:221-233)
+ *   hyb_cycle            goCycle without the IO call                                (hybird.cpp:35-66)
+ *   hyb_lb_get / hyb_dem_get   host views of LB::nodes/types and DEM::elmts/particles read by IO
+ *                        (IO.cpp:711-831, 835-1106)
+ * The reference has no foreign-function interface; a C++ caller reaches these
+ * through the facade classes in hybird_amd/host (same class and method names as
+ * the reference), and Python reaches them through ctypes (hybird_amd/_abi.py).
+ */
+#ifndef HYBIRD_AMD_H
+#define HYBIRD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HYB_OK 0
+#define HYB_E_ARG (-1)     /* invalid argument or configuration */
+#define HYB_E_HIP (-2)     /* HIP runtime error */
+#define HYB_E_STATE (-3)   /* call out of order (e.g. step before init) */
+#define HYB_E_CAPACITY (-4) /* a device list overflowed its capacity */
+#define HYB_E_UNSUPPORTED (-5)
+#define HYB_E_NUMERIC (-6) /* non-finite value detected */
+
+/* problemName values (vector.h:21), plus an extension for a generic liquid box */
+enum hyb_problem {
+    HYB_PROBLEM_NONE = 0,
+    HYB_PROBLEM_SPLASH = 2,     /* gas above z = 20 (LB.cpp:628-640) */
+    HYB_PROBLEM_LIQUID_BOX = 100 /* liquid inside liquidBox[], gas elsewhere (not in the reference) */
+};
+
+enum hyb_contact { HYB_CONTACT_LINEAR = 0, HYB_CONTACT_HERTZIAN = 1 }; /* elmt.h:141 */
+enum hyb_collision { HYB_COLLISION_BGK = 0, HYB_COLLISION_TRT = 1 };
+enum hyb_precision { HYB_FP64 = 0, HYB_FP32_SHIFTED = 1 };
+
+/* Configuration: the .cfg keys of the reference, in the reference's (physical)
+ * units; the library converts to lattice units exactly as LB.cpp:85-188 and
+ * DEM.cpp:13-183 do. Fields after `collision` are extensions whose defaults
+ * reproduce the reference. */
+typedef struct hyb_config {
+    /* run control, hybird.cpp:136-190 */
+    int32_t problem;
+    int32_t demSolve, lbSolve, freeSurface, forceField, nonNewtonian, turbulence;
+    double demInitialRepeat;
+    /* lattice, LB.cpp:93-185 */
+    double unitLength, unitTime, unitDensity;
+    double lbSize[3];
+    double plasticVisc, yieldStress, initVisc;
+    double initVelocity[3];
+    double lbF[3];
+    int32_t boundary[6];
+    double slipCoefficient, turbConst;
+    /* DEM, DEM.cpp:15-61, 177-181 */
+    double density;
+    int32_t contactModel;
+    double youngMod, poisson, linearStiff, restitution, viscTang, frictionCoefPart, frictionCoefWall;
+    double numVisc;
+    int32_t multiStep;
+    double criticalRatio;
+    double translate[3];
+    double scale;
+    /* extensions */
+    int32_t collision;      /* HYB_COLLISION_BGK (reference) or HYB_COLLISION_TRT */
+    double trtMagic;        /* TRT magic parameter Lambda (default 1/4) */
+    int32_t precision;      /* HYB_FP64 or HYB_FP32_SHIFTED */
+    double liquidBox[6];    /* HYB_PROBLEM_LIQUID_BOX: xmin,xmax,ymin,ymax,zmin,zmax (node coordinates, inclusive) */
+    int32_t reserved[8];
+} hyb_config;
+
+/* Fills *cfg with the reference's parse defaults (LB.cpp:93-185, DEM.cpp:15-181)
+ * overridden by the lattice-unit conventions (unit* = 1). */
+void hyb_config_defaults(hyb_config* cfg);
+
+/* One particle-file record, DEM.cpp:85-120:
+ * index size radius x y z vx vy vz wx wy wz q0 q1 q2 q3 dq0 dq1 dq2 dq3 */
+#define HYB_ELEMENT_RECORD 20
+
+typedef struct hyb_lb hyb_lb;
+typedef struct hyb_dem hyb_dem;
+
+const char* hyb_last_error(void);
+int hyb_device_count(int* count);
+int hyb_sync(hyb_lb* lb);
+
+/* ---- DEM ---- */
+int hyb_dem_create(const hyb_config* cfg, int32_t n_elements, const double* records, hyb_dem** out);
+int hyb_dem_init(hyb_dem* dem, const hyb_lb* lb);
+int hyb_dem_step(hyb_dem* dem);
+int hyb_dem_destroy(hyb_dem* dem);
+
+/* ---- LB ---- */
+int hyb_lb_create(const hyb_config* cfg, hyb_lb** out);
+int hyb_lb_init(hyb_lb* lb, hyb_dem* dem);
+int hyb_lb_fs_step(hyb_lb* lb);
+int hyb_lb_couple(hyb_lb* lb, hyb_dem* dem, int32_t* newNeighborList);
+int hyb_lb_step(hyb_lb* lb, hyb_dem* dem);
+int hyb_lb_destroy(hyb_lb* lb);
+
+/* goCycle (hybird.cpp:35-66) repeated n times, IO excluded. */
+int hyb_cycle(hyb_lb* lb, hyb_dem* dem, int32_t n);
+
+/* ---- host views ----
+ * Field names and layouts:
+ *   LB  "f"          f64 [N][19]  populations in the reference's step-end (post-stream) state
+ *       "n"          f64 [N]      density from the last reconstruction
+ *       "u"          f64 [N][3]   velocity kept by the node (Guo-shifted, node.cpp:85-88)
+ *       "hydroForce" f64 [N][3]
+ *       "mass"       f64 [N]      free-surface mass function
+ *       "visc"       f64 [N]      kinematic viscosity (lattice units)
+ *       "types"      u8  [N]      node type code (node.h:73-83)
+ *       "pflag"      u8  [N]      inside-particle flag
+ *       "solidIndex" u32 [N]      particle index for pflag nodes
+ *   DEM "e_x" f64[E][6][3], "e_q" f64[E][6][4], "e_w" f64[E][6][3], "e_forces" f64[E][7][3]
+ *       (FHydro, MHydro, FParticle, MParticle, FWall, MWall, FGrav), "e_misc" f64[E][19],
+ *       "p_data" f64[P][10] (x0, x1, radiusVec, r), "p_idx" u32[P][4],
+ *       "neighborTable" u32[2*pairs], "nearWallTable" u32[2*k], "ghostIndex" u32[G], "ghostVec" f64[G][3],
+ *       "dem_scalars" f64[17]
+ * *bytes_needed receives the size; nothing is copied when capacity is too small.
+ * N = lbSize product; E elements; P particles including ghosts. */
+int hyb_lb_get(hyb_lb* lb, const char* name, void* host, size_t capacity, size_t* bytes_needed);
+int hyb_dem_get(hyb_dem* dem, const char* name, void* host, size_t capacity, size_t* bytes_needed);
+
+/* Profiling of the fused stream-collide kernel: when enabled, every LB step
+ * records HIP events around that kernel on the handle's stream. */
+int hyb_lb_set_profiling(hyb_lb* lb, int32_t enabled);
+int hyb_lb_kernel_time(hyb_lb* lb, double* total_ms, int64_t* launches);
+/* Active (fluid + interface) node count, the MLUPS denominator. */
+int hyb_lb_active_nodes(hyb_lb* lb, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYBIRD_AMD_H */

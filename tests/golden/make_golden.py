@@ -1,0 +1,165 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled reference.
+
+Runs oracle/_ref/hybird_ref_dump (the reference built from its own sources by
+``make -C oracle ref``) on small seeded cases with OMP_NUM_THREADS=1 (the
+reference's OpenMP force sums are order-dependent, LB.cpp:1897, 1216) and
+stores the full-precision state at chosen cycles as compressed .npz files
+(numpy arrays only, loadable with allow_pickle=False).
+
+Usage: python tests/golden/make_golden.py [case ...]
+"""
+from __future__ import annotations
+
+import os
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from hybird_amd.cfg import base_config, write_cfg  # noqa: E402
+from hybird_amd.particles import Bed, rsa_bed, write_empty_object_file, write_particle_file  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "hybird_ref_dump")
+
+_DT = {b"d": np.float64, b"I": np.uint32, b"B": np.uint8, b"i": np.int32}
+
+
+def read_bin(path: str) -> dict:
+    out = {}
+    with open(path, "rb") as fh:
+        data = fh.read()
+    pos = 0
+    while pos < len(data):
+        (nl,) = struct.unpack_from("<I", data, pos)
+        pos += 4
+        name = data[pos:pos + nl].decode()
+        pos += nl
+        dt = data[pos:pos + 1]
+        pos += 1
+        (nd,) = struct.unpack_from("<I", data, pos)
+        pos += 4
+        shape = struct.unpack_from("<" + "Q" * nd, data, pos)
+        pos += 8 * nd
+        dtype = np.dtype(_DT[dt])
+        count = int(np.prod(shape)) if nd else 1
+        arr = np.frombuffer(data, dtype=dtype, count=count, offset=pos).reshape(shape)
+        pos += count * dtype.itemsize
+        out[name] = arr.copy()
+    return out
+
+
+# Which arrays to keep per case ("lattice" = full lattice fields).
+LATTICE_KEYS = ["types", "pflag", "solidIndex", "f", "n", "u", "hydroForce", "mass", "visc", "activeNodes",
+                "interfaceNodes", "fluidNodes", "particleNodes", "neighbors", "lbSize", "lbF", "totalMass"]
+DEM_KEYS = ["e_x", "e_xp", "e_q", "e_qp", "e_w", "e_wp", "e_misc", "e_forces", "e_comp_off", "e_comp", "p_data",
+            "p_idx", "neighborTable", "nearWallTable", "cellTable", "ghostIndex", "ghostVec", "dem_scalars", "walls"]
+
+
+def _cases():
+    cases = {}
+
+    # plain lattice, no particles (demSolve=1 with an empty bed: the LB only steps
+    # when demTime advances, hybird.cpp:43-47)
+    cases["lb16"] = dict(cfg=base_config(lbSizeX=16, lbSizeY=16, lbSizeZ=16, initVisc=0.1, lbFZ=-1.0e-5),
+                         bed=Bed(np.zeros((0, 3)), np.zeros(0)), cycles=[0, 1, 2, 10], keys=LATTICE_KEYS)
+    # non-cubic channel, walls in y and z, body force along x, tau = 0.6
+    cases["chan"] = dict(cfg=base_config(lbSizeX=18, lbSizeY=14, lbSizeZ=12, initVisc=0.1 / 3.0, lbFX=1.0e-5,
+                                         boundary2=7, boundary3=7),
+                         bed=Bed(np.zeros((0, 3)), np.zeros(0)), cycles=[0, 1, 5], keys=LATTICE_KEYS)
+    # Bingham apparent viscosity (node.cpp:103-145)
+    cases["bingham"] = dict(cfg=base_config(lbSizeX=12, lbSizeY=10, lbSizeZ=14, nonNewtonianSolve=1,
+                                            plasticVisc=0.05, yieldStress=1.0e-5, initVisc=0.05, lbFX=1.0e-4),
+                            bed=Bed(np.zeros((0, 3)), np.zeros(0)), cycles=[0, 1, 3, 6], keys=LATTICE_KEYS)
+    # two spheres, one straddling the periodic x plane (ghosts), settling under gravity
+    cases["couple2"] = dict(cfg=base_config(lbSizeX=24, lbSizeY=24, lbSizeZ=24, initVisc=0.1, lbFZ=-1.0e-5),
+                            bed=Bed(np.array([[2.5, 12.25, 9.5], [15.75, 17.5, 13.0]]), np.array([4.0, 4.0])),
+                            cycles=[0, 1, 2, 5, 12], keys=LATTICE_KEYS + DEM_KEYS)
+    # a dense bed with contacts and a strong body force, lattice + DEM
+    # hand-placed contacts: a stack on the bottom wall, a side neighbour, a pair
+    # touching through the periodic x plane, a corner particle, plus a seeded fill
+    xs = [[6.0, 6.0, 3.48], [6.1, 6.05, 9.45], [11.98, 6.0, 3.5], [0.9, 15.0, 12.0], [16.92, 15.0, 12.0],
+          [1.2, 1.3, 19.49], [12.0, 17.0, 7.0], [17.0, 9.0, 17.5]]
+    rs = [3.0, 3.0, 3.0, 3.0, 3.0, 3.0, 3.2, 2.8]
+    bed = Bed(np.array(xs), np.array(rs))
+    cases["dense"] = dict(cfg=base_config(lbSizeX=24, lbSizeY=24, lbSizeZ=24, initVisc=0.05, lbFZ=-2.0e-4,
+                                          youngMod=2.0e3),
+                          bed=bed, cycles=[0, 1, 4, 20, 40], keys=LATTICE_KEYS + DEM_KEYS)
+    # DEM only: colliding particles with initial velocities, no fluid
+    bed2 = rsa_bed(11, (0.5, 0.5, 0.5), (38.5, 38.5, 38.5), lambda r: r.uniform(2.0, 4.0), 60,
+                   periodic=(True, True, False), period=(38.0, 38.0, 0.0), gap=0.05)
+    rng = np.random.default_rng(5)
+    vel = rng.uniform(-0.02, 0.02, size=(bed2.count, 3))
+    cases["dem60"] = dict(cfg=base_config(lbSolve=0, lbSizeX=40, lbSizeY=40, lbSizeZ=40, lbFZ=-1.0e-3,
+                                          youngMod=1.0e4),
+                          bed=bed2, vel=vel, cycles=[0, 1, 10, 100, 300], keys=DEM_KEYS)
+    # free surface: liquid below z = 20 (problemName SPLASH, LB.cpp:628-640), tilted body force
+    cases["fs"] = dict(cfg=base_config(problemName="SPLASH", freeSurfaceSolve=1, lbSizeX=12, lbSizeY=10, lbSizeZ=26,
+                                       boundary0=7, boundary1=7, boundary2=4, boundary3=4, initVisc=0.05,
+                                       initVelocityX=0.03, initVelocityZ=0.02, lbFX=5.0e-4, lbFZ=-5.0e-4),
+                       bed=Bed(np.zeros((0, 3)), np.zeros(0)), cycles=[0, 1, 2, 10, 30, 60], keys=LATTICE_KEYS)
+    return cases
+
+
+def run_case(name: str, case: dict, outdir: str) -> None:
+    with tempfile.TemporaryDirectory() as tmp:
+        cfg = dict(case["cfg"])
+        pf = os.path.join(tmp, "particles.dat")
+        of = os.path.join(tmp, "objects.dat")
+        write_particle_file(pf, case["bed"], case.get("vel"))
+        write_empty_object_file(of)
+        cfg["particleFile"] = pf
+        cfg["objectFile"] = of
+        cfg["maximumTimeSteps"] = max(case["cycles"])
+        cfgp = os.path.join(tmp, "case.cfg")
+        write_cfg(cfgp, cfg)
+        work = os.path.join(tmp, "work")
+        os.makedirs(work, exist_ok=True)
+        prefix = os.path.join(tmp, "dump")
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([HARNESS, cfgp, work, prefix, ",".join(str(c) for c in case["cycles"])], check=True,
+                       env=env, stdout=subprocess.DEVNULL)
+        arrays = {}
+        cyc = case["cycles"]
+        full_f = {cyc[0], cyc[1], cyc[-1]} if len(cyc) > 2 else set(cyc)
+        for c in cyc:
+            d = read_bin(f"{prefix}_{c}.bin")
+            for k in case["keys"]:
+                if k not in d:
+                    continue
+                if k == "neighbors" and c != cyc[0]:
+                    continue
+                if k == "f" and c not in full_f:
+                    continue
+                arrays[f"c{c}/{k}"] = d[k]
+        arrays["cycles"] = np.array(case["cycles"], dtype=np.int64)
+        # inputs, so the fixture is self-contained
+        keys = sorted(k for k, v in cfg.items() if not isinstance(v, str))
+        arrays["cfg_keys"] = np.array(keys)
+        arrays["cfg_vals"] = np.array([float(cfg[k]) for k in keys])
+        arrays["cfg_problem"] = np.array(cfg["problemName"])
+        arrays["bed_x"] = case["bed"].x
+        arrays["bed_r"] = case["bed"].r
+        arrays["bed_v"] = np.zeros_like(case["bed"].x) if case.get("vel") is None else np.asarray(case["vel"])
+        np.savez_compressed(os.path.join(outdir, f"{name}.npz"), **arrays)
+
+
+def main(argv) -> int:
+    if not os.path.exists(HARNESS):
+        print("reference harness missing: run `make -C oracle ref` first", file=sys.stderr)
+        return 1
+    cases = _cases()
+    names = argv[1:] or list(cases)
+    for n in names:
+        run_case(n, cases[n], HERE)
+        print(n, os.path.getsize(os.path.join(HERE, f"{n}.npz")))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv))
