@@ -1,0 +1,895 @@
+// C ABI of libhybird_amd.so (declared in include/hybird_amd.h).
+//
+// The host side of each entry point restates the corresponding reference
+// method's set-up logic (parsing conversions, wall and periodic planes, element
+// mass and inertia, neighbour-cell sizing) and then drives the HIP kernels of
+// lb.hip and dem.hip on one stream per device. Nothing here synchronises with
+// the device except the host views, hyb_sync and the timing queries.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "dem_kernels.h"
+#include "hybird_amd.h"
+#include "lb_kernels.h"
+
+using namespace hyb;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) return fail(HYB_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+hipStream_t shared_stream() {
+    static hipStream_t s = nullptr;
+    if (!s) (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    return s;
+}
+
+Units make_units(double L, double T, double D) {  // node.cpp:476-488
+    Units u;
+    u.Length = L;
+    u.Time = T;
+    u.Density = D;
+    u.Volume = L * L * L;
+    u.Speed = L / T;
+    u.Accel = L / T / T;
+    u.AngVel = 1.0 / T;
+    u.KinVisc = L * L / T;
+    u.DynVisc = D * L * L / T;
+    u.Force = D * L * L * L * L / T / T;
+    u.Torque = D * L * L * L * L * L / T / T;
+    u.Mass = D * L * L * L;
+    u.Stress = D * L * L / T / T;
+    return u;
+}
+
+template <class T>
+int dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return HYB_OK;
+    HIPCHK(hipMalloc((void**)p, count * sizeof(T)));
+    return HYB_OK;
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ handles
+struct hyb_dem {
+    hyb_config cfg;
+    int P = 0;
+    std::vector<double> rec;
+    // material (DEM.cpp:15-61)
+    double knConst = 0, ksConst = 0, dampCoeff = 0;
+    Units unit;
+    double demTime = 0.0, deltat = 1.0, demInitialRepeat = 0.0;
+    uint32_t demTimeStep = 0;
+    int multiStep = 1;
+    double demSize[3] = {1, 1, 1};
+    std::vector<WallDev> walls;
+    std::vector<PbcDev> pbcs;
+    DEMParams p;
+    DEMBuffers b;
+    bool inited = false;
+    hipStream_t stream = nullptr;
+    int64_t substeps = 0;
+};
+
+struct hyb_lb {
+    hyb_config cfg;
+    Units unit;
+    int X = 1, Y = 1, Z = 1;
+    size_t N = 1;
+    double initDynVisc = 0, plasticVisc = 0, yieldStress = 0, turbConst = 0, slipCoefficient = 0, initDensity = 1.0;
+    double initVelocity[3] = {0, 0, 0}, lbF[3] = {0, 0, 0};
+    int boundary[6] = {1, 1, 1, 1, 1, 1};
+    bool freeSurface = false, forceField = false, nonNewtonian = false, turbulenceOn = false;
+    Prec prec = Prec::F64;
+    hipStream_t stream = nullptr;
+    // device state
+    void* f[2] = {nullptr, nullptr};
+    int cur = 0;
+    uint8_t* type = nullptr;
+    uint32_t* link = nullptr;
+    uint32_t* solid = nullptr;
+    void* hf = nullptr;
+    double* visc = nullptr;
+    double* mac = nullptr;
+    void* scratch = nullptr;
+    int* maxXYZ = nullptr;
+    unsigned long long* dcount = nullptr;
+    CoupleRec* cp = nullptr;
+    NodeBox* boxes = nullptr;
+    int* nslots = nullptr;  // [0] slots this step, [1] boxes marked last step
+    HydroPartial* part = nullptr;
+    int slotCap = 0;
+    hyb_dem* dem = nullptr;  // DEM handle coupled at init (read by the host views)
+    int64_t steps = 0;
+    bool inited = false;
+    bool macValid = false;
+    bool havePart = false;
+    unsigned long long activeCount = 0;
+    double totalMass = 0.0;
+    uint32_t time = 0;
+    // profiling of the fused kernel
+    bool profiling = false;
+    std::vector<hipEvent_t> evs;
+    size_t evUsed = 0;
+    double profMs = 0.0;
+    int64_t profLaunches = 0;
+
+    bool varVisc() const { return nonNewtonian || turbulenceOn; }
+    LBKParams params() const {
+        LBKParams p;
+        p.X = X; p.Y = Y; p.Z = Z;
+        p.per[0] = boundary[0] == 4; p.per[1] = boundary[2] == 4; p.per[2] = boundary[4] == 4;
+        // collision() zeroes the force permanently when forceFieldSolve = 0 (LB.cpp:1074-1076)
+        for (int k = 0; k < 3; ++k) p.F[k] = forceField ? lbF[k] : 0.0;
+        p.visc0 = initDynVisc;
+        p.plasticVisc = plasticVisc; p.yieldStress = yieldStress; p.turbConst = turbConst;
+        p.nonNewtonian = nonNewtonian; p.turbulence = turbulenceOn;
+        p.collision = cfg.collision;
+        p.trtMagic = cfg.trtMagic;
+        p.unitLength = unit.Length; p.unitSpeed = unit.Speed; p.unitAngVel = unit.AngVel;
+        p.initDensity = initDensity;
+        return p;
+    }
+    size_t popBytes() const { return (prec == Prec::F64 ? 8 : 4) * 19 * N; }
+};
+
+extern "C" {
+
+const char* hyb_last_error(void) { return g_err.c_str(); }
+
+int hyb_device_count(int* count) {
+    if (!count) return fail(HYB_E_ARG, "null count");
+    HIPCHK(hipGetDeviceCount(count));
+    return HYB_OK;
+}
+
+void hyb_config_defaults(hyb_config* c) {
+    std::memset(c, 0, sizeof(*c));
+    c->problem = HYB_PROBLEM_NONE;
+    c->unitLength = c->unitTime = c->unitDensity = 1.0;
+    c->lbSize[0] = c->lbSize[1] = c->lbSize[2] = 16;
+    c->plasticVisc = 1.0;
+    c->yieldStress = 1.0;
+    c->initVisc = 1.0;
+    for (int k = 0; k < 6; ++k) c->boundary[k] = 1;
+    c->density = 1.0;
+    c->contactModel = HYB_CONTACT_HERTZIAN;
+    c->youngMod = 1.0;
+    c->poisson = 1.0;
+    c->linearStiff = 1.0;
+    c->restitution = 1.0;
+    c->viscTang = 1.0;
+    c->frictionCoefPart = 1.0;
+    c->frictionCoefWall = 1.0;
+    c->multiStep = 1;
+    c->criticalRatio = 0.1;
+    c->scale = 1.0;
+    c->collision = HYB_COLLISION_BGK;
+    c->trtMagic = 0.25;
+    c->precision = HYB_FP64;
+}
+
+// ------------------------------------------------------------------ DEM
+int hyb_dem_create(const hyb_config* cfg, int32_t n, const double* rec, hyb_dem** out) {
+    if (!cfg || !out || n < 0 || (n > 0 && !rec)) return fail(HYB_E_ARG, "hyb_dem_create: bad arguments");
+    hyb_dem* d = new hyb_dem();
+    d->cfg = *cfg;
+    // DEM.cpp:15-61 material constants; the HERTZIAN case falls through to the
+    // LINEAR damping formula (with +pi), which is what the reference uses.
+    if (!(cfg->density > 0.0) || !(cfg->restitution > 0.0) || cfg->restitution > 1.0 || !(cfg->youngMod > 0.0)) {
+        delete d;
+        return fail(HYB_E_ARG, "invalid material parameters (density, restitution, youngMod)");
+    }
+    d->knConst = 2.0 / 3.0 * cfg->youngMod / (1 - cfg->poisson * cfg->poisson);
+    d->ksConst = 2.0 * cfg->youngMod / (2.0 - cfg->poisson) / (1.0 + cfg->poisson);
+    const double le = std::log(cfg->restitution);
+    d->dampCoeff = -1.0 * std::sqrt(2.0) * le / std::sqrt((le * le + M_PI));
+    if (!(d->dampCoeff < 1.0)) { delete d; return fail(HYB_E_ARG, "dampCoeff >= 1"); }
+    d->P = n;
+    d->rec.assign(rec, rec + (size_t)n * HYB_ELEMENT_RECORD);
+    for (int e = 0; e < n; ++e) {
+        const double* r = rec + (size_t)e * HYB_ELEMENT_RECORD;
+        if ((int)r[0] != e) { delete d; return fail(HYB_E_ARG, "element indices must be 0..n-1 in file order"); }
+        if ((int)r[1] != 1) {
+            delete d;
+            return fail(HYB_E_UNSUPPORTED, "multi-sphere elements (size > 1) are not supported by the GPU path");
+        }
+    }
+    d->multiStep = cfg->multiStep;
+    d->demInitialRepeat = cfg->demInitialRepeat;
+    d->stream = shared_stream();
+    *out = d;
+    return HYB_OK;
+}
+
+int hyb_dem_init(hyb_dem* d, const hyb_lb* lb) {
+    if (!d || !lb) return fail(HYB_E_ARG, "hyb_dem_init: null handle");
+    const hyb_config& c = d->cfg;
+    d->unit = lb->unit;
+    const Units& U = d->unit;
+    // DEM.cpp:185-233
+    for (int k = 0; k < 3; ++k) d->demSize[k] = double(k == 0 ? lb->X : (k == 1 ? lb->Y : lb->Z)) * U.Length;
+    const V3 demF = V3(lb->lbF[0], lb->lbF[1], lb->lbF[2]) * U.Accel;
+    if (d->P) {
+        if (d->multiStep == 0) return fail(HYB_E_UNSUPPORTED, "multiStep = 0 (automatic) is not supported");
+        d->deltat = U.Time / (double(d->multiStep));
+    } else {
+        d->deltat = U.Time;
+        d->multiStep = 1;
+    }
+    DEMParams& p = d->p;
+    std::memset(&p, 0, sizeof(p));
+    p.P = d->P;
+    // walls (DEM.cpp:435-638, the six box planes)
+    d->walls.clear();
+    const V3 ps[6] = {V3(0.5 * U.Length, 0, 0), V3((double(lb->X) - 1.5) * U.Length, 0, 0), V3(0, 0.5 * U.Length, 0),
+                      V3(0, (double(lb->Y) - 1.5) * U.Length, 0), V3(0, 0, 0.5 * U.Length),
+                      V3(0, 0, (double(lb->Z) - 1.5) * U.Length)};
+    const V3 ns[6] = {V3(1, 0, 0), V3(-1, 0, 0), V3(0, 1, 0), V3(0, -1, 0), V3(0, 0, 1), V3(0, 0, -1)};
+    for (int k = 0; k < 6; ++k) {
+        const int bc = lb->boundary[k];
+        if (bc == 5 || bc == 6 || bc == 7 || bc == 8) {
+            WallDev w;
+            w.n = ns[k];
+            w.p = ps[k];
+            w.rotCenter = V3();
+            w.omega = V3();
+            w.vel = V3();
+            w.moving = (bc == 6 || bc == 8);
+            w.slip = (bc == 5 || bc == 6);
+            d->walls.push_back(w);
+        }
+    }
+    // periodic pairs (DEM.cpp:937-988)
+    d->pbcs.clear();
+    for (int k = 0; k < 3; ++k) {
+        if (lb->boundary[2 * k] == 4) {
+            if (lb->boundary[2 * k + 1] != 4) return fail(HYB_E_ARG, "Periodic boundaries error!");
+            PbcDev pb;
+            V3 base, v;
+            const double sz = double(k == 0 ? lb->X : (k == 1 ? lb->Y : lb->Z));
+            if (k == 0) { base = V3(0.5 * U.Length, 0, 0); v = V3((sz - 2.0) * U.Length, 0, 0); }
+            if (k == 1) { base = V3(0, 0.5 * U.Length, 0); v = V3(0, (sz - 2.0) * U.Length, 0); }
+            if (k == 2) { base = V3(0, 0, 0.5 * U.Length); v = V3(0, 0, (sz - 2.0) * U.Length); }
+            pb.v = v;
+            pb.p1 = base;
+            pb.p2 = base + v;
+            pb.n1 = v / v.norm();
+            pb.n2 = -1.0 * pb.n1;
+            d->pbcs.push_back(pb);
+        }
+    }
+    p.npbc = (int)d->pbcs.size();
+    p.nwalls = (int)d->walls.size();
+    for (int k = 0; k < p.npbc; ++k) p.pbc[k] = d->pbcs[k];
+    for (int k = 0; k < p.nwalls; ++k) p.walls[k] = d->walls[k];
+    // neighbour-cell sizing (DEM.cpp:1270-1312)
+    double maxRad = 0.0;
+    for (int e = 0; e < d->P; ++e) maxRad = std::max(maxRad, d->rec[(size_t)e * HYB_ELEMENT_RECORD + 2] * c.scale);
+    for (int k = 0; k < 3; ++k) {
+        p.cellWidth[k] = d->P ? std::min(maxRad * 5.0, d->demSize[k]) : d->demSize[k];
+        p.nCells[k] = (unsigned)int(std::ceil(d->demSize[k] / p.cellWidth[k]));
+        p.cellWidth[k] = d->demSize[k] / double(p.nCells[k]);
+        p.nCells[k] += 2;
+    }
+    p.nebrRange = d->P ? std::max(maxRad * 3.0, 0.5 * std::min(p.cellWidth[0], std::min(p.cellWidth[1], p.cellWidth[2])))
+                       : 0.0;
+    p.trigger = 0.25 * p.nebrRange;
+    p.deltat = d->deltat;
+    const double dt = d->deltat;
+    const double cc[5] = {dt, dt * dt / 2.0, dt * dt * dt / 6.0, dt * dt * dt * dt / 24.0, dt * dt * dt * dt * dt / 120.0};
+    for (int k = 0; k < 5; ++k) { p.c1[k] = cc[k]; p.c2[k] = cc[k]; }
+    const double g1[6] = {95.0 / 288.0, 1.0, 25.0 / 24.0, 35.0 / 72.0, 5.0 / 48.0, 1.0 / 120.0};
+    const double g2[6] = {3.0 / 16.0, 251.0 / 360.0, 1.0, 11.0 / 18.0, 1.0 / 6.0, 1.0 / 60.0};
+    const double co1[6] = {g1[0] * cc[0], g1[1] * cc[0] / cc[0], g1[2] * cc[0] / cc[1], g1[3] * cc[0] / cc[2],
+                           g1[4] * cc[0] / cc[3], g1[5] * cc[0] / cc[4]};
+    const double co2[6] = {g2[0] * cc[1], g2[1] * cc[1] / cc[0], g2[2] * cc[1] / cc[1], g2[3] * cc[1] / cc[2],
+                           g2[4] * cc[1] / cc[3], g2[5] * cc[1] / cc[4]};
+    for (int k = 0; k < 6; ++k) { p.co1[k] = co1[k]; p.co2[k] = co2[k]; }
+    p.knConst = d->knConst; p.ksConst = d->ksConst; p.dampCoeff = d->dampCoeff;
+    p.viscTang = c.viscTang; p.fricPart = c.frictionCoefPart; p.fricWall = c.frictionCoefWall;
+    p.linearStiff = c.linearStiff;
+    p.hertzian = c.contactModel == HYB_CONTACT_HERTZIAN;
+    p.numVisc = c.numVisc;
+    p.demF[0] = demF.x; p.demF[1] = demF.y; p.demF[2] = demF.z;
+    // capacities
+    const int np = p.npbc;
+    const int ghostsPer = np + np * (np - 1) / 2;
+    p.Gcap = std::max(1, d->P * std::max(ghostsPer, 1));
+    const int scap = d->P + p.Gcap;
+    p.pairCap = std::max(64, scap * 48);
+    DEMBuffers& b = d->b;
+    std::memset(&b, 0, sizeof(b));
+    b.scap = scap;
+    b.ncells = (int)(p.nCells[0] * p.nCells[1] * p.nCells[2]);
+    b.W = b.ncells * 14;
+    // element state, host side (elmt.cpp:17-83 with wSolver = true, single spheres)
+    const int P = d->P;
+    std::vector<double> el((size_t)EF_COUNT * std::max(P, 1), 0.0);
+    auto at = [&](int f, int e) -> double& { return el[(size_t)f * P + e]; };
+    auto set3 = [&](int f, int e, const V3& v) { at(f, e) = v.x; at(f + 1, e) = v.y; at(f + 2, e) = v.z; };
+    auto set4 = [&](int f, int e, const Q4& q) { at(f, e) = q.q0; at(f + 1, e) = q.q1; at(f + 2, e) = q.q2; at(f + 3, e) = q.q3; };
+    const V3 translate(c.translate[0], c.translate[1], c.translate[2]);
+    for (int e = 0; e < P; ++e) {
+        const double* r = &d->rec[(size_t)e * HYB_ELEMENT_RECORD];
+        const double radius = r[2] * c.scale;
+        const V3 x0 = V3(r[3] * c.scale, r[4] * c.scale, r[5] * c.scale) + translate;
+        const V3 x1(r[6], r[7], r[8]);
+        const V3 w0(r[9], r[10], r[11]);
+        const Q4 q0(r[12], r[13], r[14], r[15]);
+        const Q4 q1(r[16], r[17], r[18], r[19]);
+        set3(EF_X, e, x0); set3(EF_X + 3, e, x1);
+        set3(EF_XP, e, x0); set3(EF_XP + 3, e, x1);
+        set4(EF_Q, e, q0); set4(EF_Q + 4, e, q1);
+        set4(EF_QP, e, q0); set4(EF_QP + 4, e, q1);
+        set3(EF_W, e, w0);
+        set3(EF_WG, e, w0); set3(EF_WL, e, w0); set3(EF_WPG, e, w0); set3(EF_WPL, e, w0);
+        const double singleMass = 4.0 / 3.0 * c.density * M_PI * radius * radius * radius;
+        const double m = 1 * singleMass;
+        V3 I = 1 * 2.0 / 5.0 * singleMass * radius * radius * V3(1.0, 1.0, 1.0);
+        const V3 proto(0.0, 0.0, 0.0);
+        I += singleMass * radius * radius * V3(proto.y * proto.y + proto.z * proto.z, proto.z * proto.z + proto.x * proto.x,
+                                               proto.x * proto.x + proto.y * proto.y);
+        at(EF_M, e) = m;
+        at(EF_RAD, e) = radius;
+        set3(EF_I, e, I);
+        set3(EF_FG, e, demF * m);
+    }
+    // device buffers
+    int rc;
+    if ((rc = dalloc(&b.el, el.size())) || (rc = dalloc(&b.st, 1)) || (rc = dalloc(&b.ghostIndex, p.Gcap)) ||
+        (rc = dalloc(&b.ghostVec, 3 * (size_t)p.Gcap)) || (rc = dalloc(&b.pairs, 2 * (size_t)p.pairCap)) ||
+        (rc = dalloc(&b.nearWall, std::max(P, 1))) || (rc = dalloc(&b.cellTable, (size_t)scap + b.ncells)) ||
+        (rc = dalloc(&b.cellOf, scap)) || (rc = dalloc(&b.cellCount, b.ncells + 1)) ||
+        (rc = dalloc(&b.work, (size_t)std::max(std::max(b.W, scap), std::max(p.pairCap, b.ncells + 1)) + 1)) ||
+        (rc = dalloc(&b.csrOff, (size_t)P + 1)) || (rc = dalloc(&b.csr, 2 * (size_t)p.pairCap)) ||
+        (rc = dalloc(&b.cursor, scap)) || (rc = dalloc(&b.ghostOfCount, std::max(P, 1))) ||
+        (rc = dalloc(&b.ghostOf, 8 * (size_t)std::max(P, 1))) || (rc = dalloc(&b.side, 3 * (size_t)std::max(P, 1))) ||
+        (rc = dalloc(&b.ghostB, p.Gcap)))
+        return rc;
+    HIPCHK(hipMemcpyAsync(b.el, el.data(), el.size() * sizeof(double), hipMemcpyHostToDevice, d->stream));
+    DEMState st;
+    std::memset(&st, 0, sizeof(st));
+    st.maxDisp = 0.5 * p.nebrRange;  // DEM.cpp:1307
+    st.nSlots = P;
+    HIPCHK(hipMemcpyAsync(b.st, &st, sizeof(st), hipMemcpyHostToDevice, d->stream));
+    HIPCHK(hipMemsetAsync(b.nearWall, 0xFF, sizeof(uint32_t) * std::max(P, 1), d->stream));
+    HIPCHK(hipMemsetAsync(b.csrOff, 0, sizeof(uint32_t) * ((size_t)P + 1), d->stream));
+    HIPCHK(hipMemsetAsync(b.ghostOfCount, 0, sizeof(int) * std::max(P, 1), d->stream));
+    if (P) launch_dem_rebuild(p, b, 1, d->stream);  // evalNeighborTable at init (DEM.cpp:258-262)
+    HIPCHK(hipGetLastError());
+    d->demTime = 0.0;
+    d->demTimeStep = 0;
+    d->inited = true;
+    return HYB_OK;
+}
+
+int hyb_dem_step(hyb_dem* d) {  // DEM.cpp:331-376
+    if (!d || !d->inited) return fail(HYB_E_STATE, "hyb_dem_step before hyb_dem_init");
+    for (int it = 0; it < d->multiStep; ++it) {
+        d->demTimeStep++;
+        d->demTime += d->deltat;
+        if (d->P) {
+            launch_dem_rebuild(d->p, d->b, 0, d->stream);
+            launch_dem_predict(d->p, d->b, d->stream);
+            launch_dem_forces_correct(d->p, d->b, d->stream);
+        }
+        d->substeps++;
+    }
+    HIPCHK(hipGetLastError());
+    return HYB_OK;
+}
+
+int hyb_dem_destroy(hyb_dem* d) {
+    if (!d) return HYB_OK;
+    if (d->stream) (void)hipStreamSynchronize(d->stream);
+    DEMBuffers& b = d->b;
+    dfree(b.el); dfree(b.st); dfree(b.ghostIndex); dfree(b.ghostVec); dfree(b.pairs); dfree(b.nearWall);
+    dfree(b.cellTable); dfree(b.cellOf); dfree(b.cellCount); dfree(b.work); dfree(b.csrOff); dfree(b.csr);
+    dfree(b.cursor); dfree(b.ghostOfCount); dfree(b.ghostOf); dfree(b.side); dfree(b.ghostB);
+    delete d;
+    return HYB_OK;
+}
+
+// ------------------------------------------------------------------ LB
+int hyb_lb_create(const hyb_config* cfg, hyb_lb** out) {  // LB.cpp:85-188 + LB.cpp:42-83
+    if (!cfg || !out) return fail(HYB_E_ARG, "hyb_lb_create: null argument");
+    hyb_lb* l = new hyb_lb();
+    l->cfg = *cfg;
+    const hyb_config& c = *cfg;
+    if (!(c.unitLength > 0) || !(c.unitDensity > 0)) { delete l; return fail(HYB_E_ARG, "unit length/density must be > 0"); }
+    l->nonNewtonian = c.nonNewtonian != 0;
+    l->turbulenceOn = c.turbulence != 0;
+    l->freeSurface = c.freeSurface != 0;
+    l->forceField = c.forceField != 0;
+    l->X = int(std::floor(c.lbSize[0] / c.unitLength + 0.5));
+    l->Y = int(std::floor(c.lbSize[1] / c.unitLength + 0.5));
+    l->Z = int(std::floor(c.lbSize[2] / c.unitLength + 0.5));
+    if (l->X < 3 || l->Y < 3 || l->Z < 3) { delete l; return fail(HYB_E_ARG, "lattice must have at least 3 nodes per axis"); }
+    l->N = (size_t)l->X * l->Y * l->Z;
+    double unitTime = c.unitTime;
+    if (unitTime == 0.0) {
+        const double tauOpt = 1.0;
+        unitTime = l->nonNewtonian ? c.unitDensity * c.unitLength * c.unitLength / c.plasticVisc * (1.0 / 3.0) * (tauOpt - 0.5)
+                                   : c.unitDensity * c.unitLength * c.unitLength / c.initVisc * (1.0 / 3.0) * (tauOpt - 0.5);
+    }
+    l->unit = make_units(c.unitLength, unitTime, c.unitDensity);
+    l->initDynVisc = c.initVisc / l->unit.DynVisc;
+    l->plasticVisc = c.plasticVisc / l->unit.DynVisc;
+    l->yieldStress = c.yieldStress / l->unit.Stress;
+    l->initDensity = 1.0;
+    for (int k = 0; k < 3; ++k) {
+        l->initVelocity[k] = c.initVelocity[k] / l->unit.Speed;
+        l->lbF[k] = c.lbF[k] / l->unit.Accel;
+    }
+    for (int k = 0; k < 6; ++k) {
+        l->boundary[k] = c.boundary[k];
+        if (c.boundary[k] != 4 && c.boundary[k] != 7) {
+            delete l;
+            return fail(HYB_E_UNSUPPORTED, "only periodic (4) and static no-slip (7) box boundaries are supported");
+        }
+    }
+    for (int k = 0; k < 3; ++k)
+        if ((c.boundary[2 * k] == 4) != (c.boundary[2 * k + 1] == 4)) { delete l; return fail(HYB_E_ARG, "Periodic boundaries error!"); }
+    l->slipCoefficient = c.slipCoefficient;
+    l->turbConst = c.turbConst;
+    if (c.problem != HYB_PROBLEM_NONE || l->freeSurface) {
+        delete l;
+        return fail(HYB_E_UNSUPPORTED, "free surface and gas regions are not supported by the GPU path yet");
+    }
+    if (c.collision != HYB_COLLISION_BGK && c.collision != HYB_COLLISION_TRT) { delete l; return fail(HYB_E_ARG, "unknown collision model"); }
+    l->prec = c.precision == HYB_FP32_SHIFTED ? Prec::F32S : Prec::F64;
+    l->stream = shared_stream();
+    *out = l;
+    return HYB_OK;
+}
+
+static int lb_alloc(hyb_lb* l, int slotCap) {
+    const size_t N = l->N;
+    int rc;
+    const size_t pb = l->popBytes();
+    HIPCHK(hipMalloc(&l->f[0], pb));
+    HIPCHK(hipMalloc(&l->f[1], pb));
+    if ((rc = dalloc(&l->type, N)) || (rc = dalloc(&l->link, N)) || (rc = dalloc(&l->solid, N)) ||
+        (rc = dalloc(&l->maxXYZ, 3)) || (rc = dalloc(&l->dcount, 1)) || (rc = dalloc(&l->nslots, 2)))
+        return rc;
+    if (l->varVisc() && (rc = dalloc(&l->visc, N))) return rc;
+    l->slotCap = slotCap;
+    if (slotCap > 0) {
+        HIPCHK(hipMalloc(&l->hf, (l->prec == Prec::F64 ? 8 : 4) * 3 * N));
+        if ((rc = dalloc(&l->cp, slotCap)) || (rc = dalloc(&l->boxes, slotCap)) || (rc = dalloc(&l->part, slotCap)))
+            return rc;
+    }
+    return HYB_OK;
+}
+
+static void lb_couple_launch(hyb_lb* l, hyb_dem* d) {
+    const LBKParams p = l->params();
+    launch_dem_couple_table(d->p, d->b, l->cp, l->nslots, l->stream);
+    launch_classify_clear(l->nslots + 1, l->slotCap, l->boxes, l->solid, p, l->stream);
+    launch_classify_set(l->nslots, l->slotCap, l->cp, l->boxes, l->nslots + 1, l->solid, l->link, p, l->stream);
+}
+
+int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
+    if (!l) return fail(HYB_E_ARG, "hyb_lb_init: null handle");
+    if (d && !d->inited) return fail(HYB_E_STATE, "hyb_lb_init: DEM handle not initialised");
+    const int slotCap = d ? d->P + d->p.Gcap : 0;
+    int rc = lb_alloc(l, d && d->P ? slotCap : 0);
+    if (rc) return rc;
+    const LBKParams p = l->params();
+    hipStream_t s = l->stream;
+    // node types: box boundaries, DEM plane walls, gas/interface
+    std::vector<WallDev> walls = d ? d->walls : std::vector<WallDev>();
+    WallDev* dw = nullptr;
+    if (!walls.empty()) {
+        HIPCHK(hipMalloc((void**)&dw, walls.size() * sizeof(WallDev)));
+        HIPCHK(hipMemcpyAsync(dw, walls.data(), walls.size() * sizeof(WallDev), hipMemcpyHostToDevice, s));
+    }
+    launch_lb_init_types(l->type, l->boundary, (int)walls.size(), dw, l->cfg.problem, l->cfg.liquidBox, p, s);
+    launch_lb_init_interface(l->type, p, s);
+    launch_lb_init_links(l->type, l->link, p, s);
+    HIPCHK(hipMemsetAsync(l->maxXYZ, 0, 3 * sizeof(int), s));
+    launch_lb_max_active(l->link, l->maxXYZ, p, s);
+    HIPCHK(hipMemsetAsync(l->f[0], 0, l->popBytes(), s));
+    HIPCHK(hipMemsetAsync(l->f[1], 0, l->popBytes(), s));
+    launch_lb_init_pops(l->prec, l->f[0], l->link, l->maxXYZ, l->visc, l->initVelocity, l->lbF, p, s);
+    launch_fill_u32(l->solid, 0u, l->N, s);
+    HIPCHK(hipMemsetAsync(l->nslots, 0, 2 * sizeof(int), s));
+    HIPCHK(hipMemsetAsync(l->dcount, 0, sizeof(unsigned long long), s));
+    launch_count_active(l->link, l->dcount, p, s);
+    // particle nodes at init (LB.cpp:475-495)
+    if (d && d->P) lb_couple_launch(l, d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    if (dw) (void)hipFree(dw);
+    HIPCHK(hipMemcpy(&l->activeCount, l->dcount, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    // total mass of active nodes outside particles (LB.cpp:209-216): fluid nodes carry mass 1
+    {
+        std::vector<uint32_t> lk(l->N), so(l->N);
+        HIPCHK(hipMemcpy(lk.data(), l->link, l->N * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(so.data(), l->solid, l->N * 4, hipMemcpyDeviceToHost));
+        double tm = 0.0;
+        for (size_t i = 0; i < l->N; ++i)
+            if ((lk[i] & LINK_ACTIVE) && !so[i]) tm += (lk[i] & LINK_INTERFACE) ? 0.5 * l->initDensity : l->initDensity;
+        l->totalMass = tm;
+    }
+    l->steps = 0;
+    l->cur = 0;
+    l->time = 0;
+    l->dem = d;
+    l->inited = true;
+    return HYB_OK;
+}
+
+int hyb_lb_fs_step(hyb_lb* l) {
+    if (!l || !l->inited) return fail(HYB_E_STATE, "hyb_lb_fs_step before init");
+    if (l->freeSurface) return fail(HYB_E_UNSUPPORTED, "free surface not supported by the GPU path yet");
+    return HYB_OK;
+}
+
+int hyb_lb_couple(hyb_lb* l, hyb_dem* d, int32_t* newNeighborList) {  // LB.cpp:247-280
+    if (!l || !l->inited) return fail(HYB_E_STATE, "hyb_lb_couple before init");
+    // The GPU re-tests every node against every particle slot at each coupling
+    // step, so the rebuild handshake flag is consumed unconditionally.
+    if (newNeighborList) *newNeighborList = 0;
+    if (!d || !d->P) return HYB_OK;
+    lb_couple_launch(l, d);
+    HIPCHK(hipGetLastError());
+    return HYB_OK;
+}
+
+int hyb_lb_step(hyb_lb* l, hyb_dem* d) {  // LB.cpp:221-233 fused into one kernel
+    if (!l || !l->inited) return fail(HYB_E_STATE, "hyb_lb_step before init");
+    const LBKParams p = l->params();
+    const bool withParticles = d && d->P && l->slotCap > 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (l->profiling) {
+        if (l->evUsed + 2 > l->evs.size()) {
+            for (int k = 0; k < 64; ++k) {
+                hipEvent_t ev;
+                HIPCHK(hipEventCreate(&ev));
+                l->evs.push_back(ev);
+            }
+        }
+        e0 = l->evs[l->evUsed++];
+        e1 = l->evs[l->evUsed++];
+        HIPCHK(hipEventRecord(e0, l->stream));
+    }
+    launch_lb_fused(l->prec, l->steps > 0, l->varVisc(), l->cfg.collision, l->f[l->cur], l->f[l->cur ^ 1], l->link,
+                    withParticles ? l->solid : nullptr, l->cp, withParticles ? l->hf : nullptr, l->visc, nullptr, p,
+                    l->stream);
+    if (l->profiling) HIPCHK(hipEventRecord(e1, l->stream));
+    if (withParticles) {
+        launch_hydro_partials(l->nslots, l->slotCap, l->boxes, l->cp, l->solid, l->link, l->prec, l->hf, l->part, p,
+                              l->stream);
+        launch_dem_hydro_combine(d->p, d->b, l->part, l->unit, l->stream);
+    }
+    HIPCHK(hipGetLastError());
+    l->cur ^= 1;
+    l->steps++;
+    l->macValid = false;
+    return HYB_OK;
+}
+
+int hyb_lb_destroy(hyb_lb* l) {
+    if (!l) return HYB_OK;
+    if (l->stream) (void)hipStreamSynchronize(l->stream);
+    for (auto& ev : l->evs) (void)hipEventDestroy(ev);
+    if (l->f[0]) (void)hipFree(l->f[0]);
+    if (l->f[1]) (void)hipFree(l->f[1]);
+    if (l->hf) (void)hipFree(l->hf);
+    if (l->scratch) (void)hipFree(l->scratch);
+    dfree(l->type); dfree(l->link); dfree(l->solid); dfree(l->visc); dfree(l->mac); dfree(l->maxXYZ);
+    dfree(l->dcount); dfree(l->cp); dfree(l->boxes); dfree(l->nslots); dfree(l->part);
+    delete l;
+    return HYB_OK;
+}
+
+int hyb_cycle(hyb_lb* l, hyb_dem* d, int32_t n) {  // hybird.cpp:35-66 without IO
+    if (!l || !d) return fail(HYB_E_ARG, "hyb_cycle: null handle");
+    const bool demSolve = l->cfg.demSolve != 0, lbSolve = l->cfg.lbSolve != 0;
+    for (int k = 0; k < n; ++k) {
+        ++l->time;
+        int rc;
+        if (demSolve && (rc = hyb_dem_step(d))) return rc;
+        if (lbSolve && d->demTime > d->demInitialRepeat) {
+            if (l->freeSurface && (rc = hyb_lb_fs_step(l))) return rc;
+            int32_t flag = 1;
+            if (demSolve && (rc = hyb_lb_couple(l, d, &flag))) return rc;
+            if ((rc = hyb_lb_step(l, d))) return rc;
+        } else if (lbSolve) {
+            int32_t flag = 1;
+            if (demSolve && (rc = hyb_lb_couple(l, d, &flag))) return rc;
+        }
+    }
+    return HYB_OK;
+}
+
+int hyb_sync(hyb_lb* l) {
+    hipStream_t s = l ? l->stream : shared_stream();
+    HIPCHK(hipStreamSynchronize(s));
+    return HYB_OK;
+}
+
+// ------------------------------------------------------------------ host views
+static int put(const void* src, size_t bytes, void* host, size_t cap, size_t* need) {
+    if (need) *need = bytes;
+    if (host && cap >= bytes && bytes) std::memcpy(host, src, bytes);
+    return HYB_OK;
+}
+
+static int ensure_mac(hyb_lb* l, hyb_dem* d) {
+    // n, u (shifted) and hydroForce of the last collision, recomputed from that
+    // collision's input with the same kernel code (no population write).
+    if (l->macValid) return HYB_OK;
+    int rc;
+    if (!l->mac && (rc = dalloc(&l->mac, 7 * l->N))) return rc;
+    const LBKParams p = l->params();
+    if (l->steps == 0) {
+        // state after init: recompute the hydrostatic values the init kernel used
+        std::vector<double> h(7 * l->N, NAN);
+        std::vector<uint32_t> lk(l->N);
+        std::vector<int> mx(3);
+        HIPCHK(hipMemcpy(lk.data(), l->link, l->N * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(mx.data(), l->maxXYZ, 12, hipMemcpyDeviceToHost));
+        const V3 F(l->lbF[0], l->lbF[1], l->lbF[2]);
+        for (size_t i = 0; i < l->N; ++i) {
+            if (!(lk[i] & LINK_ACTIVE)) continue;
+            const int x = (int)(i % l->X), y = (int)((i / l->X) % l->Y), z = (int)(i / ((size_t)l->X * l->Y));
+            double n;
+            if (lk[i] & LINK_INTERFACE) n = l->initDensity;
+            else {
+                const V3 dh = V3(x, y, z) - V3(mx[0], mx[1], mx[2]);
+                n = l->initDensity + 3.0 * 1.0 * 1.0 * l->initDensity * dh.dot(F);
+            }
+            const V3 u = (1.0 * F) / 2.0 / n + V3(l->initVelocity[0], l->initVelocity[1], l->initVelocity[2]);
+            h[i] = n;
+            h[l->N + i] = u.x; h[2 * l->N + i] = u.y; h[3 * l->N + i] = u.z;
+            h[4 * l->N + i] = 0.0; h[5 * l->N + i] = 0.0; h[6 * l->N + i] = 0.0;
+        }
+        HIPCHK(hipMemcpy(l->mac, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        const bool withParticles = d && d->P && l->slotCap > 0;
+        launch_fill_u32((uint32_t*)l->mac, 0xFFFFFFFFu, 14 * l->N, l->stream);  // NaN pattern for inactive nodes
+        launch_lb_fused(l->prec, l->steps > 1, l->varVisc(), l->cfg.collision, l->f[l->cur ^ 1], nullptr, l->link,
+                        withParticles ? l->solid : nullptr, l->cp, nullptr, l->visc, l->mac, p, l->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(l->stream));
+    }
+    l->macValid = true;
+    return HYB_OK;
+}
+
+int hyb_lb_get(hyb_lb* l, const char* cname, void* host, size_t cap, size_t* need) {
+    if (!l || !cname) return fail(HYB_E_ARG, "hyb_lb_get: null argument");
+    if (!l->inited) return fail(HYB_E_STATE, "hyb_lb_get before init");
+    HIPCHK(hipStreamSynchronize(l->stream));
+    const std::string name(cname);
+    const size_t N = l->N;
+    const LBKParams p = l->params();
+    std::vector<uint32_t> lk(N);
+    HIPCHK(hipMemcpy(lk.data(), l->link, N * 4, hipMemcpyDeviceToHost));
+    if (name == "types") {
+        std::vector<uint8_t> t(N);
+        HIPCHK(hipMemcpy(t.data(), l->type, N, hipMemcpyDeviceToHost));
+        return put(t.data(), N, host, cap, need);
+    }
+    if (name == "pflag" || name == "solidIndex") {
+        std::vector<uint32_t> so(N);
+        HIPCHK(hipMemcpy(so.data(), l->solid, N * 4, hipMemcpyDeviceToHost));
+        if (name == "pflag") {
+            std::vector<uint8_t> pf(N);
+            for (size_t i = 0; i < N; ++i) pf[i] = so[i] ? 1 : 0;
+            return put(pf.data(), N, host, cap, need);
+        }
+        for (size_t i = 0; i < N; ++i) so[i] = so[i] ? so[i] - 1 : 0xFFFFFFFFu;
+        return put(so.data(), N * 4, host, cap, need);
+    }
+    if (name == "f") {
+        if (need) *need = N * 19 * 8;
+        if (!host || cap < N * 19 * 8) return HYB_OK;
+        const void* src = l->f[l->cur];
+        if (l->steps > 0) {
+            if (!l->scratch) HIPCHK(hipMalloc(&l->scratch, l->popBytes()));
+            launch_lb_stream(l->prec, l->f[l->cur], l->scratch, l->link, p, l->stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(l->stream));
+            src = l->scratch;
+        }
+        double* out = (double*)host;
+        if (l->prec == Prec::F64) {
+            std::vector<double> f(19 * N);
+            HIPCHK(hipMemcpy(f.data(), src, 19 * N * 8, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < N; ++i)
+                for (int j = 0; j < 19; ++j) out[i * 19 + j] = (lk[i] & LINK_ACTIVE) ? f[(size_t)j * N + i] : NAN;
+        } else {
+            std::vector<float> f(19 * N);
+            HIPCHK(hipMemcpy(f.data(), src, 19 * N * 4, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < N; ++i)
+                for (int j = 0; j < 19; ++j)
+                    out[i * 19 + j] = (lk[i] & LINK_ACTIVE) ? (double)f[(size_t)j * N + i] + wgt(j) : NAN;
+        }
+        return HYB_OK;
+    }
+    if (name == "n" || name == "u" || name == "hydroForce") {
+        const size_t w = name == "n" ? 1 : 3;
+        if (need) *need = N * w * 8;
+        if (!host || cap < N * w * 8) return HYB_OK;
+        int rc = ensure_mac(l, l->dem);
+        if (rc) return rc;
+        std::vector<double> m(7 * N);
+        HIPCHK(hipMemcpy(m.data(), l->mac, 7 * N * 8, hipMemcpyDeviceToHost));
+        double* out = (double*)host;
+        const size_t off = name == "n" ? 0 : (name == "u" ? 1 : 4);
+        for (size_t i = 0; i < N; ++i)
+            for (size_t k = 0; k < w; ++k) out[i * w + k] = (lk[i] & LINK_ACTIVE) ? m[(off + k) * N + i] : NAN;
+        return HYB_OK;
+    }
+    if (name == "visc" || name == "mass") {
+        std::vector<double> v(N, NAN);
+        if (name == "visc") {
+            if (l->visc) HIPCHK(hipMemcpy(v.data(), l->visc, N * 8, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < N; ++i) v[i] = (lk[i] & LINK_ACTIVE) ? (l->visc ? v[i] : l->initDynVisc) : NAN;
+        } else {
+            for (size_t i = 0; i < N; ++i)
+                if (lk[i] & LINK_ACTIVE) v[i] = (lk[i] & LINK_INTERFACE) ? 0.5 * l->initDensity : l->initDensity;
+        }
+        return put(v.data(), N * 8, host, cap, need);
+    }
+    if (name == "totalMass") return put(&l->totalMass, 8, host, cap, need);
+    return fail(HYB_E_ARG, "unknown LB field " + name);
+}
+
+int hyb_dem_get(hyb_dem* d, const char* cname, void* host, size_t cap, size_t* need) {
+    if (!d || !cname) return fail(HYB_E_ARG, "hyb_dem_get: null argument");
+    if (!d->inited) return fail(HYB_E_STATE, "hyb_dem_get before init");
+    HIPCHK(hipStreamSynchronize(d->stream));
+    const std::string name(cname);
+    const int P = d->P;
+    DEMState st;
+    std::memset(&st, 0, sizeof(st));
+    if (P) HIPCHK(hipMemcpy(&st, d->b.st, sizeof(st), hipMemcpyDeviceToHost));
+    if (st.error) return fail(HYB_E_CAPACITY, "DEM device error " + std::to_string(st.error) +
+                                                   " (1/4: ghost or pair capacity, 2: particle outside the cell grid, 3: stencil outside the cell table)");
+    std::vector<double> el;
+    if (P) {
+        el.resize((size_t)EF_COUNT * P);
+        HIPCHK(hipMemcpy(el.data(), d->b.el, el.size() * 8, hipMemcpyDeviceToHost));
+    }
+    auto at = [&](int f, int e) { return el[(size_t)f * P + e]; };
+    std::vector<double> v;
+    std::vector<uint32_t> u;
+    const int G = P ? st.nGhost : 0;
+    std::vector<uint32_t> gi(G);
+    std::vector<double> gv(3 * (size_t)G);
+    if (G) {
+        HIPCHK(hipMemcpy(gi.data(), d->b.ghostIndex, 4 * (size_t)G, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(gv.data(), d->b.ghostVec, 24 * (size_t)G, hipMemcpyDeviceToHost));
+    }
+    if (name == "e_x" || name == "e_xp" || name == "e_w" || name == "e_wp") {
+        // at step end the predicted copies equal the corrected ones (elmt.cpp:190-224)
+        const int base = (name == "e_x" || name == "e_xp") ? EF_X : EF_W;
+        for (int e = 0; e < P; ++e)
+            for (int k = 0; k < 18; ++k) v.push_back(at(base + k, e));
+        return put(v.data(), v.size() * 8, host, cap, need);
+    }
+    if (name == "e_q" || name == "e_qp") {
+        for (int e = 0; e < P; ++e)
+            for (int k = 0; k < 24; ++k) v.push_back(at(EF_Q + k, e));
+        return put(v.data(), v.size() * 8, host, cap, need);
+    }
+    if (name == "e_forces") {
+        for (int e = 0; e < P; ++e)
+            for (int k = 0; k < 21; ++k) v.push_back(at(EF_FH + k, e));
+        return put(v.data(), v.size() * 8, host, cap, need);
+    }
+    if (name == "e_misc") {
+        for (int e = 0; e < P; ++e) {
+            v.push_back(at(EF_M, e)); v.push_back(at(EF_RAD, e));
+            for (int k = 0; k < 3; ++k) v.push_back(at(EF_I + k, e));
+            for (int k = 0; k < 12; ++k) v.push_back(at(EF_WG + k, e));
+            v.push_back(at(EF_FVOL, e)); v.push_back(1.0);
+        }
+        return put(v.data(), v.size() * 8, host, cap, need);
+    }
+    if (name == "p_data" || name == "p_idx") {
+        for (int s = 0; s < P + G; ++s) {
+            const int o = s < P ? s : (int)gi[s - P];
+            if (name == "p_data") {
+                // ghosts: origin position + period vector (elmt.cpp:292-298)
+                for (int k = 0; k < 3; ++k) v.push_back(s < P ? at(EF_X + k, o) : at(EF_X + k, o) + gv[3 * (s - P) + k]);
+                for (int k = 0; k < 3; ++k) v.push_back(at(EF_X + 3 + k, o));
+                for (int k = 0; k < 3; ++k) v.push_back(0.0);
+                v.push_back(at(EF_RAD, o));
+            } else {
+                u.push_back((uint32_t)s); u.push_back((uint32_t)o); u.push_back(0u); u.push_back(0u);
+            }
+        }
+        if (name == "p_data") return put(v.data(), v.size() * 8, host, cap, need);
+        return put(u.data(), u.size() * 4, host, cap, need);
+    }
+    if (name == "neighborTable") {
+        u.resize(2 * (size_t)(P ? st.nPairs : 0));
+        if (!u.empty()) HIPCHK(hipMemcpy(u.data(), d->b.pairs, u.size() * 4, hipMemcpyDeviceToHost));
+        return put(u.data(), u.size() * 4, host, cap, need);
+    }
+    if (name == "nearWallTable") {
+        std::vector<uint32_t> nw(P);
+        if (P) HIPCHK(hipMemcpy(nw.data(), d->b.nearWall, 4 * (size_t)P, hipMemcpyDeviceToHost));
+        for (int e = 0; e < P; ++e)
+            if (nw[e] != 0xFFFFFFFFu) { u.push_back((uint32_t)e); u.push_back(nw[e]); }
+        return put(u.data(), u.size() * 4, host, cap, need);
+    }
+    if (name == "ghostIndex") return put(gi.data(), gi.size() * 4, host, cap, need);
+    if (name == "ghostVec") return put(gv.data(), gv.size() * 8, host, cap, need);
+    if (name == "dem_scalars") {
+        const DEMParams& p = d->p;
+        v = {P ? st.maxDisp : 0.0, p.nebrRange, p.cellWidth[0], p.cellWidth[1], p.cellWidth[2], (double)p.nCells[0],
+             (double)p.nCells[1], (double)p.nCells[2], d->demTime, (double)d->demTimeStep, d->deltat, (double)d->multiStep,
+             (double)P, d->dampCoeff, d->knConst, d->ksConst, (double)st.newNeighborList};
+        return put(v.data(), v.size() * 8, host, cap, need);
+    }
+    if (name == "rebuilds") {
+        const double r = st.rebuilds;
+        return put(&r, 8, host, cap, need);
+    }
+    return fail(HYB_E_ARG, "unknown DEM field " + name);
+}
+
+int hyb_lb_set_profiling(hyb_lb* l, int32_t enabled) {
+    if (!l) return fail(HYB_E_ARG, "null handle");
+    l->profiling = enabled != 0;
+    l->evUsed = 0;
+    l->profMs = 0.0;
+    l->profLaunches = 0;
+    return HYB_OK;
+}
+
+int hyb_lb_kernel_time(hyb_lb* l, double* total_ms, int64_t* launches) {
+    if (!l) return fail(HYB_E_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(l->stream));
+    for (size_t k = 0; k + 1 < l->evUsed; k += 2) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, l->evs[k], l->evs[k + 1]));
+        l->profMs += ms;
+        l->profLaunches += 1;
+    }
+    l->evUsed = 0;
+    if (total_ms) *total_ms = l->profMs;
+    if (launches) *launches = l->profLaunches;
+    return HYB_OK;
+}
+
+int hyb_lb_active_nodes(hyb_lb* l, int64_t* count) {
+    if (!l || !count) return fail(HYB_E_ARG, "null argument");
+    *count = (int64_t)l->activeCount;
+    return HYB_OK;
+}
+
+}  // extern "C"

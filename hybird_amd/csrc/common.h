@@ -1,0 +1,123 @@
+// Shared device/host helpers for the hybird_amd HIP library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace hyb {
+
+// D3Q19 velocity set in the reference's order O, x, y, z, xy, yz, zx (lattice.h:36-60),
+// opposite directions (lattice.h:85) and weights (lattice.h:98-101).
+__host__ __device__ constexpr int cx(int j) {
+    constexpr int v[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+    return v[j];
+}
+__host__ __device__ constexpr int cy(int j) {
+    constexpr int v[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 0, 0, 0, 0};
+    return v[j];
+}
+__host__ __device__ constexpr int cz(int j) {
+    constexpr int v[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1};
+    return v[j];
+}
+__host__ __device__ constexpr int opp(int j) {
+    constexpr int v[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+    return v[j];
+}
+__host__ __device__ constexpr double wgt(int j) {
+    return j == 0 ? 12.0 / 36.0 : (j < 7 ? 2.0 / 36.0 : 1.0 / 36.0);
+}
+
+// Node type codes (node.h:73-83).
+enum NodeType : uint8_t { T_FLUID = 0, T_GAS = 2, T_INTERFACE = 3, T_PERIODIC = 4, T_SLIP = 5, T_SLIP_MOVING = 6,
+                          T_WALL = 7, T_WALL_MOVING = 8, T_CURVED = 9 };
+
+// Per-node link word: bit j (1..18) = the link in direction j is a static wall;
+// bit 30 = node is interface; bit 31 = node is active (fluid or interface).
+constexpr uint32_t LINK_ACTIVE = 1u << 31;
+constexpr uint32_t LINK_INTERFACE = 1u << 30;
+constexpr uint32_t LINK_WALLS = 0x7FFFEu;
+
+// 3-vector with the reference's operation order (vector.cpp:38-180): every
+// operation is component-wise, dot = x*x' + y*y' + z*z' evaluated left to right.
+struct V3 {
+    double x, y, z;
+    __host__ __device__ V3() : x(0.0), y(0.0), z(0.0) {}
+    __host__ __device__ V3(double a, double b, double c) : x(a), y(b), z(c) {}
+    __host__ __device__ V3 operator+(const V3& o) const { return V3(x + o.x, y + o.y, z + o.z); }
+    __host__ __device__ V3 operator-(const V3& o) const { return V3(x - o.x, y - o.y, z - o.z); }
+    __host__ __device__ V3 operator*(double s) const { return V3(x * s, y * s, z * s); }
+    __host__ __device__ V3 operator/(double s) const { return V3(x / s, y / s, z / s); }
+    __host__ __device__ V3& operator+=(const V3& o) { x += o.x; y += o.y; z += o.z; return *this; }
+    __host__ __device__ V3& operator*=(double s) { x *= s; y *= s; z *= s; return *this; }
+    __host__ __device__ double dot(const V3& o) const { return x * o.x + y * o.y + z * o.z; }
+    __host__ __device__ V3 cross(const V3& o) const { return V3(y * o.z - z * o.y, z * o.x - x * o.z, x * o.y - y * o.x); }
+    __host__ __device__ double norm2() const { return x * x + y * y + z * z; }
+    __host__ __device__ double norm() const { return sqrt(x * x + y * y + z * z); }
+};
+__host__ __device__ inline V3 operator*(double s, const V3& v) { return V3(v.x * s, v.y * s, v.z * s); }
+
+struct Q4 {
+    double q0, q1, q2, q3;
+    __host__ __device__ Q4() : q0(1.0), q1(0.0), q2(0.0), q3(0.0) {}
+    __host__ __device__ Q4(double a, double b, double c, double d) : q0(a), q1(b), q2(c), q3(d) {}
+    __host__ __device__ Q4 operator+(const Q4& o) const { return Q4(q0 + o.q0, q1 + o.q1, q2 + o.q2, q3 + o.q3); }
+    __host__ __device__ Q4 operator-(const Q4& o) const { return Q4(q0 - o.q0, q1 - o.q1, q2 - o.q2, q3 - o.q3); }
+    __host__ __device__ Q4 operator*(double s) const { return Q4(q0 * s, q1 * s, q2 * s, q3 * s); }
+    __host__ __device__ double norm() const { return sqrt(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3); }
+    __host__ __device__ double norm2() const { return q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3; }
+    __host__ __device__ void normalize() { const double n = norm(); q0 /= n; q1 /= n; q2 /= n; q3 /= n; }
+    __host__ __device__ Q4 adjoint() const { return Q4(q0, -q1, -q2, -q3); }
+    // this * r, vector.cpp:296-305
+    __host__ __device__ Q4 multiply(const Q4& r) const {
+        return Q4(r.q0 * q0 - r.q1 * q1 - r.q2 * q2 - r.q3 * q3, r.q0 * q1 + r.q1 * q0 - r.q2 * q3 + r.q3 * q2,
+                  r.q0 * q2 + r.q1 * q3 + r.q2 * q0 - r.q3 * q1, r.q0 * q3 - r.q1 * q2 + r.q2 * q1 + r.q3 * q0);
+    }
+};
+__host__ __device__ inline Q4 operator*(double s, const Q4& q) { return Q4(q.q0 * s, q.q1 * s, q.q2 * s, q.q3 * s); }
+
+__host__ __device__ inline V3 project(const V3& v, const Q4& q) {  // vector.cpp:481-491
+    const Q4 qa = q.adjoint();
+    const Q4 vq(0.0, v.x, v.y, v.z);
+    Q4 r = q.multiply(vq);
+    r = r.multiply(qa);
+    return V3(r.q1, r.q2, r.q3);
+}
+__host__ __device__ inline V3 newtonAcc(const V3& m, const V3& I, const V3& w) {  // vector.cpp:493-498
+    return V3((m.x + (I.y - I.z) * w.y * w.z) / I.x, (m.y + (I.z - I.x) * w.z * w.x) / I.y,
+              (m.z + (I.x - I.y) * w.x * w.y) / I.z);
+}
+__host__ __device__ inline Q4 quatAcc(const V3& wa, const Q4& Q1) {  // vector.cpp:500-503
+    Q4 w(0.0, wa.x, wa.y, wa.z);
+    w.q0 = -2.0 * Q1.norm2();
+    return w;
+}
+
+// Unit conversions (node.cpp:476-488), all in double.
+struct Units {
+    double Length, Time, Density, Volume, Speed, Accel, AngVel, Force, Torque, Mass, KinVisc, DynVisc, Stress;
+};
+
+// Plane wall (utils.h:18-80) and periodic pair (utils.h:144-163), in device-friendly form.
+struct WallDev {
+    V3 n, p, rotCenter, omega, vel;
+    int moving, slip;
+    __host__ __device__ double dist(const V3& x) const { return n.dot(x - p); }
+    __host__ __device__ V3 getSpeed(const V3& pt) const {
+        if (moving) {
+            const V3 dc = pt - rotCenter;
+            const V3 da = dc - (dc.dot(n)) * n;
+            return vel + omega.cross(da);
+        }
+        return V3(0.0, 0.0, 0.0);
+    }
+};
+
+struct PbcDev {
+    V3 v;              // period vector
+    V3 p1, n1, p2, n2; // the two planes
+};
+
+}  // namespace hyb

@@ -1,0 +1,685 @@
+// Discrete-element kernels for gfx950 (single-sphere elements).
+//
+// One DEM sub-step (DEM.cpp:331-376) is three launches on the handle's stream:
+//  1. k_dem_rebuild (one workgroup): evalMaxDisp and, when the displacement
+//     trigger fires, the whole neighbour rebuild of DEM.cpp:1369-1513 —
+//     periodic shift, ghosts, linked-cell table, the 14-cell pair list and the
+//     near-wall list — reproducing the reference's cell index (vector.cpp:147-149)
+//     and unsigned stencil wrap, in the reference's pair order. It also builds a
+//     per-particle list of the pairs each particle takes part in.
+//  2. k_dem_predict: Gear predictor (elmt.cpp:139-177), one thread per element.
+//  3. k_dem_forces_correct: per element, the contact forces of its pairs in
+//     pair-list order (DEM.cpp:1668-1717, 1801-1982, 2138-2222) and its wall
+//     contact, Newton-Euler (DEM.cpp:1149-1181) and the Gear corrector
+//     (elmt.cpp:179-254). Each element only accumulates its own forces, in the
+//     same order as the reference's sequential loop, so there are no atomics.
+// The predicted state (xp*, qp*, wp*) is written only by the predictor; the
+// corrector leaves it in place (its copy x -> xp is dead until the next
+// predictor), so other threads can read a partner's predicted state safely.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "dem_kernels.h"
+#include "hybird_amd.h"
+
+namespace hyb {
+
+namespace {
+
+struct El {
+    double* el;
+    int E;
+    __device__ double& at(int f, int e) const { return el[(size_t)f * E + e]; }
+    __device__ V3 v3(int f, int e) const { return V3(at(f, e), at(f + 1, e), at(f + 2, e)); }
+    __device__ void set3(int f, int e, const V3& v) const { at(f, e) = v.x; at(f + 1, e) = v.y; at(f + 2, e) = v.z; }
+    __device__ Q4 q4(int f, int e) const { return Q4(at(f, e), at(f + 1, e), at(f + 2, e), at(f + 3, e)); }
+    __device__ void set4(int f, int e, const Q4& q) const {
+        at(f, e) = q.q0; at(f + 1, e) = q.q1; at(f + 2, e) = q.q2; at(f + 3, e) = q.q3;
+    }
+};
+
+constexpr int RB = 1024;  // rebuild workgroup size
+
+// In-place exclusive scan of a[0..n) by the whole (single) workgroup; returns the total.
+__device__ int block_exscan(int* a, int n) {
+    __shared__ int sh[RB];
+    __shared__ int carry;
+    const int t = threadIdx.x;
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += RB) {
+        const int i = base + t;
+        const int v = i < n ? a[i] : 0;
+        sh[t] = v;
+        __syncthreads();
+        for (int off = 1; off < RB; off <<= 1) {
+            const int add = t >= off ? sh[t - off] : 0;
+            __syncthreads();
+            sh[t] += add;
+            __syncthreads();
+        }
+        const int incl = sh[t];
+        const int c0 = carry;
+        if (i < n) a[i] = c0 + incl - v;
+        __syncthreads();
+        if (t == RB - 1) carry = c0 + incl;
+        __syncthreads();
+    }
+    const int total = carry;
+    __syncthreads();
+    return total;
+}
+
+// vector.cpp:147-149 (offsets +1, +2, 0 through the parenthesisation).
+__device__ __forceinline__ int linearize(const V3& x, const double cw[3], const unsigned nC[3]) {
+    return (int)(floor(x.x / cw[0]) + 1 +
+                 (double)nC[0] * (floor(x.y / cw[1]) + 1 + ((double)nC[1] * (floor(x.z / cw[2])) + 1)));
+}
+
+__device__ __forceinline__ V3 slot_pos(const El& E, const DEMBuffers& b, int P, int s) {
+    if (s < P) return E.v3(EF_X, s);
+    const int g = s - P;
+    const int o = (int)b.ghostIndex[g];
+    return E.v3(EF_X, o) + V3(b.ghostVec[3 * g], b.ghostVec[3 * g + 1], b.ghostVec[3 * g + 2]);
+}
+
+__device__ __forceinline__ int slot_cluster(const DEMBuffers& b, int P, int s) {
+    return s < P ? s : (int)b.ghostIndex[s - P];
+}
+
+// Walks the chains of the cell pair (cell0, cell1) exactly as DEM.cpp:1455-1481.
+// With out == nullptr only counts. Returns -1 when the stencil leaves the table.
+__device__ int walk_pairs(const DEMParams& p, const DEMBuffers& b, const El& E, int S, unsigned w, uint32_t* out) {
+    static constexpr int sc[14][3] = {{0, 0, 0},  {1, 0, 0},  {1, 1, 0},  {0, 1, 0},  {-1, 1, 0},
+                                      {0, 0, 1},  {1, 0, 1},  {1, 1, 1},  {0, 1, 1},  {-1, 1, 1},
+                                      {-1, 0, 1}, {-1, -1, 1}, {0, -1, 1}, {1, -1, 1}};
+    const unsigned sh = w % 14u;
+    unsigned r = w / 14u;
+    unsigned i0[3];
+    i0[2] = r % p.nCells[2];
+    r /= p.nCells[2];
+    i0[1] = r % p.nCells[1];
+    i0[0] = r / p.nCells[1];
+    unsigned i1[3];
+    for (int k = 0; k < 3; ++k) {
+        i1[k] = i0[k] + (unsigned)sc[sh][k];
+        if (i1[k] == p.nCells[k]) i1[k] = 0;
+    }
+    const uint32_t Su = (uint32_t)S;
+    const uint32_t cell0 = (i0[2] * p.nCells[1] + i0[1]) * p.nCells[0] + i0[0] + Su;
+    const uint32_t cell1 = (i1[2] * p.nCells[1] + i1[1]) * p.nCells[0] + i1[0] + Su;
+    const uint32_t tsize = Su + (uint32_t)b.ncells;
+    const double r2 = p.nebrRange * p.nebrRange;
+    int cnt = 0;
+    int n0 = b.cellTable[cell0];
+    while (n0 > -1) {
+        if (cell1 >= tsize) return -1;
+        int n1 = b.cellTable[cell1];
+        const V3 x0 = slot_pos(E, b, p.P, n0);
+        const int c0 = slot_cluster(b, p.P, n0);
+        while (n1 > -1) {
+            if (cell0 != cell1 || n1 < n0) {
+                if (c0 != slot_cluster(b, p.P, n1)) {
+                    const V3 r10 = slot_pos(E, b, p.P, n1) - x0;
+                    if (r10.norm2() < r2) {
+                        if (out) {
+                            out[2 * cnt] = (uint32_t)n0;
+                            out[2 * cnt + 1] = (uint32_t)n1;
+                        }
+                        ++cnt;
+                    }
+                }
+            }
+            n1 = b.cellTable[n1];
+        }
+        n0 = b.cellTable[n0];
+    }
+    return cnt;
+}
+
+__global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, int force) {
+#pragma clang fp contract(off)
+    const El E{b.el, p.P};
+    const int t = threadIdx.x;
+    const int P = p.P;
+    __shared__ double red[RB];
+    __shared__ int doit;
+    __shared__ int sh_err;
+    if (t == 0) sh_err = 0;
+    // ---- evalMaxDisp (DEM.cpp:1314-1324) and the trigger (DEM.cpp:342-347)
+    if (!force) {
+        double m = 0.0;
+        for (int e = t; e < P; e += RB) {
+            const double v = E.v3(EF_X + 3, e).norm2();
+            if (v > m) m = v;
+        }
+        red[t] = m;
+        __syncthreads();
+        for (int w = RB / 2; w > 0; w >>= 1) {
+            if (t < w && red[t + w] > red[t]) red[t] = red[t + w];
+            __syncthreads();
+        }
+        if (t == 0) {
+            const double maxVel = sqrt(red[0]);
+            b.st->maxDisp += maxVel * p.deltat;
+            doit = b.st->maxDisp > p.trigger;
+            if (doit) b.st->maxDisp = 0.0;
+        }
+    } else if (t == 0) {
+        doit = 1;
+    }
+    __syncthreads();
+    if (!doit || P == 0) return;
+    if (t == 0) b.st->rebuilds += 1;
+    int G = 0;
+    if (p.npbc) {
+        // ---- pbcShift (DEM.cpp:1554-1584)
+        for (int e = t; e < P; e += RB) {
+            V3 x0 = E.v3(EF_X, e), xp0 = E.v3(EF_XP, e);
+            for (int k = 0; k < p.npbc; ++k) {
+                const PbcDev& pb = p.pbc[k];
+                const double left = pb.n1.dot(x0 - pb.p1);
+                const double right = pb.n2.dot(x0 - pb.p2);
+                if (left < 0.0) { x0 += pb.v; xp0 += pb.v; }
+                if (right < 0.0) { const V3 m = -1.0 * pb.v; x0 += m; xp0 += m; }
+            }
+            E.set3(EF_X, e, x0);
+            E.set3(EF_XP, e, xp0);
+        }
+        __syncthreads();
+        // ---- side ghosts in (pbc, element) order (DEM.cpp:1590-1618)
+        int nSide = 0;
+        for (int k = 0; k < p.npbc; ++k) {
+            const PbcDev& pb = p.pbc[k];
+            for (int e = t; e < P; e += RB) {
+                const V3 x0 = E.v3(EF_X, e);
+                const double left = pb.n1.dot(x0 - pb.p1);
+                const double right = pb.n2.dot(x0 - pb.p2);
+                const int flag = left < p.nebrRange ? 1 : (right < p.nebrRange ? 2 : 0);
+                b.work[e] = flag ? 1 : 0;
+                b.side[e * 3 + k] = flag;  // temporarily the flag
+            }
+            __syncthreads();
+            const int tot = block_exscan(b.work, P);
+            for (int e = t; e < P; e += RB) {
+                const int flag = b.side[e * 3 + k];
+                if (flag) {
+                    const int g = nSide + b.work[e];
+                    if (g < p.Gcap) {
+                        b.ghostIndex[g] = (uint32_t)e;
+                        const V3 v = flag == 1 ? pb.v : -1.0 * pb.v;
+                        b.ghostVec[3 * g] = v.x;
+                        b.ghostVec[3 * g + 1] = v.y;
+                        b.ghostVec[3 * g + 2] = v.z;
+                        b.ghostB[g] = k;
+                    }
+                    b.side[e * 3 + k] = g;
+                } else {
+                    b.side[e * 3 + k] = -1;
+                }
+            }
+            nSide += tot;
+            __syncthreads();
+        }
+        if (nSide > p.Gcap) { if (t == 0) sh_err = 1; }
+        __syncthreads();
+        if (sh_err) { if (t == 0) b.st->error = 1; return; }
+        // ---- corner ghosts: every pair g1 < g2 of side ghosts of one particle (DEM.cpp:1620-1635)
+        for (int g = t; g < nSide; g += RB) {
+            const int e = (int)b.ghostIndex[g];
+            int c = 0;
+            for (int k2 = b.ghostB[g] + 1; k2 < p.npbc; ++k2) c += b.side[e * 3 + k2] >= 0 ? 1 : 0;
+            b.work[g] = c;
+        }
+        __syncthreads();
+        const int nCorner = block_exscan(b.work, nSide);
+        G = nSide + nCorner;
+        if (G > p.Gcap) { if (t == 0) b.st->error = 1; return; }
+        for (int g = t; g < nSide; g += RB) {
+            const int e = (int)b.ghostIndex[g];
+            int slot = nSide + b.work[g];
+            const V3 v1(b.ghostVec[3 * g], b.ghostVec[3 * g + 1], b.ghostVec[3 * g + 2]);
+            for (int k2 = b.ghostB[g] + 1; k2 < p.npbc; ++k2) {
+                const int g2 = b.side[e * 3 + k2];
+                if (g2 < 0) continue;
+                const V3 v2(b.ghostVec[3 * g2], b.ghostVec[3 * g2 + 1], b.ghostVec[3 * g2 + 2]);
+                const V3 v = v1 + v2;
+                b.ghostIndex[slot] = (uint32_t)e;
+                b.ghostVec[3 * slot] = v.x;
+                b.ghostVec[3 * slot + 1] = v.y;
+                b.ghostVec[3 * slot + 2] = v.z;
+                ++slot;
+            }
+        }
+        if (t == 0) b.st->newNeighborList = 1;
+        __syncthreads();
+    }
+    const int S = P + G;
+    if (t == 0) { b.st->nGhost = G; b.st->nSlots = S; }
+    // ---- linked-cell table (DEM.cpp:1326-1367): chains in descending slot order
+    const int nc = b.ncells;
+    for (int c = t; c <= nc; c += RB) b.cellCount[c] = 0;
+    __syncthreads();
+    for (int s = t; s < S; s += RB) {
+        const int c = linearize(slot_pos(E, b, P, s), p.cellWidth, p.nCells) + S;
+        if (c < 0 || c >= S + nc) {
+            sh_err = 2;
+            b.cellOf[s] = 0;
+        } else {
+            b.cellOf[s] = c - S;
+            atomicAdd(&b.cellCount[c - S], 1);
+        }
+    }
+    __syncthreads();
+    if (sh_err) { if (t == 0) b.st->error = 2; return; }
+    block_exscan(b.cellCount, nc + 1);
+    for (int c = t; c < nc; c += RB) b.work[c] = b.cellCount[c];
+    __syncthreads();
+    for (int s = t; s < S; s += RB) {
+        const int pos = atomicAdd(&b.work[b.cellOf[s]], 1);
+        b.cursor[pos] = s;  // cursor holds the slots sorted by cell (order within a cell fixed below)
+    }
+    __syncthreads();
+    for (int c = t; c < nc; c += RB) {
+        const int lo = b.cellCount[c], hi = b.cellCount[c + 1];
+        for (int i = lo + 1; i < hi; ++i) {  // insertion sort, descending
+            const int v = b.cursor[i];
+            int j = i - 1;
+            while (j >= lo && b.cursor[j] < v) { b.cursor[j + 1] = b.cursor[j]; --j; }
+            b.cursor[j + 1] = v;
+        }
+        b.cellTable[S + c] = hi > lo ? b.cursor[lo] : -1;
+        for (int i = lo; i < hi; ++i) b.cellTable[b.cursor[i]] = (i + 1 < hi) ? b.cursor[i + 1] : -1;
+    }
+    __syncthreads();
+    // ---- pair list in the reference's order (DEM.cpp:1425-1484): count, scan, fill
+    const int W = b.W;
+    for (int w = t; w < W; w += RB) {
+        const int c = walk_pairs(p, b, E, S, (unsigned)w, nullptr);
+        if (c < 0) { sh_err = 3; b.work[w] = 0; }
+        else b.work[w] = c;
+    }
+    __syncthreads();
+    if (sh_err) { if (t == 0) b.st->error = 3; return; }
+    const int T = block_exscan(b.work, W);
+    if (T > p.pairCap) { if (t == 0) b.st->error = 4; return; }
+    for (int w = t; w < W; w += RB) walk_pairs(p, b, E, S, (unsigned)w, b.pairs + 2 * (size_t)b.work[w]);
+    if (t == 0) b.st->nPairs = T;
+    // ---- near-wall list (DEM.cpp:1496-1513): the first wall within range
+    int nw = 0;
+    for (int e = t; e < P; e += RB) {
+        const V3 x0 = E.v3(EF_X, e);
+        uint32_t wi = 0xFFFFFFFFu;
+        for (int k = 0; k < p.nwalls; ++k)
+            if (p.walls[k].dist(x0) < p.nebrRange) { wi = (uint32_t)k; break; }
+        b.nearWall[e] = wi;
+        nw += wi != 0xFFFFFFFFu;
+    }
+    red[t] = (double)nw;
+    __syncthreads();
+    // ---- per-particle pair lists, ascending pair index
+    for (int e = t; e < P; e += RB) b.cursor[e] = 0;
+    __syncthreads();
+    for (int k = t; k < T; k += RB) {
+        const uint32_t n0 = b.pairs[2 * k], n1 = b.pairs[2 * k + 1];
+        if ((int)n0 < P) atomicAdd(&b.cursor[n0], 1);
+        if ((int)n1 < P) atomicAdd(&b.cursor[n1], 1);
+    }
+    __syncthreads();
+    for (int e = t; e < P; e += RB) b.work[e] = b.cursor[e];
+    __syncthreads();
+    const int tot2 = block_exscan(b.work, P);
+    for (int e = t; e < P; e += RB) { b.csrOff[e] = (uint32_t)b.work[e]; b.cursor[e] = b.work[e]; }
+    if (t == 0) b.csrOff[P] = (uint32_t)tot2;
+    __syncthreads();
+    for (int k = t; k < T; k += RB) {
+        const uint32_t n0 = b.pairs[2 * k], n1 = b.pairs[2 * k + 1];
+        if ((int)n0 < P) b.csr[atomicAdd(&b.cursor[n0], 1)] = (uint32_t)k;
+        if ((int)n1 < P) b.csr[atomicAdd(&b.cursor[n1], 1)] = (uint32_t)k;
+    }
+    __syncthreads();
+    for (int e = t; e < P; e += RB) {
+        const int lo = (int)b.csrOff[e], hi = (int)b.csrOff[e + 1];
+        for (int i = lo + 1; i < hi; ++i) {
+            const uint32_t v = b.csr[i];
+            int j = i - 1;
+            while (j >= lo && b.csr[j] > v) { b.csr[j + 1] = b.csr[j]; --j; }
+            b.csr[j + 1] = v;
+        }
+    }
+    // ---- ghost slots of each particle, ascending (for the hydrodynamic sums)
+    for (int e = t; e < P; e += RB) b.ghostOfCount[e] = 0;
+    __syncthreads();
+    for (int g = t; g < G; g += RB) {
+        const int e = (int)b.ghostIndex[g];
+        const int k = atomicAdd(&b.ghostOfCount[e], 1);
+        if (k < 8) b.ghostOf[e * 8 + k] = P + g;
+    }
+    __syncthreads();
+    for (int e = t; e < P; e += RB) {
+        const int n = b.ghostOfCount[e] < 8 ? b.ghostOfCount[e] : 8;
+        int* l = b.ghostOf + e * 8;
+        for (int i = 1; i < n; ++i) {
+            const int v = l[i];
+            int j = i - 1;
+            while (j >= 0 && l[j] > v) { l[j + 1] = l[j]; --j; }
+            l[j + 1] = v;
+        }
+    }
+    // near-wall count
+    for (int w = RB / 2; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) b.st->nNearWall = (int)red[0];
+}
+
+// ------------------------------------------------------------------ predictor
+__global__ void k_dem_predict(DEMParams p, DEMBuffers b) {
+#pragma clang fp contract(off)
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p.P) return;
+    const El E{b.el, p.P};
+    const double* c1 = p.c1;
+    const double* c2 = p.c2;
+    V3 x[6];
+    Q4 q[6];
+    V3 w[6];
+    for (int k = 0; k < 6; ++k) { x[k] = E.v3(EF_X + 3 * k, e); q[k] = E.q4(EF_Q + 4 * k, e); w[k] = E.v3(EF_W + 3 * k, e); }
+    const V3 xp0 = x[0] + x[1] * c2[0] + x[2] * c2[1] + x[3] * c2[2] + x[4] * c2[3] + x[5] * c2[4];
+    const V3 xp1 = x[1] + x[2] * c2[0] + x[3] * c2[1] + x[4] * c2[2] + x[5] * c2[3];
+    const V3 xp2 = x[2] + x[3] * c2[0] + x[4] * c2[1] + x[5] * c2[2];
+    const V3 xp3 = x[3] + x[4] * c2[0] + x[5] * c2[1];
+    const V3 xp4 = x[4] + x[5] * c2[0];
+    const V3 xp5 = x[5];
+    Q4 qp0 = q[0] + q[1] * c2[0] + q[2] * c2[1] + q[3] * c2[2] + q[4] * c2[3] + q[5] * c2[4];
+    const Q4 qp1 = q[1] + q[2] * c2[0] + q[3] * c2[1] + q[4] * c2[2] + q[5] * c2[3];
+    const Q4 qp2 = q[2] + q[3] * c2[0] + q[4] * c2[1] + q[5] * c2[2];
+    const Q4 qp3 = q[3] + q[4] * c2[0] + q[5] * c2[1];
+    const Q4 qp4 = q[4] + q[5] * c2[0];
+    const Q4 qp5 = q[5];
+    qp0.normalize();
+    const V3 wp0 = w[0] + w[1] * c1[0] + w[2] * c1[1] + w[3] * c1[2] + w[4] * c1[3] + w[5] * c1[4];
+    const V3 wp1 = w[1] + w[2] * c1[0] + w[3] * c1[1] + w[4] * c1[2] + w[5] * c1[3];
+    const V3 wp2 = w[2] + w[3] * c1[0] + w[4] * c1[1] + w[5] * c1[2];
+    const V3 wp3 = w[3] + w[4] * c1[0] + w[5] * c1[1];
+    const V3 wp4 = w[4] + w[5] * c1[0];
+    const V3 wp5 = w[5];
+    E.set3(EF_XP, e, xp0); E.set3(EF_XP + 3, e, xp1); E.set3(EF_XP + 6, e, xp2);
+    E.set3(EF_XP + 9, e, xp3); E.set3(EF_XP + 12, e, xp4); E.set3(EF_XP + 15, e, xp5);
+    E.set4(EF_QP, e, qp0); E.set4(EF_QP + 4, e, qp1); E.set4(EF_QP + 8, e, qp2);
+    E.set4(EF_QP + 12, e, qp3); E.set4(EF_QP + 16, e, qp4); E.set4(EF_QP + 20, e, qp5);
+    E.set3(EF_WP, e, wp0); E.set3(EF_WP + 3, e, wp1); E.set3(EF_WP + 6, e, wp2);
+    E.set3(EF_WP + 9, e, wp3); E.set3(EF_WP + 12, e, wp4); E.set3(EF_WP + 15, e, wp5);
+    E.set3(EF_WPG, e, wp0);
+    E.set3(EF_WPL, e, project(wp0, qp0.adjoint()));
+}
+
+// ------------------------------------------------------------------ contacts + Newton-Euler + corrector
+__device__ __forceinline__ double normal_contact(const DEMParams& p, double overlap, double vrel, double effRad,
+                                                 double effMass) {  // DEM.cpp:2138-2174
+    if (p.hertzian) {
+        const double sqrtEffRad = sqrt(effRad);
+        const double sqrtOverlap = sqrt(overlap);
+        const double kn = p.knConst * sqrtEffRad * sqrtOverlap;
+        const double gamman = 2.0 * p.dampCoeff * sqrt(kn * effMass);
+        const double dumpfn = -gamman * vrel;
+        const double elasticfn = kn * overlap;
+        const double fn = elasticfn + dumpfn;
+        return (fn < 0.0) ? 0.0 : fn;  // std::max(fn, 0.0)
+    }
+    const double elasticfn = p.linearStiff * overlap;
+    const double gamman = 2.0 * p.dampCoeff * sqrt(p.linearStiff * effMass);
+    const double dumpfn = -gamman * vrel;
+    const double fn = elasticfn + dumpfn;
+    return (fn < 0.0) ? 0.0 : fn;
+}
+
+__device__ __forceinline__ double tangential_contact(const DEMParams& p, double vrelt, double fn, double effRad,
+                                                     double effMass, double friction) {  // DEM.cpp:2176-2222
+    double ks;
+    if (p.hertzian) ks = p.ksConst * sqrt(effRad) * pow(fabs(fn), 1.0 / 3.0);
+    else ks = p.linearStiff;
+    const double fsMax = friction * fn;
+    const double gammas = 2.0 * p.viscTang * sqrt(effMass * ks);
+    const double dumpfs = gammas * vrelt;
+    return (fsMax < dumpfs) ? fsMax : dumpfs;  // std::min(dumpfs, fsMax)
+}
+
+// Predicted position / velocity of a particle slot (updateParticlesPredicted, DEM.cpp:1184-1204).
+__device__ __forceinline__ void slot_pred(const El& E, const DEMBuffers& b, int P, int s, V3& x, V3& v, int& elem) {
+    if (s < P) {
+        x = E.v3(EF_XP, s);
+        v = E.v3(EF_XP + 3, s);
+        elem = s;
+    } else {
+        const int g = s - P;
+        elem = (int)b.ghostIndex[g];
+        x = E.v3(EF_XP, elem) + V3(b.ghostVec[3 * g], b.ghostVec[3 * g + 1], b.ghostVec[3 * g + 2]);
+        v = E.v3(EF_XP + 3, elem);
+    }
+}
+
+__global__ void k_dem_forces_correct(DEMParams p, DEMBuffers b) {
+#pragma clang fp contract(off)
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p.P) return;
+    const El E{b.el, p.P};
+    const int P = p.P;
+    V3 FP(0.0, 0.0, 0.0), MP(0.0, 0.0, 0.0), FW(0.0, 0.0, 0.0), MW(0.0, 0.0, 0.0);
+    // ---- particle-particle contacts, in pair-list order (DEM.cpp:1668-1690, 1801-1900)
+    const uint32_t lo = b.csrOff[e], hi = b.csrOff[e + 1];
+    for (uint32_t k = lo; k < hi; ++k) {
+        const uint32_t pr = b.csr[k];
+        const int n0 = (int)b.pairs[2 * pr], n1 = (int)b.pairs[2 * pr + 1];
+        V3 xI, vI, xJ, vJ;
+        int eI, eJ;
+        slot_pred(E, b, P, n0, xI, vI, eI);
+        slot_pred(E, b, P, n1, xJ, vJ, eJ);
+        const double rI = E.at(EF_RAD, eI), rJ = E.at(EF_RAD, eJ);
+        const double sigij = rI + rJ;
+        const double sigij2 = sigij * sigij;
+        const V3 vd = xJ - xI;
+        if (!(vd.norm2() < sigij2)) continue;
+        const double distance = vd.norm();
+        const double overlap = rI + rJ - distance;
+        const V3 relVel = vJ - vI;
+        const V3 en = vd / distance;
+        const double normRelVel = relVel.dot(en);
+        const V3 normalRelVel = en * normRelVel;
+        const double mI = E.at(EF_M, eI), mJ = E.at(EF_M, eJ);
+        const double effMass = mI * mJ / (mI + mJ);
+        const double effRad = rI * rJ / (rI + rJ);
+        const double fn = normal_contact(p, overlap, normRelVel, effRad, effMass);
+        const V3 normalForce = en * fn;
+        const V3 vecRadI = rI * en;
+        const V3 vecRadJ = -rJ * en;
+        // radiusVec is zero for single-sphere elements
+        const V3 centerDistI = vecRadI + V3(0.0, 0.0, 0.0);
+        const V3 centerDistJ = vecRadJ + V3(0.0, 0.0, 0.0);
+        const bool isI = (n0 == e);
+        if (isI) FP = FP - normalForce;
+        else FP = FP + normalForce;
+        const V3 wI = E.v3(EF_WPG, eI), wJ = E.v3(EF_WPG, eJ);
+        const V3 relVelContact = relVel - wI.cross(vecRadI) + wJ.cross(vecRadJ);
+        const V3 tangRelVelContact = relVelContact - normalRelVel;
+        const double nt = tangRelVelContact.norm();
+        if (nt != 0.0) {
+            const double ft = tangential_contact(p, nt, fn, effRad, effMass, p.fricPart);
+            const V3 et = tangRelVelContact / nt;
+            const V3 tangForce = ft * et;
+            if (isI) {
+                MP = MP + centerDistI.cross(tangForce);
+                FP = FP + tangForce;
+            } else {
+                MP = MP - centerDistJ.cross(tangForce);
+                FP = FP - tangForce;
+            }
+        }
+    }
+    // ---- wall contact (DEM.cpp:1692-1717, 1902-1982)
+    const uint32_t wi = b.nearWall[e];
+    const V3 xpe = E.v3(EF_XP, e);
+    const V3 vpe = E.v3(EF_XP + 3, e);
+    const double re = E.at(EF_RAD, e), me = E.at(EF_M, e);
+    const V3 wpg = E.v3(EF_WPG, e);
+    if (wi != 0xFFFFFFFFu) {
+        const WallDev& w = p.walls[wi];
+        const double distance = w.dist(xpe);
+        const double overlap = re - distance;
+        if (overlap > 0.0) {
+            const V3 en = w.n;
+            const V3 cpv = w.getSpeed(xpe);
+            const V3 relVel = vpe - cpv;
+            const double normRelVel = relVel.dot(en);
+            const V3 normalRelVel = en * normRelVel;
+            const double fn = normal_contact(p, 2.0 * overlap, normRelVel, re, me);
+            const V3 normalForce = en * fn;
+            const V3 vecRadJ = -re * en;
+            const V3 centerDistJ = vecRadJ;
+            FW = FW + normalForce;
+            const V3 relVelContact = relVel + wpg.cross(vecRadJ);
+            const V3 tangRelVelContact = relVelContact - normalRelVel;
+            const double nt = tangRelVelContact.norm();
+            if (nt != 0.0) {
+                const double ft = tangential_contact(p, nt, fn, re, me, p.fricWall);
+                const V3 et = tangRelVelContact / tangRelVelContact.norm();
+                const V3 tangForce = ft * et;
+                MW = MW - centerDistJ.cross(tangForce);
+                FW = FW - tangForce;
+            }
+        }
+    }
+    // ---- Newton-Euler (DEM.cpp:1149-1181)
+    const V3 xp1 = vpe;
+    const double rad = re;
+    const V3 FVisc = -6.0 * M_PI * p.numVisc * rad * xp1;
+    const V3 MVisc = -8.0 * M_PI * p.numVisc * rad * rad * rad * wpg;
+    const V3 FH = E.v3(EF_FH, e), MH = E.v3(EF_MH, e);
+    const V3 demF(p.demF[0], p.demF[1], p.demF[2]);
+    const V3 x2 = (FVisc + FH + FP + FW) / me + demF;
+    const V3 moment = MVisc + MH + MP + MW;
+    const Q4 qp0 = E.q4(EF_QP, e);
+    const V3 momentBf = project(moment, qp0.adjoint());
+    const V3 waBf = newtonAcc(momentBf, E.v3(EF_I, e), E.v3(EF_WPL, e));
+    const V3 w1 = project(waBf, qp0);
+    E.set3(EF_FP, e, FP); E.set3(EF_MP, e, MP); E.set3(EF_FW, e, FW); E.set3(EF_MW, e, MW);
+    // ---- corrector (elmt.cpp:179-254); the predicted copies stay untouched
+    const double* co1 = p.co1;
+    const double* co2 = p.co2;
+    const V3 xp[6] = {xpe, xp1, E.v3(EF_XP + 6, e), E.v3(EF_XP + 9, e), E.v3(EF_XP + 12, e), E.v3(EF_XP + 15, e)};
+    const V3 x2Corr = x2 - xp[2];
+    E.set3(EF_X, e, xp[0] + x2Corr * co2[0]);
+    E.set3(EF_X + 3, e, xp[1] + x2Corr * co2[1]);
+    E.set3(EF_X + 6, e, x2);
+    E.set3(EF_X + 9, e, xp[3] + x2Corr * co2[3]);
+    E.set3(EF_X + 12, e, xp[4] + x2Corr * co2[4]);
+    E.set3(EF_X + 15, e, xp[5] + x2Corr * co2[5]);
+    const V3 wp[6] = {E.v3(EF_WP, e), E.v3(EF_WP + 3, e), E.v3(EF_WP + 6, e), E.v3(EF_WP + 9, e), E.v3(EF_WP + 12, e),
+                      E.v3(EF_WP + 15, e)};
+    const V3 w1Corr = w1 - wp[1];
+    const V3 w0 = wp[0] + w1Corr * co1[0];
+    E.set3(EF_W, e, w0);
+    E.set3(EF_W + 3, e, w1);
+    E.set3(EF_W + 6, e, wp[2] + w1Corr * co1[2]);
+    E.set3(EF_W + 9, e, wp[3] + w1Corr * co1[3]);
+    E.set3(EF_W + 12, e, wp[4] + w1Corr * co1[4]);
+    E.set3(EF_W + 15, e, wp[5] + w1Corr * co1[5]);
+    const Q4 q2 = E.q4(EF_Q + 8, e);  // single spheres: q2 is never set by the force routine
+    const Q4 qp[6] = {qp0, E.q4(EF_QP + 4, e), E.q4(EF_QP + 8, e), E.q4(EF_QP + 12, e), E.q4(EF_QP + 16, e),
+                      E.q4(EF_QP + 20, e)};
+    const Q4 q2Corr = q2 - qp[2];
+    Q4 q0 = qp[0] + q2Corr * co2[0];
+    E.set4(EF_Q + 4, e, qp[1] + q2Corr * co2[1]);
+    E.set4(EF_Q + 12, e, qp[3] + q2Corr * co2[3]);
+    E.set4(EF_Q + 16, e, qp[4] + q2Corr * co2[4]);
+    E.set4(EF_Q + 20, e, qp[5] + q2Corr * co2[5]);
+    q0.normalize();
+    E.set4(EF_Q, e, q0);
+    E.set3(EF_WG, e, w0);
+    E.set3(EF_WL, e, project(w0, q0.adjoint()));
+}
+
+// Coupling records of every particle slot after the DEM step: corrected
+// position (ghosts: origin + period vector, elmt.cpp:292-298), element velocity
+// and angular velocity, radius.
+__global__ void k_dem_couple_table(DEMParams p, DEMBuffers b, CoupleRec* __restrict__ cp, int* __restrict__ nslots) {
+#pragma clang fp contract(off)
+    const El E{b.el, p.P};
+    const int S = b.st->nSlots;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) *nslots = S;
+    if (s >= S) return;
+    int o = s;
+    V3 x0 = E.v3(EF_X, s < p.P ? s : 0);
+    if (s >= p.P) {
+        const int g = s - p.P;
+        o = (int)b.ghostIndex[g];
+        x0 = E.v3(EF_X, o) + V3(b.ghostVec[3 * g], b.ghostVec[3 * g + 1], b.ghostVec[3 * g + 2]);
+    }
+    CoupleRec r;
+    r.x0[0] = x0.x; r.x0[1] = x0.y; r.x0[2] = x0.z;
+    r.rv[0] = r.rv[1] = r.rv[2] = 0.0;
+    const V3 v = E.v3(EF_X + 3, o), w = E.v3(EF_WG, o);
+    r.v[0] = v.x; r.v[1] = v.y; r.v[2] = v.z;
+    r.w[0] = w.x; r.w[1] = w.y; r.w[2] = w.z;
+    r.r = E.at(EF_RAD, o);
+    r.elem = o;
+    r.pad = 0;
+    cp[s] = r;
+}
+
+// Element hydrodynamic force and torque: own slot first, then its ghosts in
+// slot order, compensated FP64; then scaled to physical units (LB.cpp:1907-1918).
+__global__ void k_dem_hydro_combine(DEMParams p, DEMBuffers b, const HydroPartial* __restrict__ part, Units u) {
+#pragma clang fp contract(off)
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p.P) return;
+    const El E{b.el, p.P};
+    double s[7], c[7];
+    for (int k = 0; k < 7; ++k) { s[k] = part[e].s[k]; c[k] = 0.0; }
+    const int n = b.ghostOfCount[e] < 8 ? b.ghostOfCount[e] : 8;
+    for (int i = 0; i < n; ++i) {
+        const HydroPartial& q = part[b.ghostOf[e * 8 + i]];
+        for (int k = 0; k < 7; ++k) {
+            const double v = q.s[k];
+            const double t = s[k] + v;
+            if (fabs(s[k]) >= fabs(v)) c[k] += (s[k] - t) + v;
+            else c[k] += (v - t) + s[k];
+            s[k] = t;
+        }
+    }
+    for (int k = 0; k < 7; ++k) s[k] += c[k];
+    E.set3(EF_FH, e, V3(s[0], s[1], s[2]) * u.Force);
+    E.set3(EF_MH, e, V3(s[3], s[4], s[5]) * u.Torque);
+    E.at(EF_FVOL, e) = s[6] * u.Volume;
+}
+
+}  // namespace
+
+void launch_dem_rebuild(const DEMParams& p, const DEMBuffers& b, int force, hipStream_t s) {
+    k_dem_rebuild<<<1, RB, 0, s>>>(p, b, force);
+}
+
+void launch_dem_predict(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {
+    if (p.P <= 0) return;
+    k_dem_predict<<<(p.P + 255) / 256, 256, 0, s>>>(p, b);
+}
+
+void launch_dem_forces_correct(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {
+    if (p.P <= 0) return;
+    k_dem_forces_correct<<<(p.P + 255) / 256, 256, 0, s>>>(p, b);
+}
+
+void launch_dem_couple_table(const DEMParams& p, const DEMBuffers& b, CoupleRec* cp, int* nslots, hipStream_t s) {
+    const int cap = p.P + p.Gcap;
+    k_dem_couple_table<<<(cap + 255) / 256 > 0 ? (cap + 255) / 256 : 1, 256, 0, s>>>(p, b, cp, nslots);
+}
+
+void launch_dem_hydro_combine(const DEMParams& p, const DEMBuffers& b, const HydroPartial* part, const Units& u,
+                              hipStream_t s) {
+    if (p.P <= 0) return;
+    k_dem_hydro_combine<<<(p.P + 255) / 256, 256, 0, s>>>(p, b, part, u);
+}
+
+}  // namespace hyb

@@ -1,0 +1,91 @@
+// Host-side launch interface of the discrete-element kernels (dem.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.h"
+#include "lb_kernels.h"
+
+namespace hyb {
+
+// Element state is stored field-major: field f of element e at el[f * Ecap + e].
+// Offsets (in doubles) of each field; vectors take 3 slots, quaternions 4.
+enum ElemField : int {
+    EF_X = 0,      // x0..x5  (6 x 3)
+    EF_XP = 18,    // xp0..xp5
+    EF_Q = 36,     // q0..q5  (6 x 4)
+    EF_QP = 60,    // qp0..qp5
+    EF_W = 84,     // w0..w5  (6 x 3)
+    EF_WP = 102,   // wp0..wp5
+    EF_WG = 120,   // wGlobal
+    EF_WL = 123,   // wLocal
+    EF_WPG = 126,  // wpGlobal
+    EF_WPL = 129,  // wpLocal
+    EF_FH = 132, EF_MH = 135, EF_FP = 138, EF_MP = 141, EF_FW = 144, EF_MW = 147, EF_FG = 150,
+    EF_M = 153, EF_RAD = 154, EF_I = 155, EF_FVOL = 158,
+    EF_COUNT = 159
+};
+
+struct DEMParams {
+    int P;                 // standard particles (= elements; single-sphere elements)
+    int Gcap;              // ghost capacity
+    int pairCap;           // neighbour-pair capacity
+    int npbc, nwalls;
+    PbcDev pbc[3];
+    WallDev walls[6];
+    double cellWidth[3];
+    unsigned nCells[3];
+    double nebrRange, trigger, deltat;
+    double c1[5], c2[5], co1[6], co2[6];
+    double knConst, ksConst, dampCoeff, viscTang, fricPart, fricWall, linearStiff;
+    int hertzian;
+    double numVisc;
+    double demF[3];
+};
+
+// Device-side mutable scalars of the DEM.
+struct DEMState {
+    double maxDisp;
+    int nGhost;            // current ghost count
+    int nSlots;            // P + nGhost
+    int nPairs;
+    int nNearWall;
+    int newNeighborList;
+    int error;             // nonzero: capacity overflow or particle outside the cell grid
+    int rebuilds;
+};
+
+// Device buffers of the neighbour machinery.
+struct DEMBuffers {
+    double* el;            // [EF_COUNT][P]
+    DEMState* st;
+    uint32_t* ghostIndex;  // [Gcap]
+    double* ghostVec;      // [Gcap][3]
+    uint32_t* pairs;       // [pairCap][2]
+    uint32_t* nearWall;    // [P] wall index or 0xFFFFFFFF
+    int* cellTable;        // [Scap + ncells]
+    int* cellOf;           // [Scap]
+    int* cellCount;        // [ncells + 1]
+    int* work;             // [max(W, Scap, pairCap) + 1] scan scratch
+    uint32_t* csrOff;      // [P + 1]
+    uint32_t* csr;         // [2 * pairCap] pair indices per particle, ascending
+    int* cursor;           // [Scap]
+    int* ghostOfCount;     // [P]
+    int* ghostOf;          // [P][8] ghost slots of each particle, ascending
+    int* side;             // [P][3] side-ghost index of each particle per periodic pair, -1 if none
+    int* ghostB;           // [Gcap] periodic pair that created a side ghost
+    int scap;
+    int ncells;
+    int W;                 // ncells * 14
+};
+
+void launch_dem_rebuild(const DEMParams& p, const DEMBuffers& b, int force, hipStream_t s);
+void launch_dem_predict(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
+void launch_dem_forces_correct(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
+void launch_dem_couple_table(const DEMParams& p, const DEMBuffers& b, CoupleRec* cp, int* nslots, hipStream_t s);
+void launch_dem_hydro_combine(const DEMParams& p, const DEMBuffers& b, const HydroPartial* part, const Units& u,
+                              hipStream_t s);
+
+}  // namespace hyb

@@ -1,0 +1,796 @@
+// Lattice-Boltzmann kernels for gfx950.
+//
+// Populations live in HBM as structure-of-arrays planes f[q][z][y][x] (two
+// copies, ping-pong), so every population load and store of a wavefront is a
+// contiguous run along x. One fused kernel does, per active node: pull
+// streaming with half-way bounce-back and periodic wrap (LB.cpp:1144-1488),
+// moments (node.cpp:63-83), the volumetric penalty force of particle nodes
+// (LB.cpp:1851-1919), the Guo velocity shift (node.cpp:85-88), equilibrium
+// (node.cpp:90-101), optional Bingham/Smagorinsky viscosity (node.cpp:103-145),
+// BGK (node.cpp:158-165) or TRT relaxation and Guo forcing (node.cpp:167-184).
+//
+// FP64 form: the reference's operation order, compiled without FMA contraction,
+// so that results reproduce the CPU reference. FP32 form: populations are stored
+// as deviations g = f - w (rho0 = 1) and the collision runs in FP32 with FMA.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "hybird_amd.h"
+#include "lb_kernels.h"
+
+namespace hyb {
+
+namespace {
+
+constexpr int BX = 64;
+constexpr int BY = 4;
+
+struct Coord {
+    int x, y, z;
+    size_t idx;
+};
+
+__device__ __forceinline__ bool node_coord(const LBKParams& p, Coord& c) {
+    c.x = blockIdx.x * BX + threadIdx.x;
+    c.y = blockIdx.y * BY + threadIdx.y;
+    c.z = blockIdx.z;
+    if (c.x >= p.X || c.y >= p.Y) return false;
+    c.idx = (size_t)c.x + (size_t)p.X * ((size_t)c.y + (size_t)p.Y * (size_t)c.z);
+    return true;
+}
+
+// Neighbour coordinates x-1, x, x+1 along one axis for an interior node, with the
+// periodic rewiring of LB.cpp:437-472 (layers 0 and X-1 are ghosts, period X-2).
+struct Nb3 {
+    int m, c, pl;
+};
+__device__ __forceinline__ Nb3 nb3(int x, int X, int per) {
+    Nb3 r;
+    r.c = x;
+    r.m = (per && x == 1) ? X - 2 : x - 1;
+    r.pl = (per && x == X - 2) ? 1 : x + 1;
+    return r;
+}
+__device__ __forceinline__ int pick(const Nb3& n, int d) { return d > 0 ? n.pl : (d < 0 ? n.m : n.c); }
+
+// Gather the 19 pre-collision populations of node c into f[]: pull from the
+// neighbour x + c_j for f_opp(j), or bounce back the node's own f_j when that
+// link is a static wall.
+template <typename T>
+__device__ __forceinline__ void pull19(const T* __restrict__ fin, size_t N, const LBKParams& p, const Coord& c,
+                                       uint32_t lw, T f[19]) {
+    const Nb3 nx = nb3(c.x, p.X, p.per[0]);
+    const Nb3 ny = nb3(c.y, p.Y, p.per[1]);
+    const Nb3 nz = nb3(c.z, p.Z, p.per[2]);
+    f[0] = fin[c.idx];
+#pragma unroll
+    for (int i = 1; i < 19; ++i) {
+        const int j = opp(i);
+        if (lw & (1u << j)) {
+            f[i] = fin[(size_t)j * N + c.idx];
+        } else {
+            const size_t s = (size_t)pick(nx, cx(j)) +
+                             (size_t)p.X * ((size_t)pick(ny, cy(j)) + (size_t)p.Y * (size_t)pick(nz, cz(j)));
+            f[i] = fin[(size_t)i * N + s];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ FP64 path
+// u.c_j with the zero components left out (j is a constant after unrolling);
+// equal to the reference's three-term dot product (vector.cpp:134-136) because
+// adding +-0 to a nonzero partial sum never changes it, and a zero result only
+// enters later terms through squares or a sum with 1.
+__device__ __forceinline__ double cdot_rt(int j, const V3& u) {
+    double r = 0.0;
+    bool first = true;
+    if (cx(j) != 0) { r = cx(j) > 0 ? u.x : -u.x; first = false; }
+    if (cy(j) != 0) { const double t = cy(j) > 0 ? u.y : -u.y; r = first ? t : r + t; first = false; }
+    if (cz(j) != 0) { const double t = cz(j) > 0 ? u.z : -u.z; r = first ? t : r + t; first = false; }
+    return r;
+}
+
+// node.cpp:63-83: n = sum f_j; u = sum f_j c_j / n (zero terms left out).
+__device__ __forceinline__ void moments64(const double f[19], double& n, V3& u) {
+    n = 0.0;
+#pragma unroll
+    for (int j = 0; j < 19; ++j) n += f[j];
+    double ux = 0.0, uy = 0.0, uz = 0.0;
+#pragma unroll
+    for (int j = 1; j < 19; ++j) {
+        if (cx(j) > 0) ux += f[j];
+        if (cx(j) < 0) ux -= f[j];
+        if (cy(j) > 0) uy += f[j];
+        if (cy(j) < 0) uy -= f[j];
+        if (cz(j) > 0) uz += f[j];
+        if (cz(j) < 0) uz -= f[j];
+    }
+    u = V3(ux / n, uy / n, uz / n);
+}
+
+// The penalty force of a particle node (LB.cpp:1865-1890): returns the
+// element-side increment diffVel and sets the node force hF = -diffVel.
+__device__ __forceinline__ V3 penalty64(const CoupleRec& r, const Coord& c, double mass, double n, const V3& u,
+                                        const LBKParams& p, V3& hF) {
+    const V3 pos((double)c.x, (double)c.y, (double)c.z);
+    const V3 x0(r.x0[0], r.x0[1], r.x0[2]);
+    const V3 rv(r.rv[0], r.rv[1], r.rv[2]);
+    const V3 v(r.v[0], r.v[1], r.v[2]);
+    const V3 w(r.w[0], r.w[1], r.w[2]);
+    const V3 radius = pos - x0 / p.unitLength + rv / p.unitLength;
+    const V3 localVel = v / p.unitSpeed + (w.cross(radius)) / p.unitAngVel;
+    const V3 diff = (mass / n) * (u - localVel);
+    hF = V3(0.0 + diff.x * -1.0, 0.0 + diff.y * -1.0, 0.0 + diff.z * -1.0);
+    return diff;
+}
+
+// node.cpp:103-145 with the node's previous viscosity.
+__device__ __forceinline__ double shear_visc64(const double f[19], const double feq[19], double n, double viscOld,
+                                               const LBKParams& p) {
+    const double minVisc = (0.501 - 0.5) / 3 / 1.0;
+    const double maxVisc = (1.8 - 0.5) / 3 / 1.0;
+    const double tau = 0.5 + 3.0 * viscOld * 1.0;
+    double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int j = 1; j < 19; ++j) {
+        const double d = f[j] - feq[j];
+        const int c[3] = {cx(j), cy(j), cz(j)};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const int cc = c[a] * c[b];
+                if (cc > 0) g[a * 3 + b] += d;
+                if (cc < 0) g[a * 3 + b] -= d;
+            }
+    }
+    const double s = 1.5 * 1.0 / (tau * n);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) g[k] *= s;
+    const double sr = sqrt(0.5 * (g[0] * g[0] + g[4] * g[4] + g[8] * g[8] + 2.0 * (g[1] * g[3] + g[6] * g[2] + g[5] * g[7])));
+    double nuTurb = 0.0;
+    if (p.turbulence) nuTurb = p.turbConst * sr;
+    double nuApp = p.nonNewtonian ? p.plasticVisc + p.yieldStress / (n * 2.0 * sr) : viscOld;
+    const double a = nuApp + nuTurb;
+    // std::min(maxVisc, a) then std::max(minVisc, .): NaN and +inf end at maxVisc
+    const double lo = (a < maxVisc) ? a : maxVisc;
+    return (minVisc < lo) ? lo : minVisc;
+}
+
+template <bool PULL, bool VARVISC, int COLL>
+__global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ fin, double* __restrict__ fout,
+                                                     const uint32_t* __restrict__ link,
+                                                     const uint32_t* __restrict__ solid,
+                                                     const CoupleRec* __restrict__ cp, double* __restrict__ hf,
+                                                     double* __restrict__ visc, double* __restrict__ mac,
+                                                     LBKParams p) {
+#pragma clang fp contract(off)
+    Coord c;
+    if (!node_coord(p, c)) return;
+    const size_t N = (size_t)p.X * p.Y * p.Z;
+    const uint32_t lw = link[c.idx];
+    if (!(lw & LINK_ACTIVE)) return;
+    double f[19];
+    if (PULL) pull19<double>(fin, N, p, c, lw, f);
+    else {
+#pragma unroll
+        for (int j = 0; j < 19; ++j) f[j] = fin[(size_t)j * N + c.idx];
+    }
+    double n;
+    V3 u;
+    moments64(f, n, u);
+    // penalty coupling force
+    V3 hF(0.0, 0.0, 0.0);
+    const uint32_t sid = solid ? solid[c.idx] : 0u;
+    if (sid) {
+        const double mass = (lw & LINK_INTERFACE) ? 0.5 * p.initDensity : p.initDensity;
+        const V3 d = penalty64(cp[sid - 1], c, mass, n, u, p, hF);
+        if (hf) {
+            hf[c.idx] = d.x;
+            hf[N + c.idx] = d.y;
+            hf[2 * N + c.idx] = d.z;
+        }
+    }
+    const V3 Fb(p.F[0], p.F[1], p.F[2]);
+    const V3 T = Fb + hF;
+    u += 0.5 * T / n;  // node.cpp:85-88
+    if (mac) {
+        mac[c.idx] = n;
+        mac[N + c.idx] = u.x;
+        mac[2 * N + c.idx] = u.y;
+        mac[3 * N + c.idx] = u.z;
+        mac[4 * N + c.idx] = hF.x;
+        mac[5 * N + c.idx] = hF.y;
+        mac[6 * N + c.idx] = hF.z;
+    }
+    if (!fout) return;
+    const double usq = u.norm2();
+    double feq[19];
+#pragma unroll
+    for (int j = 0; j < 19; ++j) {
+        const double vu = cdot_rt(j, u);
+        feq[j] = wgt(j) * n * (1.0 + 3.0 * vu + 4.5 * vu * vu - 1.5 * usq);
+    }
+    double nu = p.visc0;
+    if (VARVISC) {
+        nu = shear_visc64(f, feq, n, visc[c.idx], p);
+        visc[c.idx] = nu;
+    }
+    if (COLL == HYB_COLLISION_TRT) {
+        const double omP = 1.0 / (0.5 + 3.0 * nu);
+        const double omM = 1.0 / (0.5 + p.trtMagic / (1.0 / omP - 0.5));
+        const double uT = u.dot(T);
+        double out[19];
+#pragma unroll
+        for (int j = 0; j < 19; ++j) {
+            const int o = opp(j);
+            const double fp = 0.5 * (f[j] + f[o]), fm = 0.5 * (f[j] - f[o]);
+            const double ep = 0.5 * (feq[j] + feq[o]), em = 0.5 * (feq[j] - feq[o]);
+            const double vu = cdot_rt(j, u);
+            const double cF = cdot_rt(j, T);
+            const double sp = wgt(j) * (9.0 * vu * cF - 3.0 * uT);
+            const double sm = wgt(j) * (3.0 * cF);
+            out[j] = f[j] - omP * (fp - ep) - omM * (fm - em) + (1.0 - 0.5 * omP) * sp + (1.0 - 0.5 * omM) * sm;
+        }
+#pragma unroll
+        for (int j = 0; j < 19; ++j) fout[(size_t)j * N + c.idx] = out[j];
+    } else {
+        const double omega = 1.0 / (0.5 + 3.0 * nu * 1.0);
+        const double omegaf = 1.0 - 1.0 / (1.0 + 6.0 * nu * 1.0);
+#pragma unroll
+        for (int j = 0; j < 19; ++j) {
+            double fj = f[j] + omega * (feq[j] - f[j]);
+            // Guo forcing: forcePart = 9 (c.u) c + 3 (c - u), dotted with the total force
+            const double vu = cdot_rt(j, u);
+            const double a = 9.0 * vu;
+            const V3 fp((double)cx(j) * a + ((double)cx(j) - u.x) * 3.0, (double)cy(j) * a + ((double)cy(j) - u.y) * 3.0,
+                        (double)cz(j) * a + ((double)cz(j) - u.z) * 3.0);
+            fj += 1.0 * omegaf * wgt(j) * fp.dot(T);
+            fout[(size_t)j * N + c.idx] = fj;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ FP32 deviation path
+// g_j = f_j - w_j (rho0 = 1). Moments: rho = 1 + sum g, j = sum g c (sum w c = 0).
+// geq_j = w_j (drho + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u^2)).
+__device__ __forceinline__ float cdotf(int j, float ux, float uy, float uz) {
+    float r = 0.0f;
+    if (cx(j) != 0) r += cx(j) > 0 ? ux : -ux;
+    if (cy(j) != 0) r += cy(j) > 0 ? uy : -uy;
+    if (cz(j) != 0) r += cz(j) > 0 ? uz : -uz;
+    return r;
+}
+
+template <bool PULL, bool VARVISC, int COLL>
+__global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ fin, float* __restrict__ fout,
+                                                     const uint32_t* __restrict__ link,
+                                                     const uint32_t* __restrict__ solid,
+                                                     const CoupleRec* __restrict__ cp, float* __restrict__ hf,
+                                                     double* __restrict__ visc, double* __restrict__ mac,
+                                                     LBKParams p) {
+#pragma clang fp contract(fast)
+    Coord c;
+    if (!node_coord(p, c)) return;
+    const size_t N = (size_t)p.X * p.Y * p.Z;
+    const uint32_t lw = link[c.idx];
+    if (!(lw & LINK_ACTIVE)) return;
+    float g[19];
+    if (PULL) pull19<float>(fin, N, p, c, lw, g);
+    else {
+#pragma unroll
+        for (int j = 0; j < 19; ++j) g[j] = fin[(size_t)j * N + c.idx];
+    }
+    float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 19; ++j) {
+        drho += g[j];
+        if (cx(j) > 0) jx += g[j];
+        if (cx(j) < 0) jx -= g[j];
+        if (cy(j) > 0) jy += g[j];
+        if (cy(j) < 0) jy -= g[j];
+        if (cz(j) > 0) jz += g[j];
+        if (cz(j) < 0) jz -= g[j];
+    }
+    const float rho = 1.0f + drho;
+    const float irho = 1.0f / rho;
+    float ux = jx * irho, uy = jy * irho, uz = jz * irho;
+    float Tx = (float)p.F[0], Ty = (float)p.F[1], Tz = (float)p.F[2];
+    float hx = 0.0f, hy = 0.0f, hz = 0.0f;
+    const uint32_t sid = solid ? solid[c.idx] : 0u;
+    if (sid) {
+        const CoupleRec& r = cp[sid - 1];
+        const double mass = (lw & LINK_INTERFACE) ? 0.5 * p.initDensity : p.initDensity;
+        // rigid-body velocity at the node in double, then the difference in float
+        const V3 pos((double)c.x, (double)c.y, (double)c.z);
+        const V3 radius = pos - V3(r.x0[0], r.x0[1], r.x0[2]) / p.unitLength + V3(r.rv[0], r.rv[1], r.rv[2]) / p.unitLength;
+        const V3 lv = V3(r.v[0], r.v[1], r.v[2]) / p.unitSpeed + (V3(r.w[0], r.w[1], r.w[2]).cross(radius)) / p.unitAngVel;
+        const float lf = (float)mass * irho;
+        const float dx = lf * (ux - (float)lv.x), dy = lf * (uy - (float)lv.y), dz = lf * (uz - (float)lv.z);
+        hx = -dx; hy = -dy; hz = -dz;
+        if (hf) {
+            hf[c.idx] = dx;
+            hf[N + c.idx] = dy;
+            hf[2 * N + c.idx] = dz;
+        }
+        Tx += hx; Ty += hy; Tz += hz;
+    }
+    ux += 0.5f * Tx * irho;
+    uy += 0.5f * Ty * irho;
+    uz += 0.5f * Tz * irho;
+    if (mac) {
+        mac[c.idx] = (double)rho;
+        mac[N + c.idx] = ux;
+        mac[2 * N + c.idx] = uy;
+        mac[3 * N + c.idx] = uz;
+        mac[4 * N + c.idx] = hx;
+        mac[5 * N + c.idx] = hy;
+        mac[6 * N + c.idx] = hz;
+    }
+    if (!fout) return;
+    const float usq = ux * ux + uy * uy + uz * uz;
+    float geq[19];
+#pragma unroll
+    for (int j = 0; j < 19; ++j) {
+        const float vu = cdotf(j, ux, uy, uz);
+        geq[j] = (float)wgt(j) * (drho + rho * (3.0f * vu + 4.5f * vu * vu - 1.5f * usq));
+    }
+    float nu = (float)p.visc0;
+    if (VARVISC) {
+        // Bingham / Smagorinsky on the non-equilibrium stress (node.cpp:103-145)
+        float s[6] = {0, 0, 0, 0, 0, 0};  // xx yy zz xy yz zx
+#pragma unroll
+        for (int j = 1; j < 19; ++j) {
+            const float d = g[j] - geq[j];
+            if (cx(j)) s[0] += d;
+            if (cy(j)) s[1] += d;
+            if (cz(j)) s[2] += d;
+            if (cx(j) * cy(j) > 0) s[3] += d;
+            if (cx(j) * cy(j) < 0) s[3] -= d;
+            if (cy(j) * cz(j) > 0) s[4] += d;
+            if (cy(j) * cz(j) < 0) s[4] -= d;
+            if (cz(j) * cx(j) > 0) s[5] += d;
+            if (cz(j) * cx(j) < 0) s[5] -= d;
+        }
+        const double viscOld = visc[c.idx];
+        const float tau = 0.5f + 3.0f * (float)viscOld;
+        const float k = 1.5f / (tau * rho);
+        const float g00 = s[0] * k, g11 = s[1] * k, g22 = s[2] * k, g01 = s[3] * k, g12 = s[4] * k, g20 = s[5] * k;
+        const float sr = sqrtf(0.5f * (g00 * g00 + g11 * g11 + g22 * g22 + 2.0f * (g01 * g01 + g20 * g20 + g12 * g12)));
+        float nuTurb = p.turbulence ? (float)p.turbConst * sr : 0.0f;
+        float nuApp = p.nonNewtonian ? (float)p.plasticVisc + (float)p.yieldStress / (rho * 2.0f * sr) : (float)viscOld;
+        const float a = nuApp + nuTurb;
+        const float maxV = (float)((1.8 - 0.5) / 3.0), minV = (float)((0.501 - 0.5) / 3.0);
+        const float lo = (a < maxV) ? a : maxV;
+        nu = (minV < lo) ? lo : minV;
+        visc[c.idx] = (double)nu;
+    }
+    const float omP = 1.0f / (0.5f + 3.0f * nu);
+    const float uT = ux * Tx + uy * Ty + uz * Tz;
+    if (COLL == HYB_COLLISION_TRT) {
+        const float omM = 1.0f / (0.5f + (float)p.trtMagic / (1.0f / omP - 0.5f));
+        const float kp = 1.0f - 0.5f * omP, km = 1.0f - 0.5f * omM;
+        float out[19];
+#pragma unroll
+        for (int j = 0; j < 19; ++j) {
+            const int o = opp(j);
+            const float fpl = 0.5f * (g[j] + g[o]), fmi = 0.5f * (g[j] - g[o]);
+            const float ep = 0.5f * (geq[j] + geq[o]), em = 0.5f * (geq[j] - geq[o]);
+            const float vu = cdotf(j, ux, uy, uz);
+            const float cF = cdotf(j, Tx, Ty, Tz);
+            const float w = (float)wgt(j);
+            const float sp = w * (9.0f * vu * cF - 3.0f * uT);
+            const float sm = w * (3.0f * cF);
+            out[j] = g[j] - omP * (fpl - ep) - omM * (fmi - em) + kp * sp + km * sm;
+        }
+#pragma unroll
+        for (int j = 0; j < 19; ++j) fout[(size_t)j * N + c.idx] = out[j];
+    } else {
+        const float kp = 1.0f - 0.5f * omP;
+#pragma unroll
+        for (int j = 0; j < 19; ++j) {
+            const float vu = cdotf(j, ux, uy, uz);
+            const float cF = cdotf(j, Tx, Ty, Tz);
+            const float S = (float)wgt(j) * (9.0f * vu * cF + 3.0f * (cF - uT));
+            fout[(size_t)j * N + c.idx] = g[j] + omP * (geq[j] - g[j]) + kp * S;
+        }
+    }
+}
+
+// Standalone pull streaming: turns the post-collision state into the
+// reference's step-end (post-stream) populations, for host views only.
+template <typename T>
+__global__ void __launch_bounds__(BX * BY) k_stream(const T* __restrict__ fin, T* __restrict__ fout,
+                                                    const uint32_t* __restrict__ link, LBKParams p) {
+    Coord c;
+    if (!node_coord(p, c)) return;
+    const size_t N = (size_t)p.X * p.Y * p.Z;
+    const uint32_t lw = link[c.idx];
+    if (!(lw & LINK_ACTIVE)) return;
+    T f[19];
+    pull19<T>(fin, N, p, c, lw, f);
+#pragma unroll
+    for (int j = 0; j < 19; ++j) fout[(size_t)j * N + c.idx] = f[j];
+}
+
+// ------------------------------------------------------------------ initialisation
+// Boundary typing (LB.cpp:394-431), plane walls (LB.cpp:497-533) and the
+// problem's gas region (LB.cpp:611-784: SPLASH plane, or a liquid box).
+__global__ void k_init_types(uint8_t* __restrict__ type, LBKParams p, int b0, int b1, int b2, int b3, int b4, int b5,
+                             int nwalls, const WallDev* __restrict__ walls, int problem, double lx0, double lx1,
+                             double ly0, double ly1, double lz0, double lz1) {
+    Coord c;
+    if (!node_coord(p, c)) return;
+    uint8_t t = T_FLUID;
+    auto isWall = [](uint8_t tt) { return tt == 5 || tt == 6 || tt == 7 || tt == 8 || tt == 9; };
+    if (c.x == 0 && !isWall(t)) t = (uint8_t)b0;
+    else if (c.x == p.X - 1 && !isWall(t)) t = (uint8_t)b1;
+    if (c.y == 0 && !isWall(t)) t = (uint8_t)b2;
+    else if (c.y == p.Y - 1 && !isWall(t)) t = (uint8_t)b3;
+    if (c.z == 0 && !isWall(t)) t = (uint8_t)b4;
+    else if (c.z == p.Z - 1 && !isWall(t)) t = (uint8_t)b5;
+    const V3 pos((double)c.x, (double)c.y, (double)c.z);
+    for (int w = 0; w < nwalls; ++w) {
+        const V3 d = pos - walls[w].p / p.unitLength;
+        if (walls[w].n.dot(d) < 0.0) {
+            if (walls[w].slip) t = walls[w].moving ? 6 : 5;
+            else t = walls[w].moving ? 8 : 7;
+        }
+    }
+    if (t == T_FLUID) {
+        if (problem == HYB_PROBLEM_SPLASH) {
+            if ((pos - V3(0.0, 0.0, 20.0)).dot(V3(0.0, 0.0, 1.0)) > 0.0) t = T_GAS;
+        } else if (problem == HYB_PROBLEM_LIQUID_BOX) {
+            if (pos.x < lx0 || pos.x > lx1 || pos.y < ly0 || pos.y > ly1 || pos.z < lz0 || pos.z > lz1) t = T_GAS;
+        }
+    }
+    type[c.idx] = t;
+}
+
+// Neighbour index of an interior node along direction j with periodic rewiring.
+__device__ __forceinline__ size_t nb_index(const LBKParams& p, const Coord& c, int j) {
+    const Nb3 nx = nb3(c.x, p.X, p.per[0]);
+    const Nb3 ny = nb3(c.y, p.Y, p.per[1]);
+    const Nb3 nz = nb3(c.z, p.Z, p.per[2]);
+    return (size_t)pick(nx, cx(j)) + (size_t)p.X * ((size_t)pick(ny, cy(j)) + (size_t)p.Y * (size_t)pick(nz, cz(j)));
+}
+
+__device__ __forceinline__ bool interior(const LBKParams& p, const Coord& c) {
+    return c.x > 0 && c.x < p.X - 1 && c.y > 0 && c.y < p.Y - 1 && c.z > 0 && c.z < p.Z - 1;
+}
+
+// Interface = fluid with a gas neighbour (LB.cpp:787-797). Each pass reads the
+// previous pass's types, which is equivalent to the reference's in-place loops
+// because neither loop changes the property it tests on its neighbours.
+__global__ void k_tag_interface(const uint8_t* __restrict__ tin, uint8_t* __restrict__ tout, LBKParams p) {
+    Coord c;
+    if (!node_coord(p, c)) return;
+    uint8_t t = tin[c.idx];
+    if (t == T_FLUID && interior(p, c)) {
+        for (int j = 1; j < 19; ++j)
+            if (tin[nb_index(p, c, j)] == T_GAS) t = T_INTERFACE;
+    }
+    tout[c.idx] = t;
+}
+
+// Isolated interface nodes become gas (LB.cpp:799-814).
+__global__ void k_remove_isolated(const uint8_t* __restrict__ tin, uint8_t* __restrict__ tout, LBKParams p) {
+    Coord c;
+    if (!node_coord(p, c)) return;
+    uint8_t t = tin[c.idx];
+    if (t == T_INTERFACE && interior(p, c)) {
+        bool close = false;
+        for (int j = 1; j < 19; ++j)
+            if (tin[nb_index(p, c, j)] == T_FLUID) { close = true; break; }
+        if (!close) t = T_GAS;
+    }
+    tout[c.idx] = t;
+}
+
+__global__ void k_init_links(const uint8_t* __restrict__ type, uint32_t* __restrict__ link, LBKParams p) {
+    Coord c;
+    if (!node_coord(p, c)) return;
+    const uint8_t t = type[c.idx];
+    uint32_t lw = 0;
+    if ((t == T_FLUID || t == T_INTERFACE) && interior(p, c)) {
+        lw |= LINK_ACTIVE;
+        if (t == T_INTERFACE) lw |= LINK_INTERFACE;
+        for (int j = 1; j < 19; ++j)
+            if (type[nb_index(p, c, j)] == T_WALL) lw |= (1u << j);
+    }
+    link[c.idx] = lw;
+}
+
+__global__ void k_max_active(const uint32_t* __restrict__ link, int* __restrict__ mx, LBKParams p) {
+    Coord c;
+    if (!node_coord(p, c)) return;
+    if (link[c.idx] & LINK_ACTIVE) {
+        atomicMax(&mx[0], c.x);
+        atomicMax(&mx[1], c.y);
+        atomicMax(&mx[2], c.z);
+    }
+}
+
+// node::initialize + setEquilibrium (node.cpp:26-61) with the hydrostatic
+// density of LB.cpp:917-942. Interface nodes start at rho = 1.
+template <typename T, bool SHIFT>
+__global__ void k_init_pops(T* __restrict__ f, const uint32_t* __restrict__ link, const int* __restrict__ mx,
+                            double* __restrict__ visc, LBKParams p, double u0x, double u0y, double u0z, double Fx,
+                            double Fy, double Fz) {
+#pragma clang fp contract(off)
+    Coord c;
+    if (!node_coord(p, c)) return;
+    const size_t N = (size_t)p.X * p.Y * p.Z;
+    const uint32_t lw = link[c.idx];
+    if (!(lw & LINK_ACTIVE)) return;
+    // the cfg body force, before collision() zeroes it when forceFieldSolve = 0 (LB.cpp:1074-1076)
+    const V3 F(Fx, Fy, Fz);
+    double n;
+    if (lw & LINK_INTERFACE) {
+        n = p.initDensity;
+    } else {
+        const V3 dh = V3((double)c.x, (double)c.y, (double)c.z) - V3((double)mx[0], (double)mx[1], (double)mx[2]);
+        n = p.initDensity + 3.0 * 1.0 * 1.0 * p.initDensity * dh.dot(F);
+    }
+    const V3 u = (1.0 * F) / 2.0 / n + V3(u0x, u0y, u0z);
+    const double usq = u.norm2();
+    for (int j = 0; j < 19; ++j) {
+        const V3 cj((double)cx(j), (double)cy(j), (double)cz(j));
+        const double vu = u.dot(cj);
+        const double feq = wgt(j) * n * (1.0 + 3.0 * vu + 4.5 * vu * vu - 1.5 * usq);
+        if (SHIFT) f[(size_t)j * N + c.idx] = (T)(feq - wgt(j));
+        else f[(size_t)j * N + c.idx] = (T)feq;
+    }
+    if (visc) visc[c.idx] = p.visc0;
+}
+
+__global__ void k_count_active(const uint32_t* __restrict__ link, unsigned long long* __restrict__ cnt, LBKParams p) {
+    Coord c;
+    bool a = false;
+    if (node_coord(p, c)) a = (link[c.idx] & LINK_ACTIVE) != 0;
+    const unsigned long long b = __ballot(a);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(cnt, (unsigned long long)__popcll(b));
+}
+
+// ------------------------------------------------------------------ particle classification
+// Each particle slot (real particles, then ghosts) marks the active nodes
+// strictly inside it (vector.cpp:153-158). Where spheres overlap the highest
+// slot index wins, as in the full re-test of LB.cpp:475-495.
+__global__ void k_classify_clear(const int* __restrict__ nboxes, const NodeBox* __restrict__ boxes,
+                                 uint32_t* __restrict__ solid, LBKParams p) {
+    const int s = blockIdx.x;
+    if (s >= *nboxes) return;
+    const NodeBox b = boxes[s];
+    const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return;
+    const int tot = nx * ny * nz;
+    for (int k = threadIdx.x; k < tot; k += blockDim.x) {
+        const int x = b.lo[0] + k % nx;
+        const int y = b.lo[1] + (k / nx) % ny;
+        const int z = b.lo[2] + k / (nx * ny);
+        solid[(size_t)x + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z)] = 0u;
+    }
+}
+
+__device__ __forceinline__ NodeBox slot_box(const CoupleRec& r, const LBKParams& p) {
+    const double c[3] = {r.x0[0] / p.unitLength, r.x0[1] / p.unitLength, r.x0[2] / p.unitLength};
+    const double rad = r.r / p.unitLength;
+    const int lim[3] = {p.X, p.Y, p.Z};
+    NodeBox b;
+    for (int k = 0; k < 3; ++k) {
+        int lo = (int)floor(c[k] - rad);
+        int hi = (int)ceil(c[k] + rad);
+        lo = lo < 1 ? 1 : lo;
+        hi = hi > lim[k] - 2 ? lim[k] - 2 : hi;
+        b.lo[k] = lo;
+        b.hi[k] = hi;
+    }
+    return b;
+}
+
+__global__ void k_classify_set(const int* __restrict__ nslots_p, const CoupleRec* __restrict__ cp,
+                               NodeBox* __restrict__ boxes, int* __restrict__ nboxes, uint32_t* __restrict__ solid,
+                               const uint32_t* __restrict__ link, LBKParams p) {
+#pragma clang fp contract(off)
+    const int s = blockIdx.x;
+    if (s == 0 && threadIdx.x == 0) *nboxes = *nslots_p;
+    if (s >= *nslots_p) return;
+    const CoupleRec r = cp[s];
+    const NodeBox b = slot_box(r, p);
+    if (threadIdx.x == 0) boxes[s] = b;
+    const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return;
+    const V3 cen(r.x0[0] / p.unitLength, r.x0[1] / p.unitLength, r.x0[2] / p.unitLength);
+    const double rad = r.r / p.unitLength;
+    const int tot = nx * ny * nz;
+    for (int k = threadIdx.x; k < tot; k += blockDim.x) {
+        const int x = b.lo[0] + k % nx;
+        const int y = b.lo[1] + (k / nx) % ny;
+        const int z = b.lo[2] + k / (nx * ny);
+        const size_t idx = (size_t)x + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z);
+        if (!(link[idx] & LINK_ACTIVE)) continue;
+        const V3 d = V3((double)x, (double)y, (double)z) - cen;
+        if (d.norm2() < rad * rad) atomicMax(&solid[idx], (uint32_t)(s + 1));
+    }
+}
+
+// Neumaier-compensated FP64 accumulator.
+struct KSum {
+    double s, c;
+    __device__ void add(double v) {
+        const double t = s + v;
+        if (fabs(s) >= fabs(v)) c += (s - t) + v;
+        else c += (v - t) + s;
+        s = t;
+    }
+    __device__ double value() const { return s + c; }
+};
+
+// Per particle slot: sum over the nodes it owns of diffVel, radius x diffVel and
+// the node mass (LB.cpp:1897-1902), with compensated FP64 sums and a
+// fixed-order LDS tree across the workgroup (no atomics, deterministic).
+template <typename HT>
+__global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ nslots_p, const NodeBox* __restrict__ boxes,
+                                                        const CoupleRec* __restrict__ cp,
+                                                        const uint32_t* __restrict__ solid,
+                                                        const uint32_t* __restrict__ link, const HT* __restrict__ hf,
+                                                        HydroPartial* __restrict__ out, LBKParams p) {
+#pragma clang fp contract(off)
+    const int s = blockIdx.x;
+    if (s >= *nslots_p) return;
+    const size_t N = (size_t)p.X * p.Y * p.Z;
+    const CoupleRec r = cp[s];
+    const NodeBox b = boxes[s];
+    KSum acc[7];
+    for (int k = 0; k < 7; ++k) acc[k] = KSum{0.0, 0.0};
+    const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
+    const int tot = (nx > 0 && ny > 0 && nz > 0) ? nx * ny * nz : 0;
+    const V3 x0(r.x0[0], r.x0[1], r.x0[2]);
+    const V3 rv(r.rv[0], r.rv[1], r.rv[2]);
+    for (int k = threadIdx.x; k < tot; k += blockDim.x) {
+        const int x = b.lo[0] + k % nx;
+        const int y = b.lo[1] + (k / nx) % ny;
+        const int z = b.lo[2] + k / (nx * ny);
+        const size_t idx = (size_t)x + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z);
+        if (solid[idx] != (uint32_t)(s + 1)) continue;
+        const V3 d((double)hf[idx], (double)hf[N + idx], (double)hf[2 * N + idx]);
+        const V3 radius = V3((double)x, (double)y, (double)z) - x0 / p.unitLength + rv / p.unitLength;
+        const V3 m = radius.cross(d);
+        const double mass = (link[idx] & LINK_INTERFACE) ? 0.5 * p.initDensity : p.initDensity;
+        acc[0].add(d.x); acc[1].add(d.y); acc[2].add(d.z);
+        acc[3].add(m.x); acc[4].add(m.y); acc[5].add(m.z);
+        acc[6].add(mass);
+    }
+    __shared__ double sh_s[7][256];
+    __shared__ double sh_c[7][256];
+    for (int k = 0; k < 7; ++k) { sh_s[k][threadIdx.x] = acc[k].s; sh_c[k][threadIdx.x] = acc[k].c; }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int k = 0; k < 7; ++k) {
+                KSum a{sh_s[k][threadIdx.x], sh_c[k][threadIdx.x]};
+                a.add(sh_s[k][threadIdx.x + w]);
+                a.c += sh_c[k][threadIdx.x + w];
+                sh_s[k][threadIdx.x] = a.s;
+                sh_c[k][threadIdx.x] = a.c;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 7; ++k) out[s].s[k] = sh_s[k][0] + sh_c[k][0];
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+dim3 lattice_grid(const LBKParams& p) { return dim3((p.X + BX - 1) / BX, (p.Y + BY - 1) / BY, p.Z); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+#define HYB_FUSED_CASE(PULL, VV, CO)                                                                               \
+    if (pull == PULL && varVisc == VV && collision == CO) {                                                        \
+        if (prec == Prec::F64)                                                                                     \
+            k_fused64<PULL, VV, CO><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link, solid, cp,            \
+                                                    (double*)hf, visc, mac, p);                                    \
+        else                                                                                                       \
+            k_fused32<PULL, VV, CO><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid, cp, (float*)hf, \
+                                                    visc, mac, p);                                                 \
+        return;                                                                                                    \
+    }
+
+void launch_lb_fused(Prec prec, bool pull, bool varVisc, int collision, const void* fin, void* fout,
+                     const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, void* hf, double* visc,
+                     double* mac, const LBKParams& p, hipStream_t s) {
+    const dim3 g = lattice_grid(p), b(BX, BY, 1);
+    HYB_FUSED_CASE(true, false, HYB_COLLISION_BGK)
+    HYB_FUSED_CASE(true, false, HYB_COLLISION_TRT)
+    HYB_FUSED_CASE(true, true, HYB_COLLISION_BGK)
+    HYB_FUSED_CASE(true, true, HYB_COLLISION_TRT)
+    HYB_FUSED_CASE(false, false, HYB_COLLISION_BGK)
+    HYB_FUSED_CASE(false, false, HYB_COLLISION_TRT)
+    HYB_FUSED_CASE(false, true, HYB_COLLISION_BGK)
+    HYB_FUSED_CASE(false, true, HYB_COLLISION_TRT)
+}
+
+void launch_lb_stream(Prec prec, const void* fin, void* fout, const uint32_t* link, const LBKParams& p, hipStream_t s) {
+    const dim3 g = lattice_grid(p), b(BX, BY, 1);
+    if (prec == Prec::F64) k_stream<double><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link, p);
+    else k_stream<float><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, p);
+}
+
+void launch_lb_init_types(uint8_t* type, const int* bnd, int nwalls, const WallDev* walls, int problem,
+                          const double* lbx, const LBKParams& p, hipStream_t s) {
+    const dim3 g = lattice_grid(p), b(BX, BY, 1);
+    k_init_types<<<g, b, 0, s>>>(type, p, bnd[0], bnd[1], bnd[2], bnd[3], bnd[4], bnd[5], nwalls, walls, problem, lbx[0],
+                                 lbx[1], lbx[2], lbx[3], lbx[4], lbx[5]);
+}
+
+void launch_lb_init_interface(uint8_t* type, const LBKParams& p, hipStream_t s) {
+    const dim3 g = lattice_grid(p), b(BX, BY, 1);
+    const size_t N = (size_t)p.X * p.Y * p.Z;
+    uint8_t* tmp = nullptr;
+    if (hipMalloc((void**)&tmp, N) != hipSuccess) return;
+    k_tag_interface<<<g, b, 0, s>>>(type, tmp, p);
+    k_remove_isolated<<<g, b, 0, s>>>(tmp, type, p);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(tmp);
+}
+
+void launch_lb_init_links(const uint8_t* type, uint32_t* link, const LBKParams& p, hipStream_t s) {
+    const dim3 g = lattice_grid(p), b(BX, BY, 1);
+    k_init_links<<<g, b, 0, s>>>(type, link, p);
+}
+
+void launch_lb_max_active(const uint32_t* link, int* mx, const LBKParams& p, hipStream_t s) {
+    const dim3 g = lattice_grid(p), b(BX, BY, 1);
+    k_max_active<<<g, b, 0, s>>>(link, mx, p);
+}
+
+void launch_lb_init_pops(Prec prec, void* f, const uint32_t* link, const int* mx, double* visc, const double* u0,
+                         const double* Finit, const LBKParams& p, hipStream_t s) {
+    const dim3 g = lattice_grid(p), b(BX, BY, 1);
+    if (prec == Prec::F64)
+        k_init_pops<double, false><<<g, b, 0, s>>>((double*)f, link, mx, visc, p, u0[0], u0[1], u0[2], Finit[0],
+                                                   Finit[1], Finit[2]);
+    else
+        k_init_pops<float, true><<<g, b, 0, s>>>((float*)f, link, mx, visc, p, u0[0], u0[1], u0[2], Finit[0], Finit[1],
+                                                 Finit[2]);
+}
+
+void launch_classify_clear(const int* nboxes, int max_slots, const NodeBox* boxes, uint32_t* solid, const LBKParams& p,
+                           hipStream_t s) {
+    if (max_slots <= 0) return;
+    k_classify_clear<<<max_slots, 256, 0, s>>>(nboxes, boxes, solid, p);
+}
+
+void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, NodeBox* boxes, int* nboxes,
+                         uint32_t* solid, const uint32_t* link, const LBKParams& p, hipStream_t s) {
+    if (max_slots <= 0) return;
+    k_classify_set<<<max_slots, 256, 0, s>>>(nslots, cp, boxes, nboxes, solid, link, p);
+}
+
+void launch_hydro_partials(const int* nslots, int max_slots, const NodeBox* boxes, const CoupleRec* cp,
+                           const uint32_t* solid, const uint32_t* link, Prec prec, const void* hf, HydroPartial* out,
+                           const LBKParams& p, hipStream_t s) {
+    if (max_slots <= 0) return;
+    if (prec == Prec::F64)
+        k_hydro_partials<double><<<max_slots, 256, 0, s>>>(nslots, boxes, cp, solid, link, (const double*)hf, out, p);
+    else
+        k_hydro_partials<float><<<max_slots, 256, 0, s>>>(nslots, boxes, cp, solid, link, (const float*)hf, out, p);
+}
+
+void launch_fill_u32(uint32_t* a, uint32_t v, size_t n, hipStream_t s) {
+    if (!n) return;
+    k_fill_u32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, v, n);
+}
+
+void launch_count_active(const uint32_t* link, unsigned long long* count, const LBKParams& p, hipStream_t s) {
+    const dim3 g = lattice_grid(p), b(BX, BY, 1);
+    k_count_active<<<g, b, 0, s>>>(link, count, p);
+}
+
+}  // namespace hyb

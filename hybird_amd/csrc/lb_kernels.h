@@ -1,0 +1,74 @@
+// Host-side launch interface of the lattice-Boltzmann kernels (lb.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.h"
+
+namespace hyb {
+
+// Parameters every LB kernel receives by value.
+struct LBKParams {
+    int X, Y, Z;
+    int per[3];          // periodic along x, y, z
+    double F[3];         // body force (lattice units), zero when forceFieldSolve = 0
+    double visc0;        // initial kinematic viscosity (lattice units)
+    double plasticVisc, yieldStress, turbConst;
+    int nonNewtonian, turbulence;
+    int collision;       // HYB_COLLISION_BGK / HYB_COLLISION_TRT
+    double trtMagic;
+    double unitLength, unitSpeed, unitAngVel;
+    double initDensity;
+};
+
+// Per-particle record the coupling reads (particle slots include ghosts).
+// Raw (DEM, physical-unit) values: centre, vector centre-of-element -> centre,
+// element velocity and angular velocity; radius.
+struct CoupleRec {
+    double x0[3];
+    double rv[3];
+    double v[3];
+    double w[3];
+    double r;
+    int32_t elem;
+    int32_t pad;
+};
+
+// Box of lattice nodes a particle slot may cover (inclusive bounds).
+struct NodeBox {
+    int lo[3];
+    int hi[3];
+};
+
+// Partial hydrodynamic sums of one particle slot: F(3), M(3), fluidVolume.
+struct HydroPartial {
+    double s[7];
+};
+
+enum class Prec { F64, F32S };
+
+// f planes: FP64 -> double[19][N]; FP32S -> float[19][N] holding f - w.
+void launch_lb_fused(Prec prec, bool pull, bool varVisc, int collision, const void* fin, void* fout,
+                     const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, void* hf, double* visc,
+                     double* mac, const LBKParams& p, hipStream_t s);
+void launch_lb_stream(Prec prec, const void* fin, void* fout, const uint32_t* link, const LBKParams& p, hipStream_t s);
+void launch_lb_init_types(uint8_t* type, const int* boundary, int nwalls, const WallDev* walls, int problem,
+                          const double* liquidBox, const LBKParams& p, hipStream_t s);
+void launch_lb_init_interface(uint8_t* type, const LBKParams& p, hipStream_t s);
+void launch_lb_init_links(const uint8_t* type, uint32_t* link, const LBKParams& p, hipStream_t s);
+void launch_lb_max_active(const uint32_t* link, int* maxXYZ, const LBKParams& p, hipStream_t s);
+void launch_lb_init_pops(Prec prec, void* f, const uint32_t* link, const int* maxXYZ, double* visc, const double* u0,
+                         const double* Finit, const LBKParams& p, hipStream_t s);
+void launch_classify_clear(const int* nboxes, int max_slots, const NodeBox* boxes, uint32_t* solid, const LBKParams& p,
+                           hipStream_t s);
+void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, NodeBox* boxes, int* nboxes,
+                         uint32_t* solid, const uint32_t* link, const LBKParams& p, hipStream_t s);
+void launch_hydro_partials(const int* nslots, int max_slots, const NodeBox* boxes, const CoupleRec* cp,
+                           const uint32_t* solid, const uint32_t* link, Prec prec, const void* hf, HydroPartial* out,
+                           const LBKParams& p, hipStream_t s);
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+void launch_count_active(const uint32_t* link, unsigned long long* count, const LBKParams& p, hipStream_t s);
+
+}  // namespace hyb

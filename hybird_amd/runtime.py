@@ -1,0 +1,147 @@
+"""Python access to libhybird_amd.so through its C ABI (ctypes).
+
+``Simulation`` owns one LB handle and one DEM handle on the GPU and runs the
+coupled cycle of the reference (goCycle, hybird.cpp:35-66). The library is
+required: nothing here falls back to a CPU path, and loading fails loudly when
+the shared library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._abi import HybConfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhybird_amd.so")
+
+_lib = None
+
+LB_FIELDS = {"f": (np.float64, (-1, 19)), "n": (np.float64, None), "u": (np.float64, (-1, 3)),
+             "hydroForce": (np.float64, (-1, 3)), "mass": (np.float64, None), "visc": (np.float64, None),
+             "types": (np.uint8, None), "pflag": (np.uint8, None), "solidIndex": (np.uint32, None),
+             "totalMass": (np.float64, None)}
+DEM_FIELDS = {"e_x": (np.float64, (-1, 6, 3)), "e_xp": (np.float64, (-1, 6, 3)), "e_q": (np.float64, (-1, 6, 4)),
+              "e_qp": (np.float64, (-1, 6, 4)), "e_w": (np.float64, (-1, 6, 3)), "e_wp": (np.float64, (-1, 6, 3)),
+              "e_forces": (np.float64, (-1, 7, 3)), "e_misc": (np.float64, (-1, 19)),
+              "p_data": (np.float64, (-1, 10)), "p_idx": (np.uint32, (-1, 4)), "neighborTable": (np.uint32, None),
+              "nearWallTable": (np.uint32, None), "ghostIndex": (np.uint32, None), "ghostVec": (np.float64, (-1, 3)),
+              "dem_scalars": (np.float64, None), "rebuilds": (np.float64, None)}
+
+# every extern "C" entry point declared in include/hybird_amd.h
+EXPORTS = ["hyb_config_defaults", "hyb_last_error", "hyb_device_count", "hyb_sync", "hyb_dem_create", "hyb_dem_init",
+           "hyb_dem_step", "hyb_dem_destroy", "hyb_lb_create", "hyb_lb_init", "hyb_lb_fs_step", "hyb_lb_couple",
+           "hyb_lb_step", "hyb_lb_destroy", "hyb_cycle", "hyb_lb_get", "hyb_dem_get", "hyb_lb_set_profiling",
+           "hyb_lb_kernel_time", "hyb_lb_active_nodes"]
+
+
+class HybError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libhybird_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HybError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+    L.hyb_last_error.restype = C.c_char_p
+    L.hyb_config_defaults.argtypes = [vp]
+    L.hyb_config_defaults.restype = None
+    for name, args in [("hyb_device_count", [C.POINTER(C.c_int)]), ("hyb_sync", [vp]),
+                       ("hyb_dem_create", [vp, i32, vp, C.POINTER(vp)]), ("hyb_dem_init", [vp, vp]),
+                       ("hyb_dem_step", [vp]), ("hyb_dem_destroy", [vp]), ("hyb_lb_create", [vp, C.POINTER(vp)]),
+                       ("hyb_lb_init", [vp, vp]), ("hyb_lb_fs_step", [vp]),
+                       ("hyb_lb_couple", [vp, vp, C.POINTER(i32)]), ("hyb_lb_step", [vp, vp]),
+                       ("hyb_lb_destroy", [vp]), ("hyb_cycle", [vp, vp, i32]),
+                       ("hyb_lb_get", [vp, C.c_char_p, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
+                       ("hyb_dem_get", [vp, C.c_char_p, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
+                       ("hyb_lb_set_profiling", [vp, i32]),
+                       ("hyb_lb_kernel_time", [vp, C.POINTER(C.c_double), C.POINTER(i64)]),
+                       ("hyb_lb_active_nodes", [vp, C.POINTER(i64)])]:
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise HybError(f"{what} failed ({rc}): {load().hyb_last_error().decode()}")
+
+
+def default_config() -> HybConfig:
+    c = HybConfig()
+    load().hyb_config_defaults(C.byref(c))
+    return c
+
+
+class Simulation:
+    """Coupled LB + DEM state on the GPU, created as the reference's main() does."""
+
+    def __init__(self, config: HybConfig, records: np.ndarray):
+        L = load()
+        rec = np.ascontiguousarray(records, dtype=np.float64).reshape(-1, 20)
+        self.config = config
+        self._dem = C.c_void_p()
+        self._lb = C.c_void_p()
+        _check(L.hyb_dem_create(C.byref(config), int(rec.shape[0]), rec.ctypes.data if rec.size else None,
+                                C.byref(self._dem)), "hyb_dem_create")
+        _check(L.hyb_lb_create(C.byref(config), C.byref(self._lb)), "hyb_lb_create")
+        _check(L.hyb_dem_init(self._dem, self._lb), "hyb_dem_init")
+        _check(L.hyb_lb_init(self._lb, self._dem), "hyb_lb_init")
+
+    def cycle(self, n: int = 1) -> None:
+        _check(load().hyb_cycle(self._lb, self._dem, int(n)), "hyb_cycle")
+
+    def sync(self) -> None:
+        _check(load().hyb_sync(self._lb), "hyb_sync")
+
+    def get(self, name: str) -> np.ndarray:
+        L = load()
+        if name in LB_FIELDS:
+            fn, h, (dt, shp) = L.hyb_lb_get, self._lb, LB_FIELDS[name]
+        elif name in DEM_FIELDS:
+            fn, h, (dt, shp) = L.hyb_dem_get, self._dem, DEM_FIELDS[name]
+        else:
+            raise KeyError(name)
+        need = C.c_size_t(0)
+        _check(fn(h, name.encode(), None, 0, C.byref(need)), f"get {name}")
+        out = np.empty(need.value // np.dtype(dt).itemsize, dtype=dt)
+        if need.value:
+            _check(fn(h, name.encode(), out.ctypes.data, need.value, C.byref(need)), f"get {name}")
+        return out.reshape(shp) if shp else out
+
+    def set_profiling(self, on: bool) -> None:
+        _check(load().hyb_lb_set_profiling(self._lb, 1 if on else 0), "hyb_lb_set_profiling")
+
+    def kernel_time(self):
+        ms = C.c_double(0.0)
+        n = C.c_int64(0)
+        _check(load().hyb_lb_kernel_time(self._lb, C.byref(ms), C.byref(n)), "hyb_lb_kernel_time")
+        return ms.value, n.value
+
+    def active_nodes(self) -> int:
+        n = C.c_int64(0)
+        _check(load().hyb_lb_active_nodes(self._lb, C.byref(n)), "hyb_lb_active_nodes")
+        return n.value
+
+    def close(self) -> None:
+        if self._lb:
+            load().hyb_lb_destroy(self._lb)
+            self._lb = C.c_void_p()
+        if self._dem:
+            load().hyb_dem_destroy(self._dem)
+            self._dem = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -6,6 +6,9 @@
 //
 // Usage: hybird_ref_dump <cfg> <workdir> <outprefix> <cycle,cycle,...>
 // Writes <outprefix>_<cycle>.bin for every listed cycle (0 = after init).
+// With outprefix "-" nothing is dumped: the harness runs up to the largest
+// listed cycle and prints the steady-state seconds per cycle, (T(n) - T(1))/(n-1),
+// as one JSON line (CPU-baseline timing).
 //
 // Format of a .bin file: repeated records
 //   u32 name_len | name bytes | u8 dtype ('d' f64, 'I' u32, 'B' u8, 'i' i32)
@@ -241,6 +244,20 @@ int main(int argc, char** argv) {
     dem.demTime = 0.0;
     dem.demTimeStep = 0;
 
+    if (prefix == "-") {
+        auto now = []() {
+            timespec ts;
+            clock_gettime(CLOCK_MONOTONIC, &ts);
+            return ts.tv_sec + 1e-9 * ts.tv_nsec;
+        };
+        goCycle(io, dem, lb);
+        const double t1 = now();
+        for (unsigned c = 2; c <= last; ++c) goCycle(io, dem, lb);
+        const double t2 = now();
+        std::printf("{\"cycles\": %u, \"seconds_per_cycle\": %.9g, \"active_nodes\": %zu, \"particles\": %zu}\n",
+                    last - 1, (t2 - t1) / double(last > 1 ? last - 1 : 1), lb.activeNodes.size(), dem.elmts.size());
+        return 0;
+    }
     if (cycles.count(0)) dump(prefix + "_0.bin", io, dem, lb);
     for (unsigned c = 1; c <= last; ++c) {
         goCycle(io, dem, lb);
