@@ -121,10 +121,12 @@ struct hyb_lb {
     int* maxXYZ = nullptr;
     unsigned long long* dcount = nullptr;
     CoupleRec* cp = nullptr;
+    CoupleRecF* cpf = nullptr;
     NodeBox* boxes = nullptr;
     int* nslots = nullptr;  // [0] slots this step, [1] boxes marked last step
     HydroPartial* part = nullptr;
     int slotCap = 0;
+    uint32_t ctag = 0;       // coupling-step tag of the node ownership words
     hyb_dem* dem = nullptr;  // DEM handle coupled at init (read by the host views)
     int64_t steps = 0;
     bool inited = false;
@@ -144,6 +146,7 @@ struct hyb_lb {
     LBKParams params() const {
         LBKParams p;
         p.X = X; p.Y = Y; p.Z = Z;
+        p.PS = planeStride();
         p.per[0] = boundary[0] == 4; p.per[1] = boundary[2] == 4; p.per[2] = boundary[4] == 4;
         // collision() zeroes the force permanently when forceFieldSolve = 0 (LB.cpp:1074-1076)
         for (int k = 0; k < 3; ++k) p.F[k] = forceField ? lbF[k] : 0.0;
@@ -154,9 +157,14 @@ struct hyb_lb {
         p.trtMagic = cfg.trtMagic;
         p.unitLength = unit.Length; p.unitSpeed = unit.Speed; p.unitAngVel = unit.AngVel;
         p.initDensity = initDensity;
+        p.ctag = ctag;
         return p;
     }
-    size_t popBytes() const { return (prec == Prec::F64 ? 8 : 4) * 19 * N; }
+    // planes are padded so that successive planes do not start on the same
+    // memory-channel boundary: N rounded up to 256 elements plus planePad
+    size_t planePad = 0;
+    size_t planeStride() const { return ((N + 255) / 256) * 256 + planePad; }
+    size_t popBytes() const { return (prec == Prec::F64 ? 8 : 4) * 19 * planeStride(); }
 };
 
 extern "C" {
@@ -323,7 +331,7 @@ int hyb_dem_init(hyb_dem* d, const hyb_lb* lb) {
     const int ghostsPer = np + np * (np - 1) / 2;
     p.Gcap = std::max(1, d->P * std::max(ghostsPer, 1));
     const int scap = d->P + p.Gcap;
-    p.pairCap = std::max(64, scap * 48);
+    p.pairCap = std::max(1024, scap * 16 + 1024);
     DEMBuffers& b = d->b;
     std::memset(&b, 0, sizeof(b));
     b.scap = scap;
@@ -371,7 +379,8 @@ int hyb_dem_init(hyb_dem* d, const hyb_lb* lb) {
         (rc = dalloc(&b.csrOff, (size_t)P + 1)) || (rc = dalloc(&b.csr, 2 * (size_t)p.pairCap)) ||
         (rc = dalloc(&b.cursor, scap)) || (rc = dalloc(&b.ghostOfCount, std::max(P, 1))) ||
         (rc = dalloc(&b.ghostOf, 8 * (size_t)std::max(P, 1))) || (rc = dalloc(&b.side, 3 * (size_t)std::max(P, 1))) ||
-        (rc = dalloc(&b.ghostB, p.Gcap)))
+        (rc = dalloc(&b.ghostB, p.Gcap)) || (rc = dalloc(&b.posI, p.pairCap)) || (rc = dalloc(&b.posJ, p.pairCap)) ||
+        (rc = dalloc(&b.prec, 2 * (size_t)p.pairCap)))
         return rc;
     HIPCHK(hipMemcpyAsync(b.el, el.data(), el.size() * sizeof(double), hipMemcpyHostToDevice, d->stream));
     DEMState st;
@@ -382,7 +391,10 @@ int hyb_dem_init(hyb_dem* d, const hyb_lb* lb) {
     HIPCHK(hipMemsetAsync(b.nearWall, 0xFF, sizeof(uint32_t) * std::max(P, 1), d->stream));
     HIPCHK(hipMemsetAsync(b.csrOff, 0, sizeof(uint32_t) * ((size_t)P + 1), d->stream));
     HIPCHK(hipMemsetAsync(b.ghostOfCount, 0, sizeof(int) * std::max(P, 1), d->stream));
-    if (P) launch_dem_rebuild(p, b, 1, d->stream);  // evalNeighborTable at init (DEM.cpp:258-262)
+    if (P) {
+        launch_dem_rebuild(p, b, 1, d->stream);  // evalNeighborTable at init (DEM.cpp:258-262)
+        launch_dem_predict(p, b, d->stream);     // the first sub-step's predictor
+    }
     HIPCHK(hipGetLastError());
     d->demTime = 0.0;
     d->demTimeStep = 0;
@@ -397,8 +409,8 @@ int hyb_dem_step(hyb_dem* d) {  // DEM.cpp:331-376
         d->demTime += d->deltat;
         if (d->P) {
             launch_dem_rebuild(d->p, d->b, 0, d->stream);
-            launch_dem_predict(d->p, d->b, d->stream);
-            launch_dem_forces_correct(d->p, d->b, d->stream);
+            launch_dem_pairs(d->p, d->b, d->stream);
+            launch_dem_gather(d->p, d->b, d->stream);
         }
         d->substeps++;
     }
@@ -413,6 +425,7 @@ int hyb_dem_destroy(hyb_dem* d) {
     dfree(b.el); dfree(b.st); dfree(b.ghostIndex); dfree(b.ghostVec); dfree(b.pairs); dfree(b.nearWall);
     dfree(b.cellTable); dfree(b.cellOf); dfree(b.cellCount); dfree(b.work); dfree(b.csrOff); dfree(b.csr);
     dfree(b.cursor); dfree(b.ghostOfCount); dfree(b.ghostOf); dfree(b.side); dfree(b.ghostB);
+    dfree(b.posI); dfree(b.posJ); dfree(b.prec);
     delete d;
     return HYB_OK;
 }
@@ -465,6 +478,11 @@ int hyb_lb_create(const hyb_config* cfg, hyb_lb** out) {  // LB.cpp:85-188 + LB.
     }
     if (c.collision != HYB_COLLISION_BGK && c.collision != HYB_COLLISION_TRT) { delete l; return fail(HYB_E_ARG, "unknown collision model"); }
     l->prec = c.precision == HYB_FP32_SHIFTED ? Prec::F32S : Prec::F64;
+#ifdef HYB_PLANE_PAD
+    l->planePad = HYB_PLANE_PAD;
+#else
+    l->planePad = 0;
+#endif
     l->stream = shared_stream();
     *out = l;
     return HYB_OK;
@@ -483,16 +501,22 @@ static int lb_alloc(hyb_lb* l, int slotCap) {
     l->slotCap = slotCap;
     if (slotCap > 0) {
         HIPCHK(hipMalloc(&l->hf, (l->prec == Prec::F64 ? 8 : 4) * 3 * N));
-        if ((rc = dalloc(&l->cp, slotCap)) || (rc = dalloc(&l->boxes, slotCap)) || (rc = dalloc(&l->part, slotCap)))
+        if ((rc = dalloc(&l->cp, slotCap)) || (rc = dalloc(&l->cpf, slotCap)) || (rc = dalloc(&l->boxes, slotCap)) ||
+            (rc = dalloc(&l->part, slotCap)))
             return rc;
     }
     return HYB_OK;
 }
 
+// Particle-node classification of one coupling step: every slot marks the
+// active nodes strictly inside it with this step's tag (LB.cpp:475-495).
 static void lb_couple_launch(hyb_lb* l, hyb_dem* d) {
+    if (++l->ctag > SOLID_TAG_MAX) {
+        launch_fill_u32(l->solid, 0u, l->N, l->stream);
+        l->ctag = 1;
+    }
     const LBKParams p = l->params();
-    launch_dem_couple_table(d->p, d->b, l->cp, l->nslots, l->stream);
-    launch_classify_clear(l->nslots + 1, l->slotCap, l->boxes, l->solid, p, l->stream);
+    launch_dem_couple_table(d->p, d->b, l->cp, l->cpf, l->nslots, l->unit, l->stream);
     launch_classify_set(l->nslots, l->slotCap, l->cp, l->boxes, l->nslots + 1, l->solid, l->link, p, l->stream);
 }
 
@@ -500,6 +524,7 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
     if (!l) return fail(HYB_E_ARG, "hyb_lb_init: null handle");
     if (d && !d->inited) return fail(HYB_E_STATE, "hyb_lb_init: DEM handle not initialised");
     const int slotCap = d ? d->P + d->p.Gcap : 0;
+    if (slotCap >= (int)SOLID_SLOT_MASK) return fail(HYB_E_CAPACITY, "too many particle slots for the ownership word");
     int rc = lb_alloc(l, d && d->P ? slotCap : 0);
     if (rc) return rc;
     const LBKParams p = l->params();
@@ -535,8 +560,10 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
         HIPCHK(hipMemcpy(lk.data(), l->link, l->N * 4, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(so.data(), l->solid, l->N * 4, hipMemcpyDeviceToHost));
         double tm = 0.0;
-        for (size_t i = 0; i < l->N; ++i)
-            if ((lk[i] & LINK_ACTIVE) && !so[i]) tm += (lk[i] & LINK_INTERFACE) ? 0.5 * l->initDensity : l->initDensity;
+        for (size_t i = 0; i < l->N; ++i) {
+            const bool owned = (so[i] >> SOLID_SLOT_BITS) == l->ctag && (so[i] & SOLID_SLOT_MASK);
+            if ((lk[i] & LINK_ACTIVE) && !owned) tm += (lk[i] & LINK_INTERFACE) ? 0.5 * l->initDensity : l->initDensity;
+        }
         l->totalMass = tm;
     }
     l->steps = 0;
@@ -582,8 +609,8 @@ int hyb_lb_step(hyb_lb* l, hyb_dem* d) {  // LB.cpp:221-233 fused into one kerne
         HIPCHK(hipEventRecord(e0, l->stream));
     }
     launch_lb_fused(l->prec, l->steps > 0, l->varVisc(), l->cfg.collision, l->f[l->cur], l->f[l->cur ^ 1], l->link,
-                    withParticles ? l->solid : nullptr, l->cp, withParticles ? l->hf : nullptr, l->visc, nullptr, p,
-                    l->stream);
+                    withParticles ? l->solid : nullptr, l->cp, l->cpf, withParticles ? l->hf : nullptr, l->visc, nullptr,
+                    p, l->stream);
     if (l->profiling) HIPCHK(hipEventRecord(e1, l->stream));
     if (withParticles) {
         launch_hydro_partials(l->nslots, l->slotCap, l->boxes, l->cp, l->solid, l->link, l->prec, l->hf, l->part, p,
@@ -606,7 +633,7 @@ int hyb_lb_destroy(hyb_lb* l) {
     if (l->hf) (void)hipFree(l->hf);
     if (l->scratch) (void)hipFree(l->scratch);
     dfree(l->type); dfree(l->link); dfree(l->solid); dfree(l->visc); dfree(l->mac); dfree(l->maxXYZ);
-    dfree(l->dcount); dfree(l->cp); dfree(l->boxes); dfree(l->nslots); dfree(l->part);
+    dfree(l->dcount); dfree(l->cp); dfree(l->cpf); dfree(l->boxes); dfree(l->nslots); dfree(l->part);
     delete l;
     return HYB_OK;
 }
@@ -678,7 +705,7 @@ static int ensure_mac(hyb_lb* l, hyb_dem* d) {
         const bool withParticles = d && d->P && l->slotCap > 0;
         launch_fill_u32((uint32_t*)l->mac, 0xFFFFFFFFu, 14 * l->N, l->stream);  // NaN pattern for inactive nodes
         launch_lb_fused(l->prec, l->steps > 1, l->varVisc(), l->cfg.collision, l->f[l->cur ^ 1], nullptr, l->link,
-                        withParticles ? l->solid : nullptr, l->cp, nullptr, l->visc, l->mac, p, l->stream);
+                        withParticles ? l->solid : nullptr, l->cp, l->cpf, nullptr, l->visc, l->mac, p, l->stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(l->stream));
     }
@@ -703,6 +730,7 @@ int hyb_lb_get(hyb_lb* l, const char* cname, void* host, size_t cap, size_t* nee
     if (name == "pflag" || name == "solidIndex") {
         std::vector<uint32_t> so(N);
         HIPCHK(hipMemcpy(so.data(), l->solid, N * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; ++i) so[i] = (so[i] >> SOLID_SLOT_BITS) == l->ctag ? (so[i] & SOLID_SLOT_MASK) : 0u;
         if (name == "pflag") {
             std::vector<uint8_t> pf(N);
             for (size_t i = 0; i < N; ++i) pf[i] = so[i] ? 1 : 0;
@@ -723,17 +751,18 @@ int hyb_lb_get(hyb_lb* l, const char* cname, void* host, size_t cap, size_t* nee
             src = l->scratch;
         }
         double* out = (double*)host;
+        const size_t PS = l->planeStride();
         if (l->prec == Prec::F64) {
-            std::vector<double> f(19 * N);
-            HIPCHK(hipMemcpy(f.data(), src, 19 * N * 8, hipMemcpyDeviceToHost));
+            std::vector<double> f(19 * PS);
+            HIPCHK(hipMemcpy(f.data(), src, 19 * PS * 8, hipMemcpyDeviceToHost));
             for (size_t i = 0; i < N; ++i)
-                for (int j = 0; j < 19; ++j) out[i * 19 + j] = (lk[i] & LINK_ACTIVE) ? f[(size_t)j * N + i] : NAN;
+                for (int j = 0; j < 19; ++j) out[i * 19 + j] = (lk[i] & LINK_ACTIVE) ? f[(size_t)j * PS + i] : NAN;
         } else {
-            std::vector<float> f(19 * N);
-            HIPCHK(hipMemcpy(f.data(), src, 19 * N * 4, hipMemcpyDeviceToHost));
+            std::vector<float> f(19 * PS);
+            HIPCHK(hipMemcpy(f.data(), src, 19 * PS * 4, hipMemcpyDeviceToHost));
             for (size_t i = 0; i < N; ++i)
                 for (int j = 0; j < 19; ++j)
-                    out[i * 19 + j] = (lk[i] & LINK_ACTIVE) ? (double)f[(size_t)j * N + i] + wgt(j) : NAN;
+                    out[i * 19 + j] = (lk[i] & LINK_ACTIVE) ? (double)f[(size_t)j * PS + i] + wgt(j) : NAN;
         }
         return HYB_OK;
     }

@@ -7,15 +7,17 @@
 //     near-wall list — reproducing the reference's cell index (vector.cpp:147-149)
 //     and unsigned stencil wrap, in the reference's pair order. It also builds a
 //     per-particle list of the pairs each particle takes part in.
-//  2. k_dem_predict: Gear predictor (elmt.cpp:139-177), one thread per element.
-//  3. k_dem_forces_correct: per element, the contact forces of its pairs in
-//     pair-list order (DEM.cpp:1668-1717, 1801-1982, 2138-2222) and its wall
-//     contact, Newton-Euler (DEM.cpp:1149-1181) and the Gear corrector
-//     (elmt.cpp:179-254). Each element only accumulates its own forces, in the
-//     same order as the reference's sequential loop, so there are no atomics.
-// The predicted state (xp*, qp*, wp*) is written only by the predictor; the
-// corrector leaves it in place (its copy x -> xp is dead until the next
-// predictor), so other threads can read a partner's predicted state safely.
+//  2. k_dem_pairs: one thread per pair evaluates the contact (DEM.cpp:1801-1900,
+//     2138-2222) and writes each element's signed increments into that
+//     element's slot of the pair.
+//  3. k_dem_gather: per element, the increments in pair-list order (the same
+//     order as the reference's sequential loop, so no atomics and bit-identical
+//     sums), the wall contact, Newton-Euler (DEM.cpp:1149-1181), the Gear
+//     corrector (elmt.cpp:179-254) and the predictor (elmt.cpp:139-177) of the
+//     next sub-step. An initial predictor runs once after initialisation.
+// The corrector keeps the predicted copies it has just used until the new
+// prediction overwrites them, so partner reads in k_dem_pairs always see the
+// predicted state of the current sub-step.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -174,17 +176,26 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
     int G = 0;
     if (p.npbc) {
         // ---- pbcShift (DEM.cpp:1554-1584)
+        // The predictor of this sub-step already ran (fused into the previous
+        // corrector), so a shifted element gets its predicted position
+        // recomputed from the shifted x0 exactly as the predictor would.
         for (int e = t; e < P; e += RB) {
-            V3 x0 = E.v3(EF_X, e), xp0 = E.v3(EF_XP, e);
+            V3 x0 = E.v3(EF_X, e);
+            bool moved = false;
             for (int k = 0; k < p.npbc; ++k) {
                 const PbcDev& pb = p.pbc[k];
                 const double left = pb.n1.dot(x0 - pb.p1);
                 const double right = pb.n2.dot(x0 - pb.p2);
-                if (left < 0.0) { x0 += pb.v; xp0 += pb.v; }
-                if (right < 0.0) { const V3 m = -1.0 * pb.v; x0 += m; xp0 += m; }
+                if (left < 0.0) { x0 += pb.v; moved = true; }
+                if (right < 0.0) { const V3 m = -1.0 * pb.v; x0 += m; moved = true; }
             }
-            E.set3(EF_X, e, x0);
-            E.set3(EF_XP, e, xp0);
+            if (moved) {
+                E.set3(EF_X, e, x0);
+                const double* c2 = p.c2;
+                const V3 xp0 = x0 + E.v3(EF_X + 3, e) * c2[0] + E.v3(EF_X + 6, e) * c2[1] + E.v3(EF_X + 9, e) * c2[2] +
+                               E.v3(EF_X + 12, e) * c2[3] + E.v3(EF_X + 15, e) * c2[4];
+                E.set3(EF_XP, e, xp0);
+            }
         }
         __syncthreads();
         // ---- side ghosts in (pbc, element) order (DEM.cpp:1590-1618)
@@ -347,6 +358,17 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
             b.csr[j + 1] = v;
         }
     }
+    // ---- slot of each pair in its particles' lists
+    for (int k = t; k < T; k += RB) { b.posI[k] = 0xFFFFFFFFu; b.posJ[k] = 0xFFFFFFFFu; }
+    __syncthreads();
+    for (int e = t; e < P; e += RB) {
+        const int lo = (int)b.csrOff[e], hi = (int)b.csrOff[e + 1];
+        for (int i = lo; i < hi; ++i) {
+            const uint32_t k = b.csr[i];
+            if ((int)b.pairs[2 * k] == e) b.posI[k] = (uint32_t)i;
+            else b.posJ[k] = (uint32_t)i;
+        }
+    }
     // ---- ghost slots of each particle, ascending (for the hydrodynamic sums)
     for (int e = t; e < P; e += RB) b.ghostOfCount[e] = 0;
     __syncthreads();
@@ -374,48 +396,7 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
     if (t == 0) b.st->nNearWall = (int)red[0];
 }
 
-// ------------------------------------------------------------------ predictor
-__global__ void k_dem_predict(DEMParams p, DEMBuffers b) {
-#pragma clang fp contract(off)
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= p.P) return;
-    const El E{b.el, p.P};
-    const double* c1 = p.c1;
-    const double* c2 = p.c2;
-    V3 x[6];
-    Q4 q[6];
-    V3 w[6];
-    for (int k = 0; k < 6; ++k) { x[k] = E.v3(EF_X + 3 * k, e); q[k] = E.q4(EF_Q + 4 * k, e); w[k] = E.v3(EF_W + 3 * k, e); }
-    const V3 xp0 = x[0] + x[1] * c2[0] + x[2] * c2[1] + x[3] * c2[2] + x[4] * c2[3] + x[5] * c2[4];
-    const V3 xp1 = x[1] + x[2] * c2[0] + x[3] * c2[1] + x[4] * c2[2] + x[5] * c2[3];
-    const V3 xp2 = x[2] + x[3] * c2[0] + x[4] * c2[1] + x[5] * c2[2];
-    const V3 xp3 = x[3] + x[4] * c2[0] + x[5] * c2[1];
-    const V3 xp4 = x[4] + x[5] * c2[0];
-    const V3 xp5 = x[5];
-    Q4 qp0 = q[0] + q[1] * c2[0] + q[2] * c2[1] + q[3] * c2[2] + q[4] * c2[3] + q[5] * c2[4];
-    const Q4 qp1 = q[1] + q[2] * c2[0] + q[3] * c2[1] + q[4] * c2[2] + q[5] * c2[3];
-    const Q4 qp2 = q[2] + q[3] * c2[0] + q[4] * c2[1] + q[5] * c2[2];
-    const Q4 qp3 = q[3] + q[4] * c2[0] + q[5] * c2[1];
-    const Q4 qp4 = q[4] + q[5] * c2[0];
-    const Q4 qp5 = q[5];
-    qp0.normalize();
-    const V3 wp0 = w[0] + w[1] * c1[0] + w[2] * c1[1] + w[3] * c1[2] + w[4] * c1[3] + w[5] * c1[4];
-    const V3 wp1 = w[1] + w[2] * c1[0] + w[3] * c1[1] + w[4] * c1[2] + w[5] * c1[3];
-    const V3 wp2 = w[2] + w[3] * c1[0] + w[4] * c1[1] + w[5] * c1[2];
-    const V3 wp3 = w[3] + w[4] * c1[0] + w[5] * c1[1];
-    const V3 wp4 = w[4] + w[5] * c1[0];
-    const V3 wp5 = w[5];
-    E.set3(EF_XP, e, xp0); E.set3(EF_XP + 3, e, xp1); E.set3(EF_XP + 6, e, xp2);
-    E.set3(EF_XP + 9, e, xp3); E.set3(EF_XP + 12, e, xp4); E.set3(EF_XP + 15, e, xp5);
-    E.set4(EF_QP, e, qp0); E.set4(EF_QP + 4, e, qp1); E.set4(EF_QP + 8, e, qp2);
-    E.set4(EF_QP + 12, e, qp3); E.set4(EF_QP + 16, e, qp4); E.set4(EF_QP + 20, e, qp5);
-    E.set3(EF_WP, e, wp0); E.set3(EF_WP + 3, e, wp1); E.set3(EF_WP + 6, e, wp2);
-    E.set3(EF_WP + 9, e, wp3); E.set3(EF_WP + 12, e, wp4); E.set3(EF_WP + 15, e, wp5);
-    E.set3(EF_WPG, e, wp0);
-    E.set3(EF_WPL, e, project(wp0, qp0.adjoint()));
-}
-
-// ------------------------------------------------------------------ contacts + Newton-Euler + corrector
+// ------------------------------------------------------------------ contacts, Newton-Euler, Gear
 __device__ __forceinline__ double normal_contact(const DEMParams& p, double overlap, double vrel, double effRad,
                                                  double effMass) {  // DEM.cpp:2138-2174
     if (p.hertzian) {
@@ -460,18 +441,18 @@ __device__ __forceinline__ void slot_pred(const El& E, const DEMBuffers& b, int 
     }
 }
 
-__global__ void k_dem_forces_correct(DEMParams p, DEMBuffers b) {
+// One thread per neighbour pair (grid-stride): the contact of DEM.cpp:1801-1900
+// evaluated at the predicted state. The signed increments for each of the two
+// elements are written into that element's slot of the pair in its csr list, so
+// that k_dem_gather adds them in pair-list order.
+__global__ void k_dem_pairs(DEMParams p, DEMBuffers b) {
 #pragma clang fp contract(off)
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= p.P) return;
     const El E{b.el, p.P};
     const int P = p.P;
-    V3 FP(0.0, 0.0, 0.0), MP(0.0, 0.0, 0.0), FW(0.0, 0.0, 0.0), MW(0.0, 0.0, 0.0);
-    // ---- particle-particle contacts, in pair-list order (DEM.cpp:1668-1690, 1801-1900)
-    const uint32_t lo = b.csrOff[e], hi = b.csrOff[e + 1];
-    for (uint32_t k = lo; k < hi; ++k) {
-        const uint32_t pr = b.csr[k];
-        const int n0 = (int)b.pairs[2 * pr], n1 = (int)b.pairs[2 * pr + 1];
+    const int T = b.st->nPairs;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < T; k += gridDim.x * blockDim.x) {
+        const uint32_t si = b.posI[k], sj = b.posJ[k];
+        const int n0 = (int)b.pairs[2 * k], n1 = (int)b.pairs[2 * k + 1];
         V3 xI, vI, xJ, vJ;
         int eI, eJ;
         slot_pred(E, b, P, n0, xI, vI, eI);
@@ -480,44 +461,117 @@ __global__ void k_dem_forces_correct(DEMParams p, DEMBuffers b) {
         const double sigij = rI + rJ;
         const double sigij2 = sigij * sigij;
         const V3 vd = xJ - xI;
-        if (!(vd.norm2() < sigij2)) continue;
-        const double distance = vd.norm();
-        const double overlap = rI + rJ - distance;
-        const V3 relVel = vJ - vI;
-        const V3 en = vd / distance;
-        const double normRelVel = relVel.dot(en);
-        const V3 normalRelVel = en * normRelVel;
-        const double mI = E.at(EF_M, eI), mJ = E.at(EF_M, eJ);
-        const double effMass = mI * mJ / (mI + mJ);
-        const double effRad = rI * rJ / (rI + rJ);
-        const double fn = normal_contact(p, overlap, normRelVel, effRad, effMass);
-        const V3 normalForce = en * fn;
-        const V3 vecRadI = rI * en;
-        const V3 vecRadJ = -rJ * en;
-        // radiusVec is zero for single-sphere elements
-        const V3 centerDistI = vecRadI + V3(0.0, 0.0, 0.0);
-        const V3 centerDistJ = vecRadJ + V3(0.0, 0.0, 0.0);
-        const bool isI = (n0 == e);
-        if (isI) FP = FP - normalForce;
-        else FP = FP + normalForce;
-        const V3 wI = E.v3(EF_WPG, eI), wJ = E.v3(EF_WPG, eJ);
-        const V3 relVelContact = relVel - wI.cross(vecRadI) + wJ.cross(vecRadJ);
-        const V3 tangRelVelContact = relVelContact - normalRelVel;
-        const double nt = tangRelVelContact.norm();
-        if (nt != 0.0) {
-            const double ft = tangential_contact(p, nt, fn, effRad, effMass, p.fricPart);
-            const V3 et = tangRelVelContact / nt;
-            const V3 tangForce = ft * et;
-            if (isI) {
-                MP = MP + centerDistI.cross(tangForce);
-                FP = FP + tangForce;
-            } else {
-                MP = MP - centerDistJ.cross(tangForce);
-                FP = FP - tangForce;
+        PairRec ri, rj;
+        ri.flag = rj.flag = 0;
+        ri.pad = rj.pad = 0;
+        if (vd.norm2() < sigij2) {
+            const double distance = vd.norm();
+            const double overlap = rI + rJ - distance;
+            const V3 relVel = vJ - vI;
+            const V3 en = vd / distance;
+            const double normRelVel = relVel.dot(en);
+            const V3 normalRelVel = en * normRelVel;
+            const double mI = E.at(EF_M, eI), mJ = E.at(EF_M, eJ);
+            const double effMass = mI * mJ / (mI + mJ);
+            const double effRad = rI * rJ / (rI + rJ);
+            const double fn = normal_contact(p, overlap, normRelVel, effRad, effMass);
+            const V3 normalForce = en * fn;
+            const V3 vecRadI = rI * en;
+            const V3 vecRadJ = -rJ * en;
+            // radiusVec is zero for single-sphere elements
+            const V3 centerDistI = vecRadI + V3(0.0, 0.0, 0.0);
+            const V3 centerDistJ = vecRadJ + V3(0.0, 0.0, 0.0);
+            const V3 mn = -1.0 * normalForce;
+            ri.fn[0] = mn.x; ri.fn[1] = mn.y; ri.fn[2] = mn.z;  // F - n == F + (-n)
+            rj.fn[0] = normalForce.x; rj.fn[1] = normalForce.y; rj.fn[2] = normalForce.z;
+            ri.flag = rj.flag = 1;
+            const V3 wI = E.v3(EF_WPG, eI), wJ = E.v3(EF_WPG, eJ);
+            const V3 relVelContact = relVel - wI.cross(vecRadI) + wJ.cross(vecRadJ);
+            const V3 tangRelVelContact = relVelContact - normalRelVel;
+            const double nt = tangRelVelContact.norm();
+            if (nt != 0.0) {
+                const double ft = tangential_contact(p, nt, fn, effRad, effMass, p.fricPart);
+                const V3 et = tangRelVelContact / nt;
+                const V3 tangForce = ft * et;
+                const V3 mI3 = centerDistI.cross(tangForce);
+                const V3 mJ3 = -1.0 * centerDistJ.cross(tangForce);
+                const V3 mt = -1.0 * tangForce;
+                ri.ft[0] = tangForce.x; ri.ft[1] = tangForce.y; ri.ft[2] = tangForce.z;
+                ri.m[0] = mI3.x; ri.m[1] = mI3.y; ri.m[2] = mI3.z;
+                rj.ft[0] = mt.x; rj.ft[1] = mt.y; rj.ft[2] = mt.z;
+                rj.m[0] = mJ3.x; rj.m[1] = mJ3.y; rj.m[2] = mJ3.z;
+                ri.flag = rj.flag = 3;
             }
         }
+        if (si != 0xFFFFFFFFu) b.prec[si] = ri;
+        if (sj != 0xFFFFFFFFu) b.prec[sj] = rj;
     }
-    // ---- wall contact (DEM.cpp:1692-1717, 1902-1982)
+}
+
+// Gear predictor of one element from its corrected state (elmt.cpp:139-177).
+__device__ __forceinline__ void predict_elem(const El& E, const DEMParams& p, int e, const V3 x[6], const Q4 q[6],
+                                             const V3 w[6]) {
+    const double* c1 = p.c1;
+    const double* c2 = p.c2;
+    const V3 xp0 = x[0] + x[1] * c2[0] + x[2] * c2[1] + x[3] * c2[2] + x[4] * c2[3] + x[5] * c2[4];
+    const V3 xp1 = x[1] + x[2] * c2[0] + x[3] * c2[1] + x[4] * c2[2] + x[5] * c2[3];
+    const V3 xp2 = x[2] + x[3] * c2[0] + x[4] * c2[1] + x[5] * c2[2];
+    const V3 xp3 = x[3] + x[4] * c2[0] + x[5] * c2[1];
+    const V3 xp4 = x[4] + x[5] * c2[0];
+    const V3 xp5 = x[5];
+    Q4 qp0 = q[0] + q[1] * c2[0] + q[2] * c2[1] + q[3] * c2[2] + q[4] * c2[3] + q[5] * c2[4];
+    const Q4 qp1 = q[1] + q[2] * c2[0] + q[3] * c2[1] + q[4] * c2[2] + q[5] * c2[3];
+    const Q4 qp2 = q[2] + q[3] * c2[0] + q[4] * c2[1] + q[5] * c2[2];
+    const Q4 qp3 = q[3] + q[4] * c2[0] + q[5] * c2[1];
+    const Q4 qp4 = q[4] + q[5] * c2[0];
+    const Q4 qp5 = q[5];
+    qp0.normalize();
+    const V3 wp0 = w[0] + w[1] * c1[0] + w[2] * c1[1] + w[3] * c1[2] + w[4] * c1[3] + w[5] * c1[4];
+    const V3 wp1 = w[1] + w[2] * c1[0] + w[3] * c1[1] + w[4] * c1[2] + w[5] * c1[3];
+    const V3 wp2 = w[2] + w[3] * c1[0] + w[4] * c1[1] + w[5] * c1[2];
+    const V3 wp3 = w[3] + w[4] * c1[0] + w[5] * c1[1];
+    const V3 wp4 = w[4] + w[5] * c1[0];
+    const V3 wp5 = w[5];
+    E.set3(EF_XP, e, xp0); E.set3(EF_XP + 3, e, xp1); E.set3(EF_XP + 6, e, xp2);
+    E.set3(EF_XP + 9, e, xp3); E.set3(EF_XP + 12, e, xp4); E.set3(EF_XP + 15, e, xp5);
+    E.set4(EF_QP, e, qp0); E.set4(EF_QP + 4, e, qp1); E.set4(EF_QP + 8, e, qp2);
+    E.set4(EF_QP + 12, e, qp3); E.set4(EF_QP + 16, e, qp4); E.set4(EF_QP + 20, e, qp5);
+    E.set3(EF_WP, e, wp0); E.set3(EF_WP + 3, e, wp1); E.set3(EF_WP + 6, e, wp2);
+    E.set3(EF_WP + 9, e, wp3); E.set3(EF_WP + 12, e, wp4); E.set3(EF_WP + 15, e, wp5);
+    E.set3(EF_WPG, e, wp0);
+    E.set3(EF_WPL, e, project(wp0, qp0.adjoint()));
+}
+
+__global__ void k_dem_predict(DEMParams p, DEMBuffers b) {
+#pragma clang fp contract(off)
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p.P) return;
+    const El E{b.el, p.P};
+    V3 x[6], w[6];
+    Q4 q[6];
+    for (int k = 0; k < 6; ++k) { x[k] = E.v3(EF_X + 3 * k, e); q[k] = E.q4(EF_Q + 4 * k, e); w[k] = E.v3(EF_W + 3 * k, e); }
+    predict_elem(E, p, e, x, q, w);
+}
+
+// Per element: particle-contact increments in pair-list order, the wall contact
+// (DEM.cpp:1692-1717, 1902-1982), Newton-Euler (DEM.cpp:1149-1181), the Gear
+// corrector (elmt.cpp:179-254) and the predictor of the next sub-step.
+__global__ void k_dem_gather(DEMParams p, DEMBuffers b) {
+#pragma clang fp contract(off)
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p.P) return;
+    const El E{b.el, p.P};
+    V3 FP(0.0, 0.0, 0.0), MP(0.0, 0.0, 0.0), FW(0.0, 0.0, 0.0), MW(0.0, 0.0, 0.0);
+    const uint32_t lo = b.csrOff[e], hi = b.csrOff[e + 1];
+    for (uint32_t i = lo; i < hi; ++i) {
+        const PairRec& r = b.prec[i];
+        const int flag = r.flag;
+        if (flag & 1) FP = FP + V3(r.fn[0], r.fn[1], r.fn[2]);
+        if (flag & 2) {
+            MP = MP + V3(r.m[0], r.m[1], r.m[2]);
+            FP = FP + V3(r.ft[0], r.ft[1], r.ft[2]);
+        }
+    }
     const uint32_t wi = b.nearWall[e];
     const V3 xpe = E.v3(EF_XP, e);
     const V3 vpe = E.v3(EF_XP + 3, e);
@@ -550,7 +604,7 @@ __global__ void k_dem_forces_correct(DEMParams p, DEMBuffers b) {
             }
         }
     }
-    // ---- Newton-Euler (DEM.cpp:1149-1181)
+    // Newton-Euler
     const V3 xp1 = vpe;
     const double rad = re;
     const V3 FVisc = -6.0 * M_PI * p.numVisc * rad * xp1;
@@ -564,46 +618,56 @@ __global__ void k_dem_forces_correct(DEMParams p, DEMBuffers b) {
     const V3 waBf = newtonAcc(momentBf, E.v3(EF_I, e), E.v3(EF_WPL, e));
     const V3 w1 = project(waBf, qp0);
     E.set3(EF_FP, e, FP); E.set3(EF_MP, e, MP); E.set3(EF_FW, e, FW); E.set3(EF_MW, e, MW);
-    // ---- corrector (elmt.cpp:179-254); the predicted copies stay untouched
+    // corrector
     const double* co1 = p.co1;
     const double* co2 = p.co2;
     const V3 xp[6] = {xpe, xp1, E.v3(EF_XP + 6, e), E.v3(EF_XP + 9, e), E.v3(EF_XP + 12, e), E.v3(EF_XP + 15, e)};
     const V3 x2Corr = x2 - xp[2];
-    E.set3(EF_X, e, xp[0] + x2Corr * co2[0]);
-    E.set3(EF_X + 3, e, xp[1] + x2Corr * co2[1]);
-    E.set3(EF_X + 6, e, x2);
-    E.set3(EF_X + 9, e, xp[3] + x2Corr * co2[3]);
-    E.set3(EF_X + 12, e, xp[4] + x2Corr * co2[4]);
-    E.set3(EF_X + 15, e, xp[5] + x2Corr * co2[5]);
+    V3 x[6];
+    x[0] = xp[0] + x2Corr * co2[0];
+    x[1] = xp[1] + x2Corr * co2[1];
+    x[2] = x2;
+    x[3] = xp[3] + x2Corr * co2[3];
+    x[4] = xp[4] + x2Corr * co2[4];
+    x[5] = xp[5] + x2Corr * co2[5];
     const V3 wp[6] = {E.v3(EF_WP, e), E.v3(EF_WP + 3, e), E.v3(EF_WP + 6, e), E.v3(EF_WP + 9, e), E.v3(EF_WP + 12, e),
                       E.v3(EF_WP + 15, e)};
     const V3 w1Corr = w1 - wp[1];
-    const V3 w0 = wp[0] + w1Corr * co1[0];
-    E.set3(EF_W, e, w0);
-    E.set3(EF_W + 3, e, w1);
-    E.set3(EF_W + 6, e, wp[2] + w1Corr * co1[2]);
-    E.set3(EF_W + 9, e, wp[3] + w1Corr * co1[3]);
-    E.set3(EF_W + 12, e, wp[4] + w1Corr * co1[4]);
-    E.set3(EF_W + 15, e, wp[5] + w1Corr * co1[5]);
+    V3 w[6];
+    w[0] = wp[0] + w1Corr * co1[0];
+    w[1] = w1;
+    w[2] = wp[2] + w1Corr * co1[2];
+    w[3] = wp[3] + w1Corr * co1[3];
+    w[4] = wp[4] + w1Corr * co1[4];
+    w[5] = wp[5] + w1Corr * co1[5];
     const Q4 q2 = E.q4(EF_Q + 8, e);  // single spheres: q2 is never set by the force routine
     const Q4 qp[6] = {qp0, E.q4(EF_QP + 4, e), E.q4(EF_QP + 8, e), E.q4(EF_QP + 12, e), E.q4(EF_QP + 16, e),
                       E.q4(EF_QP + 20, e)};
     const Q4 q2Corr = q2 - qp[2];
-    Q4 q0 = qp[0] + q2Corr * co2[0];
-    E.set4(EF_Q + 4, e, qp[1] + q2Corr * co2[1]);
-    E.set4(EF_Q + 12, e, qp[3] + q2Corr * co2[3]);
-    E.set4(EF_Q + 16, e, qp[4] + q2Corr * co2[4]);
-    E.set4(EF_Q + 20, e, qp[5] + q2Corr * co2[5]);
-    q0.normalize();
-    E.set4(EF_Q, e, q0);
-    E.set3(EF_WG, e, w0);
-    E.set3(EF_WL, e, project(w0, q0.adjoint()));
+    Q4 q[6];
+    q[0] = qp[0] + q2Corr * co2[0];
+    q[1] = qp[1] + q2Corr * co2[1];
+    q[2] = q2;
+    q[3] = qp[3] + q2Corr * co2[3];
+    q[4] = qp[4] + q2Corr * co2[4];
+    q[5] = qp[5] + q2Corr * co2[5];
+    q[0].normalize();
+    for (int k = 0; k < 6; ++k) {
+        E.set3(EF_X + 3 * k, e, x[k]);
+        E.set3(EF_W + 3 * k, e, w[k]);
+        if (k != 2) E.set4(EF_Q + 4 * k, e, q[k]);
+    }
+    E.set3(EF_WG, e, w[0]);
+    E.set3(EF_WL, e, project(w[0], q[0].adjoint()));
+    // predictor of the next sub-step
+    predict_elem(E, p, e, x, q, w);
 }
 
 // Coupling records of every particle slot after the DEM step: corrected
 // position (ghosts: origin + period vector, elmt.cpp:292-298), element velocity
 // and angular velocity, radius.
-__global__ void k_dem_couple_table(DEMParams p, DEMBuffers b, CoupleRec* __restrict__ cp, int* __restrict__ nslots) {
+__global__ void k_dem_couple_table(DEMParams p, DEMBuffers b, CoupleRec* __restrict__ cp, CoupleRecF* __restrict__ cpf,
+                                   int* __restrict__ nslots, Units u) {
 #pragma clang fp contract(off)
     const El E{b.el, p.P};
     const int S = b.st->nSlots;
@@ -627,6 +691,14 @@ __global__ void k_dem_couple_table(DEMParams p, DEMBuffers b, CoupleRec* __restr
     r.elem = o;
     r.pad = 0;
     cp[s] = r;
+    CoupleRecF q;
+    const V3 cc = x0 / u.Length;  // radiusVec is zero for single spheres
+    const V3 vs = v / u.Speed, ws = w / u.AngVel;
+    q.c[0] = (float)cc.x; q.c[1] = (float)cc.y; q.c[2] = (float)cc.z;
+    q.v[0] = (float)vs.x; q.v[1] = (float)vs.y; q.v[2] = (float)vs.z;
+    q.w[0] = (float)ws.x; q.w[1] = (float)ws.y; q.w[2] = (float)ws.z;
+    q.pad[0] = q.pad[1] = q.pad[2] = 0.0f;
+    cpf[s] = q;
 }
 
 // Element hydrodynamic force and torque: own slot first, then its ghosts in
@@ -666,14 +738,22 @@ void launch_dem_predict(const DEMParams& p, const DEMBuffers& b, hipStream_t s) 
     k_dem_predict<<<(p.P + 255) / 256, 256, 0, s>>>(p, b);
 }
 
-void launch_dem_forces_correct(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {
+void launch_dem_pairs(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {
     if (p.P <= 0) return;
-    k_dem_forces_correct<<<(p.P + 255) / 256, 256, 0, s>>>(p, b);
+    int blocks = (p.pairCap + 255) / 256;
+    if (blocks > 512) blocks = 512;
+    k_dem_pairs<<<blocks, 256, 0, s>>>(p, b);
 }
 
-void launch_dem_couple_table(const DEMParams& p, const DEMBuffers& b, CoupleRec* cp, int* nslots, hipStream_t s) {
+void launch_dem_gather(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {
+    if (p.P <= 0) return;
+    k_dem_gather<<<(p.P + 127) / 128, 128, 0, s>>>(p, b);
+}
+
+void launch_dem_couple_table(const DEMParams& p, const DEMBuffers& b, CoupleRec* cp, CoupleRecF* cpf, int* nslots,
+                             const Units& u, hipStream_t s) {
     const int cap = p.P + p.Gcap;
-    k_dem_couple_table<<<(cap + 255) / 256 > 0 ? (cap + 255) / 256 : 1, 256, 0, s>>>(p, b, cp, nslots);
+    k_dem_couple_table<<<(cap + 255) / 256 > 0 ? (cap + 255) / 256 : 1, 256, 0, s>>>(p, b, cp, cpf, nslots, u);
 }
 
 void launch_dem_hydro_combine(const DEMParams& p, const DEMBuffers& b, const HydroPartial* part, const Units& u,

@@ -76,15 +76,30 @@ struct DEMBuffers {
     int* ghostOf;          // [P][8] ghost slots of each particle, ascending
     int* side;             // [P][3] side-ghost index of each particle per periodic pair, -1 if none
     int* ghostB;           // [Gcap] periodic pair that created a side ghost
+    uint32_t* posI;        // [pairCap] csr slot of a pair in its first particle's list (0xFFFFFFFF: ghost)
+    uint32_t* posJ;        // [pairCap] csr slot in its second particle's list
+    struct PairRec* prec;  // [2 * pairCap] per csr slot: signed force/torque increments of one pair
     int scap;
     int ncells;
     int W;                 // ncells * 14
 };
 
+// Increments one contact adds to one element (size-1 elements): the normal force
+// (applied when flag & 1), then torque and tangential force (when flag & 2).
+struct PairRec {
+    double fn[3];
+    double ft[3];
+    double m[3];
+    int flag;
+    int pad;
+};
+
 void launch_dem_rebuild(const DEMParams& p, const DEMBuffers& b, int force, hipStream_t s);
+void launch_dem_pairs(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
+void launch_dem_gather(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
 void launch_dem_predict(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
-void launch_dem_forces_correct(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
-void launch_dem_couple_table(const DEMParams& p, const DEMBuffers& b, CoupleRec* cp, int* nslots, hipStream_t s);
+void launch_dem_couple_table(const DEMParams& p, const DEMBuffers& b, CoupleRec* cp, CoupleRecF* cpf, int* nslots,
+                             const Units& u, hipStream_t s);
 void launch_dem_hydro_combine(const DEMParams& p, const DEMBuffers& b, const HydroPartial* part, const Units& u,
                               hipStream_t s);
 

@@ -22,8 +22,49 @@ namespace hyb {
 
 namespace {
 
-constexpr int BX = 64;
-constexpr int BY = 4;
+// Workgroup tile of the lattice kernels: a full 256-node x-row per workgroup
+// keeps the +-1 x-shifted loads of a row inside one workgroup's cache.
+#ifndef HYB_LB_BX
+#define HYB_LB_BX 256
+#endif
+#ifndef HYB_LB_BY
+#define HYB_LB_BY 1
+#endif
+#ifndef HYB_LB_NT
+#define HYB_LB_NT 1
+#endif
+constexpr int BX = HYB_LB_BX;
+
+// Particle ownership word of a node: (coupling step tag << 20) | (slot + 1).
+// Entries written by an earlier coupling step carry an older tag and read as
+// "no particle", so the array never needs clearing between steps.
+__device__ __forceinline__ uint32_t owner_slot(uint32_t w, uint32_t tag) {
+    return (w >> SOLID_SLOT_BITS) == tag ? (w & SOLID_SLOT_MASK) : 0u;
+}
+constexpr int BY = HYB_LB_BY;
+
+#ifndef HYB_LB_NTLOAD
+#define HYB_LB_NTLOAD 0
+#endif
+
+// Population stores and loads: optionally non-temporal (every population is
+// written once and read once per step, so neither benefits from caching).
+template <typename T>
+__device__ __forceinline__ void st_pop(T* p, T v) {
+#if HYB_LB_NT
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+template <typename T>
+__device__ __forceinline__ T ld_pop(const T* p) {
+#if HYB_LB_NTLOAD
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
 
 struct Coord {
     int x, y, z;
@@ -53,25 +94,31 @@ __device__ __forceinline__ Nb3 nb3(int x, int X, int per) {
 }
 __device__ __forceinline__ int pick(const Nb3& n, int d) { return d > 0 ? n.pl : (d < 0 ? n.m : n.c); }
 
-// Gather the 19 pre-collision populations of node c into f[]: pull from the
-// neighbour x + c_j for f_opp(j), or bounce back the node's own f_j when that
-// link is a static wall.
+// Gather the 19 pre-collision populations of node c into f[]: f_i is pulled
+// from the neighbour x - c_i (32-bit node index into plane i). Links to static
+// walls bounce back the node's own f_opp(i) instead; they only occur next to
+// walls, so that fix-up runs as a separate, rarely taken branch and the common
+// path issues its 19 loads back to back.
 template <typename T>
-__device__ __forceinline__ void pull19(const T* __restrict__ fin, size_t N, const LBKParams& p, const Coord& c,
+__device__ __forceinline__ void pull19(const T* __restrict__ fin, size_t PS, const LBKParams& p, const Coord& c,
                                        uint32_t lw, T f[19]) {
     const Nb3 nx = nb3(c.x, p.X, p.per[0]);
     const Nb3 ny = nb3(c.y, p.Y, p.per[1]);
     const Nb3 nz = nb3(c.z, p.Z, p.per[2]);
-    f[0] = fin[c.idx];
+    const uint32_t sX = (uint32_t)p.X, sXY = (uint32_t)p.X * (uint32_t)p.Y;
+    const uint32_t self = (uint32_t)c.idx;
 #pragma unroll
-    for (int i = 1; i < 19; ++i) {
+    for (int i = 0; i < 19; ++i) {
         const int j = opp(i);
-        if (lw & (1u << j)) {
-            f[i] = fin[(size_t)j * N + c.idx];
-        } else {
-            const size_t s = (size_t)pick(nx, cx(j)) +
-                             (size_t)p.X * ((size_t)pick(ny, cy(j)) + (size_t)p.Y * (size_t)pick(nz, cz(j)));
-            f[i] = fin[(size_t)i * N + s];
+        const uint32_t nb = (uint32_t)pick(nx, cx(j)) + sX * (uint32_t)pick(ny, cy(j)) + sXY * (uint32_t)pick(nz, cz(j));
+        // a wall neighbour's index is valid memory; its value is replaced below
+        f[i] = ld_pop(&fin[(size_t)i * PS + (i == 0 ? self : nb)]);
+    }
+    if (lw & LINK_WALLS) {
+#pragma unroll
+        for (int i = 1; i < 19; ++i) {
+            const int j = opp(i);
+            if (lw & (1u << j)) f[i] = ld_pop(&fin[(size_t)j * PS + self]);
         }
     }
 }
@@ -171,17 +218,17 @@ __global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ 
     const uint32_t lw = link[c.idx];
     if (!(lw & LINK_ACTIVE)) return;
     double f[19];
-    if (PULL) pull19<double>(fin, N, p, c, lw, f);
+    if (PULL) pull19<double>(fin, p.PS, p, c, lw, f);
     else {
 #pragma unroll
-        for (int j = 0; j < 19; ++j) f[j] = fin[(size_t)j * N + c.idx];
+        for (int j = 0; j < 19; ++j) f[j] = ld_pop(&fin[(size_t)j * p.PS + c.idx]);
     }
     double n;
     V3 u;
     moments64(f, n, u);
     // penalty coupling force
     V3 hF(0.0, 0.0, 0.0);
-    const uint32_t sid = solid ? solid[c.idx] : 0u;
+    const uint32_t sid = solid ? owner_slot(solid[c.idx], p.ctag) : 0u;
     if (sid) {
         const double mass = (lw & LINK_INTERFACE) ? 0.5 * p.initDensity : p.initDensity;
         const V3 d = penalty64(cp[sid - 1], c, mass, n, u, p, hF);
@@ -205,48 +252,60 @@ __global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ 
     }
     if (!fout) return;
     const double usq = u.norm2();
-    double feq[19];
-#pragma unroll
-    for (int j = 0; j < 19; ++j) {
+    // feq_j = w n (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2), node.cpp:90-101, evaluated where used
+    auto feq = [&](int j) {
         const double vu = cdot_rt(j, u);
-        feq[j] = wgt(j) * n * (1.0 + 3.0 * vu + 4.5 * vu * vu - 1.5 * usq);
-    }
+        return wgt(j) * n * (1.0 + 3.0 * vu + 4.5 * vu * vu - 1.5 * usq);
+    };
     double nu = p.visc0;
     if (VARVISC) {
-        nu = shear_visc64(f, feq, n, visc[c.idx], p);
+        double fe[19];
+#pragma unroll
+        for (int j = 0; j < 19; ++j) fe[j] = feq(j);
+        nu = shear_visc64(f, fe, n, visc[c.idx], p);
         visc[c.idx] = nu;
     }
     if (COLL == HYB_COLLISION_TRT) {
         const double omP = 1.0 / (0.5 + 3.0 * nu);
         const double omM = 1.0 / (0.5 + p.trtMagic / (1.0 / omP - 0.5));
         const double uT = u.dot(T);
-        double out[19];
 #pragma unroll
-        for (int j = 0; j < 19; ++j) {
+        for (int j = 0; j < 19; j += (j == 0 ? 1 : 2)) {
             const int o = opp(j);
-            const double fp = 0.5 * (f[j] + f[o]), fm = 0.5 * (f[j] - f[o]);
-            const double ep = 0.5 * (feq[j] + feq[o]), em = 0.5 * (feq[j] - feq[o]);
+            const double fj = f[j], fo = f[o], ej = feq(j), eo = (o == j) ? ej : feq(o);
+            const double fp = 0.5 * (fj + fo), fm = 0.5 * (fj - fo);
+            const double ep = 0.5 * (ej + eo), em = 0.5 * (ej - eo);
             const double vu = cdot_rt(j, u);
             const double cF = cdot_rt(j, T);
             const double sp = wgt(j) * (9.0 * vu * cF - 3.0 * uT);
             const double sm = wgt(j) * (3.0 * cF);
-            out[j] = f[j] - omP * (fp - ep) - omM * (fm - em) + (1.0 - 0.5 * omP) * sp + (1.0 - 0.5 * omM) * sm;
+            st_pop(&fout[(size_t)j * p.PS + c.idx],
+                   fj - omP * (fp - ep) - omM * (fm - em) + (1.0 - 0.5 * omP) * sp + (1.0 - 0.5 * omM) * sm);
+            if (o != j) {
+                // the same expression evaluated for the opposite direction
+                const double fp2 = 0.5 * (fo + fj), fm2 = 0.5 * (fo - fj);
+                const double ep2 = 0.5 * (eo + ej), em2 = 0.5 * (eo - ej);
+                const double vu2 = cdot_rt(o, u);
+                const double cF2 = cdot_rt(o, T);
+                const double sp2 = wgt(o) * (9.0 * vu2 * cF2 - 3.0 * uT);
+                const double sm2 = wgt(o) * (3.0 * cF2);
+                st_pop(&fout[(size_t)o * p.PS + c.idx],
+                       fo - omP * (fp2 - ep2) - omM * (fm2 - em2) + (1.0 - 0.5 * omP) * sp2 + (1.0 - 0.5 * omM) * sm2);
+            }
         }
-#pragma unroll
-        for (int j = 0; j < 19; ++j) fout[(size_t)j * N + c.idx] = out[j];
     } else {
         const double omega = 1.0 / (0.5 + 3.0 * nu * 1.0);
         const double omegaf = 1.0 - 1.0 / (1.0 + 6.0 * nu * 1.0);
 #pragma unroll
         for (int j = 0; j < 19; ++j) {
-            double fj = f[j] + omega * (feq[j] - f[j]);
+            double fj = f[j] + omega * (feq(j) - f[j]);
             // Guo forcing: forcePart = 9 (c.u) c + 3 (c - u), dotted with the total force
             const double vu = cdot_rt(j, u);
             const double a = 9.0 * vu;
             const V3 fp((double)cx(j) * a + ((double)cx(j) - u.x) * 3.0, (double)cy(j) * a + ((double)cy(j) - u.y) * 3.0,
                         (double)cz(j) * a + ((double)cz(j) - u.z) * 3.0);
             fj += 1.0 * omegaf * wgt(j) * fp.dot(T);
-            fout[(size_t)j * N + c.idx] = fj;
+            st_pop(&fout[(size_t)j * p.PS + c.idx], fj);
         }
     }
 }
@@ -266,7 +325,7 @@ template <bool PULL, bool VARVISC, int COLL>
 __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ fin, float* __restrict__ fout,
                                                      const uint32_t* __restrict__ link,
                                                      const uint32_t* __restrict__ solid,
-                                                     const CoupleRec* __restrict__ cp, float* __restrict__ hf,
+                                                     const CoupleRecF* __restrict__ cp, float* __restrict__ hf,
                                                      double* __restrict__ visc, double* __restrict__ mac,
                                                      LBKParams p) {
 #pragma clang fp contract(fast)
@@ -276,10 +335,10 @@ __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ f
     const uint32_t lw = link[c.idx];
     if (!(lw & LINK_ACTIVE)) return;
     float g[19];
-    if (PULL) pull19<float>(fin, N, p, c, lw, g);
+    if (PULL) pull19<float>(fin, p.PS, p, c, lw, g);
     else {
 #pragma unroll
-        for (int j = 0; j < 19; ++j) g[j] = fin[(size_t)j * N + c.idx];
+        for (int j = 0; j < 19; ++j) g[j] = ld_pop(&fin[(size_t)j * p.PS + c.idx]);
     }
     float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
 #pragma unroll
@@ -297,16 +356,17 @@ __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ f
     float ux = jx * irho, uy = jy * irho, uz = jz * irho;
     float Tx = (float)p.F[0], Ty = (float)p.F[1], Tz = (float)p.F[2];
     float hx = 0.0f, hy = 0.0f, hz = 0.0f;
-    const uint32_t sid = solid ? solid[c.idx] : 0u;
+    const uint32_t sid = solid ? owner_slot(solid[c.idx], p.ctag) : 0u;
     if (sid) {
-        const CoupleRec& r = cp[sid - 1];
-        const double mass = (lw & LINK_INTERFACE) ? 0.5 * p.initDensity : p.initDensity;
-        // rigid-body velocity at the node in double, then the difference in float
-        const V3 pos((double)c.x, (double)c.y, (double)c.z);
-        const V3 radius = pos - V3(r.x0[0], r.x0[1], r.x0[2]) / p.unitLength + V3(r.rv[0], r.rv[1], r.rv[2]) / p.unitLength;
-        const V3 lv = V3(r.v[0], r.v[1], r.v[2]) / p.unitSpeed + (V3(r.w[0], r.w[1], r.w[2]).cross(radius)) / p.unitAngVel;
-        const float lf = (float)mass * irho;
-        const float dx = lf * (ux - (float)lv.x), dy = lf * (uy - (float)lv.y), dz = lf * (uz - (float)lv.z);
+        const CoupleRecF& r = cp[sid - 1];
+        const float mass = (lw & LINK_INTERFACE) ? 0.5f * (float)p.initDensity : (float)p.initDensity;
+        // rigid-body velocity at the node: v + w x (node - c)
+        const float rx = (float)c.x - r.c[0], ry = (float)c.y - r.c[1], rz = (float)c.z - r.c[2];
+        const float lvx = r.v[0] + (r.w[1] * rz - r.w[2] * ry);
+        const float lvy = r.v[1] + (r.w[2] * rx - r.w[0] * rz);
+        const float lvz = r.v[2] + (r.w[0] * ry - r.w[1] * rx);
+        const float lf = mass * irho;
+        const float dx = lf * (ux - lvx), dy = lf * (uy - lvy), dz = lf * (uz - lvz);
         hx = -dx; hy = -dy; hz = -dz;
         if (hf) {
             hf[c.idx] = dx;
@@ -329,19 +389,16 @@ __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ f
     }
     if (!fout) return;
     const float usq = ux * ux + uy * uy + uz * uz;
-    float geq[19];
-#pragma unroll
-    for (int j = 0; j < 19; ++j) {
-        const float vu = cdotf(j, ux, uy, uz);
-        geq[j] = (float)wgt(j) * (drho + rho * (3.0f * vu + 4.5f * vu * vu - 1.5f * usq));
-    }
+    const float base = 1.5f * usq;
     float nu = (float)p.visc0;
     if (VARVISC) {
         // Bingham / Smagorinsky on the non-equilibrium stress (node.cpp:103-145)
         float s[6] = {0, 0, 0, 0, 0, 0};  // xx yy zz xy yz zx
 #pragma unroll
         for (int j = 1; j < 19; ++j) {
-            const float d = g[j] - geq[j];
+            const float vu = cdotf(j, ux, uy, uz);
+            const float geq = (float)wgt(j) * (drho + rho * (3.0f * vu + 4.5f * vu * vu - base));
+            const float d = g[j] - geq;
             if (cx(j)) s[0] += d;
             if (cy(j)) s[1] += d;
             if (cz(j)) s[2] += d;
@@ -367,33 +424,31 @@ __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ f
     }
     const float omP = 1.0f / (0.5f + 3.0f * nu);
     const float uT = ux * Tx + uy * Ty + uz * Tz;
-    if (COLL == HYB_COLLISION_TRT) {
-        const float omM = 1.0f / (0.5f + (float)p.trtMagic / (1.0f / omP - 0.5f));
-        const float kp = 1.0f - 0.5f * omP, km = 1.0f - 0.5f * omM;
-        float out[19];
+    const float kp = 1.0f - 0.5f * omP;
+    // rest population: even part only
+    {
+        const float w0 = (float)wgt(0);
+        const float geq0 = w0 * (drho - rho * base);
+        st_pop(&fout[c.idx], g[0] - omP * (g[0] - geq0) + kp * (w0 * (-3.0f * uT)));
+    }
+    const float omM = (COLL == HYB_COLLISION_TRT) ? 1.0f / (0.5f + (float)p.trtMagic / (1.0f / omP - 0.5f)) : omP;
+    const float km = 1.0f - 0.5f * omM;
 #pragma unroll
-        for (int j = 0; j < 19; ++j) {
-            const int o = opp(j);
-            const float fpl = 0.5f * (g[j] + g[o]), fmi = 0.5f * (g[j] - g[o]);
-            const float ep = 0.5f * (geq[j] + geq[o]), em = 0.5f * (geq[j] - geq[o]);
-            const float vu = cdotf(j, ux, uy, uz);
-            const float cF = cdotf(j, Tx, Ty, Tz);
-            const float w = (float)wgt(j);
-            const float sp = w * (9.0f * vu * cF - 3.0f * uT);
-            const float sm = w * (3.0f * cF);
-            out[j] = g[j] - omP * (fpl - ep) - omM * (fmi - em) + kp * sp + km * sm;
-        }
-#pragma unroll
-        for (int j = 0; j < 19; ++j) fout[(size_t)j * N + c.idx] = out[j];
-    } else {
-        const float kp = 1.0f - 0.5f * omP;
-#pragma unroll
-        for (int j = 0; j < 19; ++j) {
-            const float vu = cdotf(j, ux, uy, uz);
-            const float cF = cdotf(j, Tx, Ty, Tz);
-            const float S = (float)wgt(j) * (9.0f * vu * cF + 3.0f * (cF - uT));
-            fout[(size_t)j * N + c.idx] = g[j] + omP * (geq[j] - g[j]) + kp * S;
-        }
+    for (int j = 1; j < 19; j += 2) {
+        // j and opp(j) = j + 1 form a pair; with vu = c_j.u, geq_j,o = w (drho + rho (4.5 vu^2 - 1.5 u^2 +- 3 vu))
+        const int o = j + 1;
+        const float w = (float)wgt(j);
+        const float vu = cdotf(j, ux, uy, uz);
+        const float cF = cdotf(j, Tx, Ty, Tz);
+        const float ep = w * (drho + rho * (4.5f * vu * vu - base));
+        const float em = w * rho * 3.0f * vu;
+        const float fpl = 0.5f * (g[j] + g[o]), fmi = 0.5f * (g[j] - g[o]);
+        const float sp = w * (9.0f * vu * cF - 3.0f * uT);
+        const float sm = w * (3.0f * cF);
+        const float relP = omP * (fpl - ep) - kp * sp;
+        const float relM = omM * (fmi - em) - km * sm;
+        st_pop(&fout[(size_t)j * p.PS + c.idx], g[j] - relP - relM);
+        st_pop(&fout[(size_t)o * p.PS + c.idx], g[o] - relP + relM);
     }
 }
 
@@ -408,9 +463,9 @@ __global__ void __launch_bounds__(BX * BY) k_stream(const T* __restrict__ fin, T
     const uint32_t lw = link[c.idx];
     if (!(lw & LINK_ACTIVE)) return;
     T f[19];
-    pull19<T>(fin, N, p, c, lw, f);
+    pull19<T>(fin, p.PS, p, c, lw, f);
 #pragma unroll
-    for (int j = 0; j < 19; ++j) fout[(size_t)j * N + c.idx] = f[j];
+    for (int j = 0; j < 19; ++j) fout[(size_t)j * p.PS + c.idx] = f[j];
 }
 
 // ------------------------------------------------------------------ initialisation
@@ -538,8 +593,8 @@ __global__ void k_init_pops(T* __restrict__ f, const uint32_t* __restrict__ link
         const V3 cj((double)cx(j), (double)cy(j), (double)cz(j));
         const double vu = u.dot(cj);
         const double feq = wgt(j) * n * (1.0 + 3.0 * vu + 4.5 * vu * vu - 1.5 * usq);
-        if (SHIFT) f[(size_t)j * N + c.idx] = (T)(feq - wgt(j));
-        else f[(size_t)j * N + c.idx] = (T)feq;
+        if (SHIFT) f[(size_t)j * p.PS + c.idx] = (T)(feq - wgt(j));
+        else f[(size_t)j * p.PS + c.idx] = (T)feq;
     }
     if (visc) visc[c.idx] = p.visc0;
 }
@@ -554,24 +609,9 @@ __global__ void k_count_active(const uint32_t* __restrict__ link, unsigned long 
 
 // ------------------------------------------------------------------ particle classification
 // Each particle slot (real particles, then ghosts) marks the active nodes
-// strictly inside it (vector.cpp:153-158). Where spheres overlap the highest
-// slot index wins, as in the full re-test of LB.cpp:475-495.
-__global__ void k_classify_clear(const int* __restrict__ nboxes, const NodeBox* __restrict__ boxes,
-                                 uint32_t* __restrict__ solid, LBKParams p) {
-    const int s = blockIdx.x;
-    if (s >= *nboxes) return;
-    const NodeBox b = boxes[s];
-    const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
-    if (nx <= 0 || ny <= 0 || nz <= 0) return;
-    const int tot = nx * ny * nz;
-    for (int k = threadIdx.x; k < tot; k += blockDim.x) {
-        const int x = b.lo[0] + k % nx;
-        const int y = b.lo[1] + (k / nx) % ny;
-        const int z = b.lo[2] + k / (nx * ny);
-        solid[(size_t)x + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z)] = 0u;
-    }
-}
-
+// strictly inside it (vector.cpp:153-158) with the current coupling-step tag.
+// Where spheres overlap the highest slot index wins, as in the full re-test of
+// LB.cpp:475-495.
 __device__ __forceinline__ NodeBox slot_box(const CoupleRec& r, const LBKParams& p) {
     const double c[3] = {r.x0[0] / p.unitLength, r.x0[1] / p.unitLength, r.x0[2] / p.unitLength};
     const double rad = r.r / p.unitLength;
@@ -592,25 +632,26 @@ __global__ void k_classify_set(const int* __restrict__ nslots_p, const CoupleRec
                                NodeBox* __restrict__ boxes, int* __restrict__ nboxes, uint32_t* __restrict__ solid,
                                const uint32_t* __restrict__ link, LBKParams p) {
 #pragma clang fp contract(off)
-    const int s = blockIdx.x;
-    if (s == 0 && threadIdx.x == 0) *nboxes = *nslots_p;
-    if (s >= *nslots_p) return;
-    const CoupleRec r = cp[s];
-    const NodeBox b = slot_box(r, p);
-    if (threadIdx.x == 0) boxes[s] = b;
-    const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
-    if (nx <= 0 || ny <= 0 || nz <= 0) return;
-    const V3 cen(r.x0[0] / p.unitLength, r.x0[1] / p.unitLength, r.x0[2] / p.unitLength);
-    const double rad = r.r / p.unitLength;
-    const int tot = nx * ny * nz;
-    for (int k = threadIdx.x; k < tot; k += blockDim.x) {
-        const int x = b.lo[0] + k % nx;
-        const int y = b.lo[1] + (k / nx) % ny;
-        const int z = b.lo[2] + k / (nx * ny);
-        const size_t idx = (size_t)x + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z);
-        if (!(link[idx] & LINK_ACTIVE)) continue;
-        const V3 d = V3((double)x, (double)y, (double)z) - cen;
-        if (d.norm2() < rad * rad) atomicMax(&solid[idx], (uint32_t)(s + 1));
+    const int ns = *nslots_p;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nboxes = ns;
+    for (int s = blockIdx.x; s < ns; s += gridDim.x) {
+        const CoupleRec r = cp[s];
+        const NodeBox b = slot_box(r, p);
+        if (threadIdx.x == 0) boxes[s] = b;
+        const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
+        if (nx <= 0 || ny <= 0 || nz <= 0) continue;
+        const V3 cen(r.x0[0] / p.unitLength, r.x0[1] / p.unitLength, r.x0[2] / p.unitLength);
+        const double rad = r.r / p.unitLength;
+        const int tot = nx * ny * nz;
+        for (int k = threadIdx.x; k < tot; k += blockDim.x) {
+            const int x = b.lo[0] + k % nx;
+            const int y = b.lo[1] + (k / nx) % ny;
+            const int z = b.lo[2] + k / (nx * ny);
+            const V3 d = V3((double)x, (double)y, (double)z) - cen;
+            if (!(d.norm2() < rad * rad)) continue;
+            const size_t idx = (size_t)x + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z);
+            if (link[idx] & LINK_ACTIVE) atomicMax(&solid[idx], (p.ctag << SOLID_SLOT_BITS) | (uint32_t)(s + 1));
+        }
     }
 }
 
@@ -627,8 +668,15 @@ struct KSum {
 };
 
 // Per particle slot: sum over the nodes it owns of diffVel, radius x diffVel and
-// the node mass (LB.cpp:1897-1902), with compensated FP64 sums and a
-// fixed-order LDS tree across the workgroup (no atomics, deterministic).
+// the node mass (LB.cpp:1897-1902). One wavefront per slot; each lane keeps
+// Neumaier-compensated FP64 sums and the lanes are merged by a fixed butterfly
+// (deterministic, no atomics). Only nodes strictly inside the sphere can be
+// owned by the slot, so the others are skipped before any load.
+__device__ __forceinline__ void ksum_merge(KSum& a, double s2, double c2) {
+    a.add(s2);
+    a.c += c2;
+}
+
 template <typename HT>
 __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ nslots_p, const NodeBox* __restrict__ boxes,
                                                         const CoupleRec* __restrict__ cp,
@@ -636,49 +684,56 @@ __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ 
                                                         const uint32_t* __restrict__ link, const HT* __restrict__ hf,
                                                         HydroPartial* __restrict__ out, LBKParams p) {
 #pragma clang fp contract(off)
-    const int s = blockIdx.x;
-    if (s >= *nslots_p) return;
+    const int ns = *nslots_p;
     const size_t N = (size_t)p.X * p.Y * p.Z;
-    const CoupleRec r = cp[s];
-    const NodeBox b = boxes[s];
-    KSum acc[7];
-    for (int k = 0; k < 7; ++k) acc[k] = KSum{0.0, 0.0};
-    const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
-    const int tot = (nx > 0 && ny > 0 && nz > 0) ? nx * ny * nz : 0;
-    const V3 x0(r.x0[0], r.x0[1], r.x0[2]);
-    const V3 rv(r.rv[0], r.rv[1], r.rv[2]);
-    for (int k = threadIdx.x; k < tot; k += blockDim.x) {
-        const int x = b.lo[0] + k % nx;
-        const int y = b.lo[1] + (k / nx) % ny;
-        const int z = b.lo[2] + k / (nx * ny);
-        const size_t idx = (size_t)x + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z);
-        if (solid[idx] != (uint32_t)(s + 1)) continue;
-        const V3 d((double)hf[idx], (double)hf[N + idx], (double)hf[2 * N + idx]);
-        const V3 radius = V3((double)x, (double)y, (double)z) - x0 / p.unitLength + rv / p.unitLength;
-        const V3 m = radius.cross(d);
-        const double mass = (link[idx] & LINK_INTERFACE) ? 0.5 * p.initDensity : p.initDensity;
-        acc[0].add(d.x); acc[1].add(d.y); acc[2].add(d.z);
-        acc[3].add(m.x); acc[4].add(m.y); acc[5].add(m.z);
-        acc[6].add(mass);
-    }
-    __shared__ double sh_s[7][256];
-    __shared__ double sh_c[7][256];
-    for (int k = 0; k < 7; ++k) { sh_s[k][threadIdx.x] = acc[k].s; sh_c[k][threadIdx.x] = acc[k].c; }
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int s = wave; s < ns; s += nwaves) {
+        const CoupleRec r = cp[s];
+        const NodeBox b = boxes[s];
+        const uint32_t want = (p.ctag << SOLID_SLOT_BITS) | (uint32_t)(s + 1);
+        KSum acc[7];
+        for (int k = 0; k < 7; ++k) acc[k] = KSum{0.0, 0.0};
+        const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
+        const int tot = (nx > 0 && ny > 0 && nz > 0) ? nx * ny * nz : 0;
+        const V3 x0(r.x0[0], r.x0[1], r.x0[2]);
+        const V3 rv(r.rv[0], r.rv[1], r.rv[2]);
+        const V3 cen = x0 / p.unitLength;
+        const double rad = r.r / p.unitLength;
+        for (int k = lane; k < tot; k += 64) {
+            const int x = b.lo[0] + k % nx;
+            const int y = b.lo[1] + (k / nx) % ny;
+            const int z = b.lo[2] + k / (nx * ny);
+            const V3 dd = V3((double)x, (double)y, (double)z) - cen;
+            if (!(dd.norm2() < rad * rad)) continue;
+            const size_t idx = (size_t)x + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z);
+            if (solid[idx] != want) continue;
+            const V3 d((double)hf[idx], (double)hf[N + idx], (double)hf[2 * N + idx]);
+            const V3 radius = V3((double)x, (double)y, (double)z) - x0 / p.unitLength + rv / p.unitLength;
+            const V3 m = radius.cross(d);
+            const double mass = (link[idx] & LINK_INTERFACE) ? 0.5 * p.initDensity : p.initDensity;
+            acc[0].add(d.x); acc[1].add(d.y); acc[2].add(d.z);
+            acc[3].add(m.x); acc[4].add(m.y); acc[5].add(m.z);
+            acc[6].add(mass);
+        }
+        for (int off = 32; off > 0; off >>= 1) {
             for (int k = 0; k < 7; ++k) {
-                KSum a{sh_s[k][threadIdx.x], sh_c[k][threadIdx.x]};
-                a.add(sh_s[k][threadIdx.x + w]);
-                a.c += sh_c[k][threadIdx.x + w];
-                sh_s[k][threadIdx.x] = a.s;
-                sh_c[k][threadIdx.x] = a.c;
+                const double s2 = __shfl_xor(acc[k].s, off, 64);
+                const double c2 = __shfl_xor(acc[k].c, off, 64);
+                // both partners combine in the same (lower lane first) order
+                if (lane & off) {
+                    KSum a{s2, c2};
+                    ksum_merge(a, acc[k].s, acc[k].c);
+                    acc[k] = a;
+                } else {
+                    ksum_merge(acc[k], s2, c2);
+                }
             }
         }
-        __syncthreads();
+        if (lane == 0)
+            for (int k = 0; k < 7; ++k) out[s].s[k] = acc[k].s + acc[k].c;
     }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 7; ++k) out[s].s[k] = sh_s[k][0] + sh_c[k][0];
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t v, size_t n) {
@@ -697,14 +752,14 @@ dim3 lattice_grid(const LBKParams& p) { return dim3((p.X + BX - 1) / BX, (p.Y + 
             k_fused64<PULL, VV, CO><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link, solid, cp,            \
                                                     (double*)hf, visc, mac, p);                                    \
         else                                                                                                       \
-            k_fused32<PULL, VV, CO><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid, cp, (float*)hf, \
-                                                    visc, mac, p);                                                 \
+            k_fused32<PULL, VV, CO><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid, cpf,            \
+                                                    (float*)hf, visc, mac, p);                                     \
         return;                                                                                                    \
     }
 
 void launch_lb_fused(Prec prec, bool pull, bool varVisc, int collision, const void* fin, void* fout,
-                     const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, void* hf, double* visc,
-                     double* mac, const LBKParams& p, hipStream_t s) {
+                     const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, const CoupleRecF* cpf, void* hf,
+                     double* visc, double* mac, const LBKParams& p, hipStream_t s) {
     const dim3 g = lattice_grid(p), b(BX, BY, 1);
     HYB_FUSED_CASE(true, false, HYB_COLLISION_BGK)
     HYB_FUSED_CASE(true, false, HYB_COLLISION_TRT)
@@ -761,26 +816,22 @@ void launch_lb_init_pops(Prec prec, void* f, const uint32_t* link, const int* mx
                                                  Finit[2]);
 }
 
-void launch_classify_clear(const int* nboxes, int max_slots, const NodeBox* boxes, uint32_t* solid, const LBKParams& p,
-                           hipStream_t s) {
-    if (max_slots <= 0) return;
-    k_classify_clear<<<max_slots, 256, 0, s>>>(nboxes, boxes, solid, p);
-}
-
 void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, NodeBox* boxes, int* nboxes,
                          uint32_t* solid, const uint32_t* link, const LBKParams& p, hipStream_t s) {
     if (max_slots <= 0) return;
-    k_classify_set<<<max_slots, 256, 0, s>>>(nslots, cp, boxes, nboxes, solid, link, p);
+    k_classify_set<<<max_slots < 2048 ? max_slots : 2048, 256, 0, s>>>(nslots, cp, boxes, nboxes, solid, link, p);
 }
 
 void launch_hydro_partials(const int* nslots, int max_slots, const NodeBox* boxes, const CoupleRec* cp,
                            const uint32_t* solid, const uint32_t* link, Prec prec, const void* hf, HydroPartial* out,
                            const LBKParams& p, hipStream_t s) {
     if (max_slots <= 0) return;
+    int grid = (max_slots + 3) / 4;  // four wavefronts per workgroup
+    if (grid > 2048) grid = 2048;
     if (prec == Prec::F64)
-        k_hydro_partials<double><<<max_slots, 256, 0, s>>>(nslots, boxes, cp, solid, link, (const double*)hf, out, p);
+        k_hydro_partials<double><<<grid, 256, 0, s>>>(nslots, boxes, cp, solid, link, (const double*)hf, out, p);
     else
-        k_hydro_partials<float><<<max_slots, 256, 0, s>>>(nslots, boxes, cp, solid, link, (const float*)hf, out, p);
+        k_hydro_partials<float><<<grid, 256, 0, s>>>(nslots, boxes, cp, solid, link, (const float*)hf, out, p);
 }
 
 void launch_fill_u32(uint32_t* a, uint32_t v, size_t n, hipStream_t s) {

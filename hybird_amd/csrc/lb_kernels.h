@@ -12,6 +12,7 @@ namespace hyb {
 // Parameters every LB kernel receives by value.
 struct LBKParams {
     int X, Y, Z;
+    size_t PS;           // population plane stride (elements), N padded against channel aliasing
     int per[3];          // periodic along x, y, z
     double F[3];         // body force (lattice units), zero when forceFieldSolve = 0
     double visc0;        // initial kinematic viscosity (lattice units)
@@ -21,7 +22,12 @@ struct LBKParams {
     double trtMagic;
     double unitLength, unitSpeed, unitAngVel;
     double initDensity;
+    uint32_t ctag;       // tag of the current coupling step in the node ownership words
 };
+
+constexpr int SOLID_SLOT_BITS = 20;
+constexpr uint32_t SOLID_SLOT_MASK = (1u << SOLID_SLOT_BITS) - 1u;
+constexpr uint32_t SOLID_TAG_MAX = (1u << (32 - SOLID_SLOT_BITS)) - 1u;
 
 // Per-particle record the coupling reads (particle slots include ghosts).
 // Raw (DEM, physical-unit) values: centre, vector centre-of-element -> centre,
@@ -34,6 +40,15 @@ struct CoupleRec {
     double r;
     int32_t elem;
     int32_t pad;
+};
+
+// The same record for the FP32 path, prescaled to lattice units:
+// c = x0/L - radiusVec/L (so radius = node - c), v/Speed, w/AngVel.
+struct CoupleRecF {
+    float c[3];
+    float v[3];
+    float w[3];
+    float pad[3];
 };
 
 // Box of lattice nodes a particle slot may cover (inclusive bounds).
@@ -49,10 +64,10 @@ struct HydroPartial {
 
 enum class Prec { F64, F32S };
 
-// f planes: FP64 -> double[19][N]; FP32S -> float[19][N] holding f - w.
+// f planes: FP64 -> double[19][PS]; FP32S -> float[19][PS] holding f - w (PS >= N).
 void launch_lb_fused(Prec prec, bool pull, bool varVisc, int collision, const void* fin, void* fout,
-                     const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, void* hf, double* visc,
-                     double* mac, const LBKParams& p, hipStream_t s);
+                     const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, const CoupleRecF* cpf, void* hf,
+                     double* visc, double* mac, const LBKParams& p, hipStream_t s);
 void launch_lb_stream(Prec prec, const void* fin, void* fout, const uint32_t* link, const LBKParams& p, hipStream_t s);
 void launch_lb_init_types(uint8_t* type, const int* boundary, int nwalls, const WallDev* walls, int problem,
                           const double* liquidBox, const LBKParams& p, hipStream_t s);
@@ -61,8 +76,6 @@ void launch_lb_init_links(const uint8_t* type, uint32_t* link, const LBKParams& 
 void launch_lb_max_active(const uint32_t* link, int* maxXYZ, const LBKParams& p, hipStream_t s);
 void launch_lb_init_pops(Prec prec, void* f, const uint32_t* link, const int* maxXYZ, double* visc, const double* u0,
                          const double* Finit, const LBKParams& p, hipStream_t s);
-void launch_classify_clear(const int* nboxes, int max_slots, const NodeBox* boxes, uint32_t* solid, const LBKParams& p,
-                           hipStream_t s);
 void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, NodeBox* boxes, int* nboxes,
                          uint32_t* solid, const uint32_t* link, const LBKParams& p, hipStream_t s);
 void launch_hydro_partials(const int* nslots, int max_slots, const NodeBox* boxes, const CoupleRec* cp,

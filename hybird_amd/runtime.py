@@ -15,7 +15,8 @@ import numpy as np
 from ._abi import HybConfig
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libhybird_amd.so")
+# HYBIRD_AMD_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("HYBIRD_AMD_LIB", os.path.join(HERE, "libhybird_amd.so"))
 
 _lib = None
 
