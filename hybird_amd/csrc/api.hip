@@ -380,7 +380,7 @@ int hyb_dem_init(hyb_dem* d, const hyb_lb* lb) {
         (rc = dalloc(&b.cursor, scap)) || (rc = dalloc(&b.ghostOfCount, std::max(P, 1))) ||
         (rc = dalloc(&b.ghostOf, 8 * (size_t)std::max(P, 1))) || (rc = dalloc(&b.side, 3 * (size_t)std::max(P, 1))) ||
         (rc = dalloc(&b.ghostB, p.Gcap)) || (rc = dalloc(&b.posI, p.pairCap)) || (rc = dalloc(&b.posJ, p.pairCap)) ||
-        (rc = dalloc(&b.prec, 2 * (size_t)p.pairCap)))
+        (rc = dalloc(&b.prec, 2 * (size_t)p.pairCap)) || (rc = dalloc(&b.vmax, (size_t)(std::max(P, 1) + 63) / 64)))
         return rc;
     HIPCHK(hipMemcpyAsync(b.el, el.data(), el.size() * sizeof(double), hipMemcpyHostToDevice, d->stream));
     DEMState st;
@@ -425,7 +425,7 @@ int hyb_dem_destroy(hyb_dem* d) {
     dfree(b.el); dfree(b.st); dfree(b.ghostIndex); dfree(b.ghostVec); dfree(b.pairs); dfree(b.nearWall);
     dfree(b.cellTable); dfree(b.cellOf); dfree(b.cellCount); dfree(b.work); dfree(b.csrOff); dfree(b.csr);
     dfree(b.cursor); dfree(b.ghostOfCount); dfree(b.ghostOf); dfree(b.side); dfree(b.ghostB);
-    dfree(b.posI); dfree(b.posJ); dfree(b.prec);
+    dfree(b.posI); dfree(b.posJ); dfree(b.prec); dfree(b.vmax);
     delete d;
     return HYB_OK;
 }
@@ -495,7 +495,8 @@ static int lb_alloc(hyb_lb* l, int slotCap) {
     HIPCHK(hipMalloc(&l->f[0], pb));
     HIPCHK(hipMalloc(&l->f[1], pb));
     if ((rc = dalloc(&l->type, N)) || (rc = dalloc(&l->link, N)) || (rc = dalloc(&l->solid, N)) ||
-        (rc = dalloc(&l->maxXYZ, 3)) || (rc = dalloc(&l->dcount, 1)) || (rc = dalloc(&l->nslots, 2)))
+        (rc = dalloc(&l->maxXYZ, 3 * (size_t)l->Z)) || (rc = dalloc(&l->dcount, (size_t)l->Z)) ||
+        (rc = dalloc(&l->nslots, 2)))
         return rc;
     if (l->varVisc() && (rc = dalloc(&l->visc, N))) return rc;
     l->slotCap = slotCap;
@@ -539,21 +540,35 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
     launch_lb_init_types(l->type, l->boundary, (int)walls.size(), dw, l->cfg.problem, l->cfg.liquidBox, p, s);
     launch_lb_init_interface(l->type, p, s);
     launch_lb_init_links(l->type, l->link, p, s);
-    HIPCHK(hipMemsetAsync(l->maxXYZ, 0, 3 * sizeof(int), s));
+    HIPCHK(hipMemsetAsync(l->maxXYZ, 0, 3 * sizeof(int) * l->Z, s));
     launch_lb_max_active(l->link, l->maxXYZ, p, s);
+    {
+        std::vector<int> mz(3 * (size_t)l->Z);
+        HIPCHK(hipMemcpyAsync(mz.data(), l->maxXYZ, mz.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        int m3[3] = {0, 0, 0};
+        for (int z = 0; z < l->Z; ++z)
+            for (int k = 0; k < 3; ++k) m3[k] = std::max(m3[k], mz[3 * z + k]);
+        HIPCHK(hipMemcpy(l->maxXYZ, m3, sizeof(m3), hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemsetAsync(l->f[0], 0, l->popBytes(), s));
     HIPCHK(hipMemsetAsync(l->f[1], 0, l->popBytes(), s));
     launch_lb_init_pops(l->prec, l->f[0], l->link, l->maxXYZ, l->visc, l->initVelocity, l->lbF, p, s);
     launch_fill_u32(l->solid, 0u, l->N, s);
     HIPCHK(hipMemsetAsync(l->nslots, 0, 2 * sizeof(int), s));
-    HIPCHK(hipMemsetAsync(l->dcount, 0, sizeof(unsigned long long), s));
+    HIPCHK(hipMemsetAsync(l->dcount, 0, sizeof(unsigned long long) * l->Z, s));
     launch_count_active(l->link, l->dcount, p, s);
     // particle nodes at init (LB.cpp:475-495)
     if (d && d->P) lb_couple_launch(l, d);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
     if (dw) (void)hipFree(dw);
-    HIPCHK(hipMemcpy(&l->activeCount, l->dcount, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    {
+        std::vector<unsigned long long> cz(l->Z);
+        HIPCHK(hipMemcpy(cz.data(), l->dcount, cz.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        l->activeCount = 0;
+        for (unsigned long long v : cz) l->activeCount += v;
+    }
     // total mass of active nodes outside particles (LB.cpp:209-216): fluid nodes carry mass 1
     {
         std::vector<uint32_t> lk(l->N), so(l->N);

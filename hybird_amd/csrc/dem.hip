@@ -150,9 +150,11 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
     if (t == 0) sh_err = 0;
     // ---- evalMaxDisp (DEM.cpp:1314-1324) and the trigger (DEM.cpp:342-347)
     if (!force) {
+        // max |x1|^2 over elements, from the per-wavefront maxima the corrector left
         double m = 0.0;
-        for (int e = t; e < P; e += RB) {
-            const double v = E.v3(EF_X + 3, e).norm2();
+        const int nw = (P + 63) / 64;
+        for (int k = t; k < nw; k += RB) {
+            const double v = b.vmax[k];
             if (v > m) m = v;
         }
         red[t] = m;
@@ -445,7 +447,7 @@ __device__ __forceinline__ void slot_pred(const El& E, const DEMBuffers& b, int 
 // evaluated at the predicted state. The signed increments for each of the two
 // elements are written into that element's slot of the pair in its csr list, so
 // that k_dem_gather adds them in pair-list order.
-__global__ void k_dem_pairs(DEMParams p, DEMBuffers b) {
+__global__ void __launch_bounds__(256) k_dem_pairs(DEMParams p, DEMBuffers b) {
 #pragma clang fp contract(off)
     const El E{b.el, p.P};
     const int P = p.P;
@@ -508,6 +510,15 @@ __global__ void k_dem_pairs(DEMParams p, DEMBuffers b) {
     }
 }
 
+// Maximum over the 64 lanes of a wavefront (order-free, so deterministic).
+__device__ __forceinline__ double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
 // Gear predictor of one element from its corrected state (elmt.cpp:139-177).
 __device__ __forceinline__ void predict_elem(const El& E, const DEMParams& p, int e, const V3 x[6], const Q4 q[6],
                                              const V3 w[6]) {
@@ -542,24 +553,37 @@ __device__ __forceinline__ void predict_elem(const El& E, const DEMParams& p, in
     E.set3(EF_WPL, e, project(wp0, qp0.adjoint()));
 }
 
-__global__ void k_dem_predict(DEMParams p, DEMBuffers b) {
+__global__ void __launch_bounds__(64) k_dem_predict(DEMParams p, DEMBuffers b) {
 #pragma clang fp contract(off)
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= p.P) return;
     const El E{b.el, p.P};
-    V3 x[6], w[6];
-    Q4 q[6];
-    for (int k = 0; k < 6; ++k) { x[k] = E.v3(EF_X + 3 * k, e); q[k] = E.q4(EF_Q + 4 * k, e); w[k] = E.v3(EF_W + 3 * k, e); }
-    predict_elem(E, p, e, x, q, w);
+    double v2 = 0.0;
+    if (e < p.P) {
+        V3 x[6], w[6];
+        Q4 q[6];
+        for (int k = 0; k < 6; ++k) { x[k] = E.v3(EF_X + 3 * k, e); q[k] = E.q4(EF_Q + 4 * k, e); w[k] = E.v3(EF_W + 3 * k, e); }
+        predict_elem(E, p, e, x, q, w);
+        v2 = x[1].norm2();
+    }
+    v2 = wave_max(v2);
+    if (threadIdx.x == 0) b.vmax[blockIdx.x] = v2;
 }
 
 // Per element: particle-contact increments in pair-list order, the wall contact
 // (DEM.cpp:1692-1717, 1902-1982), Newton-Euler (DEM.cpp:1149-1181), the Gear
 // corrector (elmt.cpp:179-254) and the predictor of the next sub-step.
-__global__ void k_dem_gather(DEMParams p, DEMBuffers b) {
-#pragma clang fp contract(off)
+__device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffers& b, int e);
+
+__global__ void __launch_bounds__(64) k_dem_gather(DEMParams p, DEMBuffers b) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= p.P) return;
+    double v2 = e < p.P ? gather_elem(p, b, e) : 0.0;
+    v2 = wave_max(v2);
+    if (threadIdx.x == 0) b.vmax[blockIdx.x] = v2;
+}
+
+// Returns |x1|^2 of the corrected element for the next displacement check.
+__device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffers& b, int e) {
+#pragma clang fp contract(off)
     const El E{b.el, p.P};
     V3 FP(0.0, 0.0, 0.0), MP(0.0, 0.0, 0.0), FW(0.0, 0.0, 0.0), MW(0.0, 0.0, 0.0);
     const uint32_t lo = b.csrOff[e], hi = b.csrOff[e + 1];
@@ -661,6 +685,7 @@ __global__ void k_dem_gather(DEMParams p, DEMBuffers b) {
     E.set3(EF_WL, e, project(w[0], q[0].adjoint()));
     // predictor of the next sub-step
     predict_elem(E, p, e, x, q, w);
+    return x[1].norm2();
 }
 
 // Coupling records of every particle slot after the DEM step: corrected
@@ -735,7 +760,7 @@ void launch_dem_rebuild(const DEMParams& p, const DEMBuffers& b, int force, hipS
 
 void launch_dem_predict(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {
     if (p.P <= 0) return;
-    k_dem_predict<<<(p.P + 255) / 256, 256, 0, s>>>(p, b);
+    k_dem_predict<<<(p.P + 63) / 64, 64, 0, s>>>(p, b);
 }
 
 void launch_dem_pairs(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {
@@ -747,7 +772,7 @@ void launch_dem_pairs(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {
 
 void launch_dem_gather(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {
     if (p.P <= 0) return;
-    k_dem_gather<<<(p.P + 127) / 128, 128, 0, s>>>(p, b);
+    k_dem_gather<<<(p.P + 63) / 64, 64, 0, s>>>(p, b);
 }
 
 void launch_dem_couple_table(const DEMParams& p, const DEMBuffers& b, CoupleRec* cp, CoupleRecF* cpf, int* nslots,

@@ -79,6 +79,7 @@ struct DEMBuffers {
     uint32_t* posI;        // [pairCap] csr slot of a pair in its first particle's list (0xFFFFFFFF: ghost)
     uint32_t* posJ;        // [pairCap] csr slot in its second particle's list
     struct PairRec* prec;  // [2 * pairCap] per csr slot: signed force/torque increments of one pair
+    double* vmax;          // [(P + 63) / 64] per-wavefront max |x1|^2 written by the corrector
     int scap;
     int ncells;
     int W;                 // ncells * 14

@@ -44,7 +44,7 @@ __device__ __forceinline__ uint32_t owner_slot(uint32_t w, uint32_t tag) {
 constexpr int BY = HYB_LB_BY;
 
 #ifndef HYB_LB_NTLOAD
-#define HYB_LB_NTLOAD 0
+#define HYB_LB_NTLOAD 1
 #endif
 
 // Population stores and loads: optionally non-temporal (every population is
@@ -556,13 +556,19 @@ __global__ void k_init_links(const uint8_t* __restrict__ type, uint32_t* __restr
     link[c.idx] = lw;
 }
 
+// Largest coordinates of active nodes (LB.cpp:913-921). Wavefront maxima are
+// combined per z-plane (mx[3 * z + k]); the host takes the maximum over planes.
 __global__ void k_max_active(const uint32_t* __restrict__ link, int* __restrict__ mx, LBKParams p) {
     Coord c;
-    if (!node_coord(p, c)) return;
-    if (link[c.idx] & LINK_ACTIVE) {
-        atomicMax(&mx[0], c.x);
-        atomicMax(&mx[1], c.y);
-        atomicMax(&mx[2], c.z);
+    int v[3] = {-1, -1, -1};
+    if (node_coord(p, c) && (link[c.idx] & LINK_ACTIVE)) { v[0] = c.x; v[1] = c.y; v[2] = c.z; }
+    for (int k = 0; k < 3; ++k) {
+        int m = v[k];
+        for (int off = 32; off > 0; off >>= 1) {
+            const int o = __shfl_xor(m, off, 64);
+            m = o > m ? o : m;
+        }
+        if ((threadIdx.x & 63) == 0 && m >= 0) atomicMax(&mx[3 * blockIdx.z + k], m);
     }
 }
 
@@ -604,7 +610,8 @@ __global__ void k_count_active(const uint32_t* __restrict__ link, unsigned long 
     bool a = false;
     if (node_coord(p, c)) a = (link[c.idx] & LINK_ACTIVE) != 0;
     const unsigned long long b = __ballot(a);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(cnt, (unsigned long long)__popcll(b));
+    // one counter per z-plane; the host adds the planes
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&cnt[blockIdx.z], (unsigned long long)__popcll(b));
 }
 
 // ------------------------------------------------------------------ particle classification
@@ -628,30 +635,54 @@ __device__ __forceinline__ NodeBox slot_box(const CoupleRec& r, const LBKParams&
     return b;
 }
 
+// Visits the nodes of a slot's box with one wavefront: lanes run along x within
+// a row (16, 32 or 64 lanes per row, so loads coalesce) and a wavefront covers
+// several rows per iteration. Rows whose (y, z) distance already reaches the
+// radius are skipped: rounded sums of non-negative squares are monotone, so no
+// node of such a row passes the strict inside test.
+template <class F>
+__device__ __forceinline__ void for_slot_nodes(const NodeBox& b, int lane, const V3& cen, double rad, F&& f) {
+    const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return;
+    const int shift = nx <= 16 ? 4 : (nx <= 32 ? 5 : 6);
+    const int lpr = 1 << shift;
+    const int rpi = 64 >> shift;
+    const int xo = lane & (lpr - 1);
+    const int rsub = lane >> shift;
+    const int rows = ny * nz;
+    for (int base = 0; base < rows; base += rpi) {
+        const int row = base + rsub;
+        if (row >= rows) continue;
+        const int y = b.lo[1] + row % ny;
+        const int z = b.lo[2] + row / ny;
+        const double dy = (double)y - cen.y, dz = (double)z - cen.z;
+        if (!(dy * dy + dz * dz < rad * rad)) continue;
+        for (int xx = xo; xx < nx; xx += lpr) f(b.lo[0] + xx, y, z);
+    }
+}
+
 __global__ void k_classify_set(const int* __restrict__ nslots_p, const CoupleRec* __restrict__ cp,
                                NodeBox* __restrict__ boxes, int* __restrict__ nboxes, uint32_t* __restrict__ solid,
                                const uint32_t* __restrict__ link, LBKParams p) {
 #pragma clang fp contract(off)
     const int ns = *nslots_p;
     if (blockIdx.x == 0 && threadIdx.x == 0) *nboxes = ns;
-    for (int s = blockIdx.x; s < ns; s += gridDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int s = wave; s < ns; s += nwaves) {
         const CoupleRec r = cp[s];
         const NodeBox b = slot_box(r, p);
-        if (threadIdx.x == 0) boxes[s] = b;
-        const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
-        if (nx <= 0 || ny <= 0 || nz <= 0) continue;
+        if (lane == 0) boxes[s] = b;
         const V3 cen(r.x0[0] / p.unitLength, r.x0[1] / p.unitLength, r.x0[2] / p.unitLength);
         const double rad = r.r / p.unitLength;
-        const int tot = nx * ny * nz;
-        for (int k = threadIdx.x; k < tot; k += blockDim.x) {
-            const int x = b.lo[0] + k % nx;
-            const int y = b.lo[1] + (k / nx) % ny;
-            const int z = b.lo[2] + k / (nx * ny);
+        const uint32_t val = (p.ctag << SOLID_SLOT_BITS) | (uint32_t)(s + 1);
+        for_slot_nodes(b, lane, cen, rad, [&](int x, int y, int z) {
             const V3 d = V3((double)x, (double)y, (double)z) - cen;
-            if (!(d.norm2() < rad * rad)) continue;
+            if (!(d.norm2() < rad * rad)) return;  // vector.cpp:153-158
             const size_t idx = (size_t)x + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z);
-            if (link[idx] & LINK_ACTIVE) atomicMax(&solid[idx], (p.ctag << SOLID_SLOT_BITS) | (uint32_t)(s + 1));
-        }
+            if (link[idx] & LINK_ACTIVE) atomicMax(&solid[idx], val);
+        });
     }
 }
 
@@ -695,20 +726,13 @@ __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ 
         const uint32_t want = (p.ctag << SOLID_SLOT_BITS) | (uint32_t)(s + 1);
         KSum acc[7];
         for (int k = 0; k < 7; ++k) acc[k] = KSum{0.0, 0.0};
-        const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
-        const int tot = (nx > 0 && ny > 0 && nz > 0) ? nx * ny * nz : 0;
         const V3 x0(r.x0[0], r.x0[1], r.x0[2]);
         const V3 rv(r.rv[0], r.rv[1], r.rv[2]);
         const V3 cen = x0 / p.unitLength;
         const double rad = r.r / p.unitLength;
-        for (int k = lane; k < tot; k += 64) {
-            const int x = b.lo[0] + k % nx;
-            const int y = b.lo[1] + (k / nx) % ny;
-            const int z = b.lo[2] + k / (nx * ny);
-            const V3 dd = V3((double)x, (double)y, (double)z) - cen;
-            if (!(dd.norm2() < rad * rad)) continue;
+        for_slot_nodes(b, lane, cen, rad, [&](int x, int y, int z) {
             const size_t idx = (size_t)x + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z);
-            if (solid[idx] != want) continue;
+            if (solid[idx] != want) return;
             const V3 d((double)hf[idx], (double)hf[N + idx], (double)hf[2 * N + idx]);
             const V3 radius = V3((double)x, (double)y, (double)z) - x0 / p.unitLength + rv / p.unitLength;
             const V3 m = radius.cross(d);
@@ -716,7 +740,7 @@ __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ 
             acc[0].add(d.x); acc[1].add(d.y); acc[2].add(d.z);
             acc[3].add(m.x); acc[4].add(m.y); acc[5].add(m.z);
             acc[6].add(mass);
-        }
+        });
         for (int off = 32; off > 0; off >>= 1) {
             for (int k = 0; k < 7; ++k) {
                 const double s2 = __shfl_xor(acc[k].s, off, 64);
@@ -819,7 +843,9 @@ void launch_lb_init_pops(Prec prec, void* f, const uint32_t* link, const int* mx
 void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, NodeBox* boxes, int* nboxes,
                          uint32_t* solid, const uint32_t* link, const LBKParams& p, hipStream_t s) {
     if (max_slots <= 0) return;
-    k_classify_set<<<max_slots < 2048 ? max_slots : 2048, 256, 0, s>>>(nslots, cp, boxes, nboxes, solid, link, p);
+    int grid = (max_slots + 3) / 4;  // one wavefront per slot, four per workgroup
+    if (grid > 2048) grid = 2048;
+    k_classify_set<<<grid, 256, 0, s>>>(nslots, cp, boxes, nboxes, solid, link, p);
 }
 
 void launch_hydro_partials(const int* nslots, int max_slots, const NodeBox* boxes, const CoupleRec* cp,
