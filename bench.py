@@ -110,6 +110,20 @@ def cpu_baseline(size_sub: int, cycles: int):
         return {"value": round(act / dt / 1e6, 3), "unit": "MLUPS", "cores": 1, "kind": "port", "sample": sample}
 
 
+def copy_probe(nbytes: int = 1 << 31, reps: int = 5) -> int:
+    """A plain device-to-device copy of nbytes, reps times: the reference point that
+    scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "reps": reps}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,7 +133,11 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-size", type=int, default=64)
     ap.add_argument("--cpu-cycles", type=int, default=150)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain device copy of known size (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU slab design is "
               f"described but not built)", file=sys.stderr)

@@ -16,6 +16,7 @@
 
 #include "common.h"
 #include "dem_kernels.h"
+#include "fs_kernels.h"
 #include "hybird_amd.h"
 #include "lb_kernels.h"
 
@@ -136,6 +137,14 @@ struct hyb_lb {
     double totalMass = 0.0;
     uint32_t time = 0;
     // profiling of the fused kernel
+    // free surface (LB.cpp:1492-1804)
+    FSBuffers fsb;
+    int fsCur = 0;         // current interface list
+    int rhoCur = 0;        // density buffer of the last collision
+    int fluidMassBuf = 0;  // density buffer that is the fluid-node mass (set by the free-surface step)
+    uint32_t fsTag = 0;
+    int64_t fsSteps = 0;
+    bool capture = false;  // keep n, u, hydroForce of every collision for the host views
     bool profiling = false;
     std::vector<hipEvent_t> evs;
     size_t evUsed = 0;
@@ -236,8 +245,42 @@ int hyb_dem_create(const hyb_config* cfg, int32_t n, const double* rec, hyb_dem*
     return HYB_OK;
 }
 
+// What DEM::discreteElementInit receives from the lattice (DEM.cpp:185-297):
+// boundary codes, node counts, units and the body force in lattice units.
+struct LatticeInfo {
+    int boundary[6];
+    int X, Y, Z;
+    Units unit;
+    double lbF[3];
+};
+
+static int dem_init(hyb_dem* d, const LatticeInfo& lb);
+
 int hyb_dem_init(hyb_dem* d, const hyb_lb* lb) {
     if (!d || !lb) return fail(HYB_E_ARG, "hyb_dem_init: null handle");
+    LatticeInfo li;
+    for (int k = 0; k < 6; ++k) li.boundary[k] = lb->boundary[k];
+    li.X = lb->X; li.Y = lb->Y; li.Z = lb->Z;
+    li.unit = lb->unit;
+    for (int k = 0; k < 3; ++k) li.lbF[k] = lb->lbF[k];
+    return dem_init(d, li);
+}
+
+int hyb_dem_init_lattice(hyb_dem* d, const int32_t boundary[6], const int32_t lbSize[3], double unitLength,
+                         double unitTime, double unitDensity, const double lbF[3]) {
+    if (!d || !boundary || !lbSize || !lbF) return fail(HYB_E_ARG, "hyb_dem_init_lattice: null argument");
+    if (!(unitLength > 0) || !(unitTime > 0) || !(unitDensity > 0)) return fail(HYB_E_ARG, "units must be > 0");
+    LatticeInfo li;
+    for (int k = 0; k < 6; ++k) li.boundary[k] = boundary[k];
+    li.X = lbSize[0]; li.Y = lbSize[1]; li.Z = lbSize[2];
+    if (li.X < 3 || li.Y < 3 || li.Z < 3) return fail(HYB_E_ARG, "lattice must have at least 3 nodes per axis");
+    li.unit = make_units(unitLength, unitTime, unitDensity);
+    for (int k = 0; k < 3; ++k) li.lbF[k] = lbF[k];
+    return dem_init(d, li);
+}
+
+static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
+    const LatticeInfo* lb = &lbi;
     const hyb_config& c = d->cfg;
     d->unit = lb->unit;
     const Units& U = d->unit;
@@ -472,9 +515,17 @@ int hyb_lb_create(const hyb_config* cfg, hyb_lb** out) {  // LB.cpp:85-188 + LB.
         if ((c.boundary[2 * k] == 4) != (c.boundary[2 * k + 1] == 4)) { delete l; return fail(HYB_E_ARG, "Periodic boundaries error!"); }
     l->slipCoefficient = c.slipCoefficient;
     l->turbConst = c.turbConst;
-    if (c.problem != HYB_PROBLEM_NONE || l->freeSurface) {
+    if (c.problem != HYB_PROBLEM_NONE && c.problem != HYB_PROBLEM_SPLASH && c.problem != HYB_PROBLEM_LIQUID_BOX) {
         delete l;
-        return fail(HYB_E_UNSUPPORTED, "free surface and gas regions are not supported by the GPU path yet");
+        return fail(HYB_E_UNSUPPORTED, "problemName not supported");
+    }
+    if (c.problem != HYB_PROBLEM_NONE && !l->freeSurface) {
+        delete l;
+        return fail(HYB_E_UNSUPPORTED, "gas regions without free-surface tracking (freeSurfaceSolve = 0) are not supported");
+    }
+    if (l->freeSurface && l->N >= 0xFFFFFFFFull) {
+        delete l;
+        return fail(HYB_E_CAPACITY, "free-surface lattices are limited to 2^32 - 1 nodes");
     }
     if (c.collision != HYB_COLLISION_BGK && c.collision != HYB_COLLISION_TRT) { delete l; return fail(HYB_E_ARG, "unknown collision model"); }
     l->prec = c.precision == HYB_FP32_SHIFTED ? Prec::F32S : Prec::F64;
@@ -499,6 +550,17 @@ static int lb_alloc(hyb_lb* l, int slotCap) {
         (rc = dalloc(&l->nslots, 2)))
         return rc;
     if (l->varVisc() && (rc = dalloc(&l->visc, N))) return rc;
+    if (l->freeSurface) {
+        FSBuffers& b = l->fsb;
+        const size_t eb = l->prec == Prec::F64 ? 8 : 4;
+        HIPCHK(hipMalloc(&b.rho[0], eb * N));
+        HIPCHK(hipMalloc(&b.rho[1], eb * N));
+        if ((rc = dalloc(&b.mass, N)) || (rc = dalloc(&b.uS, 3 * N)) || (rc = dalloc(&b.hFi, 3 * N)) ||
+            (rc = dalloc(&b.claim, N)) || (rc = dalloc(&b.list[0], N)) || (rc = dalloc(&b.list[1], N)) ||
+            (rc = dalloc(&b.listA, N)) || (rc = dalloc(&b.listB, N)) || (rc = dalloc(&b.mnew, N)) ||
+            (rc = dalloc(&b.state, N)) || (rc = dalloc(&b.ctr, 1)))
+            return rc;
+    }
     l->slotCap = slotCap;
     if (slotCap > 0) {
         HIPCHK(hipMalloc(&l->hf, (l->prec == Prec::F64 ? 8 : 4) * 3 * N));
@@ -553,7 +615,17 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
     }
     HIPCHK(hipMemsetAsync(l->f[0], 0, l->popBytes(), s));
     HIPCHK(hipMemsetAsync(l->f[1], 0, l->popBytes(), s));
-    launch_lb_init_pops(l->prec, l->f[0], l->link, l->maxXYZ, l->visc, l->initVelocity, l->lbF, p, s);
+    if (l->freeSurface) {
+        FSBuffers& b = l->fsb;
+        HIPCHK(hipMemsetAsync(b.mass, 0, 8 * l->N, s));
+        HIPCHK(hipMemsetAsync(b.uS, 0, 24 * l->N, s));
+        HIPCHK(hipMemsetAsync(b.hFi, 0, 24 * l->N, s));
+        HIPCHK(hipMemsetAsync(b.claim, 0, 8 * l->N, s));
+        HIPCHK(hipMemsetAsync(b.ctr, 0, sizeof(FSCounters), s));
+        launch_fs_build_list(l->type, b, 0, p, s);
+    }
+    launch_lb_init_pops(l->prec, l->f[0], l->link, l->maxXYZ, l->visc, l->initVelocity, l->lbF,
+                        l->freeSurface ? l->fsb.rho[0] : nullptr, l->fsb.mass, l->fsb.uS, p, s);
     launch_fill_u32(l->solid, 0u, l->N, s);
     HIPCHK(hipMemsetAsync(l->nslots, 0, 2 * sizeof(int), s));
     HIPCHK(hipMemsetAsync(l->dcount, 0, sizeof(unsigned long long) * l->Z, s));
@@ -584,14 +656,34 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
     l->steps = 0;
     l->cur = 0;
     l->time = 0;
+    l->fsCur = 0;
+    l->rhoCur = 0;
+    l->fluidMassBuf = 0;
+    l->fsTag = 0;
+    l->fsSteps = 0;
     l->dem = d;
     l->inited = true;
     return HYB_OK;
 }
 
-int hyb_lb_fs_step(hyb_lb* l) {
+int hyb_lb_fs_step(hyb_lb* l) {  // LB.cpp:235-245
     if (!l || !l->inited) return fail(HYB_E_STATE, "hyb_lb_fs_step before init");
-    if (l->freeSurface) return fail(HYB_E_UNSUPPORTED, "free surface not supported by the GPU path yet");
+    if (!l->freeSurface) return HYB_OK;
+    if (++l->fsTag == 0) {  // tag wrap-around: clear the ownership words once
+        launch_fill_u32((uint32_t*)l->fsb.claim, 0u, 2 * l->N, l->stream);
+        l->fsTag = 1;
+    }
+    const LBKParams p = l->params();
+    // new interface nodes get hydroForce + lbF; lbF is zeroed by the first collision
+    // when forceFieldSolve = 0 (LB.cpp:1074-1076)
+    double Fnew[3];
+    for (int k = 0; k < 3; ++k) Fnew[k] = (l->forceField || l->steps == 0) ? l->lbF[k] : 0.0;
+    launch_fs_step(l->prec, l->f[l->cur], l->f[l->cur ^ 1], l->type, l->link, l->visc, l->fsb, l->fsCur, l->rhoCur,
+                   l->fsTag, Fnew, l->steps > 0, p, l->stream);
+    HIPCHK(hipGetLastError());
+    l->fsCur ^= 1;
+    l->fluidMassBuf = l->rhoCur;
+    l->fsSteps++;
     return HYB_OK;
 }
 
@@ -623,19 +715,31 @@ int hyb_lb_step(hyb_lb* l, hyb_dem* d) {  // LB.cpp:221-233 fused into one kerne
         e1 = l->evs[l->evUsed++];
         HIPCHK(hipEventRecord(e0, l->stream));
     }
+    FSView fv{};
+    if (l->freeSurface) {
+        fv.pre = l->f[l->cur ^ 1];
+        fv.rhoPrev = l->fsb.rho[l->rhoCur];
+        fv.rhoCur = l->fsb.rho[l->rhoCur ^ 1];
+        fv.mass = l->fsb.mass;
+        fv.uS = l->fsb.uS;
+        fv.hFi = l->fsb.hFi;
+        fv.linkw = l->link;
+    }
     launch_lb_fused(l->prec, l->steps > 0, l->varVisc(), l->cfg.collision, l->f[l->cur], l->f[l->cur ^ 1], l->link,
-                    withParticles ? l->solid : nullptr, l->cp, l->cpf, withParticles ? l->hf : nullptr, l->visc, nullptr,
-                    p, l->stream);
+                    withParticles ? l->solid : nullptr, l->cp, l->cpf, withParticles ? l->hf : nullptr, l->visc,
+                    l->capture ? l->mac : nullptr, l->freeSurface ? &fv : nullptr, p, l->stream);
     if (l->profiling) HIPCHK(hipEventRecord(e1, l->stream));
     if (withParticles) {
-        launch_hydro_partials(l->nslots, l->slotCap, l->boxes, l->cp, l->solid, l->link, l->prec, l->hf, l->part, p,
-                              l->stream);
+        launch_hydro_partials(l->nslots, l->slotCap, l->boxes, l->cp, l->solid, l->link, l->prec, l->hf,
+                              l->freeSurface ? l->fsb.mass : nullptr, l->freeSurface ? fv.rhoPrev : nullptr, l->part,
+                              p, l->stream);
         launch_dem_hydro_combine(d->p, d->b, l->part, l->unit, l->stream);
     }
     HIPCHK(hipGetLastError());
     l->cur ^= 1;
+    if (l->freeSurface) l->rhoCur ^= 1;
     l->steps++;
-    l->macValid = false;
+    l->macValid = l->capture;
     return HYB_OK;
 }
 
@@ -649,6 +753,11 @@ int hyb_lb_destroy(hyb_lb* l) {
     if (l->scratch) (void)hipFree(l->scratch);
     dfree(l->type); dfree(l->link); dfree(l->solid); dfree(l->visc); dfree(l->mac); dfree(l->maxXYZ);
     dfree(l->dcount); dfree(l->cp); dfree(l->cpf); dfree(l->boxes); dfree(l->nslots); dfree(l->part);
+    FSBuffers& b = l->fsb;
+    if (b.rho[0]) (void)hipFree(b.rho[0]);
+    if (b.rho[1]) (void)hipFree(b.rho[1]);
+    dfree(b.mass); dfree(b.uS); dfree(b.hFi); dfree(b.claim); dfree(b.list[0]); dfree(b.list[1]); dfree(b.listA);
+    dfree(b.listB); dfree(b.mnew); dfree(b.state); dfree(b.ctr);
     delete l;
     return HYB_OK;
 }
@@ -716,15 +825,58 @@ static int ensure_mac(hyb_lb* l, hyb_dem* d) {
             h[4 * l->N + i] = 0.0; h[5 * l->N + i] = 0.0; h[6 * l->N + i] = 0.0;
         }
         HIPCHK(hipMemcpy(l->mac, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    } else if (l->freeSurface) {
+        // the free-surface pass prepares some collision inputs in place, so they
+        // cannot be recomputed afterwards
+        return fail(HYB_E_STATE, "free-surface mode: enable hyb_lb_set_capture before stepping to read n/u/hydroForce");
     } else {
         const bool withParticles = d && d->P && l->slotCap > 0;
         launch_fill_u32((uint32_t*)l->mac, 0xFFFFFFFFu, 14 * l->N, l->stream);  // NaN pattern for inactive nodes
         launch_lb_fused(l->prec, l->steps > 1, l->varVisc(), l->cfg.collision, l->f[l->cur ^ 1], nullptr, l->link,
-                        withParticles ? l->solid : nullptr, l->cp, l->cpf, nullptr, l->visc, l->mac, p, l->stream);
+                        withParticles ? l->solid : nullptr, l->cp, l->cpf, nullptr, l->visc, l->mac, nullptr, p,
+                        l->stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(l->stream));
     }
     l->macValid = true;
+    return HYB_OK;
+}
+
+// Density buffer k of the free-surface state, as doubles.
+static int fs_density(hyb_lb* l, int k, std::vector<double>& out) {
+    out.resize(l->N);
+    if (l->prec == Prec::F64) {
+        HIPCHK(hipMemcpy(out.data(), l->fsb.rho[k], 8 * l->N, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> r(l->N);
+        HIPCHK(hipMemcpy(r.data(), l->fsb.rho[k], 4 * l->N, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < l->N; ++i) out[i] = r[i];
+    }
+    return HYB_OK;
+}
+
+// Node mass of active nodes: without free surface the initial values (LB.cpp:
+// 909-944); with free surface the tracked interface mass and, for fluid nodes,
+// the density that the last free-surface step assigned (LB.cpp:1582-1584).
+static int mass_view(hyb_lb* l, const std::vector<uint32_t>& lk, std::vector<double>& v) {
+    const size_t N = l->N;
+    if (!l->freeSurface || l->fsSteps == 0) {
+        std::vector<double> m;
+        if (l->freeSurface) {
+            m.resize(N);
+            HIPCHK(hipMemcpy(m.data(), l->fsb.mass, 8 * N, hipMemcpyDeviceToHost));
+        }
+        for (size_t i = 0; i < N; ++i)
+            if (lk[i] & LINK_ACTIVE)
+                v[i] = (lk[i] & LINK_INTERFACE) ? (l->freeSurface ? m[i] : 0.5 * l->initDensity) : l->initDensity;
+        return HYB_OK;
+    }
+    std::vector<double> m(N), r;
+    HIPCHK(hipMemcpy(m.data(), l->fsb.mass, 8 * N, hipMemcpyDeviceToHost));
+    int rc = fs_density(l, l->fluidMassBuf, r);
+    if (rc) return rc;
+    for (size_t i = 0; i < N; ++i)
+        if (lk[i] & LINK_ACTIVE) v[i] = (lk[i] & LINK_INTERFACE) ? m[i] : r[i];
     return HYB_OK;
 }
 
@@ -760,7 +912,10 @@ int hyb_lb_get(hyb_lb* l, const char* cname, void* host, size_t cap, size_t* nee
         const void* src = l->f[l->cur];
         if (l->steps > 0) {
             if (!l->scratch) HIPCHK(hipMalloc(&l->scratch, l->popBytes()));
-            launch_lb_stream(l->prec, l->f[l->cur], l->scratch, l->link, p, l->stream);
+            if (l->freeSurface)
+                launch_fs_stream_view(l->prec, l->f[l->cur], l->scratch, l->type, l->link, l->fsb.uS, p, l->stream);
+            else
+                launch_lb_stream(l->prec, l->f[l->cur], l->scratch, l->link, p, l->stream);
             HIPCHK(hipGetLastError());
             HIPCHK(hipStreamSynchronize(l->stream));
             src = l->scratch;
@@ -779,6 +934,16 @@ int hyb_lb_get(hyb_lb* l, const char* cname, void* host, size_t cap, size_t* nee
                 for (int j = 0; j < 19; ++j)
                     out[i * 19 + j] = (lk[i] & LINK_ACTIVE) ? (double)f[(size_t)j * PS + i] + wgt(j) : NAN;
         }
+        return HYB_OK;
+    }
+    if (name == "n" && l->freeSurface) {
+        if (need) *need = N * 8;
+        if (!host || cap < N * 8) return HYB_OK;
+        std::vector<double> r;
+        int rc = fs_density(l, l->rhoCur, r);
+        if (rc) return rc;
+        double* out = (double*)host;
+        for (size_t i = 0; i < N; ++i) out[i] = (lk[i] & LINK_ACTIVE) ? r[i] : NAN;
         return HYB_OK;
     }
     if (name == "n" || name == "u" || name == "hydroForce") {
@@ -801,12 +966,25 @@ int hyb_lb_get(hyb_lb* l, const char* cname, void* host, size_t cap, size_t* nee
             if (l->visc) HIPCHK(hipMemcpy(v.data(), l->visc, N * 8, hipMemcpyDeviceToHost));
             for (size_t i = 0; i < N; ++i) v[i] = (lk[i] & LINK_ACTIVE) ? (l->visc ? v[i] : l->initDynVisc) : NAN;
         } else {
-            for (size_t i = 0; i < N; ++i)
-                if (lk[i] & LINK_ACTIVE) v[i] = (lk[i] & LINK_INTERFACE) ? 0.5 * l->initDensity : l->initDensity;
+            int rc = mass_view(l, lk, v);
+            if (rc) return rc;
         }
         return put(v.data(), N * 8, host, cap, need);
     }
     if (name == "totalMass") return put(&l->totalMass, 8, host, cap, need);
+    if (name == "fluidMass") {  // IO::totFluidMass (IO.cpp:987-999), summed in node order
+        std::vector<double> v(N, NAN);
+        int rc = mass_view(l, lk, v);
+        if (rc) return rc;
+        std::vector<uint32_t> so(N);
+        HIPCHK(hipMemcpy(so.data(), l->solid, N * 4, hipMemcpyDeviceToHost));
+        double m = 0.0;
+        for (size_t i = 0; i < N; ++i) {
+            const bool owned = (so[i] >> SOLID_SLOT_BITS) == l->ctag && (so[i] & SOLID_SLOT_MASK);
+            if ((lk[i] & LINK_ACTIVE) && !owned) m += v[i];
+        }
+        return put(&m, 8, host, cap, need);
+    }
     return fail(HYB_E_ARG, "unknown LB field " + name);
 }
 
@@ -912,6 +1090,17 @@ int hyb_lb_set_profiling(hyb_lb* l, int32_t enabled) {
     l->evUsed = 0;
     l->profMs = 0.0;
     l->profLaunches = 0;
+    return HYB_OK;
+}
+
+int hyb_lb_set_capture(hyb_lb* l, int32_t enabled) {
+    if (!l) return fail(HYB_E_ARG, "null handle");
+    if (enabled && !l->mac) {
+        int rc = dalloc(&l->mac, 7 * l->N);
+        if (rc) return rc;
+    }
+    l->capture = enabled != 0;
+    if (!l->capture && l->freeSurface) l->macValid = false;
     return HYB_OK;
 }
 

@@ -35,9 +35,12 @@ enum NodeType : uint8_t { T_FLUID = 0, T_GAS = 2, T_INTERFACE = 3, T_PERIODIC = 
                           T_WALL = 7, T_WALL_MOVING = 8, T_CURVED = 9 };
 
 // Per-node link word: bit j (1..18) = the link in direction j is a static wall;
-// bit 30 = node is interface; bit 31 = node is active (fluid or interface).
+// bit 29 = the free-surface pass has already streamed this node's populations
+// for the next collision (they sit at the node's own index of the output
+// buffer); bit 30 = node is interface; bit 31 = node is active (fluid or interface).
 constexpr uint32_t LINK_ACTIVE = 1u << 31;
 constexpr uint32_t LINK_INTERFACE = 1u << 30;
+constexpr uint32_t LINK_PRESTREAMED = 1u << 29;
 constexpr uint32_t LINK_WALLS = 0x7FFFEu;
 
 // 3-vector with the reference's operation order (vector.cpp:38-180): every

@@ -16,6 +16,8 @@
 
 #include "common.h"
 #include "hybird_amd.h"
+#include "fs_kernels.h"
+#include "lattice.h"
 #include "lb_kernels.h"
 
 namespace hyb {
@@ -66,11 +68,6 @@ __device__ __forceinline__ T ld_pop(const T* p) {
 #endif
 }
 
-struct Coord {
-    int x, y, z;
-    size_t idx;
-};
-
 __device__ __forceinline__ bool node_coord(const LBKParams& p, Coord& c) {
     c.x = blockIdx.x * BX + threadIdx.x;
     c.y = blockIdx.y * BY + threadIdx.y;
@@ -79,20 +76,6 @@ __device__ __forceinline__ bool node_coord(const LBKParams& p, Coord& c) {
     c.idx = (size_t)c.x + (size_t)p.X * ((size_t)c.y + (size_t)p.Y * (size_t)c.z);
     return true;
 }
-
-// Neighbour coordinates x-1, x, x+1 along one axis for an interior node, with the
-// periodic rewiring of LB.cpp:437-472 (layers 0 and X-1 are ghosts, period X-2).
-struct Nb3 {
-    int m, c, pl;
-};
-__device__ __forceinline__ Nb3 nb3(int x, int X, int per) {
-    Nb3 r;
-    r.c = x;
-    r.m = (per && x == 1) ? X - 2 : x - 1;
-    r.pl = (per && x == X - 2) ? 1 : x + 1;
-    return r;
-}
-__device__ __forceinline__ int pick(const Nb3& n, int d) { return d > 0 ? n.pl : (d < 0 ? n.m : n.c); }
 
 // Gather the 19 pre-collision populations of node c into f[]: f_i is pulled
 // from the neighbour x - c_i (32-bit node index into plane i). Links to static
@@ -204,13 +187,35 @@ __device__ __forceinline__ double shear_visc64(const double f[19], const double 
     return (minVisc < lo) ? lo : minVisc;
 }
 
-template <bool PULL, bool VARVISC, int COLL>
+// Node mass of the penalty force (LB.cpp:1880): without free surface the
+// initial masses (rho0, interface rho0/2) never change; with free surface an
+// interface node carries its tracked mass and a fluid node the density of its
+// previous collision (updateMass, LB.cpp:1582-1584).
+template <bool FS, typename T>
+__device__ __forceinline__ double node_mass(uint32_t lw, size_t idx, const FSView& fsv, const LBKParams& p) {
+    if (FS) return (lw & LINK_INTERFACE) ? fsv.mass[idx] : (double)((const T*)fsv.rhoPrev)[idx];
+    return (lw & LINK_INTERFACE) ? 0.5 * p.initDensity : p.initDensity;
+}
+
+// Free-surface mode: nodes the free-surface pass has streamed already (or
+// initialised) hold their populations at their own index of the output buffer.
+// Each thread reads those before it writes the same addresses, and every store
+// depends on the loaded values, so the loads cannot move past the stores.
+template <typename T>
+__device__ __forceinline__ void load_prepared(const FSView& fsv, size_t PS, size_t idx, uint32_t lw, T f[19]) {
+    const T* pre = (const T*)fsv.pre;
+#pragma unroll
+    for (int j = 0; j < 19; ++j) f[j] = pre[(size_t)j * PS + idx];
+    fsv.linkw[idx] = lw & ~LINK_PRESTREAMED;
+}
+
+template <bool PULL, bool VARVISC, int COLL, bool FS>
 __global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ fin, double* __restrict__ fout,
                                                      const uint32_t* __restrict__ link,
                                                      const uint32_t* __restrict__ solid,
                                                      const CoupleRec* __restrict__ cp, double* __restrict__ hf,
                                                      double* __restrict__ visc, double* __restrict__ mac,
-                                                     LBKParams p) {
+                                                     LBKParams p, FSView fsv) {
 #pragma clang fp contract(off)
     Coord c;
     if (!node_coord(p, c)) return;
@@ -218,7 +223,8 @@ __global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ 
     const uint32_t lw = link[c.idx];
     if (!(lw & LINK_ACTIVE)) return;
     double f[19];
-    if (PULL) pull19<double>(fin, p.PS, p, c, lw, f);
+    if (FS && (lw & LINK_PRESTREAMED)) load_prepared<double>(fsv, p.PS, c.idx, lw, f);
+    else if (PULL) pull19<double>(fin, p.PS, p, c, lw, f);
     else {
 #pragma unroll
         for (int j = 0; j < 19; ++j) f[j] = ld_pop(&fin[(size_t)j * p.PS + c.idx]);
@@ -230,7 +236,7 @@ __global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ 
     V3 hF(0.0, 0.0, 0.0);
     const uint32_t sid = solid ? owner_slot(solid[c.idx], p.ctag) : 0u;
     if (sid) {
-        const double mass = (lw & LINK_INTERFACE) ? 0.5 * p.initDensity : p.initDensity;
+        const double mass = node_mass<FS, double>(lw, c.idx, fsv, p);
         const V3 d = penalty64(cp[sid - 1], c, mass, n, u, p, hF);
         if (hf) {
             hf[c.idx] = d.x;
@@ -251,6 +257,13 @@ __global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ 
         mac[6 * N + c.idx] = hF.z;
     }
     if (!fout) return;
+    if (FS) {
+        ((double*)fsv.rhoCur)[c.idx] = n;
+        if (lw & LINK_INTERFACE) {
+            fsv.uS[c.idx] = u.x; fsv.uS[N + c.idx] = u.y; fsv.uS[2 * N + c.idx] = u.z;
+            fsv.hFi[c.idx] = hF.x; fsv.hFi[N + c.idx] = hF.y; fsv.hFi[2 * N + c.idx] = hF.z;
+        }
+    }
     const double usq = u.norm2();
     // feq_j = w n (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2), node.cpp:90-101, evaluated where used
     auto feq = [&](int j) {
@@ -321,13 +334,13 @@ __device__ __forceinline__ float cdotf(int j, float ux, float uy, float uz) {
     return r;
 }
 
-template <bool PULL, bool VARVISC, int COLL>
+template <bool PULL, bool VARVISC, int COLL, bool FS>
 __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ fin, float* __restrict__ fout,
                                                      const uint32_t* __restrict__ link,
                                                      const uint32_t* __restrict__ solid,
                                                      const CoupleRecF* __restrict__ cp, float* __restrict__ hf,
                                                      double* __restrict__ visc, double* __restrict__ mac,
-                                                     LBKParams p) {
+                                                     LBKParams p, FSView fsv) {
 #pragma clang fp contract(fast)
     Coord c;
     if (!node_coord(p, c)) return;
@@ -335,7 +348,8 @@ __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ f
     const uint32_t lw = link[c.idx];
     if (!(lw & LINK_ACTIVE)) return;
     float g[19];
-    if (PULL) pull19<float>(fin, p.PS, p, c, lw, g);
+    if (FS && (lw & LINK_PRESTREAMED)) load_prepared<float>(fsv, p.PS, c.idx, lw, g);
+    else if (PULL) pull19<float>(fin, p.PS, p, c, lw, g);
     else {
 #pragma unroll
         for (int j = 0; j < 19; ++j) g[j] = ld_pop(&fin[(size_t)j * p.PS + c.idx]);
@@ -359,7 +373,7 @@ __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ f
     const uint32_t sid = solid ? owner_slot(solid[c.idx], p.ctag) : 0u;
     if (sid) {
         const CoupleRecF& r = cp[sid - 1];
-        const float mass = (lw & LINK_INTERFACE) ? 0.5f * (float)p.initDensity : (float)p.initDensity;
+        const float mass = (float)node_mass<FS, float>(lw, c.idx, fsv, p);
         // rigid-body velocity at the node: v + w x (node - c)
         const float rx = (float)c.x - r.c[0], ry = (float)c.y - r.c[1], rz = (float)c.z - r.c[2];
         const float lvx = r.v[0] + (r.w[1] * rz - r.w[2] * ry);
@@ -388,6 +402,13 @@ __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ f
         mac[6 * N + c.idx] = hz;
     }
     if (!fout) return;
+    if (FS) {
+        ((float*)fsv.rhoCur)[c.idx] = rho;
+        if (lw & LINK_INTERFACE) {
+            fsv.uS[c.idx] = ux; fsv.uS[N + c.idx] = uy; fsv.uS[2 * N + c.idx] = uz;
+            fsv.hFi[c.idx] = hx; fsv.hFi[N + c.idx] = hy; fsv.hFi[2 * N + c.idx] = hz;
+        }
+    }
     const float usq = ux * ux + uy * uy + uz * uz;
     const float base = 1.5f * usq;
     float nu = (float)p.visc0;
@@ -502,18 +523,6 @@ __global__ void k_init_types(uint8_t* __restrict__ type, LBKParams p, int b0, in
     type[c.idx] = t;
 }
 
-// Neighbour index of an interior node along direction j with periodic rewiring.
-__device__ __forceinline__ size_t nb_index(const LBKParams& p, const Coord& c, int j) {
-    const Nb3 nx = nb3(c.x, p.X, p.per[0]);
-    const Nb3 ny = nb3(c.y, p.Y, p.per[1]);
-    const Nb3 nz = nb3(c.z, p.Z, p.per[2]);
-    return (size_t)pick(nx, cx(j)) + (size_t)p.X * ((size_t)pick(ny, cy(j)) + (size_t)p.Y * (size_t)pick(nz, cz(j)));
-}
-
-__device__ __forceinline__ bool interior(const LBKParams& p, const Coord& c) {
-    return c.x > 0 && c.x < p.X - 1 && c.y > 0 && c.y < p.Y - 1 && c.z > 0 && c.z < p.Z - 1;
-}
-
 // Interface = fluid with a gas neighbour (LB.cpp:787-797). Each pass reads the
 // previous pass's types, which is equivalent to the reference's in-place loops
 // because neither loop changes the property it tests on its neighbours.
@@ -577,7 +586,8 @@ __global__ void k_max_active(const uint32_t* __restrict__ link, int* __restrict_
 template <typename T, bool SHIFT>
 __global__ void k_init_pops(T* __restrict__ f, const uint32_t* __restrict__ link, const int* __restrict__ mx,
                             double* __restrict__ visc, LBKParams p, double u0x, double u0y, double u0z, double Fx,
-                            double Fy, double Fz) {
+                            double Fy, double Fz, T* __restrict__ rho, double* __restrict__ mass,
+                            double* __restrict__ uS) {
 #pragma clang fp contract(off)
     Coord c;
     if (!node_coord(p, c)) return;
@@ -603,6 +613,14 @@ __global__ void k_init_pops(T* __restrict__ f, const uint32_t* __restrict__ link
         else f[(size_t)j * p.PS + c.idx] = (T)feq;
     }
     if (visc) visc[c.idx] = p.visc0;
+    if (rho) {
+        // free surface: density of the "previous collision" and the interface state
+        rho[c.idx] = (T)n;
+        if (lw & LINK_INTERFACE) {
+            mass[c.idx] = 0.5 * p.initDensity;
+            uS[c.idx] = u.x; uS[N + c.idx] = u.y; uS[2 * N + c.idx] = u.z;
+        }
+    }
 }
 
 __global__ void k_count_active(const uint32_t* __restrict__ link, unsigned long long* __restrict__ cnt, LBKParams p) {
@@ -713,6 +731,8 @@ __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ 
                                                         const CoupleRec* __restrict__ cp,
                                                         const uint32_t* __restrict__ solid,
                                                         const uint32_t* __restrict__ link, const HT* __restrict__ hf,
+                                                        const double* __restrict__ fsMass,
+                                                        const HT* __restrict__ fsRho,
                                                         HydroPartial* __restrict__ out, LBKParams p) {
 #pragma clang fp contract(off)
     const int ns = *nslots_p;
@@ -736,7 +756,9 @@ __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ 
             const V3 d((double)hf[idx], (double)hf[N + idx], (double)hf[2 * N + idx]);
             const V3 radius = V3((double)x, (double)y, (double)z) - x0 / p.unitLength + rv / p.unitLength;
             const V3 m = radius.cross(d);
-            const double mass = (link[idx] & LINK_INTERFACE) ? 0.5 * p.initDensity : p.initDensity;
+            const bool itf = (link[idx] & LINK_INTERFACE) != 0;
+            const double mass = fsMass ? (itf ? fsMass[idx] : (double)fsRho[idx])
+                                       : (itf ? 0.5 * p.initDensity : p.initDensity);
             acc[0].add(d.x); acc[1].add(d.y); acc[2].add(d.z);
             acc[3].add(m.x); acc[4].add(m.y); acc[5].add(m.z);
             acc[6].add(mass);
@@ -772,19 +794,29 @@ dim3 lattice_grid(const LBKParams& p) { return dim3((p.X + BX - 1) / BX, (p.Y + 
 // ------------------------------------------------------------------ launchers
 #define HYB_FUSED_CASE(PULL, VV, CO)                                                                               \
     if (pull == PULL && varVisc == VV && collision == CO) {                                                        \
-        if (prec == Prec::F64)                                                                                     \
-            k_fused64<PULL, VV, CO><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link, solid, cp,            \
-                                                    (double*)hf, visc, mac, p);                                    \
-        else                                                                                                       \
-            k_fused32<PULL, VV, CO><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid, cpf,            \
-                                                    (float*)hf, visc, mac, p);                                     \
+        if (prec == Prec::F64) {                                                                                   \
+            if (fs)                                                                                                \
+                k_fused64<PULL, VV, CO, true><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link, solid, cp,  \
+                                                              (double*)hf, visc, mac, p, *fs);                     \
+            else                                                                                                   \
+                k_fused64<PULL, VV, CO, false><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link, solid, cp, \
+                                                               (double*)hf, visc, mac, p, none);                   \
+        } else {                                                                                                   \
+            if (fs)                                                                                                \
+                k_fused32<PULL, VV, CO, true><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid, cpf,   \
+                                                              (float*)hf, visc, mac, p, *fs);                      \
+            else                                                                                                   \
+                k_fused32<PULL, VV, CO, false><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid, cpf,  \
+                                                               (float*)hf, visc, mac, p, none);                    \
+        }                                                                                                          \
         return;                                                                                                    \
     }
 
 void launch_lb_fused(Prec prec, bool pull, bool varVisc, int collision, const void* fin, void* fout,
                      const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, const CoupleRecF* cpf, void* hf,
-                     double* visc, double* mac, const LBKParams& p, hipStream_t s) {
+                     double* visc, double* mac, const FSView* fs, const LBKParams& p, hipStream_t s) {
     const dim3 g = lattice_grid(p), b(BX, BY, 1);
+    const FSView none{};
     HYB_FUSED_CASE(true, false, HYB_COLLISION_BGK)
     HYB_FUSED_CASE(true, false, HYB_COLLISION_TRT)
     HYB_FUSED_CASE(true, true, HYB_COLLISION_BGK)
@@ -830,14 +862,14 @@ void launch_lb_max_active(const uint32_t* link, int* mx, const LBKParams& p, hip
 }
 
 void launch_lb_init_pops(Prec prec, void* f, const uint32_t* link, const int* mx, double* visc, const double* u0,
-                         const double* Finit, const LBKParams& p, hipStream_t s) {
+                         const double* Finit, void* rho, double* mass, double* uS, const LBKParams& p, hipStream_t s) {
     const dim3 g = lattice_grid(p), b(BX, BY, 1);
     if (prec == Prec::F64)
         k_init_pops<double, false><<<g, b, 0, s>>>((double*)f, link, mx, visc, p, u0[0], u0[1], u0[2], Finit[0],
-                                                   Finit[1], Finit[2]);
+                                                   Finit[1], Finit[2], (double*)rho, mass, uS);
     else
         k_init_pops<float, true><<<g, b, 0, s>>>((float*)f, link, mx, visc, p, u0[0], u0[1], u0[2], Finit[0], Finit[1],
-                                                 Finit[2]);
+                                                 Finit[2], (float*)rho, mass, uS);
 }
 
 void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, NodeBox* boxes, int* nboxes,
@@ -849,15 +881,17 @@ void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, 
 }
 
 void launch_hydro_partials(const int* nslots, int max_slots, const NodeBox* boxes, const CoupleRec* cp,
-                           const uint32_t* solid, const uint32_t* link, Prec prec, const void* hf, HydroPartial* out,
-                           const LBKParams& p, hipStream_t s) {
+                           const uint32_t* solid, const uint32_t* link, Prec prec, const void* hf, const double* fsMass,
+                           const void* fsRho, HydroPartial* out, const LBKParams& p, hipStream_t s) {
     if (max_slots <= 0) return;
     int grid = (max_slots + 3) / 4;  // four wavefronts per workgroup
     if (grid > 2048) grid = 2048;
     if (prec == Prec::F64)
-        k_hydro_partials<double><<<grid, 256, 0, s>>>(nslots, boxes, cp, solid, link, (const double*)hf, out, p);
+        k_hydro_partials<double><<<grid, 256, 0, s>>>(nslots, boxes, cp, solid, link, (const double*)hf, fsMass,
+                                                      (const double*)fsRho, out, p);
     else
-        k_hydro_partials<float><<<grid, 256, 0, s>>>(nslots, boxes, cp, solid, link, (const float*)hf, out, p);
+        k_hydro_partials<float><<<grid, 256, 0, s>>>(nslots, boxes, cp, solid, link, (const float*)hf, fsMass,
+                                                     (const float*)fsRho, out, p);
 }
 
 void launch_fill_u32(uint32_t* a, uint32_t v, size_t n, hipStream_t s) {

@@ -64,23 +64,27 @@ struct HydroPartial {
 
 enum class Prec { F64, F32S };
 
+struct FSView;
 // f planes: FP64 -> double[19][PS]; FP32S -> float[19][PS] holding f - w (PS >= N).
+// fs: free-surface state (nullptr without free surface), see fs_kernels.h
 void launch_lb_fused(Prec prec, bool pull, bool varVisc, int collision, const void* fin, void* fout,
                      const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, const CoupleRecF* cpf, void* hf,
-                     double* visc, double* mac, const LBKParams& p, hipStream_t s);
+                     double* visc, double* mac, const FSView* fs, const LBKParams& p, hipStream_t s);
 void launch_lb_stream(Prec prec, const void* fin, void* fout, const uint32_t* link, const LBKParams& p, hipStream_t s);
 void launch_lb_init_types(uint8_t* type, const int* boundary, int nwalls, const WallDev* walls, int problem,
                           const double* liquidBox, const LBKParams& p, hipStream_t s);
 void launch_lb_init_interface(uint8_t* type, const LBKParams& p, hipStream_t s);
 void launch_lb_init_links(const uint8_t* type, uint32_t* link, const LBKParams& p, hipStream_t s);
 void launch_lb_max_active(const uint32_t* link, int* maxXYZ, const LBKParams& p, hipStream_t s);
+// With free surface, also the initial density (rho), interface mass and interface velocity.
 void launch_lb_init_pops(Prec prec, void* f, const uint32_t* link, const int* maxXYZ, double* visc, const double* u0,
-                         const double* Finit, const LBKParams& p, hipStream_t s);
+                         const double* Finit, void* rho, double* mass, double* uS, const LBKParams& p, hipStream_t s);
 void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, NodeBox* boxes, int* nboxes,
                          uint32_t* solid, const uint32_t* link, const LBKParams& p, hipStream_t s);
+// fsMass/fsRho: free-surface node masses (interface mass, previous density), nullptr without free surface
 void launch_hydro_partials(const int* nslots, int max_slots, const NodeBox* boxes, const CoupleRec* cp,
-                           const uint32_t* solid, const uint32_t* link, Prec prec, const void* hf, HydroPartial* out,
-                           const LBKParams& p, hipStream_t s);
+                           const uint32_t* solid, const uint32_t* link, Prec prec, const void* hf, const double* fsMass,
+                           const void* fsRho, HydroPartial* out, const LBKParams& p, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_count_active(const uint32_t* link, unsigned long long* count, const LBKParams& p, hipStream_t s);
 

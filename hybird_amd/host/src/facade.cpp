@@ -1,0 +1,467 @@
+// C++ host layer: the reference's LB / DEM / IO classes over the C ABI of
+// libhybird_amd.so. Parameter parsing reproduces LB.cpp:85-188,
+// DEM.cpp:13-183 and hybird.cpp's flags; all solver work happens on the GPU.
+#include <cerrno>
+#include <cstdio>
+#include <fstream>
+
+#include "DEM.h"
+#include "IO.h"
+#include "LB.h"
+
+void hybird_check(int rc, const char* what, const char* file, int line) {
+    if (rc == HYB_OK) return;
+    cout << file << ":" << line << " ERROR: " << what << " failed (" << rc << "): " << hyb_last_error() << std::endl;
+    exit(1);
+}
+
+namespace {
+
+// Optional extension keys: read when present, without the missing-parameter
+// warning that the reference's keys produce. "-name value" overrides them too.
+template <class T>
+T extension_key(GetPot& cfg, GetPot& command_line, const char* name, T def) {
+    T v = cfg(name, def);
+    const std::string flag = std::string("-") + name;
+    if (command_line.search(flag.c_str())) v = command_line.next(v);
+    return v;
+}
+
+std::string upper(std::string s) {
+    for (char& c : s) c = (char)toupper((unsigned char)c);
+    return s;
+}
+
+// Writes a host view to <dir>/<name>.bin (raw little-endian).
+template <class T>
+void dump(const std::string& dir, const char* name, const std::vector<T>& v) {
+    FILE* f = std::fopen((dir + "/" + name + ".bin").c_str(), "wb");
+    if (!f) return;
+    if (!v.empty()) std::fwrite(v.data(), sizeof(T), v.size(), f);
+    std::fclose(f);
+}
+
+template <class T, class H, class G>
+bool view(H* h, G get, const char* name, std::vector<T>& out) {
+    size_t need = 0;
+    if (get(h, name, nullptr, 0, &need) != HYB_OK) return false;
+    out.resize(need / sizeof(T));
+    if (need && get(h, name, out.data(), need, &need) != HYB_OK) return false;
+    return true;
+}
+
+}  // namespace
+
+// ======================================================================== LB
+LB::LB() {
+    hyb_config_defaults(&cfg_);
+    boundary.assign(6, 1);
+    lbSize.assign(3, 1);
+}
+
+LB::~LB() {
+    const char* dir = std::getenv("HYBIRD_AMD_DUMP_DIR");
+    if (h_ && inited_ && dir && *dir) {
+        std::vector<double> d;
+        std::vector<unsigned char> t;
+        if (view(h_, hyb_lb_get, "types", t)) dump(dir, "lb_types", t);
+        for (const char* name : {"f", "n", "u", "hydroForce", "mass", "fluidMass"})
+            if (view(h_, hyb_lb_get, name, d)) dump(dir, (std::string("lb_") + name).c_str(), d);
+    }
+    if (h_) hyb_lb_destroy(h_);
+}
+
+void LB::latticeBoltzmannGet(GetPot& lbmCfgFile, GetPot& command_line) {  // LB.cpp:85-188
+    PARSE_CLASS_MEMBER(lbmCfgFile, unit.Length, "unitLength", 1.0);
+    ASSERT(unit.Length > 0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, unit.Time, "unitTime", 1.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, unit.Density, "unitDensity", 1.0);
+    ASSERT(unit.Density > 0);
+    double lbSizeX = 0.0, lbSizeY = 0.0, lbSizeZ = 0.0;
+    PARSE_CLASS_MEMBER(lbmCfgFile, lbSizeX, "lbSizeX", 0.0);
+    ASSERT(lbSizeX > 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, lbSizeY, "lbSizeY", 0.0);
+    ASSERT(lbSizeY > 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, lbSizeZ, "lbSizeZ", 0.0);
+    ASSERT(lbSizeZ > 0.0);
+    lbSize.assign(3, 0);
+    lbSize[0] = int(std::floor(lbSizeX / unit.Length + 0.5));
+    lbSize[1] = int(std::floor(lbSizeY / unit.Length + 0.5));
+    lbSize[2] = int(std::floor(lbSizeZ / unit.Length + 0.5));
+    PARSE_CLASS_MEMBER(lbmCfgFile, plasticVisc, "plasticVisc", 1.0);
+    ASSERT(plasticVisc >= 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, yieldStress, "yieldStress", 1.0);
+    ASSERT(yieldStress >= 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, initDynVisc, "initVisc", 1.0);
+    ASSERT(initDynVisc > 0.0);
+    cfg_.unitLength = unit.Length;
+    cfg_.unitTime = unit.Time;
+    cfg_.unitDensity = unit.Density;
+    cfg_.lbSize[0] = lbSizeX; cfg_.lbSize[1] = lbSizeY; cfg_.lbSize[2] = lbSizeZ;
+    cfg_.plasticVisc = plasticVisc;
+    cfg_.yieldStress = yieldStress;
+    cfg_.initVisc = initDynVisc;
+    if (unit.Time == 0.0) {  // time step for tau = 1 (LB.cpp:124-133)
+        const double tauOpt = 1.0;
+        unit.Time = nonNewtonian ? unit.Density * unit.Length * unit.Length / plasticVisc * (1.0 / 3.0) * (tauOpt - 0.5)
+                                 : unit.Density * unit.Length * unit.Length / initDynVisc * (1.0 / 3.0) * (tauOpt - 0.5);
+    }
+    unit.setComposite();
+    initDynVisc /= unit.DynVisc;
+    plasticVisc /= unit.DynVisc;
+    yieldStress /= unit.Stress;
+    initDensity = 1.0;
+    double initVelocityX = 0.0, initVelocityY = 0.0, initVelocityZ = 0.0;
+    PARSE_CLASS_MEMBER(lbmCfgFile, initVelocityX, "initVelocityX", 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, initVelocityY, "initVelocityY", 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, initVelocityZ, "initVelocityZ", 0.0);
+    cfg_.initVelocity[0] = initVelocityX; cfg_.initVelocity[1] = initVelocityY; cfg_.initVelocity[2] = initVelocityZ;
+    initVelocity = tVect(initVelocityX, initVelocityY, initVelocityZ) / unit.Speed;
+    double lbFX = 0.0, lbFY = 0.0, lbFZ = 0.0;
+    PARSE_CLASS_MEMBER(lbmCfgFile, lbFX, "lbFX", 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, lbFY, "lbFY", 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, lbFZ, "lbFZ", 0.0);
+    cfg_.lbF[0] = lbFX; cfg_.lbF[1] = lbFY; cfg_.lbF[2] = lbFZ;
+    lbF = tVect(lbFX, lbFY, lbFZ) / unit.Accel;
+    boundary.assign(6, 1);
+    PARSE_CLASS_MEMBER(lbmCfgFile, boundary[0], "boundary0", 1);
+    PARSE_CLASS_MEMBER(lbmCfgFile, boundary[1], "boundary1", 1);
+    PARSE_CLASS_MEMBER(lbmCfgFile, boundary[2], "boundary2", 1);
+    PARSE_CLASS_MEMBER(lbmCfgFile, boundary[3], "boundary3", 1);
+    PARSE_CLASS_MEMBER(lbmCfgFile, boundary[4], "boundary4", 1);
+    PARSE_CLASS_MEMBER(lbmCfgFile, boundary[5], "boundary5", 1);
+    for (int k = 0; k < 6; ++k) cfg_.boundary[k] = (int32_t)boundary[k];
+    PARSE_CLASS_MEMBER(lbmCfgFile, slipCoefficient, "slipCoefficient", 0.0);
+    ASSERT(slipCoefficient >= 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, turbConst, "turbConst", 0.0);
+    ASSERT(turbConst >= 0.0);
+    cfg_.slipCoefficient = slipCoefficient;
+    cfg_.turbConst = turbConst;
+    // extensions: collision model, storage precision, generic liquid box
+    const std::string coll = upper(extension_key<std::string>(lbmCfgFile, command_line, "collisionModel", "BGK"));
+    ASSERT(coll == "BGK" || coll == "TRT");
+    cfg_.collision = coll == "TRT" ? HYB_COLLISION_TRT : HYB_COLLISION_BGK;
+    cfg_.trtMagic = extension_key<double>(lbmCfgFile, command_line, "trtMagic", 0.25);
+    const std::string prec = upper(extension_key<std::string>(lbmCfgFile, command_line, "precision", "FP64"));
+    ASSERT(prec == "FP64" || prec == "FP32");
+    cfg_.precision = prec == "FP32" ? HYB_FP32_SHIFTED : HYB_FP64;
+    const char* boxKeys[6] = {"liquidBoxXMin", "liquidBoxXMax", "liquidBoxYMin", "liquidBoxYMax", "liquidBoxZMin",
+                              "liquidBoxZMax"};
+    bool box = false;
+    for (int k = 0; k < 6; ++k) box |= lbmCfgFile.have_variable(boxKeys[k]);
+    if (box) {
+        const double inf = 1.0e300;
+        for (int k = 0; k < 6; ++k)
+            cfg_.liquidBox[k] = extension_key<double>(lbmCfgFile, command_line, boxKeys[k], (k % 2) ? inf : -inf);
+        cfg_.problem = HYB_PROBLEM_LIQUID_BOX;
+    } else {
+        cfg_.problem = -1;  // taken from problemName in latticeDefinition
+    }
+}
+
+void LB::latticeDefinition() {  // LB.cpp:42-83 (the lattice itself lives on the device)
+    if (cfg_.problem != HYB_PROBLEM_LIQUID_BOX) cfg_.problem = (int32_t)problemName;
+    cfg_.freeSurface = freeSurface;
+    cfg_.forceField = forceField;
+    cfg_.nonNewtonian = nonNewtonian;
+    cfg_.turbulence = turbulenceOn;
+    cfg_.lbSolve = 1;
+    HYB_CALL(hyb_lb_create(&cfg_, &h_));
+    totNodes = lbSize[0] * lbSize[1] * lbSize[2];
+}
+
+void LB::LBShow() const {  // LB.cpp:2290-2312
+    cout << "LATTICE CHARACTERISTICS (lattice units):\n";
+    cout << "directions=19;\n";
+    cout << "time unit = " << unit.Time << "; length unit = " << unit.Length << "; density unit = " << unit.Density
+         << ";\n";
+    cout << "lbSize: " << lbSize[0] << ", " << lbSize[1] << ", " << lbSize[2] << ";\n";
+    cout << "Init kin viscosity=" << initDynVisc << ", Init tau=" << (0.5 + 3.0 * initDynVisc) << ";\n";
+    cout << "Init density=" << initDensity << ";\n";
+    cout << "Init velocity=";
+    initVelocity.show();
+    cout << ";\nForce field=";
+    lbF.show();
+    cout << ";\nCollision=" << (cfg_.collision == HYB_COLLISION_TRT ? "TRT" : "BGK")
+         << "; populations=" << (cfg_.precision == HYB_FP32_SHIFTED ? "FP32 (f - w)" : "FP64") << ";\n";
+}
+
+void LB::latticeBolzmannInit(cylinderList& cylinders, wallList& walls, particleList& particles,
+                             objectList& objects) {  // LB.cpp:190-219
+    (void)walls;
+    if (!cylinders.empty() || !objects.empty()) {
+        cout << INFO << "ERROR: cylinders and objects are not supported by the GPU lattice" << std::endl;
+        exit(1);
+    }
+    DEM* dem = particles.owner;
+    HYB_CALL(hyb_lb_init(h_, dem ? dem->handle() : nullptr));
+    inited_ = true;
+    std::vector<double> tm;
+    if (view(h_, hyb_lb_get, "totalMass", tm) && !tm.empty()) totalMass = tm[0];
+}
+
+void LB::latticeBolzmannStep(elmtList& elmts, particleList& particles, wallList& walls) {  // LB.cpp:221-233
+    (void)particles;
+    (void)walls;
+    DEM* dem = elmts.owner;
+    HYB_CALL(hyb_lb_step(h_, dem ? dem->handle() : nullptr));
+}
+
+void LB::latticeBoltzmannCouplingStep(bool& newNeighborList, elmtList& elmts,
+                                      particleList& particles) {  // LB.cpp:247-280
+    (void)particles;
+    DEM* dem = elmts.owner;
+    int32_t flag = newNeighborList ? 1 : 0;
+    HYB_CALL(hyb_lb_couple(h_, dem ? dem->handle() : nullptr, &flag));
+    newNeighborList = flag != 0;
+}
+
+void LB::latticeBoltzmannFreeSurfaceStep() { HYB_CALL(hyb_lb_fs_step(h_)); }  // LB.cpp:235-245
+
+std::vector<double> LB::field(const char* name) const {
+    std::vector<double> v;
+    if (!view(h_, hyb_lb_get, name, v)) hybird_check(HYB_E_ARG, name, __FILE__, __LINE__);
+    return v;
+}
+
+std::vector<unsigned char> LB::nodeTypes() const {
+    std::vector<unsigned char> t;
+    if (!view(h_, hyb_lb_get, "types", t)) hybird_check(HYB_E_ARG, "types", __FILE__, __LINE__);
+    return t;
+}
+
+double LB::totalFluidMass() const {
+    const std::vector<double> m = field("fluidMass");
+    return m.empty() ? 0.0 : m[0];
+}
+
+// ======================================================================== DEM
+DEM::DEM() {
+    hyb_config_defaults(&cfg_);
+    walls.owner = this;
+    cylinders.owner = this;
+    pbcs.owner = this;
+    elmts.owner = this;
+    particles.owner = this;
+    objects.owner = this;
+}
+
+DEM::~DEM() {
+    const char* dir = std::getenv("HYBIRD_AMD_DUMP_DIR");
+    if (h_ && dir && *dir) {
+        std::vector<double> d;
+        std::vector<uint32_t> u;
+        for (const char* name : {"e_x", "e_q", "e_w", "e_forces", "dem_scalars"})
+            if (view(h_, hyb_dem_get, name, d)) dump(dir, (std::string("dem_") + name).c_str(), d);
+        for (const char* name : {"neighborTable", "nearWallTable"})
+            if (view(h_, hyb_dem_get, name, u)) dump(dir, (std::string("dem_") + name).c_str(), u);
+    }
+    if (h_) hyb_dem_destroy(h_);
+}
+
+void DEM::discreteElementGet(GetPot& lbmCfgFile, GetPot& command_line) {  // DEM.cpp:13-183
+    double density = 1.0;
+    PARSE_CLASS_MEMBER(lbmCfgFile, density, "density", 1.0);
+    ASSERT(density > 0.0);
+    string contactModelString;
+    PARSE_CLASS_MEMBER(lbmCfgFile, contactModelString, "contactModel", "none");
+    if (contactModelString == "HERTZIAN") cfg_.contactModel = HYB_CONTACT_HERTZIAN;
+    else if (contactModelString == "LINEAR") cfg_.contactModel = HYB_CONTACT_LINEAR;
+    double youngMod = 1.0, poisson = 1.0, linearStiff = 1.0, restitution = 1.0;
+    PARSE_CLASS_MEMBER(lbmCfgFile, youngMod, "youngMod", 1.0);
+    ASSERT(youngMod > 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, poisson, "poisson", 1.0);
+    ASSERT(poisson >= 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, linearStiff, "linearStiff", 1.0);
+    ASSERT(linearStiff >= 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, restitution, "restitution", 1.0);
+    ASSERT(restitution > 0.0);
+    ASSERT(restitution <= 1.0);
+    double viscTang = 1.0, frictionCoefPart = 1.0, frictionCoefWall = 1.0;
+    PARSE_CLASS_MEMBER(lbmCfgFile, viscTang, "viscTang", 1.0);
+    ASSERT(viscTang >= 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, frictionCoefPart, "frictionCoefPart", 1.0);
+    ASSERT(frictionCoefPart >= 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, frictionCoefWall, "frictionCoefWall", 1.0);
+    ASSERT(frictionCoefWall >= 0.0);
+    cfg_.density = density;
+    cfg_.youngMod = youngMod;
+    cfg_.poisson = poisson;
+    cfg_.linearStiff = linearStiff;
+    cfg_.restitution = restitution;
+    cfg_.viscTang = viscTang;
+    cfg_.frictionCoefPart = frictionCoefPart;
+    cfg_.frictionCoefWall = frictionCoefWall;
+    // particle initial state (DEM.cpp:63-116)
+    string particleFile;
+    PARSE_CLASS_MEMBER(lbmCfgFile, particleFile, "particleFile", "particles.dat");
+    double translateX = 0.0, translateY = 0.0, translateZ = 0.0;
+    PARSE_CLASS_MEMBER(lbmCfgFile, translateX, "translateX", 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, translateY, "translateY", 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, translateZ, "translateZ", 0.0);
+    double scale = 1.0;
+    PARSE_CLASS_MEMBER(lbmCfgFile, scale, "scale", 1.0);
+    cfg_.translate[0] = translateX; cfg_.translate[1] = translateY; cfg_.translate[2] = translateZ;
+    cfg_.scale = scale;
+    ifstream particleFileID(particleFile.c_str(), ios::in);
+    ASSERT(particleFileID.is_open());
+    unsigned int totElmt = 0;
+    particleFileID >> totElmt;
+    records_.assign((size_t)totElmt * HYB_ELEMENT_RECORD, 0.0);
+    elmts.clear();
+    for (unsigned int n = 0; n < totElmt; ++n) {
+        double* r = &records_[(size_t)n * HYB_ELEMENT_RECORD];
+        for (int k = 0; k < HYB_ELEMENT_RECORD; ++k) particleFileID >> r[k];
+        elmt e;
+        e.index = (unsigned)r[0];
+        e.size = (unsigned)r[1];
+        e.radius = r[2] * scale;
+        e.x0 = tVect(r[3] * scale, r[4] * scale, r[5] * scale) + tVect(translateX, translateY, translateZ);
+        e.x1 = tVect(r[6], r[7], r[8]);
+        e.w0 = tVect(r[9], r[10], r[11]);
+        e.q0 = tQuat(r[12], r[13], r[14], r[15]);
+        e.q1 = tQuat(r[16], r[17], r[18], r[19]);
+        elmts.push_back(e);
+    }
+    // objects (DEM.cpp:118-161): the file must exist; objects are not supported on the GPU
+    string objectFile;
+    PARSE_CLASS_MEMBER(lbmCfgFile, objectFile, "objectFile", "objects.dat");
+    ifstream objectFileID(objectFile.c_str(), ios::in);
+    ASSERT(objectFileID.is_open());
+    unsigned int totObjects = 0;
+    objectFileID >> totObjects;
+    if (totObjects) {
+        cout << INFO << "ERROR: objects are not supported by the GPU DEM" << std::endl;
+        exit(1);
+    }
+    double numVisc = 0.0;
+    PARSE_CLASS_MEMBER(lbmCfgFile, numVisc, "numVisc", 0.0);
+    PARSE_CLASS_MEMBER(lbmCfgFile, multiStep, "multiStep", 1);
+    PARSE_CLASS_MEMBER(lbmCfgFile, criticalRatio, "criticalRatio", 0.1);
+    cfg_.numVisc = numVisc;
+}
+
+void DEM::discreteElementInit(unsIntList& boundary, unsIntList& lbSize, measureUnits& unit,
+                              tVect& lbF) {  // DEM.cpp:185-297
+    cfg_.multiStep = (int32_t)multiStep;
+    cfg_.criticalRatio = criticalRatio;
+    cfg_.demInitialRepeat = demInitialRepeat;
+    cfg_.demSolve = 1;
+    const int32_t P = (int32_t)(records_.size() / HYB_ELEMENT_RECORD);
+    HYB_CALL(hyb_dem_create(&cfg_, P, P ? records_.data() : nullptr, &h_));
+    int32_t b[6], s[3];
+    for (int k = 0; k < 6; ++k) b[k] = (int32_t)boundary[k];
+    for (int k = 0; k < 3; ++k) s[k] = (int32_t)lbSize[k];
+    const double F[3] = {lbF.x, lbF.y, lbF.z};
+    HYB_CALL(hyb_dem_init_lattice(h_, b, s, unit.Length, unit.Time, unit.Density, F));
+    std::vector<double> sc;
+    if (view(h_, hyb_dem_get, "dem_scalars", sc) && sc.size() >= 12) {
+        deltat = sc[10];
+        multiStep = (unsigned)sc[11];
+    }
+    demF = lbF * unit.Accel;
+    // box walls as host records (DEM.cpp:435-638)
+    walls.clear();
+    const tVect ns[6] = {tVect(1, 0, 0), tVect(-1, 0, 0), tVect(0, 1, 0), tVect(0, -1, 0), tVect(0, 0, 1), tVect(0, 0, -1)};
+    for (int k = 0; k < 6; ++k) {
+        const unsigned bc = boundary[k];
+        if (bc < 5 || bc > 8) continue;
+        wall w;
+        w.index = (unsigned)walls.size();
+        w.n = ns[k];
+        const double pos = (k % 2 == 0) ? 0.5 : double(lbSize[k / 2]) - 1.5;
+        w.p = tVect(k / 2 == 0 ? pos : 0.0, k / 2 == 1 ? pos : 0.0, k / 2 == 2 ? pos : 0.0) * unit.Length;
+        w.moving = bc == 6 || bc == 8;
+        w.slip = bc == 5 || bc == 6;
+        walls.push_back(w);
+    }
+    particles.clear();
+    for (const elmt& e : elmts) {
+        particle p;
+        p.particleIndex = p.clusterIndex = e.index;
+        p.r = e.radius;
+        p.x0 = e.x0;
+        p.x1 = e.x1;
+        particles.push_back(p);
+    }
+    stdParticles = (unsigned)particles.size();
+}
+
+void DEM::discreteElementStep(IO& io) {  // DEM.cpp:331-376
+    (void)io;
+    HYB_CALL(hyb_dem_step(h_));
+    // the same time bookkeeping as the device side, without a device round trip
+    for (unsigned it = 0; it < multiStep; ++it) {
+        demTimeStep++;
+        demTime += deltat;
+    }
+}
+
+void DEM::syncHost() {
+    std::vector<double> x, w, fr, misc;
+    if (!view(h_, hyb_dem_get, "e_x", x) || !view(h_, hyb_dem_get, "e_w", w) ||
+        !view(h_, hyb_dem_get, "e_forces", fr) || !view(h_, hyb_dem_get, "e_misc", misc))
+        hybird_check(HYB_E_STATE, "DEM::syncHost", __FILE__, __LINE__);
+    for (size_t e = 0; e < elmts.size(); ++e) {
+        elmt& el = elmts[e];
+        const double* X = &x[e * 18];
+        el.x0 = tVect(X[0], X[1], X[2]);
+        el.x1 = tVect(X[3], X[4], X[5]);
+        el.w0 = tVect(w[e * 18], w[e * 18 + 1], w[e * 18 + 2]);
+        const double* F = &fr[e * 21];
+        el.FHydro = tVect(F[0], F[1], F[2]);
+        el.MHydro = tVect(F[3], F[4], F[5]);
+        el.FParticle = tVect(F[6], F[7], F[8]);
+        el.MParticle = tVect(F[9], F[10], F[11]);
+        el.FWall = tVect(F[12], F[13], F[14]);
+        el.MWall = tVect(F[15], F[16], F[17]);
+        el.FGrav = tVect(F[18], F[19], F[20]);
+        el.m = misc[e * 19];
+        el.fluidVolume = misc[e * 19 + 17];
+        if (e < particles.size()) {
+            particles[e].x0 = el.x0;
+            particles[e].x1 = el.x1;
+        }
+    }
+}
+
+std::vector<double> DEM::field(const char* name) const {
+    std::vector<double> v;
+    if (!view(h_, hyb_dem_get, name, v)) hybird_check(HYB_E_ARG, name, __FILE__, __LINE__);
+    return v;
+}
+
+// ======================================================================== IO
+void IO::initialize() {  // IO.cpp:25-130 (status file only)
+    if (!workDirectory.empty()) {
+        const int a = mkdir(workDirectory.c_str(), 0777);
+        if (a != 0 && errno != EEXIST) cout << "WARNING: cannot create " << workDirectory << "\n";
+    }
+    exportFileName = (workDirectory.empty() ? std::string(".") : workDirectory) + "/export.dat";
+    lastScreenExp = 0;
+    realTime = 0.0;
+}
+
+void IO::outputStep(const double& demTime, const LB& lb, const DEM& dem) {  // IO.cpp:132-198
+    (void)dem;
+    realTime = demTime;
+    const unsigned int screenExpCounter =
+        (screenExpTime > 0 ? static_cast<unsigned int>(realTime / screenExpTime) + 1 : 0);
+    if (screenExpCounter > lastScreenExp) {
+        lastScreenExp = screenExpCounter;
+        exportFile.open(exportFileName.c_str(), ios::app);
+        cout << currentTimeStep << "; time=" << realTime << "\t";
+        exportFile << currentTimeStep << "; time=" << realTime << "\t";
+        if (lbmSolve) {  // exportTotalMass (IO.cpp:878-883)
+            const double massTot = lb.totalFluidMass();
+            cout << "Volume=" << massTot * lb.unit.Volume << "; Mass = " << massTot * lb.unit.Mass << "\t";
+            exportFile << "Volume=" << massTot * lb.unit.Volume << "; Mass = " << massTot * lb.unit.Mass << "\t";
+        }
+        cout << "\n";
+        exportFile << "\n";
+        exportFile.close();
+        cout.flush();
+    }
+}
+
+void IO::outputFinal() { cout << "Simulation finished at step " << currentTimeStep << "\n"; }

@@ -23,7 +23,7 @@ _lib = None
 LB_FIELDS = {"f": (np.float64, (-1, 19)), "n": (np.float64, None), "u": (np.float64, (-1, 3)),
              "hydroForce": (np.float64, (-1, 3)), "mass": (np.float64, None), "visc": (np.float64, None),
              "types": (np.uint8, None), "pflag": (np.uint8, None), "solidIndex": (np.uint32, None),
-             "totalMass": (np.float64, None)}
+             "totalMass": (np.float64, None), "fluidMass": (np.float64, None)}
 DEM_FIELDS = {"e_x": (np.float64, (-1, 6, 3)), "e_xp": (np.float64, (-1, 6, 3)), "e_q": (np.float64, (-1, 6, 4)),
               "e_qp": (np.float64, (-1, 6, 4)), "e_w": (np.float64, (-1, 6, 3)), "e_wp": (np.float64, (-1, 6, 3)),
               "e_forces": (np.float64, (-1, 7, 3)), "e_misc": (np.float64, (-1, 19)),
@@ -35,7 +35,7 @@ DEM_FIELDS = {"e_x": (np.float64, (-1, 6, 3)), "e_xp": (np.float64, (-1, 6, 3)),
 EXPORTS = ["hyb_config_defaults", "hyb_last_error", "hyb_device_count", "hyb_sync", "hyb_dem_create", "hyb_dem_init",
            "hyb_dem_step", "hyb_dem_destroy", "hyb_lb_create", "hyb_lb_init", "hyb_lb_fs_step", "hyb_lb_couple",
            "hyb_lb_step", "hyb_lb_destroy", "hyb_cycle", "hyb_lb_get", "hyb_dem_get", "hyb_lb_set_profiling",
-           "hyb_lb_kernel_time", "hyb_lb_active_nodes"]
+           "hyb_lb_kernel_time", "hyb_lb_active_nodes", "hyb_lb_set_capture", "hyb_dem_init_lattice"]
 
 
 class HybError(RuntimeError):
@@ -56,13 +56,14 @@ def load():
     L.hyb_config_defaults.restype = None
     for name, args in [("hyb_device_count", [C.POINTER(C.c_int)]), ("hyb_sync", [vp]),
                        ("hyb_dem_create", [vp, i32, vp, C.POINTER(vp)]), ("hyb_dem_init", [vp, vp]),
+                       ("hyb_dem_init_lattice", [vp, vp, vp, C.c_double, C.c_double, C.c_double, vp]),
                        ("hyb_dem_step", [vp]), ("hyb_dem_destroy", [vp]), ("hyb_lb_create", [vp, C.POINTER(vp)]),
                        ("hyb_lb_init", [vp, vp]), ("hyb_lb_fs_step", [vp]),
                        ("hyb_lb_couple", [vp, vp, C.POINTER(i32)]), ("hyb_lb_step", [vp, vp]),
                        ("hyb_lb_destroy", [vp]), ("hyb_cycle", [vp, vp, i32]),
                        ("hyb_lb_get", [vp, C.c_char_p, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
                        ("hyb_dem_get", [vp, C.c_char_p, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
-                       ("hyb_lb_set_profiling", [vp, i32]),
+                       ("hyb_lb_set_profiling", [vp, i32]), ("hyb_lb_set_capture", [vp, i32]),
                        ("hyb_lb_kernel_time", [vp, C.POINTER(C.c_double), C.POINTER(i64)]),
                        ("hyb_lb_active_nodes", [vp, C.POINTER(i64)])]:
         fn = getattr(L, name)
@@ -118,6 +119,10 @@ class Simulation:
         if need.value:
             _check(fn(h, name.encode(), out.ctypes.data, need.value, C.byref(need)), f"get {name}")
         return out.reshape(shp) if shp else out
+
+    def set_capture(self, on: bool) -> None:
+        """Keep n, u and hydroForce of every collision (needed for those views with free surface)."""
+        _check(load().hyb_lb_set_capture(self._lb, 1 if on else 0), "hyb_lb_set_capture")
 
     def set_profiling(self, on: bool) -> None:
         _check(load().hyb_lb_set_profiling(self._lb, 1 if on else 0), "hyb_lb_set_profiling")

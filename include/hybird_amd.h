@@ -12,12 +12,15 @@
  *   hyb_lb_create        LB::LB + LB::latticeBoltzmannGet + LB::latticeDefinition  (LB.h:104-155, LB.cpp:42-188)
  *   hyb_dem_create       DEM::DEM + DEM::discreteElementGet                         (DEM.h:98-138, DEM.cpp:13-183)
  *   hyb_dem_init         DEM::discreteElementInit                                   (DEM.h:139, DEM.cpp:185-297)
+ *   hyb_dem_init_lattice DEM::discreteElementInit with its own arguments
+ *                        (boundary, lbSize, unit, lbF) instead of an LB handle   (DEM.h:139, DEM.cpp:185-297)
  *   hyb_lb_init          LB::latticeBolzmannInit                                    (LB.h:158, LB.cpp:190-219)
  *   hyb_dem_step         DEM::discreteElementStep                                   (DEM.h:137, DEM.cpp:331-376)
  *   hyb_lb_fs_step       LB::latticeBoltzmannFreeSurfaceStep                        (LB.h:161, LB.cpp:235-245)
  *   hyb_lb_couple        LB::latticeBoltzmannCouplingStep                           (LB.h:160, LB.cpp:247-280)
  *   hyb_lb_step          LB::latticeBolzmannStep                                    (LB.h:159, LB.cpp:221-233)
  *   hyb_cycle            goCycle without the IO call                                (hybird.cpp:35-66)
+ *   hyb_lb_set_capture   keeps the node fields IO reads (IO.cpp:711-831) while stepping
  *   hyb_lb_get / hyb_dem_get   host views of LB::nodes/types and DEM::elmts/particles read by IO
  *                        (IO.cpp:711-831, 835-1106)
  * The reference has no foreign-function interface; a C++ caller reaches these
@@ -105,6 +108,10 @@ int hyb_sync(hyb_lb* lb);
 /* ---- DEM ---- */
 int hyb_dem_create(const hyb_config* cfg, int32_t n_elements, const double* records, hyb_dem** out);
 int hyb_dem_init(hyb_dem* dem, const hyb_lb* lb);
+/* boundary: the six box boundary codes; lbSize: node counts; unit*: base units
+ * (after the automatic unitTime of LB.cpp:124-133); lbF: body force in lattice units. */
+int hyb_dem_init_lattice(hyb_dem* dem, const int32_t boundary[6], const int32_t lbSize[3], double unitLength,
+                         double unitTime, double unitDensity, const double lbF[3]);
 int hyb_dem_step(hyb_dem* dem);
 int hyb_dem_destroy(hyb_dem* dem);
 
@@ -126,6 +133,8 @@ int hyb_cycle(hyb_lb* lb, hyb_dem* dem, int32_t n);
  *       "u"          f64 [N][3]   velocity kept by the node (Guo-shifted, node.cpp:85-88)
  *       "hydroForce" f64 [N][3]
  *       "mass"       f64 [N]      free-surface mass function
+ *       "fluidMass"  f64 [1]      sum of "mass" over active nodes outside particles (IO.cpp:987-999)
+ *       "totalMass"  f64 [1]      the same sum at initialisation (LB.cpp:203-216)
  *       "visc"       f64 [N]      kinematic viscosity (lattice units)
  *       "types"      u8  [N]      node type code (node.h:73-83)
  *       "pflag"      u8  [N]      inside-particle flag
@@ -144,6 +153,12 @@ int hyb_dem_get(hyb_dem* dem, const char* name, void* host, size_t capacity, siz
  * records HIP events around that kernel on the handle's stream. */
 int hyb_lb_set_profiling(hyb_lb* lb, int32_t enabled);
 int hyb_lb_kernel_time(hyb_lb* lb, double* total_ms, int64_t* launches);
+/* Keep the density, velocity and node force of every collision for the host
+ * views "n", "u", "hydroForce" (what IO reads from the nodes, IO.cpp:711-831).
+ * Without free surface they can also be recomputed on demand; with free
+ * surface they must be captured while stepping. Costs 56 bytes per node and
+ * step of extra HBM writes while enabled. */
+int hyb_lb_set_capture(hyb_lb* lb, int32_t enabled);
 /* Active (fluid + interface) node count, the MLUPS denominator. */
 int hyb_lb_active_nodes(hyb_lb* lb, int64_t* count);
 

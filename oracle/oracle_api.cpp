@@ -114,6 +114,12 @@ long orc_get(void* h, const char* cname, void* out, long cap) {
     if (name == "particleNodes") return put(lb.particleNodes, out, cap);
     if (name == "neighbors") return put(lb.nb, out, cap);
     if (name == "totalMass") return put(std::vector<double>{lb.totalMass}, out, cap);
+    if (name == "fluidMass") {  // IO::totFluidMass (IO.cpp:987-999): active nodes outside particles, index order
+        double m = 0.0;
+        for (size_t it = 0; it < N; ++it)
+            if ((lb.t[it] == 0 || lb.t[it] == 3) && !lb.p[it]) m += lb.nodes[it].mass;
+        return put(std::vector<double>{m}, out, cap);
+    }
     if (name == "lbF") { std::vector<double> v; p3(v, lb.lbF); return put(v, out, cap); }
     if (name == "lbSize") return put(std::vector<uint32_t>(lb.lbSize.begin(), lb.lbSize.end()), out, cap);
     if (name.rfind("e_", 0) == 0) {
