@@ -38,6 +38,9 @@ from hybird_amd.particles import rsa_bed, write_empty_object_file, write_particl
 METRIC = "MLUPS + particle-steps/s, FP32-shift vs FP64 plain at <=1e-4 velocity L2 error"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_LUP = {"fp32": 2 * 19 * 4, "fp64": 2 * 19 * 8}
+# HBM bytes per launch of k_fused32 on the 256^3 workload: FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc,
+# scaled by the same counters on a plain copy (profiles/hbm_traffic_fused_dense_suspension_256.md)
+TRAFFIC_FUSED32_256 = 1.542e9 + 1.231e9
 
 
 def workload(size: int, seed: int = 1, phi: float = 0.2):
@@ -192,7 +195,8 @@ def main() -> int:
         "drag_rel_l2_fp32_vs_fp64": drag_err,
         "lb_kernel_ms": {"fp32": round(kms32, 4), "fp64": round(kms64, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": TRAFFIC_FUSED32_256 if args.size == 256 else None,
                      "kernel": "k_fused32 (TRT, pull)", "bytes_per_lup": BYTES_PER_LUP["fp32"]},
     }
     if not args.no_cpu_baseline:

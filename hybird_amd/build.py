@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhybird_amd.so")
-SOURCES = ["lb.hip", "fs.hip", "dem.hip", "api.hip"]
+SOURCES = ["lb.hip", "fs.hip", "adj.hip", "dem.hip", "api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # FP64 kernels keep the reference's operation order: no FMA contraction unless a

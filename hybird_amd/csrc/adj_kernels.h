@@ -1,0 +1,22 @@
+// Host-side launch interface of the adjoint step (adj.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "lb_kernels.h"
+
+namespace hyb {
+
+// Number of per-workgroup slots of the tau-gradient partial sums.
+size_t adj_tau_slots(const LBKParams& p);
+// One reverse step: lamOut = lambda_t from lamNext = lambda_{t+1} and the
+// recomputed post-collision state sPrev = s_{t-1} (pullFwd: t > 1, else sPrev = f0).
+// Accumulates dL/dF into gradF [3][N] and dL/dtau into tauPart [adj_tau_slots].
+void launch_adj_step(int collision, const double* sPrev, const double* lamNext, double* lamOut, const uint32_t* link,
+                     const uint32_t* solid, const CoupleRec* cp, double* gradF, double* tauPart, bool pullFwd,
+                     const LBKParams& p, hipStream_t s);
+
+}  // namespace hyb

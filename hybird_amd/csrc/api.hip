@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "common.h"
+#include "adj_kernels.h"
 #include "dem_kernels.h"
 #include "fs_kernels.h"
 #include "hybird_amd.h"
@@ -145,6 +146,10 @@ struct hyb_lb {
     uint32_t fsTag = 0;
     int64_t fsSteps = 0;
     bool capture = false;  // keep n, u, hydroForce of every collision for the host views
+    float* ffield = nullptr;  // optional per-node body force [3][N] (lattice units)
+    // adjoint: checkpoints of the post-collision state every adjEvery steps
+    int adjSteps = -1, adjEvery = 0;
+    std::vector<void*> adjCk;
     bool profiling = false;
     std::vector<hipEvent_t> evs;
     size_t evUsed = 0;
@@ -167,6 +172,7 @@ struct hyb_lb {
         p.unitLength = unit.Length; p.unitSpeed = unit.Speed; p.unitAngVel = unit.AngVel;
         p.initDensity = initDensity;
         p.ctag = ctag;
+        p.ffield = ffield;
         return p;
     }
     // planes are padded so that successive planes do not start on the same
@@ -743,6 +749,166 @@ int hyb_lb_step(hyb_lb* l, hyb_dem* d) {  // LB.cpp:221-233 fused into one kerne
     return HYB_OK;
 }
 
+int hyb_lb_set_force_field(hyb_lb* l, const float* F) {
+    if (!l || !l->inited) return fail(HYB_E_STATE, "hyb_lb_set_force_field before init");
+    HIPCHK(hipStreamSynchronize(l->stream));
+    if (!F) {
+        if (l->ffield) (void)hipFree(l->ffield);
+        l->ffield = nullptr;
+        return HYB_OK;
+    }
+    const size_t N = l->N;
+    std::vector<float> planes(3 * N);
+    for (size_t i = 0; i < N; ++i)
+        for (int k = 0; k < 3; ++k) planes[(size_t)k * N + i] = F[3 * i + k];
+    int rc;
+    if (!l->ffield && (rc = dalloc(&l->ffield, 3 * N))) return rc;
+    HIPCHK(hipMemcpy(l->ffield, planes.data(), 12 * N, hipMemcpyHostToDevice));
+    return HYB_OK;
+}
+
+int hyb_lb_set_populations(hyb_lb* l, const double* f) {
+    if (!l || !f) return fail(HYB_E_ARG, "hyb_lb_set_populations: null argument");
+    if (!l->inited || l->steps != 0) return fail(HYB_E_STATE, "populations can only be set before the first step");
+    HIPCHK(hipStreamSynchronize(l->stream));
+    const size_t N = l->N, PS = l->planeStride();
+    std::vector<uint32_t> lk(N);
+    HIPCHK(hipMemcpy(lk.data(), l->link, N * 4, hipMemcpyDeviceToHost));
+    if (l->prec == Prec::F64) {
+        std::vector<double> pl(19 * PS, 0.0);
+        for (size_t i = 0; i < N; ++i)
+            if (lk[i] & LINK_ACTIVE)
+                for (int j = 0; j < 19; ++j) pl[(size_t)j * PS + i] = f[i * 19 + j];
+        HIPCHK(hipMemcpy(l->f[l->cur], pl.data(), pl.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> pl(19 * PS, 0.0f);
+        for (size_t i = 0; i < N; ++i)
+            if (lk[i] & LINK_ACTIVE)
+                for (int j = 0; j < 19; ++j) pl[(size_t)j * PS + i] = (float)(f[i * 19 + j] - wgt(j));
+        HIPCHK(hipMemcpy(l->f[l->cur], pl.data(), pl.size() * 4, hipMemcpyHostToDevice));
+    }
+    l->macValid = false;
+    return HYB_OK;
+}
+
+// ------------------------------------------------------------------ adjoint
+static int adj_check(hyb_lb* l) {
+    if (l->prec != Prec::F64) return fail(HYB_E_UNSUPPORTED, "the adjoint runs on the FP64 path");
+    if (l->freeSurface) return fail(HYB_E_UNSUPPORTED, "the adjoint does not differentiate the free surface");
+    if (l->varVisc()) return fail(HYB_E_UNSUPPORTED, "the adjoint needs a constant viscosity");
+    return HYB_OK;
+}
+
+int hyb_adj_forward(hyb_lb* l, hyb_dem* d, int32_t steps, int32_t every) {
+    if (!l || !l->inited) return fail(HYB_E_STATE, "hyb_adj_forward before init");
+    if (steps < 1 || every < 1) return fail(HYB_E_ARG, "steps and checkpoint interval must be >= 1");
+    if (l->steps != 0) return fail(HYB_E_STATE, "hyb_adj_forward starts from the initial state (before any step)");
+    int rc = adj_check(l);
+    if (rc) return rc;
+    for (void* ck : l->adjCk) (void)hipFree(ck);
+    l->adjCk.clear();
+    const size_t pb = l->popBytes();
+    const int M = (steps + every - 1) / every;  // checkpoints of s_0, s_k, ..., s_{(M-1)k}
+    for (int m = 0; m < M; ++m) {
+        void* ck = nullptr;
+        HIPCHK(hipMalloc(&ck, pb));
+        l->adjCk.push_back(ck);
+    }
+    HIPCHK(hipMemcpyAsync(l->adjCk[0], l->f[l->cur], pb, hipMemcpyDeviceToDevice, l->stream));
+    for (int t = 1; t <= steps; ++t) {
+        if ((rc = hyb_lb_step(l, d))) return rc;
+        if (t % every == 0 && t / every < M)
+            HIPCHK(hipMemcpyAsync(l->adjCk[t / every], l->f[l->cur], pb, hipMemcpyDeviceToDevice, l->stream));
+    }
+    l->adjSteps = steps;
+    l->adjEvery = every;
+    return HYB_OK;
+}
+
+int hyb_adj_backward(hyb_lb* l, hyb_dem* d, const double* seed, double* grad_force, double* grad_tau, double* grad_f0) {
+    if (!l || !l->inited || !seed) return fail(HYB_E_ARG, "hyb_adj_backward: bad arguments");
+    if (l->adjSteps < 1 || l->steps != l->adjSteps) return fail(HYB_E_STATE, "hyb_adj_backward needs hyb_adj_forward just before");
+    int rc = adj_check(l);
+    if (rc) return rc;
+    const size_t N = l->N, PS = l->planeStride(), pb = l->popBytes();
+    const int T = l->adjSteps, K = l->adjEvery;
+    const LBKParams p = l->params();
+    const bool withParticles = d && d->P && l->slotCap > 0;
+    const uint32_t* solid = withParticles ? l->solid : nullptr;
+    hipStream_t s = l->stream;
+    // device buffers: segment of K states, two adjoint buffers, gradients
+    std::vector<void*> seg;
+    double *lam[2] = {nullptr, nullptr}, *gF = nullptr, *tp = nullptr;
+    const size_t slots = adj_tau_slots(p);
+    auto cleanup = [&]() {
+        for (void* q : seg) (void)hipFree(q);
+        (void)hipFree(lam[0]); (void)hipFree(lam[1]); (void)hipFree(gF); (void)hipFree(tp);
+    };
+    for (int i = 0; i < K; ++i) {
+        void* q = nullptr;
+        if (hipMalloc(&q, pb) != hipSuccess) { cleanup(); return fail(HYB_E_HIP, "adjoint segment allocation failed"); }
+        seg.push_back(q);
+    }
+    if (hipMalloc((void**)&lam[0], pb) != hipSuccess || hipMalloc((void**)&lam[1], pb) != hipSuccess ||
+        hipMalloc((void**)&gF, 24 * N) != hipSuccess || hipMalloc((void**)&tp, 8 * slots) != hipSuccess) {
+        cleanup();
+        return fail(HYB_E_HIP, "adjoint buffer allocation failed");
+    }
+    {
+        std::vector<double> pl(19 * PS, 0.0);
+        for (size_t i = 0; i < N; ++i)
+            for (int j = 0; j < 19; ++j) pl[(size_t)j * PS + i] = seed[i * 19 + j];
+        HIPCHK(hipMemcpy(lam[0], pl.data(), pb, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemsetAsync(lam[1], 0, pb, s));
+    HIPCHK(hipMemsetAsync(gF, 0, 24 * N, s));
+    HIPCHK(hipMemsetAsync(tp, 0, 8 * slots, s));
+    int cur = 0;
+    const int M = (int)l->adjCk.size();
+    for (int m = M - 1; m >= 0; --m) {
+        const int t0 = m * K;
+        const int len = std::min(K, T - t0);
+        HIPCHK(hipMemcpyAsync(seg[0], l->adjCk[m], pb, hipMemcpyDeviceToDevice, s));
+        for (int i = 1; i < len; ++i)  // recompute s_{t0+i}
+            launch_lb_fused(Prec::F64, t0 + i > 1, false, l->cfg.collision, seg[i - 1], seg[i], l->link, solid, l->cp,
+                            l->cpf, nullptr, l->visc, nullptr, nullptr, p, s);
+        for (int t = t0 + len; t > t0; --t) {
+            launch_adj_step(l->cfg.collision, (const double*)seg[t - 1 - t0], lam[cur], lam[cur ^ 1], l->link, solid,
+                            l->cp, gF, tp, t > 1, p, s);
+            cur ^= 1;
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    if (grad_f0) {
+        std::vector<double> pl(19 * PS);
+        std::vector<uint32_t> lk(N);
+        HIPCHK(hipMemcpy(pl.data(), lam[cur], pb, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(lk.data(), l->link, N * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; ++i)
+            for (int j = 0; j < 19; ++j) grad_f0[i * 19 + j] = (lk[i] & LINK_ACTIVE) ? pl[(size_t)j * PS + i] : 0.0;
+    }
+    if (grad_force) {
+        std::vector<double> g(3 * N);
+        HIPCHK(hipMemcpy(g.data(), gF, 24 * N, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; ++i)
+            for (int k = 0; k < 3; ++k) grad_force[3 * i + k] = g[(size_t)k * N + i];
+    }
+    if (grad_tau) {
+        std::vector<double> t(slots);
+        HIPCHK(hipMemcpy(t.data(), tp, 8 * slots, hipMemcpyDeviceToHost));
+        double sum = 0.0, comp = 0.0;  // Neumaier, fixed slot order
+        for (double v : t) {
+            const double u = sum + v;
+            comp += std::fabs(sum) >= std::fabs(v) ? (sum - u) + v : (v - u) + sum;
+            sum = u;
+        }
+        *grad_tau = sum + comp;
+    }
+    cleanup();
+    return HYB_OK;
+}
+
 int hyb_lb_destroy(hyb_lb* l) {
     if (!l) return HYB_OK;
     if (l->stream) (void)hipStreamSynchronize(l->stream);
@@ -753,6 +919,8 @@ int hyb_lb_destroy(hyb_lb* l) {
     if (l->scratch) (void)hipFree(l->scratch);
     dfree(l->type); dfree(l->link); dfree(l->solid); dfree(l->visc); dfree(l->mac); dfree(l->maxXYZ);
     dfree(l->dcount); dfree(l->cp); dfree(l->cpf); dfree(l->boxes); dfree(l->nslots); dfree(l->part);
+    for (void* ck : l->adjCk) (void)hipFree(ck);
+    if (l->ffield) (void)hipFree(l->ffield);
     FSBuffers& b = l->fsb;
     if (b.rho[0]) (void)hipFree(b.rho[0]);
     if (b.rho[1]) (void)hipFree(b.rho[1]);

@@ -49,6 +49,13 @@ __device__ __forceinline__ size_t nb_index(const LBKParams& p, const Coord& c, i
     return (size_t)pick(nx, cx(j)) + (size_t)p.X * ((size_t)pick(ny, cy(j)) + (size_t)p.Y * (size_t)pick(nz, cz(j)));
 }
 
+// Particle ownership word of a node: (coupling step tag << 20) | (slot + 1).
+// Entries written by an earlier coupling step carry an older tag and read as
+// "no particle", so the array never needs clearing between steps.
+__device__ __forceinline__ uint32_t owner_slot(uint32_t w, uint32_t tag) {
+    return (w >> SOLID_SLOT_BITS) == tag ? (w & SOLID_SLOT_MASK) : 0u;
+}
+
 __device__ __forceinline__ bool interior(const LBKParams& p, const Coord& c) {
     return c.x > 0 && c.x < p.X - 1 && c.y > 0 && c.y < p.Y - 1 && c.z > 0 && c.z < p.Z - 1;
 }

@@ -37,12 +37,6 @@ namespace {
 #endif
 constexpr int BX = HYB_LB_BX;
 
-// Particle ownership word of a node: (coupling step tag << 20) | (slot + 1).
-// Entries written by an earlier coupling step carry an older tag and read as
-// "no particle", so the array never needs clearing between steps.
-__device__ __forceinline__ uint32_t owner_slot(uint32_t w, uint32_t tag) {
-    return (w >> SOLID_SLOT_BITS) == tag ? (w & SOLID_SLOT_MASK) : 0u;
-}
 constexpr int BY = HYB_LB_BY;
 
 #ifndef HYB_LB_NTLOAD
@@ -244,7 +238,8 @@ __global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ 
             hf[2 * N + c.idx] = d.z;
         }
     }
-    const V3 Fb(p.F[0], p.F[1], p.F[2]);
+    V3 Fb(p.F[0], p.F[1], p.F[2]);
+    if (p.ffield) Fb = Fb + V3((double)p.ffield[c.idx], (double)p.ffield[N + c.idx], (double)p.ffield[2 * N + c.idx]);
     const V3 T = Fb + hF;
     u += 0.5 * T / n;  // node.cpp:85-88
     if (mac) {
@@ -369,6 +364,11 @@ __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ f
     const float irho = 1.0f / rho;
     float ux = jx * irho, uy = jy * irho, uz = jz * irho;
     float Tx = (float)p.F[0], Ty = (float)p.F[1], Tz = (float)p.F[2];
+    if (p.ffield) {
+        Tx += p.ffield[c.idx];
+        Ty += p.ffield[N + c.idx];
+        Tz += p.ffield[2 * N + c.idx];
+    }
     float hx = 0.0f, hy = 0.0f, hz = 0.0f;
     const uint32_t sid = solid ? owner_slot(solid[c.idx], p.ctag) : 0u;
     if (sid) {

@@ -23,6 +23,7 @@ struct LBKParams {
     double unitLength, unitSpeed, unitAngVel;
     double initDensity;
     uint32_t ctag;       // tag of the current coupling step in the node ownership words
+    const float* ffield; // optional per-node body force [3][N] added to F (nullptr: none)
 };
 
 constexpr int SOLID_SLOT_BITS = 20;

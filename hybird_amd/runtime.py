@@ -35,7 +35,8 @@ DEM_FIELDS = {"e_x": (np.float64, (-1, 6, 3)), "e_xp": (np.float64, (-1, 6, 3)),
 EXPORTS = ["hyb_config_defaults", "hyb_last_error", "hyb_device_count", "hyb_sync", "hyb_dem_create", "hyb_dem_init",
            "hyb_dem_step", "hyb_dem_destroy", "hyb_lb_create", "hyb_lb_init", "hyb_lb_fs_step", "hyb_lb_couple",
            "hyb_lb_step", "hyb_lb_destroy", "hyb_cycle", "hyb_lb_get", "hyb_dem_get", "hyb_lb_set_profiling",
-           "hyb_lb_kernel_time", "hyb_lb_active_nodes", "hyb_lb_set_capture", "hyb_dem_init_lattice"]
+           "hyb_lb_kernel_time", "hyb_lb_active_nodes", "hyb_lb_set_capture", "hyb_dem_init_lattice",
+           "hyb_lb_set_force_field", "hyb_lb_set_populations", "hyb_adj_forward", "hyb_adj_backward"]
 
 
 class HybError(RuntimeError):
@@ -64,6 +65,8 @@ def load():
                        ("hyb_lb_get", [vp, C.c_char_p, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
                        ("hyb_dem_get", [vp, C.c_char_p, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
                        ("hyb_lb_set_profiling", [vp, i32]), ("hyb_lb_set_capture", [vp, i32]),
+                       ("hyb_lb_set_force_field", [vp, vp]), ("hyb_lb_set_populations", [vp, vp]),
+                       ("hyb_adj_forward", [vp, vp, i32, i32]), ("hyb_adj_backward", [vp, vp, vp, vp, vp, vp]),
                        ("hyb_lb_kernel_time", [vp, C.POINTER(C.c_double), C.POINTER(i64)]),
                        ("hyb_lb_active_nodes", [vp, C.POINTER(i64)])]:
         fn = getattr(L, name)
@@ -132,6 +135,39 @@ class Simulation:
         n = C.c_int64(0)
         _check(load().hyb_lb_kernel_time(self._lb, C.byref(ms), C.byref(n)), "hyb_lb_kernel_time")
         return ms.value, n.value
+
+    # ---- adjoint (FP64, particles frozen)
+    def set_force_field(self, F) -> None:
+        """Per-node body force [N, 3] (float32, lattice units); None removes it."""
+        if F is None:
+            _check(load().hyb_lb_set_force_field(self._lb, None), "hyb_lb_set_force_field")
+            return
+        a = np.ascontiguousarray(F, dtype=np.float32).reshape(-1, 3)
+        _check(load().hyb_lb_set_force_field(self._lb, a.ctypes.data), "hyb_lb_set_force_field")
+
+    def set_populations(self, f) -> None:
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape(-1, 19)
+        _check(load().hyb_lb_set_populations(self._lb, a.ctypes.data), "hyb_lb_set_populations")
+
+    def adj_forward(self, steps: int, every: int) -> None:
+        _check(load().hyb_adj_forward(self._lb, self._dem, int(steps), int(every)), "hyb_adj_forward")
+
+    def adj_backward(self, seed):
+        """Returns (dL/dF [N,3], dL/dtau, dL/df0 [N,19]) for seed = dL/df_T [N,19]."""
+        sd = np.ascontiguousarray(seed, dtype=np.float64).reshape(-1, 19)
+        n = sd.shape[0]
+        gF = np.zeros((n, 3))
+        gT = np.zeros(1)
+        g0 = np.zeros((n, 19))
+        _check(load().hyb_adj_backward(self._lb, self._dem, sd.ctypes.data, gF.ctypes.data, gT.ctypes.data,
+                                       g0.ctypes.data), "hyb_adj_backward")
+        return gF, float(gT[0]), g0
+
+    def step_lb(self, n: int = 1) -> None:
+        """LB steps only (no DEM sub-steps, no coupling): particles stay frozen."""
+        L = load()
+        for _ in range(int(n)):
+            _check(L.hyb_lb_step(self._lb, self._dem), "hyb_lb_step")
 
     def active_nodes(self) -> int:
         n = C.c_int64(0)

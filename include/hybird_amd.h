@@ -21,6 +21,8 @@
  *   hyb_lb_step          LB::latticeBolzmannStep                                    (LB.h:159, LB.cpp:221-233)
  *   hyb_cycle            goCycle without the IO call                                (hybird.cpp:35-66)
  *   hyb_lb_set_capture   keeps the node fields IO reads (IO.cpp:711-831) while stepping
+ *   hyb_lb_set_force_field, hyb_lb_set_populations, hyb_adj_forward, hyb_adj_backward
+ *                        discrete adjoint of the LB step (not in the reference)
  *   hyb_lb_get / hyb_dem_get   host views of LB::nodes/types and DEM::elmts/particles read by IO
  *                        (IO.cpp:711-831, 835-1106)
  * The reference has no foreign-function interface; a C++ caller reaches these
@@ -161,6 +163,24 @@ int hyb_lb_kernel_time(hyb_lb* lb, double* total_ms, int64_t* launches);
 int hyb_lb_set_capture(hyb_lb* lb, int32_t enabled);
 /* Active (fluid + interface) node count, the MLUPS denominator. */
 int hyb_lb_active_nodes(hyb_lb* lb, int64_t* count);
+
+/* ---- extensions for the adjoint (no counterpart in the reference) ----
+ * Per-node body force F[N][3] in lattice units, added to lbF in every
+ * collision (NULL removes it). */
+int hyb_lb_set_force_field(hyb_lb* lb, const float* F);
+/* Replaces the initial populations f[N][19] (post-stream convention of the
+ * "f" view); only before the first step. */
+int hyb_lb_set_populations(hyb_lb* lb, const double* f);
+/* Runs `steps` LB steps from the initial state (particles frozen: no DEM or
+ * coupling steps) and keeps the state every `every` steps for the reverse pass
+ * (FP64, no free surface, constant viscosity). */
+int hyb_adj_forward(hyb_lb* lb, hyb_dem* dem, int32_t steps, int32_t every);
+/* Gradients of a loss L(f_T) of the final post-stream populations, given
+ * seed = dL/df_T [N][19]: grad_force [N][3] (per-node body force), *grad_tau
+ * (relaxation time tau = 1/2 + 3 nu), grad_f0 [N][19] (initial populations).
+ * Any output may be NULL. Recomputes the states between checkpoints. */
+int hyb_adj_backward(hyb_lb* lb, hyb_dem* dem, const double* seed, double* grad_force, double* grad_tau,
+                     double* grad_f0);
 
 #ifdef __cplusplus
 }

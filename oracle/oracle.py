@@ -45,6 +45,8 @@ def lib():
         _lib.orc_destroy.argtypes = [C.c_void_p]
         _lib.orc_cycle.argtypes = [C.c_void_p, C.c_int]
         _lib.orc_cycle.restype = C.c_int
+        _lib.orc_lb_step.argtypes = [C.c_void_p, C.c_int]
+        _lib.orc_lb_step.restype = C.c_int
         _lib.orc_error.argtypes = [C.c_void_p]
         _lib.orc_error.restype = C.c_char_p
         _lib.orc_get.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_long]
@@ -67,6 +69,12 @@ class OracleSim:
         rc = lib().orc_cycle(self._h, int(n))
         if rc != 0:
             raise RuntimeError(f"oracle cycle failed ({rc}): {lib().orc_error(self._h).decode()}")
+
+    def lb_step(self, n: int = 1) -> None:
+        """LB steps only: particles frozen (no DEM sub-steps, no coupling)."""
+        rc = lib().orc_lb_step(self._h, int(n))
+        if rc != 0:
+            raise RuntimeError(f"oracle LB step failed ({rc}): {lib().orc_error(self._h).decode()}")
 
     def get(self, name: str) -> np.ndarray:
         L = lib()

@@ -79,6 +79,17 @@ int orc_cycle(void* h, int n) {
     return 0;
 }
 
+// LB steps alone (LB.cpp:221-233): no DEM sub-steps and no coupling step, so
+// the particles stay where they are (the frozen-bed model of the adjoint).
+int orc_lb_step(void* h, int n) {
+    Sim* s = static_cast<Sim*>(h);
+    for (int k = 0; k < n; ++k) {
+        s->lb.step(s->dem);
+        if (s->lb.error) return s->lb.error;
+    }
+    return 0;
+}
+
 const char* orc_error(void* h) {
     Sim* s = static_cast<Sim*>(h);
     return s->dem.error ? s->dem.errmsg.c_str() : s->lb.errmsg.c_str();
