@@ -113,6 +113,58 @@ def cpu_baseline(size_sub: int, cycles: int):
         return {"value": round(act / dt / 1e6, 3), "unit": "MLUPS", "cores": 1, "kind": "port", "sample": sample}
 
 
+def adjoint_cost(size: int = 128, steps: int = 1000, every: int = 50, count: int = 2000, seed: int = 3):
+    """Cost of the hand-written adjoint against the forward on a frozen bed: 128^3 D3Q19 TRT tau 0.7,
+    2000 spheres of radius 4 (seed 3) at rest, body force 1e-6 x plus a per-node Gaussian field
+    (sigma 1e-7, float32), loss = mean x-momentum through the plane x = 120 after 1000 steps, state
+    checkpoints every 50 steps. Gradients w.r.t. the force field, tau and the initial populations."""
+    tau = 0.7
+    cfg = base_config(lbSizeX=size, lbSizeY=size, lbSizeZ=size, initVisc=(tau - 0.5) / 3.0, lbFX=1.0e-6,
+                      boundary0=4, boundary1=4, boundary2=4, boundary3=4, boundary4=7, boundary5=7, density=2.5)
+    span = size - 2.0
+    bed = rsa_bed(seed, (0.5, 0.5, 0.5), (0.5 + span, 0.5 + span, 0.5 + span), lambda r: 4.0, count,
+                  periodic=(True, True, False), period=(span, span, 0.0))
+    conf = config_from_cfg(cfg, collision="TRT", precision="FP64")
+    rec = element_records(bed.x, bed.r)
+    N = size ** 3
+    ff = np.random.default_rng(seed).normal(0.0, 1.0e-7, size=(N, 3)).astype(np.float32)
+    # plain forward of the same steps
+    plain = runtime.Simulation(conf, rec)
+    plain.set_force_field(ff)
+    plain.sync()
+    t0 = time.perf_counter()
+    plain.step_lb(steps)
+    plain.sync()
+    t_plain = time.perf_counter() - t0
+    plain.close()
+    sim = runtime.Simulation(conf, rec)
+    sim.set_force_field(ff)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.adj_forward(steps, every)
+    sim.sync()
+    t_fwd = time.perf_counter() - t0
+    fT = sim.get("f")
+    types = sim.get("types")
+    act = (types == 0) | (types == 3)
+    sel = act & ((np.arange(N) % size) == 120)
+    cxv = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1], dtype=np.float64)
+    sd = np.zeros((N, 19))
+    sd[sel] = cxv[None, :] / sel.sum()
+    loss = float(np.sum(sd[sel] * fT[sel]))
+    t0 = time.perf_counter()
+    gF, gtau, g0 = sim.adj_backward(sd)
+    t_bwd = time.perf_counter() - t0
+    sim.close()
+    return {"workload": f"frozen bed {size}^3 TRT tau {tau}, {bed.count} spheres r=4 (seed {seed}), "
+                        f"loss = mean x-momentum at x=120 after {steps} steps, checkpoints every {every}",
+            "steps": steps, "loss": loss, "forward_ms_per_step": round(t_plain / steps * 1e3, 4),
+            "forward_with_checkpoints_ms_per_step": round(t_fwd / steps * 1e3, 4),
+            "backward_ms_per_step": round(t_bwd / steps * 1e3, 4),
+            "backward_over_forward": round(t_bwd / t_plain, 3),
+            "grad_tau": gtau, "grad_force_norm": float(np.linalg.norm(gF)), "grad_f0_norm": float(np.linalg.norm(g0))}
+
+
 def copy_probe(nbytes: int = 1 << 31, reps: int = 5) -> int:
     """A plain device-to-device copy of nbytes, reps times: the reference point that
     scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -136,6 +188,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-size", type=int, default=64)
     ap.add_argument("--cpu-cycles", type=int, default=150)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of known size (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -199,6 +252,9 @@ def main() -> int:
                      "traffic": TRAFFIC_FUSED32_256 if args.size == 256 else None,
                      "kernel": "k_fused32 (TRT, pull)", "bytes_per_lup": BYTES_PER_LUP["fp32"]},
     }
+    if not args.no_adjoint:
+        print("[bench] adjoint ...", file=sys.stderr, flush=True)
+        line["adjoint"] = adjoint_cost()
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(args.cpu_size, args.cpu_cycles)

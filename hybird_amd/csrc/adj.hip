@@ -27,9 +27,22 @@ namespace hyb {
 namespace {
 
 constexpr int ABX = 256;
+// at least this many workgroups per CU: caps the registers so that several
+// wavefronts per SIMD hide the latency of the 38 neighbour loads
+#ifndef ADJ_MIN_BLOCKS
+#define ADJ_MIN_BLOCKS 2
+#endif
 
+// Neighbour index along direction j (periodic rewiring) as a 32-bit offset.
+__device__ __forceinline__ uint32_t nb32(const Nb3& nx, const Nb3& ny, const Nb3& nz, uint32_t sX, uint32_t sXY, int j) {
+    return (uint32_t)pick(nx, cx(j)) + sX * (uint32_t)pick(ny, cy(j)) + sXY * (uint32_t)pick(nz, cz(j));
+}
+
+// The vector-Jacobian product runs over the 9 direction pairs (i, opp i = i + 1)
+// and the rest direction, using c_opp = -c_i and w_opp = w_i, so that only the
+// incoming populations and the adjoint populations stay in registers.
 template <int COLL>
-__global__ void __launch_bounds__(ABX)
+__global__ void __launch_bounds__(ABX, ADJ_MIN_BLOCKS)
     k_adj_step(const double* __restrict__ sPrev, const double* __restrict__ lamNext, double* __restrict__ lamOut,
                const uint32_t* __restrict__ link, const uint32_t* __restrict__ solid, const CoupleRec* __restrict__ cp,
                double* __restrict__ gradF, double* __restrict__ tauPart, LBKParams p, int pullFwd) {
@@ -39,32 +52,30 @@ __global__ void __launch_bounds__(ABX)
     const size_t N = (size_t)p.X * p.Y * p.Z;
     double tauAcc = 0.0;
     const bool inRange = x < p.X;
-    const size_t self = inRange ? (size_t)x + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z) : 0;
+    const uint32_t self = inRange ? (uint32_t)x + (uint32_t)p.X * ((uint32_t)y + (uint32_t)p.Y * (uint32_t)z) : 0u;
     const uint32_t lw = inRange ? link[self] : 0u;
     if (lw & LINK_ACTIVE) {
-        Coord c;
-        c.x = x; c.y = y; c.z = z; c.idx = self;
-        size_t nb[19];
+        const Nb3 nx = nb3(x, p.X, p.per[0]), ny = nb3(y, p.Y, p.per[1]), nz = nb3(z, p.Z, p.per[2]);
+        const uint32_t sX = (uint32_t)p.X, sXY = (uint32_t)p.X * (uint32_t)p.Y;
+        // g_t: populations entering this node's collision; f_i arrives from the
+        // neighbour in direction opp(i) (or bounces back from a static wall there)
+        double g[19], m[19];
+        if (pullFwd) {
+            // the 19 loads back to back; a static-wall neighbour's index is valid
+            // memory and its value is replaced by the bounce-back below
 #pragma unroll
-        for (int j = 1; j < 19; ++j) nb[j] = nb_index(p, c, j);
-        // g_t: populations entering this node's collision
-        double g[19], mu[19];
-        g[0] = sPrev[self];
+            for (int i = 0; i < 19; ++i)
+                g[i] = sPrev[(size_t)i * p.PS + (i == 0 ? self : nb32(nx, ny, nz, sX, sXY, opp(i)))];
+            if (lw & LINK_WALLS) {
 #pragma unroll
-        for (int i = 1; i < 19; ++i) {
-            const int j = opp(i);  // f_i arrives from the neighbour in direction j = opp(i)
-            if (!pullFwd) g[i] = sPrev[(size_t)i * p.PS + self];
-            else if (lw & (1u << j)) g[i] = sPrev[(size_t)j * p.PS + self];
-            else g[i] = sPrev[(size_t)i * p.PS + nb[j]];
+                for (int i = 1; i < 19; ++i)
+                    if (lw & (1u << opp(i))) g[i] = sPrev[(size_t)opp(i) * p.PS + self];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) g[i] = sPrev[(size_t)i * p.PS + self];
         }
-        // mu_t = P^T lambda_{t+1}
-        mu[0] = lamNext[self];
-#pragma unroll
-        for (int j = 1; j < 19; ++j) {
-            if (lw & (1u << j)) mu[j] = lamNext[(size_t)opp(j) * p.PS + self];
-            else mu[j] = lamNext[(size_t)j * p.PS + nb[j]];
-        }
-        // ---- forward intermediates of C (same formulas as the fused kernel)
+        // ---- forward intermediates of C (the fused kernel's formulas)
         double n = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
@@ -95,34 +106,74 @@ __global__ void __launch_bounds__(ABX)
         const double tau = 1.0 / omP;
         const double omM = COLL == HYB_COLLISION_TRT ? 1.0 / (0.5 + p.trtMagic / (tau - 0.5)) : omP;
         const double q = us.norm2(), rr = us.dot(T);
-        double a[19], b[19], feq[19];
+        const double hp = 0.5 * omP, hm = 0.5 * omM;
+        // d(s)/d(omP) and d(s)/d(omM) per direction pair (even / odd parts); g is
+        // no longer needed after this, which keeps the adjoint loads below in registers
+        double ev[10], od[10];
+        ev[0] = -(g[0] - wgt(0) * n * (1.0 - 1.5 * q)) - 0.5 * (wgt(0) * -3.0 * rr);
+        od[0] = 0.0;
 #pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            a[i] = (double)cx(i) * us.x + (double)cy(i) * us.y + (double)cz(i) * us.z;
-            b[i] = (double)cx(i) * T.x + (double)cy(i) * T.y + (double)cz(i) * T.z;
-            feq[i] = wgt(i) * n * (1.0 + 3.0 * a[i] + 4.5 * a[i] * a[i] - 1.5 * q);
+        for (int i = 1; i < 19; i += 2) {
+            const int o = i + 1;
+            const double w = wgt(i);
+            const double a = (double)cx(i) * us.x + (double)cy(i) * us.y + (double)cz(i) * us.z;
+            const double b = (double)cx(i) * T.x + (double)cy(i) * T.y + (double)cz(i) * T.z;
+            const double base = 1.0 + 4.5 * a * a - 1.5 * q;
+            const double fei = w * n * (base + 3.0 * a), feo = w * n * (base - 3.0 * a);
+            ev[(i + 1) / 2] = -(0.5 * (g[i] + g[o]) - 0.5 * (fei + feo)) - 0.5 * (w * (9.0 * a * b - 3.0 * rr));
+            od[(i + 1) / 2] = -(0.5 * (g[i] - g[o]) - 0.5 * (fei - feo)) - 0.5 * (w * 3.0 * b);
+        }
+        // Opaque copies of the velocity and force: the reverse loop recomputes c.us
+        // and c.T per pair instead of keeping the values of the loop above alive.
+        double usx = us.x, usy = us.y, usz = us.z, Tx = T.x, Ty = T.y, Tz = T.z;
+        asm volatile("" : "+v"(usx), "+v"(usy), "+v"(usz), "+v"(Tx), "+v"(Ty), "+v"(Tz));
+        // mu_t = P^T lambda_{t+1}: plane j from the neighbour in direction j
+#pragma unroll
+        for (int j = 0; j < 19; ++j)
+            m[j] = lamNext[(size_t)j * p.PS + (j == 0 ? self : nb32(nx, ny, nz, sX, sXY, j))];
+        if (lw & LINK_WALLS) {
+#pragma unroll
+            for (int j = 1; j < 19; ++j)
+                if (lw & (1u << j)) m[j] = lamNext[(size_t)opp(j) * p.PS + self];
         }
         // ---- reverse
-        double gb[19];
         double nbar = 0.0, qbar = 0.0, rbar = 0.0, omPbar = 0.0, omMbar = 0.0;
         V3 usbar, Tbar;
+        {  // rest direction: c = 0, so a = b = 0, no odd part
+            const double w = wgt(0), mi = m[0];
+            const double feqb = mi * omP, spb = mi * (1.0 - hp);
+            omPbar += mi * ev[0];
+            nbar += feqb * w * (1.0 - 1.5 * q);
+            qbar += feqb * w * n * -1.5;
+            rbar += spb * w * -3.0;
+            m[0] = mi * (1.0 - omP);  // adjoint of g_0 through the relaxation
+        }
 #pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            const int o = opp(i);
-            const double mi = mu[i], mo = mu[o];
-            gb[i] = mi * (1.0 - 0.5 * omP - 0.5 * omM) + mo * (0.5 * omM - 0.5 * omP);
-            const double feqb = mi * (0.5 * omP + 0.5 * omM) + mo * (0.5 * omP - 0.5 * omM);
-            const double spb = mi * (1.0 - 0.5 * omP), smb = mi * (1.0 - 0.5 * omM);
-            const double sp = wgt(i) * (9.0 * a[i] * b[i] - 3.0 * rr), sm = wgt(i) * 3.0 * b[i];
-            omPbar += mi * (-(0.5 * (g[i] + g[o]) - 0.5 * (feq[i] + feq[o])) - 0.5 * sp);
-            omMbar += mi * (-(0.5 * (g[i] - g[o]) - 0.5 * (feq[i] - feq[o])) - 0.5 * sm);
-            nbar += feqb * wgt(i) * (1.0 + 3.0 * a[i] + 4.5 * a[i] * a[i] - 1.5 * q);
-            qbar += feqb * wgt(i) * n * -1.5;
-            const double ab = feqb * wgt(i) * n * (3.0 + 9.0 * a[i]) + spb * wgt(i) * 9.0 * b[i];
-            const double bb = spb * wgt(i) * 9.0 * a[i] + smb * wgt(i) * 3.0;
-            rbar += spb * wgt(i) * -3.0;
-            usbar += V3((double)cx(i) * ab, (double)cy(i) * ab, (double)cz(i) * ab);
-            Tbar += V3((double)cx(i) * bb, (double)cy(i) * bb, (double)cz(i) * bb);
+        for (int i = 1; i < 19; i += 2) {
+            const int o = i + 1;  // opp(i)
+            const double w = wgt(i);
+            const double mi = m[i], mo = m[o];
+            const double a = (double)cx(i) * usx + (double)cy(i) * usy + (double)cz(i) * usz;
+            const double b = (double)cx(i) * Tx + (double)cy(i) * Ty + (double)cz(i) * Tz;
+            const double base = 1.0 + 4.5 * a * a - 1.5 * q;
+            const double feqbi = mi * (hp + hm) + mo * (hp - hm), feqbo = mo * (hp + hm) + mi * (hp - hm);
+            const double spbi = mi * (1.0 - hp), spbo = mo * (1.0 - hp);
+            const double smbi = mi * (1.0 - hm), smbo = mo * (1.0 - hm);
+            omPbar += (mi + mo) * ev[(i + 1) / 2];
+            omMbar += (mi - mo) * od[(i + 1) / 2];
+            nbar += feqbi * w * (base + 3.0 * a) + feqbo * w * (base - 3.0 * a);
+            qbar += (feqbi + feqbo) * w * n * -1.5;
+            rbar += (spbi + spbo) * w * -3.0;
+            const double abi = feqbi * w * n * (3.0 + 9.0 * a) + spbi * w * 9.0 * b;
+            const double abo = feqbo * w * n * (-3.0 + 9.0 * a) - spbo * w * 9.0 * b;
+            const double bbi = spbi * w * 9.0 * a + smbi * w * 3.0;
+            const double bbo = -spbo * w * 9.0 * a + smbo * w * 3.0;
+            const double da = abi - abo, db = bbi - bbo;
+            usbar += V3((double)cx(i) * da, (double)cy(i) * da, (double)cz(i) * da);
+            Tbar += V3((double)cx(i) * db, (double)cy(i) * db, (double)cz(i) * db);
+            // adjoint of g through the relaxation (the moments' part is added below)
+            m[i] = mi * (1.0 - hp - hm) + mo * (hm - hp);
+            m[o] = mo * (1.0 - hp - hm) + mi * (hm - hp);
         }
         usbar += (2.0 * qbar) * us + rbar * T;
         Tbar += rbar * us;
@@ -151,7 +202,7 @@ __global__ void __launch_bounds__(ABX)
 #pragma unroll
         for (int i = 0; i < 19; ++i)
             lamOut[(size_t)i * p.PS + self] =
-                gb[i] + nbar + (double)cx(i) * jbar.x + (double)cy(i) * jbar.y + (double)cz(i) * jbar.z;
+                m[i] + nbar + (double)cx(i) * jbar.x + (double)cy(i) * jbar.y + (double)cz(i) * jbar.z;
     }
     // workgroup sum of the tau gradient in a fixed order
     for (int off = 32; off > 0; off >>= 1) tauAcc += __shfl_xor(tauAcc, off, 64);
@@ -165,7 +216,30 @@ __global__ void __launch_bounds__(ABX)
     }
 }
 
+// [N][Q] node-major <-> [Q][PS] plane-major (inactive nodes read as zero on the way out)
+__global__ void k_nodes_to_planes(const double* __restrict__ src, double* __restrict__ dst, size_t N, size_t PS, int Q) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N * Q; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t node = i / Q, q = i % Q;
+        dst[q * PS + node] = src[i];
+    }
+}
+__global__ void k_planes_to_nodes(const double* __restrict__ src, double* __restrict__ dst, const uint32_t* __restrict__ link,
+                                  size_t N, size_t PS, int Q) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N * Q; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t node = i / Q, q = i % Q;
+        dst[i] = (!link || (link[node] & LINK_ACTIVE)) ? src[q * PS + node] : 0.0;
+    }
+}
+
 }  // namespace
+
+void launch_nodes_to_planes(const double* src, double* dst, size_t N, size_t PS, int Q, hipStream_t s) {
+    k_nodes_to_planes<<<4096, 256, 0, s>>>(src, dst, N, PS, Q);
+}
+void launch_planes_to_nodes(const double* src, double* dst, const uint32_t* link, size_t N, size_t PS, int Q,
+                            hipStream_t s) {
+    k_planes_to_nodes<<<4096, 256, 0, s>>>(src, dst, link, N, PS, Q);
+}
 
 size_t adj_tau_slots(const LBKParams& p) { return (size_t)((p.X + ABX - 1) / ABX) * p.Y * p.Z; }
 

@@ -19,4 +19,10 @@ void launch_adj_step(int collision, const double* sPrev, const double* lamNext, 
                      const uint32_t* solid, const CoupleRec* cp, double* gradF, double* tauPart, bool pullFwd,
                      const LBKParams& p, hipStream_t s);
 
+// Layout changes between [N][Q] (node-major, host views) and [Q][PS] (planes);
+// planes_to_nodes writes zero for inactive nodes when link is given.
+void launch_nodes_to_planes(const double* src, double* dst, size_t N, size_t PS, int Q, hipStream_t s);
+void launch_planes_to_nodes(const double* src, double* dst, const uint32_t* link, size_t N, size_t PS, int Q,
+                            hipStream_t s);
+
 }  // namespace hyb

@@ -854,12 +854,10 @@ int hyb_adj_backward(hyb_lb* l, hyb_dem* d, const double* seed, double* grad_for
         cleanup();
         return fail(HYB_E_HIP, "adjoint buffer allocation failed");
     }
-    {
-        std::vector<double> pl(19 * PS, 0.0);
-        for (size_t i = 0; i < N; ++i)
-            for (int j = 0; j < 19; ++j) pl[(size_t)j * PS + i] = seed[i * 19 + j];
-        HIPCHK(hipMemcpy(lam[0], pl.data(), pb, hipMemcpyHostToDevice));
-    }
+    // the seed goes through lam[1] (node-major) into lam[0] (planes)
+    HIPCHK(hipMemsetAsync(lam[0], 0, pb, s));
+    HIPCHK(hipMemcpyAsync(lam[1], seed, 19 * 8 * N, hipMemcpyHostToDevice, s));
+    launch_nodes_to_planes(lam[1], lam[0], N, PS, 19, s);
     HIPCHK(hipMemsetAsync(lam[1], 0, pb, s));
     HIPCHK(hipMemsetAsync(gF, 0, 24 * N, s));
     HIPCHK(hipMemsetAsync(tp, 0, 8 * slots, s));
@@ -880,20 +878,15 @@ int hyb_adj_backward(hyb_lb* l, hyb_dem* d, const double* seed, double* grad_for
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
-    if (grad_f0) {
-        std::vector<double> pl(19 * PS);
-        std::vector<uint32_t> lk(N);
-        HIPCHK(hipMemcpy(pl.data(), lam[cur], pb, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(lk.data(), l->link, N * 4, hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < N; ++i)
-            for (int j = 0; j < 19; ++j) grad_f0[i * 19 + j] = (lk[i] & LINK_ACTIVE) ? pl[(size_t)j * PS + i] : 0.0;
+    if (grad_f0) {  // node-major copy in the free adjoint buffer
+        launch_planes_to_nodes(lam[cur], lam[cur ^ 1], l->link, N, PS, 19, s);
+        HIPCHK(hipMemcpyAsync(grad_f0, lam[cur ^ 1], 19 * 8 * N, hipMemcpyDeviceToHost, s));
     }
     if (grad_force) {
-        std::vector<double> g(3 * N);
-        HIPCHK(hipMemcpy(g.data(), gF, 24 * N, hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < N; ++i)
-            for (int k = 0; k < 3; ++k) grad_force[3 * i + k] = g[(size_t)k * N + i];
+        launch_planes_to_nodes(gF, lam[cur], nullptr, N, N, 3, s);
+        HIPCHK(hipMemcpyAsync(grad_force, lam[cur], 24 * N, hipMemcpyDeviceToHost, s));
     }
+    HIPCHK(hipStreamSynchronize(s));
     if (grad_tau) {
         std::vector<double> t(slots);
         HIPCHK(hipMemcpy(t.data(), tp, 8 * slots, hipMemcpyDeviceToHost));
