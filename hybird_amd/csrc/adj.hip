@@ -32,6 +32,26 @@ constexpr int ABX = 256;
 #ifndef ADJ_MIN_BLOCKS
 #define ADJ_MIN_BLOCKS 2
 #endif
+// non-temporal loads of the state and adjoint populations and stores of the
+// new adjoint populations (each is touched once per step)
+#ifndef ADJ_NT
+#define ADJ_NT 1
+#endif
+
+__device__ __forceinline__ double ld_nt(const double* p) {
+#if ADJ_NT
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+__device__ __forceinline__ void st_nt(double* p, double v) {
+#if ADJ_NT
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
 
 // Neighbour index along direction j (periodic rewiring) as a 32-bit offset.
 __device__ __forceinline__ uint32_t nb32(const Nb3& nx, const Nb3& ny, const Nb3& nz, uint32_t sX, uint32_t sXY, int j) {
@@ -65,7 +85,7 @@ __global__ void __launch_bounds__(ABX, ADJ_MIN_BLOCKS)
             // memory and its value is replaced by the bounce-back below
 #pragma unroll
             for (int i = 0; i < 19; ++i)
-                g[i] = sPrev[(size_t)i * p.PS + (i == 0 ? self : nb32(nx, ny, nz, sX, sXY, opp(i)))];
+                g[i] = ld_nt(&sPrev[(size_t)i * p.PS + (i == 0 ? self : nb32(nx, ny, nz, sX, sXY, opp(i)))]);
             if (lw & LINK_WALLS) {
 #pragma unroll
                 for (int i = 1; i < 19; ++i)
@@ -130,7 +150,7 @@ __global__ void __launch_bounds__(ABX, ADJ_MIN_BLOCKS)
         // mu_t = P^T lambda_{t+1}: plane j from the neighbour in direction j
 #pragma unroll
         for (int j = 0; j < 19; ++j)
-            m[j] = lamNext[(size_t)j * p.PS + (j == 0 ? self : nb32(nx, ny, nz, sX, sXY, j))];
+            m[j] = ld_nt(&lamNext[(size_t)j * p.PS + (j == 0 ? self : nb32(nx, ny, nz, sX, sXY, j))]);
         if (lw & LINK_WALLS) {
 #pragma unroll
             for (int j = 1; j < 19; ++j)
@@ -165,7 +185,7 @@ __global__ void __launch_bounds__(ABX, ADJ_MIN_BLOCKS)
             qbar += (feqbi + feqbo) * w * n * -1.5;
             rbar += (spbi + spbo) * w * -3.0;
             const double abi = feqbi * w * n * (3.0 + 9.0 * a) + spbi * w * 9.0 * b;
-            const double abo = feqbo * w * n * (-3.0 + 9.0 * a) - spbo * w * 9.0 * b;
+            const double abo = feqbo * w * n * (3.0 - 9.0 * a) - spbo * w * 9.0 * b;  // a_o = -a, b_o = -b
             const double bbi = spbi * w * 9.0 * a + smbi * w * 3.0;
             const double bbo = -spbo * w * 9.0 * a + smbo * w * 3.0;
             const double da = abi - abo, db = bbi - bbo;
@@ -201,8 +221,8 @@ __global__ void __launch_bounds__(ABX, ADJ_MIN_BLOCKS)
         nbar += -ubar.dot(u) / n;
 #pragma unroll
         for (int i = 0; i < 19; ++i)
-            lamOut[(size_t)i * p.PS + self] =
-                m[i] + nbar + (double)cx(i) * jbar.x + (double)cy(i) * jbar.y + (double)cz(i) * jbar.z;
+            st_nt(&lamOut[(size_t)i * p.PS + self],
+                  m[i] + nbar + (double)cx(i) * jbar.x + (double)cy(i) * jbar.y + (double)cz(i) * jbar.z);
     }
     // workgroup sum of the tau gradient in a fixed order
     for (int off = 32; off > 0; off >>= 1) tauAcc += __shfl_xor(tauAcc, off, 64);

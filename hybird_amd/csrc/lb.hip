@@ -35,6 +35,10 @@ namespace {
 #ifndef HYB_LB_NT
 #define HYB_LB_NT 1
 #endif
+// FP32 path: two nodes per thread with 8-byte population accesses
+#ifndef HYB_FP32_X2
+#define HYB_FP32_X2 0
+#endif
 constexpr int BX = HYB_LB_BX;
 
 constexpr int BY = HYB_LB_BY;
@@ -329,26 +333,17 @@ __device__ __forceinline__ float cdotf(int j, float ux, float uy, float uz) {
     return r;
 }
 
-template <bool PULL, bool VARVISC, int COLL, bool FS>
-__global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ fin, float* __restrict__ fout,
-                                                     const uint32_t* __restrict__ link,
-                                                     const uint32_t* __restrict__ solid,
-                                                     const CoupleRecF* __restrict__ cp, float* __restrict__ hf,
-                                                     double* __restrict__ visc, double* __restrict__ mac,
-                                                     LBKParams p, FSView fsv) {
+// Collision of one node in the FP32 deviation form: moments, penalty force,
+// Guo shift, [Bingham/Smagorinsky], BGK/TRT with Guo forcing. Writes the side
+// outputs (node force, macroscopic fields, viscosity, free-surface state) and,
+// when out is given, the post-collision populations into out[]; returns whether
+// out was written.
+template <bool VARVISC, int COLL, bool FS>
+__device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uint32_t lw, size_t N,
+                                          const uint32_t* __restrict__ solid, const CoupleRecF* __restrict__ cp,
+                                          float* __restrict__ hf, double* __restrict__ visc, double* __restrict__ mac,
+                                          const LBKParams& p, const FSView& fsv, bool doOut, float out[19]) {
 #pragma clang fp contract(fast)
-    Coord c;
-    if (!node_coord(p, c)) return;
-    const size_t N = (size_t)p.X * p.Y * p.Z;
-    const uint32_t lw = link[c.idx];
-    if (!(lw & LINK_ACTIVE)) return;
-    float g[19];
-    if (FS && (lw & LINK_PRESTREAMED)) load_prepared<float>(fsv, p.PS, c.idx, lw, g);
-    else if (PULL) pull19<float>(fin, p.PS, p, c, lw, g);
-    else {
-#pragma unroll
-        for (int j = 0; j < 19; ++j) g[j] = ld_pop(&fin[(size_t)j * p.PS + c.idx]);
-    }
     float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
 #pragma unroll
     for (int j = 0; j < 19; ++j) {
@@ -401,7 +396,7 @@ __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ f
         mac[5 * N + c.idx] = hy;
         mac[6 * N + c.idx] = hz;
     }
-    if (!fout) return;
+    if (!doOut) return false;
     if (FS) {
         ((float*)fsv.rhoCur)[c.idx] = rho;
         if (lw & LINK_INTERFACE) {
@@ -450,7 +445,7 @@ __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ f
     {
         const float w0 = (float)wgt(0);
         const float geq0 = w0 * (drho - rho * base);
-        st_pop(&fout[c.idx], g[0] - omP * (g[0] - geq0) + kp * (w0 * (-3.0f * uT)));
+        out[0] = g[0] - omP * (g[0] - geq0) + kp * (w0 * (-3.0f * uT));
     }
     const float omM = (COLL == HYB_COLLISION_TRT) ? 1.0f / (0.5f + (float)p.trtMagic / (1.0f / omP - 0.5f)) : omP;
     const float km = 1.0f - 0.5f * omM;
@@ -468,8 +463,112 @@ __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ f
         const float sm = w * (3.0f * cF);
         const float relP = omP * (fpl - ep) - kp * sp;
         const float relM = omM * (fmi - em) - km * sm;
-        st_pop(&fout[(size_t)j * p.PS + c.idx], g[j] - relP - relM);
-        st_pop(&fout[(size_t)o * p.PS + c.idx], g[o] - relP + relM);
+        out[j] = g[j] - relP - relM;
+        out[o] = g[o] - relP + relM;
+    }
+    return true;
+}
+
+
+template <bool PULL, bool VARVISC, int COLL, bool FS>
+__global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ fin, float* __restrict__ fout,
+                                                     const uint32_t* __restrict__ link,
+                                                     const uint32_t* __restrict__ solid,
+                                                     const CoupleRecF* __restrict__ cp, float* __restrict__ hf,
+                                                     double* __restrict__ visc, double* __restrict__ mac,
+                                                     LBKParams p, FSView fsv) {
+    Coord c;
+    if (!node_coord(p, c)) return;
+    const size_t N = (size_t)p.X * p.Y * p.Z;
+    const uint32_t lw = link[c.idx];
+    if (!(lw & LINK_ACTIVE)) return;
+    float g[19];
+    if (FS && (lw & LINK_PRESTREAMED)) load_prepared<float>(fsv, p.PS, c.idx, lw, g);
+    else if (PULL) pull19<float>(fin, p.PS, p, c, lw, g);
+    else {
+#pragma unroll
+        for (int j = 0; j < 19; ++j) g[j] = ld_pop(&fin[(size_t)j * p.PS + c.idx]);
+    }
+    float out[19];
+    if (!collide32<VARVISC, COLL, FS>(g, c, lw, N, solid, cp, hf, visc, mac, p, fsv, fout != nullptr, out)) return;
+#pragma unroll
+    for (int j = 0; j < 19; ++j) st_pop(&fout[(size_t)j * p.PS + c.idx], out[j]);
+}
+
+// Two nodes per thread (x0 = 2 t, x0 + 1): the 19 pulls of both nodes come from
+// adjacent addresses, so each is one 8-byte load (4-byte aligned for the
+// x-shifted planes) and each store one aligned 8-byte store. Pairs that touch
+// the x ghost layers or an inactive node fall back to the per-node path.
+typedef float float2_a4 __attribute__((ext_vector_type(2), aligned(4)));
+typedef float float2_a8 __attribute__((ext_vector_type(2), aligned(8)));
+
+template <bool PULL, bool VARVISC, int COLL>
+__global__ void __launch_bounds__(BX / 2) k_fused32x2(const float* __restrict__ fin, float* __restrict__ fout,
+                                                      const uint32_t* __restrict__ link,
+                                                      const uint32_t* __restrict__ solid,
+                                                      const CoupleRecF* __restrict__ cp, float* __restrict__ hf,
+                                                      double* __restrict__ visc, LBKParams p) {
+    static_assert(BY == 1, "one lattice row per workgroup");
+    const int x0 = 2 * (int)(blockIdx.x * (BX / 2) + threadIdx.x);
+    const int y = blockIdx.y, z = blockIdx.z;
+    if (x0 >= p.X) return;
+    const size_t N = (size_t)p.X * p.Y * p.Z;
+    const FSView none{};
+    Coord ca;
+    ca.x = x0; ca.y = y; ca.z = z;
+    ca.idx = (size_t)x0 + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z);
+    Coord cb = ca;
+    cb.x = x0 + 1;
+    cb.idx = ca.idx + 1;
+    const uint32_t lwa = link[ca.idx];
+    const uint32_t lwb = x0 + 1 < p.X ? link[cb.idx] : 0u;
+    const bool vec = PULL && x0 >= 2 && x0 + 1 <= p.X - 3 && (lwa & LINK_ACTIVE) && (lwb & LINK_ACTIVE);
+    if (!vec) {
+        for (int k = 0; k < 2; ++k) {
+            const Coord& c = k ? cb : ca;
+            const uint32_t lw = k ? lwb : lwa;
+            if (c.x >= p.X || !(lw & LINK_ACTIVE)) continue;
+            float g[19], out[19];
+            if (PULL) pull19<float>(fin, p.PS, p, c, lw, g);
+            else {
+#pragma unroll
+                for (int j = 0; j < 19; ++j) g[j] = ld_pop(&fin[(size_t)j * p.PS + c.idx]);
+            }
+            collide32<VARVISC, COLL, false>(g, c, lw, N, solid, cp, hf, visc, nullptr, p, none, true, out);
+#pragma unroll
+            for (int j = 0; j < 19; ++j) st_pop(&fout[(size_t)j * p.PS + c.idx], out[j]);
+        }
+        return;
+    }
+    const Nb3 ny = nb3(y, p.Y, p.per[1]);
+    const Nb3 nz = nb3(z, p.Z, p.per[2]);
+    const uint32_t sX = (uint32_t)p.X, sXY = (uint32_t)p.X * (uint32_t)p.Y;
+    float ga[19], gb[19];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const int j = opp(i);  // pulled from the neighbour in direction j
+        const uint32_t src = (uint32_t)(x0 + cx(j)) + sX * (uint32_t)pick(ny, cy(j)) + sXY * (uint32_t)pick(nz, cz(j));
+        const float2_a4 v = __builtin_nontemporal_load((const float2_a4*)&fin[(size_t)i * p.PS + src]);
+        ga[i] = v.x;
+        gb[i] = v.y;
+    }
+    if ((lwa | lwb) & LINK_WALLS) {
+#pragma unroll
+        for (int i = 1; i < 19; ++i) {
+            const int j = opp(i);
+            if (lwa & (1u << j)) ga[i] = ld_pop(&fin[(size_t)j * p.PS + ca.idx]);
+            if (lwb & (1u << j)) gb[i] = ld_pop(&fin[(size_t)j * p.PS + cb.idx]);
+        }
+    }
+    float oa[19], ob[19];
+    collide32<VARVISC, COLL, false>(ga, ca, lwa, N, solid, cp, hf, visc, nullptr, p, none, true, oa);
+    collide32<VARVISC, COLL, false>(gb, cb, lwb, N, solid, cp, hf, visc, nullptr, p, none, true, ob);
+#pragma unroll
+    for (int j = 0; j < 19; ++j) {
+        float2_a8 v;
+        v.x = oa[j];
+        v.y = ob[j];
+        __builtin_nontemporal_store(v, (float2_a8*)&fout[(size_t)j * p.PS + ca.idx]);
     }
 }
 
@@ -653,8 +752,19 @@ __device__ __forceinline__ NodeBox slot_box(const CoupleRec& r, const LBKParams&
     return b;
 }
 
-// Visits the nodes of a slot's box with one wavefront: lanes run along x within
-// a row (16, 32 or 64 lanes per row, so loads coalesce) and a wavefront covers
+// Threads per particle slot in the classification and force-sum kernels: one
+// wavefront (64) or one workgroup of four wavefronts (256).
+#ifndef HYB_SLOT_GROUP
+#define HYB_SLOT_GROUP 64
+#endif
+constexpr int SG = HYB_SLOT_GROUP;
+// rows per batch of the force-sum iterator
+#ifndef HYB_SLOT_BATCH
+#define HYB_SLOT_BATCH 1
+#endif
+
+// Visits the nodes of a slot's box with SG threads: lanes run along x within a
+// row (16, 32 or 64 lanes per row, so loads coalesce) and the group covers
 // several rows per iteration. Rows whose (y, z) distance already reaches the
 // radius are skipped: rounded sums of non-negative squares are monotone, so no
 // node of such a row passes the strict inside test.
@@ -664,7 +774,7 @@ __device__ __forceinline__ void for_slot_nodes(const NodeBox& b, int lane, const
     if (nx <= 0 || ny <= 0 || nz <= 0) return;
     const int shift = nx <= 16 ? 4 : (nx <= 32 ? 5 : 6);
     const int lpr = 1 << shift;
-    const int rpi = 64 >> shift;
+    const int rpi = SG >> shift;
     const int xo = lane & (lpr - 1);
     const int rsub = lane >> shift;
     const int rows = ny * nz;
@@ -679,16 +789,63 @@ __device__ __forceinline__ void for_slot_nodes(const NodeBox& b, int lane, const
     }
 }
 
+// The nodes of a slot's box owned by the slot (ownership word == want), visited
+// in the same per-thread order as for_slot_nodes, but CH rows at a time: the CH
+// ownership loads of a batch are issued together, so a thread waits for memory
+// once per batch instead of once per row.
+template <int CH, class F>
+__device__ __forceinline__ void for_owned_nodes(const NodeBox& b, int lane, const V3& cen, double rad, uint32_t want,
+                                                const uint32_t* __restrict__ solid, const LBKParams& p, F&& f) {
+    const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return;
+    const int shift = nx <= 16 ? 4 : (nx <= 32 ? 5 : 6);
+    const int lpr = 1 << shift;
+    const int rpi = SG >> shift;
+    const int xo = lane & (lpr - 1);
+    const int rsub = lane >> shift;
+    const int rows = ny * nz;
+    for (int base = 0; base < rows; base += rpi * CH) {
+        for (int xx = xo; xx < nx; xx += lpr) {
+            size_t idx[CH];
+            int yy[CH], zz[CH];
+            bool ok[CH];
+#pragma unroll
+            for (int k = 0; k < CH; ++k) {
+                const int row = base + k * rpi + rsub;
+                ok[k] = false;
+                idx[k] = 0;
+                yy[k] = zz[k] = 0;
+                if (row < rows) {
+                    const int y = b.lo[1] + row % ny, z = b.lo[2] + row / ny;
+                    const double dy = (double)y - cen.y, dz = (double)z - cen.z;
+                    if (dy * dy + dz * dz < rad * rad) {
+                        ok[k] = true;
+                        yy[k] = y;
+                        zz[k] = z;
+                        idx[k] = (size_t)(b.lo[0] + xx) + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z);
+                    }
+                }
+            }
+            uint32_t w[CH];
+#pragma unroll
+            for (int k = 0; k < CH; ++k) w[k] = ok[k] ? solid[idx[k]] : 0u;
+#pragma unroll
+            for (int k = 0; k < CH; ++k)
+                if (ok[k] && w[k] == want) f(b.lo[0] + xx, yy[k], zz[k], idx[k]);
+        }
+    }
+}
+
 __global__ void k_classify_set(const int* __restrict__ nslots_p, const CoupleRec* __restrict__ cp,
                                NodeBox* __restrict__ boxes, int* __restrict__ nboxes, uint32_t* __restrict__ solid,
                                const uint32_t* __restrict__ link, LBKParams p) {
 #pragma clang fp contract(off)
     const int ns = *nslots_p;
     if (blockIdx.x == 0 && threadIdx.x == 0) *nboxes = ns;
-    const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (int s = wave; s < ns; s += nwaves) {
+    const int lane = SG == 64 ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+    const int unit = SG == 64 ? (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) : (int)blockIdx.x;
+    const int nunits = SG == 64 ? (int)((gridDim.x * blockDim.x) >> 6) : (int)gridDim.x;
+    for (int s = unit; s < ns; s += nunits) {
         const CoupleRec r = cp[s];
         const NodeBox b = slot_box(r, p);
         if (lane == 0) boxes[s] = b;
@@ -717,9 +874,9 @@ struct KSum {
 };
 
 // Per particle slot: sum over the nodes it owns of diffVel, radius x diffVel and
-// the node mass (LB.cpp:1897-1902). One wavefront per slot; each lane keeps
-// Neumaier-compensated FP64 sums and the lanes are merged by a fixed butterfly
-// (deterministic, no atomics). Only nodes strictly inside the sphere can be
+// the node mass (LB.cpp:1897-1902). SG threads per slot; each keeps
+// Neumaier-compensated FP64 sums, merged by a fixed butterfly within each
+// wavefront and then in wavefront order (deterministic, no atomics). Only nodes strictly inside the sphere can be
 // owned by the slot, so the others are skipped before any load.
 __device__ __forceinline__ void ksum_merge(KSum& a, double s2, double c2) {
     a.add(s2);
@@ -735,12 +892,14 @@ __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ 
                                                         const HT* __restrict__ fsRho,
                                                         HydroPartial* __restrict__ out, LBKParams p) {
 #pragma clang fp contract(off)
+    __shared__ double sh[4][7][2];
     const int ns = *nslots_p;
     const size_t N = (size_t)p.X * p.Y * p.Z;
-    const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (int s = wave; s < ns; s += nwaves) {
+    const int wl = threadIdx.x & 63;
+    const int lane = SG == 64 ? wl : (int)threadIdx.x;
+    const int unit = SG == 64 ? (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) : (int)blockIdx.x;
+    const int nunits = SG == 64 ? (int)((gridDim.x * blockDim.x) >> 6) : (int)gridDim.x;
+    for (int s = unit; s < ns; s += nunits) {
         const CoupleRec r = cp[s];
         const NodeBox b = boxes[s];
         const uint32_t want = (p.ctag << SOLID_SLOT_BITS) | (uint32_t)(s + 1);
@@ -750,9 +909,7 @@ __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ 
         const V3 rv(r.rv[0], r.rv[1], r.rv[2]);
         const V3 cen = x0 / p.unitLength;
         const double rad = r.r / p.unitLength;
-        for_slot_nodes(b, lane, cen, rad, [&](int x, int y, int z) {
-            const size_t idx = (size_t)x + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z);
-            if (solid[idx] != want) return;
+        for_owned_nodes<HYB_SLOT_BATCH>(b, lane, cen, rad, want, solid, p, [&](int x, int y, int z, size_t idx) {
             const V3 d((double)hf[idx], (double)hf[N + idx], (double)hf[2 * N + idx]);
             const V3 radius = V3((double)x, (double)y, (double)z) - x0 / p.unitLength + rv / p.unitLength;
             const V3 m = radius.cross(d);
@@ -768,7 +925,7 @@ __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ 
                 const double s2 = __shfl_xor(acc[k].s, off, 64);
                 const double c2 = __shfl_xor(acc[k].c, off, 64);
                 // both partners combine in the same (lower lane first) order
-                if (lane & off) {
+                if (wl & off) {
                     KSum a{s2, c2};
                     ksum_merge(a, acc[k].s, acc[k].c);
                     acc[k] = a;
@@ -777,8 +934,26 @@ __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ 
                 }
             }
         }
-        if (lane == 0)
-            for (int k = 0; k < 7; ++k) out[s].s[k] = acc[k].s + acc[k].c;
+        if (SG == 64) {
+            if (wl == 0)
+                for (int k = 0; k < 7; ++k) out[s].s[k] = acc[k].s + acc[k].c;
+        } else {
+            // the four wavefront sums, merged in wavefront order
+            if (wl == 0)
+                for (int k = 0; k < 7; ++k) {
+                    sh[threadIdx.x >> 6][k][0] = acc[k].s;
+                    sh[threadIdx.x >> 6][k][1] = acc[k].c;
+                }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                for (int k = 0; k < 7; ++k) {
+                    KSum a{sh[0][k][0], sh[0][k][1]};
+                    for (int w = 1; w < 4; ++w) ksum_merge(a, sh[w][k][0], sh[w][k][1]);
+                    out[s].s[k] = a.s + a.c;
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -805,6 +980,9 @@ dim3 lattice_grid(const LBKParams& p) { return dim3((p.X + BX - 1) / BX, (p.Y + 
             if (fs)                                                                                                \
                 k_fused32<PULL, VV, CO, true><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid, cpf,   \
                                                               (float*)hf, visc, mac, p, *fs);                      \
+            else if (HYB_FP32_X2 && fout && !mac)                                                                  \
+                k_fused32x2<PULL, VV, CO><<<g, dim3(BX / 2, BY, 1), 0, s>>>((const float*)fin, (float*)fout, link, \
+                                                                            solid, cpf, (float*)hf, visc, p);      \
             else                                                                                                   \
                 k_fused32<PULL, VV, CO, false><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid, cpf,  \
                                                                (float*)hf, visc, mac, p, none);                    \
@@ -875,8 +1053,8 @@ void launch_lb_init_pops(Prec prec, void* f, const uint32_t* link, const int* mx
 void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, NodeBox* boxes, int* nboxes,
                          uint32_t* solid, const uint32_t* link, const LBKParams& p, hipStream_t s) {
     if (max_slots <= 0) return;
-    int grid = (max_slots + 3) / 4;  // one wavefront per slot, four per workgroup
-    if (grid > 2048) grid = 2048;
+    int grid = SG == 64 ? (max_slots + 3) / 4 : max_slots;  // slots per workgroup: 4 or 1
+    if (grid > 8192) grid = 8192;
     k_classify_set<<<grid, 256, 0, s>>>(nslots, cp, boxes, nboxes, solid, link, p);
 }
 
@@ -884,8 +1062,8 @@ void launch_hydro_partials(const int* nslots, int max_slots, const NodeBox* boxe
                            const uint32_t* solid, const uint32_t* link, Prec prec, const void* hf, const double* fsMass,
                            const void* fsRho, HydroPartial* out, const LBKParams& p, hipStream_t s) {
     if (max_slots <= 0) return;
-    int grid = (max_slots + 3) / 4;  // four wavefronts per workgroup
-    if (grid > 2048) grid = 2048;
+    int grid = SG == 64 ? (max_slots + 3) / 4 : max_slots;  // slots per workgroup: 4 or 1
+    if (grid > 8192) grid = 8192;
     if (prec == Prec::F64)
         k_hydro_partials<double><<<grid, 256, 0, s>>>(nslots, boxes, cp, solid, link, (const double*)hf, fsMass,
                                                       (const double*)fsRho, out, p);

@@ -4,8 +4,9 @@ CPU: the torch model of tests/adjoint_reference.py reproduces the oracle's LB
 steps (so its autograd gradient is a gradient of the reference's step).
 GPU: the hand-written adjoint (through the autograd.Function harness) matches
 the torch autograd gradient (FP64, 1e-9 relative L2) and central finite
-differences of the GPU forward (h = 1e-4 relative to tau, to f0 and to the
-uniform body force; 1e-5 relative error).
+differences of the GPU forward (h = 1e-4 relative to tau and to f0, 1e-2
+relative to the uniform body force for the force field; 1e-5 relative error
+plus a noise floor of 1e-7 of the largest gradient component).
 """
 import numpy as np
 import pytest
@@ -101,9 +102,13 @@ def test_gpu_adjoint_matches_autograd_and_fd():
     tp_eff, tm_eff = 0.5 + 3.0 * ((tp - 0.5) / 3.0), 0.5 + 3.0 * ((tm - 0.5) / 3.0)
     fd_tau = (L(tp, ff32, f0) - L(tm, ff32, f0)) / (tp_eff - tm_eff)
     assert abs(fd_tau - gtau) <= 1e-5 * abs(gtau), (fd_tau, gtau)
-    # Force-field perturbations are h relative to the uniform body force: relative to the 1e-7 field
-    # itself the loss change (~1e-14) would sink into the FP64 rounding noise of two 30-step runs.
-    fscale = float(np.hypot(cfg["lbFX"], cfg["lbFZ"]))
+    # Force-field perturbations are 100 h relative to the uniform body force: relative to the 1e-7
+    # field itself the loss change (~1e-14) would sink into the FP64 rounding noise of two 30-step
+    # runs (~1e-17 in L). Central differences cancel the quadratic term, and the loss is linear in
+    # the force to far below 1e-5 at this size. The remaining noise floor enters as 1e-7 of the
+    # largest gradient component.
+    fscale = 100.0 * float(np.hypot(cfg["lbFX"], cfg["lbFZ"]))
+    gFmax = float(np.max(np.abs(gF)))
     picks = rng.choice(act, size=6, replace=False)
     for node in picks:
         for k in (0, 2):
@@ -113,13 +118,11 @@ def test_gpu_adjoint_matches_autograd_and_fd():
             ffp[node, k] = np.float32(base + d)
             ffm[node, k] = np.float32(base - d)
             fd = (L(TAU, ffp, f0) - L(TAU, ffm, f0)) / (float(ffp[node, k]) - float(ffm[node, k]))
-            scale = max(abs(gF[node, k]), 1e-3 * np.max(np.abs(gF)))
-            assert abs(fd - gF[node, k]) <= 1e-5 * scale, (node, k, fd, gF[node, k])
+            assert abs(fd - gF[node, k]) <= 1e-5 * abs(gF[node, k]) + 1e-7 * gFmax, (node, k, fd, gF[node, k])
         j = int(rng.integers(0, 19))
         fp, fm = f0.copy(), f0.copy()
         d = h * abs(f0[node, j])
         fp[node, j] += d
         fm[node, j] -= d
         fd = (L(TAU, ff32, fp) - L(TAU, ff32, fm)) / (fp[node, j] - fm[node, j])
-        scale = max(abs(g0[node, j]), 1e-3 * np.max(np.abs(g0)))
-        assert abs(fd - g0[node, j]) <= 1e-5 * scale, (node, j, fd, g0[node, j])
+        assert abs(fd - g0[node, j]) <= 1e-5 * abs(g0[node, j]) + 1e-7 * np.max(np.abs(g0)), (node, j, fd, g0[node, j])
