@@ -115,6 +115,7 @@ struct hyb_lb {
     int cur = 0;
     uint8_t* type = nullptr;
     uint32_t* link = nullptr;
+    uint32_t* slipw = nullptr;  // slip-wall directions per node (only when a box face is a slip wall)
     uint32_t* solid = nullptr;
     void* hf = nullptr;
     double* visc = nullptr;
@@ -144,6 +145,7 @@ struct hyb_lb {
     int rhoCur = 0;        // density buffer of the last collision
     int fluidMassBuf = 0;  // density buffer that is the fluid-node mass (set by the free-surface step)
     uint32_t fsTag = 0;
+    uint32_t actTag = 0;   // tag of a free-surface pass not yet followed by a collision
     int64_t fsSteps = 0;
     bool capture = false;  // keep n, u, hydroForce of every collision for the host views
     float* ffield = nullptr;  // optional per-node body force [3][N] (lattice units)
@@ -157,6 +159,11 @@ struct hyb_lb {
     int64_t profLaunches = 0;
 
     bool varVisc() const { return nonNewtonian || turbulenceOn; }
+    bool hasSlip() const {
+        for (int k = 0; k < 6; ++k)
+            if (boundary[k] == 5 || boundary[k] == 6) return true;
+        return false;
+    }
     LBKParams params() const {
         LBKParams p;
         p.X = X; p.Y = Y; p.Z = Z;
@@ -173,6 +180,12 @@ struct hyb_lb {
         p.initDensity = initDensity;
         p.ctag = ctag;
         p.ffield = ffield;
+        p.slipS1 = slipCoefficient;  // LB.cpp:1360-1361
+        p.slipS2 = 1.0 - slipCoefficient;
+        p.slipw = slipw;
+        p.linkw = link;
+        p.actPrev = fsb.actPrev;
+        p.actTag = actTag;
         return p;
     }
     // planes are padded so that successive planes do not start on the same
@@ -429,7 +442,7 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
         (rc = dalloc(&b.cursor, scap)) || (rc = dalloc(&b.ghostOfCount, std::max(P, 1))) ||
         (rc = dalloc(&b.ghostOf, 8 * (size_t)std::max(P, 1))) || (rc = dalloc(&b.side, 3 * (size_t)std::max(P, 1))) ||
         (rc = dalloc(&b.ghostB, p.Gcap)) || (rc = dalloc(&b.posI, p.pairCap)) || (rc = dalloc(&b.posJ, p.pairCap)) ||
-        (rc = dalloc(&b.prec, 2 * (size_t)p.pairCap)) || (rc = dalloc(&b.vmax, (size_t)(std::max(P, 1) + 63) / 64)))
+        (rc = dalloc(&b.prec, 2 * (size_t)p.pairCap)) || (rc = dalloc(&b.vmax, (size_t)(std::max(P, 1) + 63) / 64)) || (rc = dalloc(&b.energy, 6)))
         return rc;
     HIPCHK(hipMemcpyAsync(b.el, el.data(), el.size() * sizeof(double), hipMemcpyHostToDevice, d->stream));
     DEMState st;
@@ -474,7 +487,7 @@ int hyb_dem_destroy(hyb_dem* d) {
     dfree(b.el); dfree(b.st); dfree(b.ghostIndex); dfree(b.ghostVec); dfree(b.pairs); dfree(b.nearWall);
     dfree(b.cellTable); dfree(b.cellOf); dfree(b.cellCount); dfree(b.work); dfree(b.csrOff); dfree(b.csr);
     dfree(b.cursor); dfree(b.ghostOfCount); dfree(b.ghostOf); dfree(b.side); dfree(b.ghostB);
-    dfree(b.posI); dfree(b.posJ); dfree(b.prec); dfree(b.vmax);
+    dfree(b.posI); dfree(b.posJ); dfree(b.prec); dfree(b.vmax); dfree(b.energy);
     delete d;
     return HYB_OK;
 }
@@ -512,9 +525,9 @@ int hyb_lb_create(const hyb_config* cfg, hyb_lb** out) {  // LB.cpp:85-188 + LB.
     }
     for (int k = 0; k < 6; ++k) {
         l->boundary[k] = c.boundary[k];
-        if (c.boundary[k] != 4 && c.boundary[k] != 7) {
+        if (c.boundary[k] < 4 || c.boundary[k] > 8) {
             delete l;
-            return fail(HYB_E_UNSUPPORTED, "only periodic (4) and static no-slip (7) box boundaries are supported");
+            return fail(HYB_E_UNSUPPORTED, "box boundaries must be periodic (4), slip (5, 6) or no-slip (7, 8) walls");
         }
     }
     for (int k = 0; k < 3; ++k)
@@ -556,6 +569,7 @@ static int lb_alloc(hyb_lb* l, int slotCap) {
         (rc = dalloc(&l->nslots, 2)))
         return rc;
     if (l->varVisc() && (rc = dalloc(&l->visc, N))) return rc;
+    if (l->hasSlip() && (rc = dalloc(&l->slipw, N))) return rc;
     if (l->freeSurface) {
         FSBuffers& b = l->fsb;
         const size_t eb = l->prec == Prec::F64 ? 8 : 4;
@@ -566,10 +580,13 @@ static int lb_alloc(hyb_lb* l, int slotCap) {
             (rc = dalloc(&b.listA, N)) || (rc = dalloc(&b.listB, N)) || (rc = dalloc(&b.mnew, N)) ||
             (rc = dalloc(&b.state, N)) || (rc = dalloc(&b.ctr, 1)))
             return rc;
+        if (l->hasSlip() && (rc = dalloc(&b.actPrev, N))) return rc;
     }
     l->slotCap = slotCap;
     if (slotCap > 0) {
-        HIPCHK(hipMalloc(&l->hf, (l->prec == Prec::F64 ? 8 : 4) * 3 * N));
+        const size_t hb = (l->prec == Prec::F64 ? sizeof(NodeForce64) : sizeof(NodeForce32)) * N;
+        HIPCHK(hipMalloc(&l->hf, hb));
+        HIPCHK(hipMemset(l->hf, 0, hb));  // no record carries a valid ownership word yet
         if ((rc = dalloc(&l->cp, slotCap)) || (rc = dalloc(&l->cpf, slotCap)) || (rc = dalloc(&l->boxes, slotCap)) ||
             (rc = dalloc(&l->part, slotCap)))
             return rc;
@@ -580,8 +597,11 @@ static int lb_alloc(hyb_lb* l, int slotCap) {
 // Particle-node classification of one coupling step: every slot marks the
 // active nodes strictly inside it with this step's tag (LB.cpp:475-495).
 static void lb_couple_launch(hyb_lb* l, hyb_dem* d) {
-    if (++l->ctag > SOLID_TAG_MAX) {
+    if (++l->ctag > SOLID_TAG_MAX) {  // tag wrap-around: clear ownership words and force records once
         launch_fill_u32(l->solid, 0u, l->N, l->stream);
+        if (l->hf)
+            launch_fill_u32((uint32_t*)l->hf,
+                            0u, (l->prec == Prec::F64 ? sizeof(NodeForce64) : sizeof(NodeForce32)) / 4 * l->N, l->stream);
         l->ctag = 1;
     }
     const LBKParams p = l->params();
@@ -607,7 +627,7 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
     }
     launch_lb_init_types(l->type, l->boundary, (int)walls.size(), dw, l->cfg.problem, l->cfg.liquidBox, p, s);
     launch_lb_init_interface(l->type, p, s);
-    launch_lb_init_links(l->type, l->link, p, s);
+    launch_lb_init_links(l->type, l->link, l->slipw, p, s);
     HIPCHK(hipMemsetAsync(l->maxXYZ, 0, 3 * sizeof(int) * l->Z, s));
     launch_lb_max_active(l->link, l->maxXYZ, p, s);
     {
@@ -627,6 +647,7 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
         HIPCHK(hipMemsetAsync(b.uS, 0, 24 * l->N, s));
         HIPCHK(hipMemsetAsync(b.hFi, 0, 24 * l->N, s));
         HIPCHK(hipMemsetAsync(b.claim, 0, 8 * l->N, s));
+        if (b.actPrev) HIPCHK(hipMemsetAsync(b.actPrev, 0, 4 * l->N, s));
         HIPCHK(hipMemsetAsync(b.ctr, 0, sizeof(FSCounters), s));
         launch_fs_build_list(l->type, b, 0, p, s);
     }
@@ -675,16 +696,18 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
 int hyb_lb_fs_step(hyb_lb* l) {  // LB.cpp:235-245
     if (!l || !l->inited) return fail(HYB_E_STATE, "hyb_lb_fs_step before init");
     if (!l->freeSurface) return HYB_OK;
-    if (++l->fsTag == 0) {  // tag wrap-around: clear the ownership words once
+    if (++l->fsTag >= 0x80000000u) {  // tag wrap-around (31-bit tags): clear the tagged words once
         launch_fill_u32((uint32_t*)l->fsb.claim, 0u, 2 * l->N, l->stream);
+        if (l->fsb.actPrev) launch_fill_u32(l->fsb.actPrev, 0u, l->N, l->stream);
         l->fsTag = 1;
     }
+    l->actTag = l->fsTag;
     const LBKParams p = l->params();
     // new interface nodes get hydroForce + lbF; lbF is zeroed by the first collision
     // when forceFieldSolve = 0 (LB.cpp:1074-1076)
     double Fnew[3];
     for (int k = 0; k < 3; ++k) Fnew[k] = (l->forceField || l->steps == 0) ? l->lbF[k] : 0.0;
-    launch_fs_step(l->prec, l->f[l->cur], l->f[l->cur ^ 1], l->type, l->link, l->visc, l->fsb, l->fsCur, l->rhoCur,
+    launch_fs_step(l->prec, l->f[l->cur], l->f[l->cur ^ 1], l->type, l->link, l->slipw, l->visc, l->fsb, l->fsCur, l->rhoCur,
                    l->fsTag, Fnew, l->steps > 0, p, l->stream);
     HIPCHK(hipGetLastError());
     l->fsCur ^= 1;
@@ -736,7 +759,7 @@ int hyb_lb_step(hyb_lb* l, hyb_dem* d) {  // LB.cpp:221-233 fused into one kerne
                     l->capture ? l->mac : nullptr, l->freeSurface ? &fv : nullptr, p, l->stream);
     if (l->profiling) HIPCHK(hipEventRecord(e1, l->stream));
     if (withParticles) {
-        launch_hydro_partials(l->nslots, l->slotCap, l->boxes, l->cp, l->solid, l->link, l->prec, l->hf,
+        launch_hydro_partials(l->nslots, l->slotCap, l->boxes, l->cp, l->link, l->prec, l->hf,
                               l->freeSurface ? l->fsb.mass : nullptr, l->freeSurface ? fv.rhoPrev : nullptr, l->part,
                               p, l->stream);
         launch_dem_hydro_combine(d->p, d->b, l->part, l->unit, l->stream);
@@ -744,6 +767,7 @@ int hyb_lb_step(hyb_lb* l, hyb_dem* d) {  // LB.cpp:221-233 fused into one kerne
     HIPCHK(hipGetLastError());
     l->cur ^= 1;
     if (l->freeSurface) l->rhoCur ^= 1;
+    l->actTag = 0;  // the next pull streams populations of this collision
     l->steps++;
     l->macValid = l->capture;
     return HYB_OK;
@@ -796,6 +820,7 @@ static int adj_check(hyb_lb* l) {
     if (l->prec != Prec::F64) return fail(HYB_E_UNSUPPORTED, "the adjoint runs on the FP64 path");
     if (l->freeSurface) return fail(HYB_E_UNSUPPORTED, "the adjoint does not differentiate the free surface");
     if (l->varVisc()) return fail(HYB_E_UNSUPPORTED, "the adjoint needs a constant viscosity");
+    if (l->hasSlip()) return fail(HYB_E_UNSUPPORTED, "the adjoint supports periodic and no-slip boundaries only");
     return HYB_OK;
 }
 
@@ -910,7 +935,7 @@ int hyb_lb_destroy(hyb_lb* l) {
     if (l->f[1]) (void)hipFree(l->f[1]);
     if (l->hf) (void)hipFree(l->hf);
     if (l->scratch) (void)hipFree(l->scratch);
-    dfree(l->type); dfree(l->link); dfree(l->solid); dfree(l->visc); dfree(l->mac); dfree(l->maxXYZ);
+    dfree(l->type); dfree(l->link); dfree(l->slipw); dfree(l->solid); dfree(l->visc); dfree(l->mac); dfree(l->maxXYZ);
     dfree(l->dcount); dfree(l->cp); dfree(l->cpf); dfree(l->boxes); dfree(l->nslots); dfree(l->part);
     for (void* ck : l->adjCk) (void)hipFree(ck);
     if (l->ffield) (void)hipFree(l->ffield);
@@ -918,7 +943,7 @@ int hyb_lb_destroy(hyb_lb* l) {
     if (b.rho[0]) (void)hipFree(b.rho[0]);
     if (b.rho[1]) (void)hipFree(b.rho[1]);
     dfree(b.mass); dfree(b.uS); dfree(b.hFi); dfree(b.claim); dfree(b.list[0]); dfree(b.list[1]); dfree(b.listA);
-    dfree(b.listB); dfree(b.mnew); dfree(b.state); dfree(b.ctr);
+    dfree(b.listB); dfree(b.mnew); dfree(b.state); dfree(b.ctr); dfree(b.actPrev);
     delete l;
     return HYB_OK;
 }
@@ -1174,6 +1199,16 @@ int hyb_dem_get(hyb_dem* d, const char* cname, void* host, size_t cap, size_t* n
     if (G) {
         HIPCHK(hipMemcpy(gi.data(), d->b.ghostIndex, 4 * (size_t)G, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(gv.data(), d->b.ghostVec, 24 * (size_t)G, hipMemcpyDeviceToHost));
+    }
+    if (name == "energy") {  // DEM::particleEnergy, updated after every sub-step (DEM.cpp:371-372)
+        double e6[6] = {0, 0, 0, 0, 0, 0};
+        if (d->substeps > 0 && d->b.energy) {
+            launch_dem_energy(d->p, d->b, d->demSize, d->b.energy, d->stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(e6, d->b.energy, sizeof(e6), hipMemcpyDeviceToHost, d->stream));
+            HIPCHK(hipStreamSynchronize(d->stream));
+        }
+        return put(e6, sizeof(e6), host, cap, need);
     }
     if (name == "e_x" || name == "e_xp" || name == "e_w" || name == "e_wp") {
         // at step end the predicted copies equal the corrected ones (elmt.cpp:190-224)

@@ -30,17 +30,46 @@ __host__ __device__ constexpr double wgt(int j) {
     return j == 0 ? 12.0 / 36.0 : (j < 7 ? 2.0 / 36.0 : 1.0 / 36.0);
 }
 
+// Slip-wall reflection (lattice.h:87-95): for a diagonal link j, the axis
+// directions whose neighbours decide the reflection and the populations
+// reflected from them.
+__host__ __device__ constexpr int slip1check(int j) {
+    constexpr int v[19] = {0, 0, 0, 0, 0, 0, 0, 1, 2, 3, 4, 3, 4, 5, 6, 1, 2, 6, 5};
+    return v[j];
+}
+__host__ __device__ constexpr int slip1dir(int j) {
+    constexpr int v[19] = {0, 0, 0, 0, 0, 0, 0, 9, 10, 8, 7, 13, 14, 12, 11, 18, 17, 15, 16};
+    return v[j];
+}
+__host__ __device__ constexpr int slip2check(int j) {
+    constexpr int v[19] = {0, 0, 0, 0, 0, 0, 0, 3, 4, 2, 1, 5, 6, 4, 3, 5, 6, 1, 2};
+    return v[j];
+}
+__host__ __device__ constexpr int slip2dir(int j) {
+    constexpr int v[19] = {0, 0, 0, 0, 0, 0, 0, 10, 9, 7, 8, 14, 13, 11, 12, 17, 18, 16, 15};
+    return v[j];
+}
+
 // Node type codes (node.h:73-83).
 enum NodeType : uint8_t { T_FLUID = 0, T_GAS = 2, T_INTERFACE = 3, T_PERIODIC = 4, T_SLIP = 5, T_SLIP_MOVING = 6,
                           T_WALL = 7, T_WALL_MOVING = 8, T_CURVED = 9 };
 
-// Per-node link word: bit j (1..18) = the link in direction j is a static wall;
+// Wall types the lattice streams against: slip (5), moving slip (6), no-slip (7),
+// moving no-slip (8). Moving walls have the generic zero velocity
+// (DEM.cpp:435-638), so they bounce back like static walls.
+__host__ __device__ constexpr bool is_plane_wall(uint8_t t) { return t >= 5 && t <= 8; }
+__host__ __device__ constexpr bool is_slip_wall(uint8_t t) { return t == 5 || t == 6; }
+
+// Per-node link word: bit j (1..18) = the link in direction j is a wall;
+// bit 0 = some of those walls are slip walls (their directions are in a
+// separate per-node slip word);
 // bit 29 = the free-surface pass has already streamed this node's populations
 // for the next collision (they sit at the node's own index of the output
 // buffer); bit 30 = node is interface; bit 31 = node is active (fluid or interface).
 constexpr uint32_t LINK_ACTIVE = 1u << 31;
 constexpr uint32_t LINK_INTERFACE = 1u << 30;
 constexpr uint32_t LINK_PRESTREAMED = 1u << 29;
+constexpr uint32_t LINK_SLIP = 1u;
 constexpr uint32_t LINK_WALLS = 0x7FFFEu;
 
 // 3-vector with the reference's operation order (vector.cpp:38-180): every

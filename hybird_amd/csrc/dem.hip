@@ -587,20 +587,40 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
     const El E{b.el, p.P};
     V3 FP(0.0, 0.0, 0.0), MP(0.0, 0.0, 0.0), FW(0.0, 0.0, 0.0), MW(0.0, 0.0, 0.0);
     const uint32_t lo = b.csrOff[e], hi = b.csrOff[e + 1];
-    for (uint32_t i = lo; i < hi; ++i) {
-        const PairRec& r = b.prec[i];
+    // the element state does not depend on the contact records: its loads are
+    // issued first, so that they overlap with the record loads below
+    const uint32_t wi = b.nearWall[e];
+    const V3 xpe = E.v3(EF_XP, e);
+    const V3 vpe = E.v3(EF_XP + 3, e);
+    const double re = E.at(EF_RAD, e), me = E.at(EF_M, e);
+    const V3 wpg = E.v3(EF_WPG, e);
+    const V3 FH = E.v3(EF_FH, e), MH = E.v3(EF_MH, e);
+    const V3 Ie = E.v3(EF_I, e), wpl = E.v3(EF_WPL, e);
+    const V3 xp[6] = {xpe, vpe, E.v3(EF_XP + 6, e), E.v3(EF_XP + 9, e), E.v3(EF_XP + 12, e), E.v3(EF_XP + 15, e)};
+    const V3 wp[6] = {E.v3(EF_WP, e), E.v3(EF_WP + 3, e), E.v3(EF_WP + 6, e), E.v3(EF_WP + 9, e), E.v3(EF_WP + 12, e),
+                      E.v3(EF_WP + 15, e)};
+    const Q4 q2 = E.q4(EF_Q + 8, e);  // single spheres: q2 is never set by the force routine
+    const Q4 qp[6] = {E.q4(EF_QP, e), E.q4(EF_QP + 4, e), E.q4(EF_QP + 8, e), E.q4(EF_QP + 12, e),
+                      E.q4(EF_QP + 16, e), E.q4(EF_QP + 20, e)};
+    // contact increments in pair-list order; records are loaded GU at a time
+    const auto add = [&](const PairRec& r) {
         const int flag = r.flag;
         if (flag & 1) FP = FP + V3(r.fn[0], r.fn[1], r.fn[2]);
         if (flag & 2) {
             MP = MP + V3(r.m[0], r.m[1], r.m[2]);
             FP = FP + V3(r.ft[0], r.ft[1], r.ft[2]);
         }
+    };
+    constexpr uint32_t GU = 4;
+    uint32_t i = lo;
+    for (; i + GU <= hi; i += GU) {
+        PairRec r[GU];
+#pragma unroll
+        for (uint32_t k = 0; k < GU; ++k) r[k] = b.prec[i + k];
+#pragma unroll
+        for (uint32_t k = 0; k < GU; ++k) add(r[k]);
     }
-    const uint32_t wi = b.nearWall[e];
-    const V3 xpe = E.v3(EF_XP, e);
-    const V3 vpe = E.v3(EF_XP + 3, e);
-    const double re = E.at(EF_RAD, e), me = E.at(EF_M, e);
-    const V3 wpg = E.v3(EF_WPG, e);
+    for (; i < hi; ++i) add(b.prec[i]);
     if (wi != 0xFFFFFFFFu) {
         const WallDev& w = p.walls[wi];
         const double distance = w.dist(xpe);
@@ -633,19 +653,17 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
     const double rad = re;
     const V3 FVisc = -6.0 * M_PI * p.numVisc * rad * xp1;
     const V3 MVisc = -8.0 * M_PI * p.numVisc * rad * rad * rad * wpg;
-    const V3 FH = E.v3(EF_FH, e), MH = E.v3(EF_MH, e);
     const V3 demF(p.demF[0], p.demF[1], p.demF[2]);
     const V3 x2 = (FVisc + FH + FP + FW) / me + demF;
     const V3 moment = MVisc + MH + MP + MW;
-    const Q4 qp0 = E.q4(EF_QP, e);
+    const Q4 qp0 = qp[0];
     const V3 momentBf = project(moment, qp0.adjoint());
-    const V3 waBf = newtonAcc(momentBf, E.v3(EF_I, e), E.v3(EF_WPL, e));
+    const V3 waBf = newtonAcc(momentBf, Ie, wpl);
     const V3 w1 = project(waBf, qp0);
     E.set3(EF_FP, e, FP); E.set3(EF_MP, e, MP); E.set3(EF_FW, e, FW); E.set3(EF_MW, e, MW);
     // corrector
     const double* co1 = p.co1;
     const double* co2 = p.co2;
-    const V3 xp[6] = {xpe, xp1, E.v3(EF_XP + 6, e), E.v3(EF_XP + 9, e), E.v3(EF_XP + 12, e), E.v3(EF_XP + 15, e)};
     const V3 x2Corr = x2 - xp[2];
     V3 x[6];
     x[0] = xp[0] + x2Corr * co2[0];
@@ -654,8 +672,6 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
     x[3] = xp[3] + x2Corr * co2[3];
     x[4] = xp[4] + x2Corr * co2[4];
     x[5] = xp[5] + x2Corr * co2[5];
-    const V3 wp[6] = {E.v3(EF_WP, e), E.v3(EF_WP + 3, e), E.v3(EF_WP + 6, e), E.v3(EF_WP + 9, e), E.v3(EF_WP + 12, e),
-                      E.v3(EF_WP + 15, e)};
     const V3 w1Corr = w1 - wp[1];
     V3 w[6];
     w[0] = wp[0] + w1Corr * co1[0];
@@ -664,9 +680,6 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
     w[3] = wp[3] + w1Corr * co1[3];
     w[4] = wp[4] + w1Corr * co1[4];
     w[5] = wp[5] + w1Corr * co1[5];
-    const Q4 q2 = E.q4(EF_Q + 8, e);  // single spheres: q2 is never set by the force routine
-    const Q4 qp[6] = {qp0, E.q4(EF_QP + 4, e), E.q4(EF_QP + 8, e), E.q4(EF_QP + 12, e), E.q4(EF_QP + 16, e),
-                      E.q4(EF_QP + 20, e)};
     const Q4 q2Corr = q2 - qp[2];
     Q4 q[6];
     q[0] = qp[0] + q2Corr * co2[0];
@@ -756,6 +769,67 @@ __global__ void k_dem_hydro_combine(DEMParams p, DEMBuffers b, const HydroPartia
 
 void launch_dem_rebuild(const DEMParams& p, const DEMBuffers& b, int force, hipStream_t s) {
     k_dem_rebuild<<<1, RB, 0, s>>>(p, b, force);
+}
+
+// particleEnergy after a sub-step (DEM.cpp:2226-2276, utils.cpp:324-327). The
+// per-element terms of a chunk are computed in parallel; one lane then adds them
+// in element order, so the sums are the reference's sequential sums.
+constexpr int EN_BLOCK = 256;
+__global__ void __launch_bounds__(EN_BLOCK) k_dem_energy(DEMParams p, DEMBuffers b, double sx, double sy, double sz,
+                                                         double* __restrict__ out) {
+#pragma clang fp contract(off)
+    __shared__ double tk[EN_BLOCK], rk[EN_BLOCK], gp[EN_BLOCK];
+    const El E{b.el, p.P};
+    const V3 F(p.demF[0], p.demF[1], p.demF[2]);
+    const double gn = sqrt(F.norm2());
+    const bool grav = gn != 0.0;
+    V3 n, c(0.0, 0.0, 0.0);
+    if (grav) {
+        n = V3(-1.0 * F.x / gn, -1.0 * F.y / gn, -1.0 * F.z / gn);
+        // the plane through the box corner the gravity points to (n . X < 0 etc.)
+        if (n.x * 1.0 + n.y * 0.0 + n.z * 0.0 < 0) c.x = c.x + sx;
+        if (n.x * 0.0 + n.y * 1.0 + n.z * 0.0 < 0) c.y = c.y + sy;
+        if (n.x * 0.0 + n.y * 0.0 + n.z * 1.0 < 0) c.z = c.z + sz;
+    }
+    double tKin = 0.0, rKin = 0.0, g = 0.0;
+    for (int base = 0; base < p.P; base += EN_BLOCK) {
+        const int e = base + threadIdx.x;
+        if (e < p.P) {
+            const double m = E.at(EF_M, e);
+            const V3 v = E.v3(EF_X + 3, e);
+            tk[threadIdx.x] = 0.5 * m * v.norm2();
+            const V3 w = E.v3(EF_WL, e), I = E.v3(EF_I, e);
+            rk[threadIdx.x] = I.dot(V3(w.x * w.x, w.y * w.y, w.z * w.z));
+            if (grav) {
+                const V3 x = E.v3(EF_X, e);
+                const double h = n.dot(x - c);
+                gp[threadIdx.x] = m * h * gn;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int cnt = min(EN_BLOCK, p.P - base);
+            for (int i = 0; i < cnt; ++i) {
+                tKin += tk[i];
+                rKin += rk[i];
+                if (grav) g += gp[i];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double elastic = 0.0;  // not computed by the reference
+        out[0] = rKin + tKin;
+        out[1] = rKin;
+        out[2] = tKin;
+        out[3] = g;
+        out[4] = elastic;
+        out[5] = rKin + tKin + g + elastic;
+    }
+}
+
+void launch_dem_energy(const DEMParams& p, const DEMBuffers& b, const double demSize[3], double* out, hipStream_t s) {
+    k_dem_energy<<<1, EN_BLOCK, 0, s>>>(p, b, demSize[0], demSize[1], demSize[2], out);
 }
 
 void launch_dem_predict(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {

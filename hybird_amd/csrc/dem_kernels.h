@@ -80,6 +80,7 @@ struct DEMBuffers {
     uint32_t* posJ;        // [pairCap] csr slot in its second particle's list
     struct PairRec* prec;  // [2 * pairCap] per csr slot: signed force/torque increments of one pair
     double* vmax;          // [(P + 63) / 64] per-wavefront max |x1|^2 written by the corrector
+    double* energy;        // [6] particleEnergy: kin, rotKin, trKin, grav, elastic, total
     int scap;
     int ncells;
     int W;                 // ncells * 14
@@ -99,6 +100,7 @@ void launch_dem_rebuild(const DEMParams& p, const DEMBuffers& b, int force, hipS
 void launch_dem_pairs(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
 void launch_dem_gather(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
 void launch_dem_predict(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
+void launch_dem_energy(const DEMParams& p, const DEMBuffers& b, const double demSize[3], double* out, hipStream_t s);
 void launch_dem_couple_table(const DEMParams& p, const DEMBuffers& b, CoupleRec* cp, CoupleRecF* cpf, int* nslots,
                              const Units& u, hipStream_t s);
 void launch_dem_hydro_combine(const DEMParams& p, const DEMBuffers& b, const HydroPartial* part, const Units& u,

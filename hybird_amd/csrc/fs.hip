@@ -111,6 +111,14 @@ __global__ void __launch_bounds__(FS_BLOCK)
         for (int j = 0; j < 19; ++j) fs[j] = fpost[(size_t)j * p.PS + idx];
         const double mi = mass[idx];
         const V3 u(uS[idx], uS[N + idx], uS[2 * N + idx]);
+        if (p.actPrev) {  // activity before this pass of the nodes it may change
+            uint32_t* ap = const_cast<uint32_t*>(p.actPrev);
+            ap[idx] = (p.actTag << 1) | 1u;
+            for (int j = 1; j < 19; ++j) {
+                const size_t L = nb_index(p, c, j);
+                if (type[L] == T_GAS) ap[L] = p.actTag << 1;
+            }
+        }
         const double usq = u.norm2();
         fi[0] = fs[0];
         double dm = 0.0;
@@ -125,6 +133,16 @@ __global__ void __launch_bounds__(FS_BLOCK)
                 v = fs[o];  // before the first collision f = fs
                 if (tl == T_INTERFACE) a = 0.5 * (mass[L] + mi);
                 else if (tl == T_FLUID) a = 1.0;
+                else if (tl == T_WALL_MOVING) a = 1.0 * mi;
+                else if (tl == T_SLIP_MOVING) {
+                    const auto active = [&](size_t n) { const uint8_t t = type[n]; return t == T_FLUID || t == T_INTERFACE; };
+                    bool half = false;
+                    if (j > 6) {
+                        const bool a1 = active(nb_index(p, c, slip1check(j))), a2 = active(nb_index(p, c, slip2check(j)));
+                        half = (a1 && !a2) || (!a1 && a2);
+                    }
+                    a = half ? 0.0 + 1.0 * p.slipS2 * mi : 0.0 + 1.0 * mi;
+                }
             } else if (tl == T_FLUID || tl == T_INTERFACE) {
                 v = fpost[(size_t)o * p.PS + L];  // LB.cpp:1234
                 a = (tl == T_INTERFACE) ? 0.5 * (mass[L] + mi) : 1.0;
@@ -135,7 +153,22 @@ __global__ void __launch_bounds__(FS_BLOCK)
                 if (SHIFT) v = (T)(-to_d(fs[j]) + (g - 2.0 * wgt(j)));
                 else v = (T)(-to_d(fs[j]) + g);
             } else {
-                v = fs[j];  // static wall: half-way bounce-back (LB.cpp:1344-1356)
+                // plane wall: bounce-back, slip reflection (LB.cpp:1321-1456); moving
+                // walls have zero velocity. Mass exchange: a = m for a moving wall,
+                // the slip rule for a moving slip wall (LB.cpp:1531-1575), else 0.
+                v = fs[j];
+                const auto active = [&](size_t n) { const uint8_t t = type[n]; return t == T_FLUID || t == T_INTERFACE; };
+                if (is_slip_wall(tl)) v = wall_link<T>(fpost, p.PS, p, c, true, j, fs[j], active);
+                if (tl == T_WALL_MOVING) {
+                    a = 1.0 * mi;
+                } else if (tl == T_SLIP_MOVING) {
+                    bool half = false;
+                    if (j > 6) {
+                        const bool a1 = active(nb_index(p, c, slip1check(j))), a2 = active(nb_index(p, c, slip2check(j)));
+                        half = (a1 && !a2) || (!a1 && a2);
+                    }
+                    a = half ? 0.0 + 1.0 * p.slipS2 * mi : 0.0 + 1.0 * mi;
+                }
             }
             fi[o] = v;
             dm += 1.0 * a * (to_d(v) - to_d(fs[j]));
@@ -204,6 +237,7 @@ __global__ void __launch_bounds__(FS_BLOCK)
                 const unsigned long long old = atomicMax(&claim[L], tagv | 1ull);
                 if (old >= tagv) continue;
                 listB[atomicAdd(&ctr->nB, 1u)] = (uint32_t)L;
+                if (p.actPrev) const_cast<uint32_t*>(p.actPrev)[L] = (p.actTag << 1) | 1u;  // was fluid
                 type[L] = T_INTERFACE;
                 link[L] |= LINK_INTERFACE;
                 const double n = to_d(rho[L]);
@@ -222,8 +256,8 @@ __global__ void __launch_bounds__(FS_BLOCK)
                 const uint8_t* __restrict__ state, uint8_t* __restrict__ type, uint32_t* __restrict__ link,
                 const unsigned long long* __restrict__ claim, const uint32_t* __restrict__ listA,
                 double* __restrict__ mass, T* __restrict__ rho, double* __restrict__ uS, double* __restrict__ hFi,
-                double* __restrict__ visc, T* __restrict__ fout, uint32_t tag, double Fx, double Fy, double Fz,
-                LBKParams p) {
+                double* __restrict__ visc, T* __restrict__ fout, uint32_t* __restrict__ slipw, uint32_t tag, double Fx,
+                double Fy, double Fz, LBKParams p) {
 #pragma clang fp contract(off)
     const size_t N = (size_t)p.X * p.Y * p.Z;
     const uint32_t nI = ctr->n[cur], nA = ctr->nA;
@@ -245,9 +279,14 @@ __global__ void __launch_bounds__(FS_BLOCK)
                 fout[(size_t)j * p.PS + L] = SHIFT ? (T)(feq - wgt(j)) : (T)feq;
             }
             const Coord c = coord_of(p, L);
-            uint32_t lw = LINK_ACTIVE | LINK_INTERFACE | LINK_PRESTREAMED;
-            for (int j = 1; j < 19; ++j)
-                if (type[nb_index(p, c, j)] == T_WALL) lw |= 1u << j;
+            uint32_t lw = LINK_ACTIVE | LINK_INTERFACE | LINK_PRESTREAMED, sw = 0;
+            for (int j = 1; j < 19; ++j) {
+                const uint8_t tl = type[nb_index(p, c, j)];
+                if (is_plane_wall(tl)) lw |= 1u << j;
+                if (is_slip_wall(tl)) sw |= 1u << j;
+            }
+            if (sw) lw |= LINK_SLIP;
+            if (slipw) slipw[L] = sw;
             link[L] = lw;
             type[L] = T_INTERFACE;
             mass[L] = 0.01 * rho0;
@@ -391,6 +430,9 @@ __global__ void k_fs_stream_view(const T* __restrict__ fin, T* __restrict__ fout
                 v = SHIFT ? (T)(-fsj + (g - 2.0 * wgt(j))) : (T)(-fsj + g);
             } else {
                 v = fin[(size_t)j * p.PS + idx];
+                if (is_slip_wall(tl))
+                    v = wall_link<T>(fin, p.PS, p, c, true, j, v,
+                                     [&](size_t n) { const uint8_t t = type[n]; return t == T_FLUID || t == T_INTERFACE; });
             }
             fout[(size_t)o * p.PS + idx] = v;
         }
@@ -407,8 +449,8 @@ void launch_fs_build_list(const uint8_t* type, FSBuffers& b, int cur, const LBKP
 }
 
 template <typename T, bool SHIFT>
-static void fs_step_t(const T* fpost, T* fout, uint8_t* type, uint32_t* link, double* visc, FSBuffers& b, int cur,
-                      int rcur, uint32_t tag, const double* Fnew, bool pull, const LBKParams& p, hipStream_t s) {
+static void fs_step_t(const T* fpost, T* fout, uint8_t* type, uint32_t* link, uint32_t* slipw, double* visc, FSBuffers& b,
+                      int cur, int rcur, uint32_t tag, const double* Fnew, bool pull, const LBKParams& p, hipStream_t s) {
     const T* rho = (const T*)b.rho[rcur];
     k_fs_mass<T, SHIFT><<<FS_GRID, FS_BLOCK, 0, s>>>(fpost, fout, type, b.mass, rho, b.uS, b.list[cur], b.ctr, cur,
                                                     b.mnew, b.state, p, pull ? 1 : 0);
@@ -416,8 +458,8 @@ static void fs_step_t(const T* fpost, T* fout, uint8_t* type, uint32_t* link, do
     k_fs_spread<T><<<FS_GRID, FS_BLOCK, 0, s>>>(b.list[cur], b.ctr, cur, b.state, type, link, b.claim, b.listA, b.listB,
                                                b.mass, rho, tag, p);
     k_fs_mutate<T, SHIFT><<<FS_GRID, FS_BLOCK, 0, s>>>(b.list[cur], b.ctr, cur, b.state, type, link, b.claim, b.listA,
-                                                      b.mass, (T*)b.rho[rcur], b.uS, b.hFi, visc, fout, tag, Fnew[0],
-                                                      Fnew[1], Fnew[2], p);
+                                                      b.mass, (T*)b.rho[rcur], b.uS, b.hFi, visc, fout, slipw, tag,
+                                                      Fnew[0], Fnew[1], Fnew[2], p);
     k_fs_iso_fluid<T><<<FS_GRID, FS_BLOCK, 0, s>>>(b.list[cur], b.ctr, cur, b.state, b.listA, b.listB, type, link,
                                                   b.mass, rho, p);
     k_fs_iso_gas<<<FS_GRID, FS_BLOCK, 0, s>>>(b.list[cur], b.ctr, cur, b.state, b.listA, b.listB, type, link, b.mass,
@@ -425,13 +467,15 @@ static void fs_step_t(const T* fpost, T* fout, uint8_t* type, uint32_t* link, do
     k_fs_redistribute<<<FS_GRID, FS_BLOCK, 0, s>>>(b.list[cur ^ 1], b.ctr, cur, b.mass);
 }
 
-void launch_fs_step(Prec prec, const void* fpost, void* fout, uint8_t* type, uint32_t* link, double* visc, FSBuffers& b,
-                    int cur, int rcur, uint32_t tag, const double* Fnew, bool pull, const LBKParams& p, hipStream_t s) {
+void launch_fs_step(Prec prec, const void* fpost, void* fout, uint8_t* type, uint32_t* link, uint32_t* slipw,
+                    double* visc, FSBuffers& b, int cur, int rcur, uint32_t tag, const double* Fnew, bool pull,
+                    const LBKParams& p, hipStream_t s) {
     if (prec == Prec::F64)
-        fs_step_t<double, false>((const double*)fpost, (double*)fout, type, link, visc, b, cur, rcur, tag, Fnew, pull, p,
-                                 s);
+        fs_step_t<double, false>((const double*)fpost, (double*)fout, type, link, slipw, visc, b, cur, rcur, tag, Fnew,
+                                 pull, p, s);
     else
-        fs_step_t<float, true>((const float*)fpost, (float*)fout, type, link, visc, b, cur, rcur, tag, Fnew, pull, p, s);
+        fs_step_t<float, true>((const float*)fpost, (float*)fout, type, link, slipw, visc, b, cur, rcur, tag, Fnew, pull,
+                               p, s);
 }
 
 void launch_fs_stream_view(Prec prec, const void* fin, void* fout, const uint8_t* type, const uint32_t* link,

@@ -31,6 +31,10 @@ struct FSBuffers {
     double* mnew = nullptr;                // [N] new mass per interface-list position
     uint8_t* state = nullptr;              // [N] filled / emptied / unchanged per interface-list position
     FSCounters* ctr = nullptr;
+    // [N] (step tag << 1) | active, written for every node whose activity the
+    // pass may change (only allocated with slip walls): the slip rule of the next
+    // collision's streaming tests the neighbours' activity before the pass
+    uint32_t* actPrev = nullptr;
 };
 
 // What the fused collision kernel reads and writes in free-surface mode.
@@ -50,8 +54,9 @@ void launch_fs_build_list(const uint8_t* type, FSBuffers& b, int cur, const LBKP
 // cur: which interface list is current; rcur: which density buffer holds the
 // last collision's density; Fnew: body force used to initialise new interface
 // nodes; pull: the populations are post-collision (false before the first collision).
-void launch_fs_step(Prec prec, const void* fpost, void* fout, uint8_t* type, uint32_t* link, double* visc, FSBuffers& b,
-                    int cur, int rcur, uint32_t tag, const double* Fnew, bool pull, const LBKParams& p, hipStream_t s);
+void launch_fs_step(Prec prec, const void* fpost, void* fout, uint8_t* type, uint32_t* link, uint32_t* slipw,
+                    double* visc, FSBuffers& b, int cur, int rcur, uint32_t tag, const double* Fnew, bool pull,
+                    const LBKParams& p, hipStream_t s);
 // Post-stream populations for the host view in free-surface mode (gas links
 // reconstructed from the stored interface velocity, LB.cpp:1237-1246).
 void launch_fs_stream_view(Prec prec, const void* fin, void* fout, const uint8_t* type, const uint32_t* link,

@@ -56,6 +56,23 @@ __device__ __forceinline__ uint32_t owner_slot(uint32_t w, uint32_t tag) {
     return (w >> SOLID_SLOT_BITS) == tag ? (w & SOLID_SLOT_MASK) : 0u;
 }
 
+// Population f_opp(j) streamed back into a node from a wall in direction j
+// (LB.cpp:1344-1456): half-way bounce-back of the node's own f_j (static or,
+// with zero wall velocity, moving walls); at a slip wall, on a diagonal link,
+// the reflection S1 f_slip(neighbour) + S2 f_j when exactly one of the two
+// checked neighbours is active. `fs` holds the post-collision planes, `own` is
+// the node's f_j and `active(n)` tells whether node n is fluid or interface.
+template <typename T, class A>
+__device__ __forceinline__ T wall_link(const T* __restrict__ fs, size_t PS, const LBKParams& p, const Coord& c,
+                                       bool slip, int j, T own, A&& active) {
+    if (!slip || j <= 6) return own;
+    const size_t n1 = nb_index(p, c, slip1check(j)), n2 = nb_index(p, c, slip2check(j));
+    const bool a1 = active(n1), a2 = active(n2);
+    if (a1 && !a2) return (T)(p.slipS1 * (double)fs[(size_t)slip1dir(j) * PS + n1] + p.slipS2 * (double)own);
+    if (!a1 && a2) return (T)(p.slipS1 * (double)fs[(size_t)slip2dir(j) * PS + n2] + p.slipS2 * (double)own);
+    return own;
+}
+
 __device__ __forceinline__ bool interior(const LBKParams& p, const Coord& c) {
     return c.x > 0 && c.x < p.X - 1 && c.y > 0 && c.y < p.Y - 1 && c.z > 0 && c.z < p.Z - 1;
 }

@@ -35,10 +35,6 @@ namespace {
 #ifndef HYB_LB_NT
 #define HYB_LB_NT 1
 #endif
-// FP32 path: two nodes per thread with 8-byte population accesses
-#ifndef HYB_FP32_X2
-#define HYB_FP32_X2 0
-#endif
 constexpr int BX = HYB_LB_BX;
 
 constexpr int BY = HYB_LB_BY;
@@ -75,6 +71,16 @@ __device__ __forceinline__ bool node_coord(const LBKParams& p, Coord& c) {
     return true;
 }
 
+// Whether node n was active when the reference streamed the populations this
+// collision pulls, i.e. before the free-surface pass that ran in between.
+__device__ __forceinline__ bool active_at_stream(const LBKParams& p, size_t n) {
+    if (p.actPrev && p.actTag) {
+        const uint32_t w = p.actPrev[n];
+        if ((w >> 1) == p.actTag) return (w & 1u) != 0u;
+    }
+    return (p.linkw[n] & LINK_ACTIVE) != 0u;
+}
+
 // Gather the 19 pre-collision populations of node c into f[]: f_i is pulled
 // from the neighbour x - c_i (32-bit node index into plane i). Links to static
 // walls bounce back the node's own f_opp(i) instead; they only occur next to
@@ -96,10 +102,16 @@ __device__ __forceinline__ void pull19(const T* __restrict__ fin, size_t PS, con
         f[i] = ld_pop(&fin[(size_t)i * PS + (i == 0 ? self : nb)]);
     }
     if (lw & LINK_WALLS) {
+        const uint32_t sw = (lw & LINK_SLIP) ? p.slipw[self] : 0u;
 #pragma unroll
         for (int i = 1; i < 19; ++i) {
             const int j = opp(i);
-            if (lw & (1u << j)) f[i] = ld_pop(&fin[(size_t)j * PS + self]);
+            if (!(lw & (1u << j))) continue;
+            const T own = ld_pop(&fin[(size_t)j * PS + self]);
+            if (sw & (1u << j))
+                f[i] = wall_link<T>(fin, PS, p, c, true, j, own, [&](size_t n) { return active_at_stream(p, n); });
+            else
+                f[i] = own;
         }
     }
 }
@@ -211,7 +223,7 @@ template <bool PULL, bool VARVISC, int COLL, bool FS>
 __global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ fin, double* __restrict__ fout,
                                                      const uint32_t* __restrict__ link,
                                                      const uint32_t* __restrict__ solid,
-                                                     const CoupleRec* __restrict__ cp, double* __restrict__ hf,
+                                                     const CoupleRec* __restrict__ cp, NodeForce64* __restrict__ hf,
                                                      double* __restrict__ visc, double* __restrict__ mac,
                                                      LBKParams p, FSView fsv) {
 #pragma clang fp contract(off)
@@ -232,15 +244,12 @@ __global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ 
     moments64(f, n, u);
     // penalty coupling force
     V3 hF(0.0, 0.0, 0.0);
-    const uint32_t sid = solid ? owner_slot(solid[c.idx], p.ctag) : 0u;
+    const uint32_t sw = solid ? solid[c.idx] : 0u;
+    const uint32_t sid = owner_slot(sw, p.ctag);
     if (sid) {
         const double mass = node_mass<FS, double>(lw, c.idx, fsv, p);
         const V3 d = penalty64(cp[sid - 1], c, mass, n, u, p, hF);
-        if (hf) {
-            hf[c.idx] = d.x;
-            hf[N + c.idx] = d.y;
-            hf[2 * N + c.idx] = d.z;
-        }
+        if (hf) hf[c.idx] = NodeForce64{{d.x, d.y, d.z}, (uint64_t)sw};
     }
     V3 Fb(p.F[0], p.F[1], p.F[2]);
     if (p.ffield) Fb = Fb + V3((double)p.ffield[c.idx], (double)p.ffield[N + c.idx], (double)p.ffield[2 * N + c.idx]);
@@ -341,7 +350,7 @@ __device__ __forceinline__ float cdotf(int j, float ux, float uy, float uz) {
 template <bool VARVISC, int COLL, bool FS>
 __device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uint32_t lw, size_t N,
                                           const uint32_t* __restrict__ solid, const CoupleRecF* __restrict__ cp,
-                                          float* __restrict__ hf, double* __restrict__ visc, double* __restrict__ mac,
+                                          NodeForce32* __restrict__ hf, double* __restrict__ visc, double* __restrict__ mac,
                                           const LBKParams& p, const FSView& fsv, bool doOut, float out[19]) {
 #pragma clang fp contract(fast)
     float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
@@ -365,7 +374,8 @@ __device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uin
         Tz += p.ffield[2 * N + c.idx];
     }
     float hx = 0.0f, hy = 0.0f, hz = 0.0f;
-    const uint32_t sid = solid ? owner_slot(solid[c.idx], p.ctag) : 0u;
+    const uint32_t sw = solid ? solid[c.idx] : 0u;
+    const uint32_t sid = owner_slot(sw, p.ctag);
     if (sid) {
         const CoupleRecF& r = cp[sid - 1];
         const float mass = (float)node_mass<FS, float>(lw, c.idx, fsv, p);
@@ -377,11 +387,7 @@ __device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uin
         const float lf = mass * irho;
         const float dx = lf * (ux - lvx), dy = lf * (uy - lvy), dz = lf * (uz - lvz);
         hx = -dx; hy = -dy; hz = -dz;
-        if (hf) {
-            hf[c.idx] = dx;
-            hf[N + c.idx] = dy;
-            hf[2 * N + c.idx] = dz;
-        }
+        if (hf) hf[c.idx] = NodeForce32{{dx, dy, dz}, sw};
         Tx += hx; Ty += hy; Tz += hz;
     }
     ux += 0.5f * Tx * irho;
@@ -474,7 +480,7 @@ template <bool PULL, bool VARVISC, int COLL, bool FS>
 __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ fin, float* __restrict__ fout,
                                                      const uint32_t* __restrict__ link,
                                                      const uint32_t* __restrict__ solid,
-                                                     const CoupleRecF* __restrict__ cp, float* __restrict__ hf,
+                                                     const CoupleRecF* __restrict__ cp, NodeForce32* __restrict__ hf,
                                                      double* __restrict__ visc, double* __restrict__ mac,
                                                      LBKParams p, FSView fsv) {
     Coord c;
@@ -493,83 +499,6 @@ __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ f
     if (!collide32<VARVISC, COLL, FS>(g, c, lw, N, solid, cp, hf, visc, mac, p, fsv, fout != nullptr, out)) return;
 #pragma unroll
     for (int j = 0; j < 19; ++j) st_pop(&fout[(size_t)j * p.PS + c.idx], out[j]);
-}
-
-// Two nodes per thread (x0 = 2 t, x0 + 1): the 19 pulls of both nodes come from
-// adjacent addresses, so each is one 8-byte load (4-byte aligned for the
-// x-shifted planes) and each store one aligned 8-byte store. Pairs that touch
-// the x ghost layers or an inactive node fall back to the per-node path.
-typedef float float2_a4 __attribute__((ext_vector_type(2), aligned(4)));
-typedef float float2_a8 __attribute__((ext_vector_type(2), aligned(8)));
-
-template <bool PULL, bool VARVISC, int COLL>
-__global__ void __launch_bounds__(BX / 2) k_fused32x2(const float* __restrict__ fin, float* __restrict__ fout,
-                                                      const uint32_t* __restrict__ link,
-                                                      const uint32_t* __restrict__ solid,
-                                                      const CoupleRecF* __restrict__ cp, float* __restrict__ hf,
-                                                      double* __restrict__ visc, LBKParams p) {
-    static_assert(BY == 1, "one lattice row per workgroup");
-    const int x0 = 2 * (int)(blockIdx.x * (BX / 2) + threadIdx.x);
-    const int y = blockIdx.y, z = blockIdx.z;
-    if (x0 >= p.X) return;
-    const size_t N = (size_t)p.X * p.Y * p.Z;
-    const FSView none{};
-    Coord ca;
-    ca.x = x0; ca.y = y; ca.z = z;
-    ca.idx = (size_t)x0 + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z);
-    Coord cb = ca;
-    cb.x = x0 + 1;
-    cb.idx = ca.idx + 1;
-    const uint32_t lwa = link[ca.idx];
-    const uint32_t lwb = x0 + 1 < p.X ? link[cb.idx] : 0u;
-    const bool vec = PULL && x0 >= 2 && x0 + 1 <= p.X - 3 && (lwa & LINK_ACTIVE) && (lwb & LINK_ACTIVE);
-    if (!vec) {
-        for (int k = 0; k < 2; ++k) {
-            const Coord& c = k ? cb : ca;
-            const uint32_t lw = k ? lwb : lwa;
-            if (c.x >= p.X || !(lw & LINK_ACTIVE)) continue;
-            float g[19], out[19];
-            if (PULL) pull19<float>(fin, p.PS, p, c, lw, g);
-            else {
-#pragma unroll
-                for (int j = 0; j < 19; ++j) g[j] = ld_pop(&fin[(size_t)j * p.PS + c.idx]);
-            }
-            collide32<VARVISC, COLL, false>(g, c, lw, N, solid, cp, hf, visc, nullptr, p, none, true, out);
-#pragma unroll
-            for (int j = 0; j < 19; ++j) st_pop(&fout[(size_t)j * p.PS + c.idx], out[j]);
-        }
-        return;
-    }
-    const Nb3 ny = nb3(y, p.Y, p.per[1]);
-    const Nb3 nz = nb3(z, p.Z, p.per[2]);
-    const uint32_t sX = (uint32_t)p.X, sXY = (uint32_t)p.X * (uint32_t)p.Y;
-    float ga[19], gb[19];
-#pragma unroll
-    for (int i = 0; i < 19; ++i) {
-        const int j = opp(i);  // pulled from the neighbour in direction j
-        const uint32_t src = (uint32_t)(x0 + cx(j)) + sX * (uint32_t)pick(ny, cy(j)) + sXY * (uint32_t)pick(nz, cz(j));
-        const float2_a4 v = __builtin_nontemporal_load((const float2_a4*)&fin[(size_t)i * p.PS + src]);
-        ga[i] = v.x;
-        gb[i] = v.y;
-    }
-    if ((lwa | lwb) & LINK_WALLS) {
-#pragma unroll
-        for (int i = 1; i < 19; ++i) {
-            const int j = opp(i);
-            if (lwa & (1u << j)) ga[i] = ld_pop(&fin[(size_t)j * p.PS + ca.idx]);
-            if (lwb & (1u << j)) gb[i] = ld_pop(&fin[(size_t)j * p.PS + cb.idx]);
-        }
-    }
-    float oa[19], ob[19];
-    collide32<VARVISC, COLL, false>(ga, ca, lwa, N, solid, cp, hf, visc, nullptr, p, none, true, oa);
-    collide32<VARVISC, COLL, false>(gb, cb, lwb, N, solid, cp, hf, visc, nullptr, p, none, true, ob);
-#pragma unroll
-    for (int j = 0; j < 19; ++j) {
-        float2_a8 v;
-        v.x = oa[j];
-        v.y = ob[j];
-        __builtin_nontemporal_store(v, (float2_a8*)&fout[(size_t)j * p.PS + ca.idx]);
-    }
 }
 
 // Standalone pull streaming: turns the post-collision state into the
@@ -650,18 +579,24 @@ __global__ void k_remove_isolated(const uint8_t* __restrict__ tin, uint8_t* __re
     tout[c.idx] = t;
 }
 
-__global__ void k_init_links(const uint8_t* __restrict__ type, uint32_t* __restrict__ link, LBKParams p) {
+__global__ void k_init_links(const uint8_t* __restrict__ type, uint32_t* __restrict__ link, uint32_t* __restrict__ slipw,
+                             LBKParams p) {
     Coord c;
     if (!node_coord(p, c)) return;
     const uint8_t t = type[c.idx];
-    uint32_t lw = 0;
+    uint32_t lw = 0, sw = 0;
     if ((t == T_FLUID || t == T_INTERFACE) && interior(p, c)) {
         lw |= LINK_ACTIVE;
         if (t == T_INTERFACE) lw |= LINK_INTERFACE;
-        for (int j = 1; j < 19; ++j)
-            if (type[nb_index(p, c, j)] == T_WALL) lw |= (1u << j);
+        for (int j = 1; j < 19; ++j) {
+            const uint8_t tl = type[nb_index(p, c, j)];
+            if (is_plane_wall(tl)) lw |= (1u << j);
+            if (is_slip_wall(tl)) sw |= (1u << j);
+        }
+        if (sw) lw |= LINK_SLIP;
     }
     link[c.idx] = lw;
+    if (slipw) slipw[c.idx] = sw;
 }
 
 // Largest coordinates of active nodes (LB.cpp:913-921). Wavefront maxima are
@@ -752,99 +687,108 @@ __device__ __forceinline__ NodeBox slot_box(const CoupleRec& r, const LBKParams&
     return b;
 }
 
-// Threads per particle slot in the classification and force-sum kernels: one
-// wavefront (64) or one workgroup of four wavefronts (256).
-#ifndef HYB_SLOT_GROUP
-#define HYB_SLOT_GROUP 64
-#endif
-constexpr int SG = HYB_SLOT_GROUP;
-// rows per batch of the force-sum iterator
+// One wavefront per particle slot in the classification and force-sum kernels;
+// each wavefront keeps its row table in LDS (SLOT_WAVES wavefronts per workgroup).
+constexpr int SLOT_WAVES = 4;
+// nodes per lane whose loads are issued together in the slot kernels
 #ifndef HYB_SLOT_BATCH
-#define HYB_SLOT_BATCH 1
+#define HYB_SLOT_BATCH 4
 #endif
+constexpr int CH = HYB_SLOT_BATCH;
 
-// Visits the nodes of a slot's box with SG threads: lanes run along x within a
-// row (16, 32 or 64 lanes per row, so loads coalesce) and the group covers
-// several rows per iteration. Rows whose (y, z) distance already reaches the
-// radius are skipped: rounded sums of non-negative squares are monotone, so no
-// node of such a row passes the strict inside test.
+struct SlotRows {  // LDS of one wavefront: inclusive node counts and first x of 64 rows
+    int scan[64];
+    int xa[64];
+};
+
+// Hands the nodes of a slot's box strictly inside its sphere (vector.cpp:153-158)
+// to the lanes of one wavefront, CH nodes per lane at a time (ok[k] marks the
+// valid entries): the caller issues the loads of a batch before using any of
+// them. Rows are taken 64 at a time; each row contributes the x interval that
+// can hold inside nodes (from the circle radius with one node of margin on each
+// side, every candidate is then tested exactly), and a wavefront prefix sum of
+// the interval lengths spreads the candidates evenly over the lanes. A row whose
+// (y, z) distance already reaches the radius holds no inside node: rounded sums
+// of non-negative squares are monotone.
 template <class F>
-__device__ __forceinline__ void for_slot_nodes(const NodeBox& b, int lane, const V3& cen, double rad, F&& f) {
+__device__ __forceinline__ void for_slot_batches(const NodeBox& b, int lane, SlotRows& sh, const V3& cen, double rad,
+                                                 const LBKParams& p, F&& f) {
+#pragma clang fp contract(off)
     const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
     if (nx <= 0 || ny <= 0 || nz <= 0) return;
-    const int shift = nx <= 16 ? 4 : (nx <= 32 ? 5 : 6);
-    const int lpr = 1 << shift;
-    const int rpi = SG >> shift;
-    const int xo = lane & (lpr - 1);
-    const int rsub = lane >> shift;
     const int rows = ny * nz;
-    for (int base = 0; base < rows; base += rpi) {
-        const int row = base + rsub;
-        if (row >= rows) continue;
-        const int y = b.lo[1] + row % ny;
-        const int z = b.lo[2] + row / ny;
-        const double dy = (double)y - cen.y, dz = (double)z - cen.z;
-        if (!(dy * dy + dz * dz < rad * rad)) continue;
-        for (int xx = xo; xx < nx; xx += lpr) f(b.lo[0] + xx, y, z);
-    }
-}
-
-// The nodes of a slot's box owned by the slot (ownership word == want), visited
-// in the same per-thread order as for_slot_nodes, but CH rows at a time: the CH
-// ownership loads of a batch are issued together, so a thread waits for memory
-// once per batch instead of once per row.
-template <int CH, class F>
-__device__ __forceinline__ void for_owned_nodes(const NodeBox& b, int lane, const V3& cen, double rad, uint32_t want,
-                                                const uint32_t* __restrict__ solid, const LBKParams& p, F&& f) {
-    const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
-    if (nx <= 0 || ny <= 0 || nz <= 0) return;
-    const int shift = nx <= 16 ? 4 : (nx <= 32 ? 5 : 6);
-    const int lpr = 1 << shift;
-    const int rpi = SG >> shift;
-    const int xo = lane & (lpr - 1);
-    const int rsub = lane >> shift;
-    const int rows = ny * nz;
-    for (int base = 0; base < rows; base += rpi * CH) {
-        for (int xx = xo; xx < nx; xx += lpr) {
+    const double r2 = rad * rad;
+    for (int base = 0; base < rows; base += 64) {
+        const int r = base + lane;
+        int xa = 0, cnt = 0;
+        if (r < rows) {
+            const double dy = (double)(b.lo[1] + r % ny) - cen.y, dz = (double)(b.lo[2] + r / ny) - cen.z;
+            const double s2 = dy * dy + dz * dz;
+            if (s2 < r2) {
+                const double w = sqrt(r2 - s2);
+                const int a = max(b.lo[0], (int)ceil(cen.x - w) - 1);
+                const int c = min(b.hi[0], (int)floor(cen.x + w) + 1);
+                if (c >= a) {
+                    xa = a;
+                    cnt = c - a + 1;
+                }
+            }
+        }
+        int incl = cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
+        }
+        const int total = __shfl(incl, 63, 64);
+        sh.scan[lane] = incl;
+        sh.xa[lane] = xa;
+        __builtin_amdgcn_wave_barrier();
+        for (int t0 = 0; t0 < total; t0 += 64 * CH) {
             size_t idx[CH];
-            int yy[CH], zz[CH];
+            int xs[CH], ys[CH], zs[CH];
             bool ok[CH];
 #pragma unroll
             for (int k = 0; k < CH; ++k) {
-                const int row = base + k * rpi + rsub;
+                const int t = t0 + k * 64 + lane;
                 ok[k] = false;
                 idx[k] = 0;
-                yy[k] = zz[k] = 0;
-                if (row < rows) {
-                    const int y = b.lo[1] + row % ny, z = b.lo[2] + row / ny;
-                    const double dy = (double)y - cen.y, dz = (double)z - cen.z;
-                    if (dy * dy + dz * dz < rad * rad) {
-                        ok[k] = true;
-                        yy[k] = y;
-                        zz[k] = z;
-                        idx[k] = (size_t)(b.lo[0] + xx) + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z);
+                xs[k] = ys[k] = zs[k] = 0;
+                if (t < total) {
+                    int lo = 0, hi = 63;  // first row whose inclusive count exceeds t
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (sh.scan[mid] > t) hi = mid;
+                        else lo = mid + 1;
                     }
+                    const int row = base + lo;
+                    const int x = sh.xa[lo] + (t - (lo ? sh.scan[lo - 1] : 0));
+                    const int y = b.lo[1] + row % ny, z = b.lo[2] + row / ny;
+                    const V3 d = V3((double)x, (double)y, (double)z) - cen;
+                    ok[k] = d.norm2() < r2;
+                    xs[k] = x;
+                    ys[k] = y;
+                    zs[k] = z;
+                    idx[k] = (size_t)x + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z);
                 }
             }
-            uint32_t w[CH];
-#pragma unroll
-            for (int k = 0; k < CH; ++k) w[k] = ok[k] ? solid[idx[k]] : 0u;
-#pragma unroll
-            for (int k = 0; k < CH; ++k)
-                if (ok[k] && w[k] == want) f(b.lo[0] + xx, yy[k], zz[k], idx[k]);
+            f(idx, xs, ys, zs, ok);
         }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
-__global__ void k_classify_set(const int* __restrict__ nslots_p, const CoupleRec* __restrict__ cp,
+__global__ void __launch_bounds__(256) k_classify_set(const int* __restrict__ nslots_p, const CoupleRec* __restrict__ cp,
                                NodeBox* __restrict__ boxes, int* __restrict__ nboxes, uint32_t* __restrict__ solid,
                                const uint32_t* __restrict__ link, LBKParams p) {
 #pragma clang fp contract(off)
+    __shared__ SlotRows shr[SLOT_WAVES];
     const int ns = *nslots_p;
     if (blockIdx.x == 0 && threadIdx.x == 0) *nboxes = ns;
-    const int lane = SG == 64 ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
-    const int unit = SG == 64 ? (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) : (int)blockIdx.x;
-    const int nunits = SG == 64 ? (int)((gridDim.x * blockDim.x) >> 6) : (int)gridDim.x;
+    const int lane = (int)(threadIdx.x & 63);
+    const int unit = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nunits = (int)((gridDim.x * blockDim.x) >> 6);
+    SlotRows& sh = shr[threadIdx.x >> 6];
     for (int s = unit; s < ns; s += nunits) {
         const CoupleRec r = cp[s];
         const NodeBox b = slot_box(r, p);
@@ -852,11 +796,13 @@ __global__ void k_classify_set(const int* __restrict__ nslots_p, const CoupleRec
         const V3 cen(r.x0[0] / p.unitLength, r.x0[1] / p.unitLength, r.x0[2] / p.unitLength);
         const double rad = r.r / p.unitLength;
         const uint32_t val = (p.ctag << SOLID_SLOT_BITS) | (uint32_t)(s + 1);
-        for_slot_nodes(b, lane, cen, rad, [&](int x, int y, int z) {
-            const V3 d = V3((double)x, (double)y, (double)z) - cen;
-            if (!(d.norm2() < rad * rad)) return;  // vector.cpp:153-158
-            const size_t idx = (size_t)x + (size_t)p.X * ((size_t)y + (size_t)p.Y * (size_t)z);
-            if (link[idx] & LINK_ACTIVE) atomicMax(&solid[idx], val);
+        for_slot_batches(b, lane, sh, cen, rad, p, [&](const size_t* idx, const int*, const int*, const int*, const bool* ok) {
+            uint32_t lk[CH];
+#pragma unroll
+            for (int k = 0; k < CH; ++k) lk[k] = ok[k] ? link[idx[k]] : 0u;
+#pragma unroll
+            for (int k = 0; k < CH; ++k)
+                if (ok[k] && (lk[k] & LINK_ACTIVE)) atomicMax(&solid[idx[k]], val);
         });
     }
 }
@@ -874,31 +820,30 @@ struct KSum {
 };
 
 // Per particle slot: sum over the nodes it owns of diffVel, radius x diffVel and
-// the node mass (LB.cpp:1897-1902). SG threads per slot; each keeps
-// Neumaier-compensated FP64 sums, merged by a fixed butterfly within each
-// wavefront and then in wavefront order (deterministic, no atomics). Only nodes strictly inside the sphere can be
-// owned by the slot, so the others are skipped before any load.
+// the node mass (LB.cpp:1897-1902). One wavefront per slot; each lane keeps
+// Neumaier-compensated FP64 sums, merged by a fixed butterfly (deterministic,
+// no atomics). Only nodes strictly inside the sphere can be owned by the slot,
+// so the others are skipped before any load.
 __device__ __forceinline__ void ksum_merge(KSum& a, double s2, double c2) {
     a.add(s2);
     a.c += c2;
 }
 
-template <typename HT>
+template <typename REC, typename HT>
 __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ nslots_p, const NodeBox* __restrict__ boxes,
                                                         const CoupleRec* __restrict__ cp,
-                                                        const uint32_t* __restrict__ solid,
-                                                        const uint32_t* __restrict__ link, const HT* __restrict__ hf,
+                                                        const uint32_t* __restrict__ link, const REC* __restrict__ hf,
                                                         const double* __restrict__ fsMass,
                                                         const HT* __restrict__ fsRho,
                                                         HydroPartial* __restrict__ out, LBKParams p) {
 #pragma clang fp contract(off)
-    __shared__ double sh[4][7][2];
+    __shared__ SlotRows shr[SLOT_WAVES];
     const int ns = *nslots_p;
-    const size_t N = (size_t)p.X * p.Y * p.Z;
     const int wl = threadIdx.x & 63;
-    const int lane = SG == 64 ? wl : (int)threadIdx.x;
-    const int unit = SG == 64 ? (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) : (int)blockIdx.x;
-    const int nunits = SG == 64 ? (int)((gridDim.x * blockDim.x) >> 6) : (int)gridDim.x;
+    const int lane = wl;
+    const int unit = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nunits = (int)((gridDim.x * blockDim.x) >> 6);
+    SlotRows& sh = shr[threadIdx.x >> 6];
     for (int s = unit; s < ns; s += nunits) {
         const CoupleRec r = cp[s];
         const NodeBox b = boxes[s];
@@ -909,16 +854,30 @@ __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ 
         const V3 rv(r.rv[0], r.rv[1], r.rv[2]);
         const V3 cen = x0 / p.unitLength;
         const double rad = r.r / p.unitLength;
-        for_owned_nodes<HYB_SLOT_BATCH>(b, lane, cen, rad, want, solid, p, [&](int x, int y, int z, size_t idx) {
-            const V3 d((double)hf[idx], (double)hf[N + idx], (double)hf[2 * N + idx]);
-            const V3 radius = V3((double)x, (double)y, (double)z) - x0 / p.unitLength + rv / p.unitLength;
-            const V3 m = radius.cross(d);
-            const bool itf = (link[idx] & LINK_INTERFACE) != 0;
-            const double mass = fsMass ? (itf ? fsMass[idx] : (double)fsRho[idx])
-                                       : (itf ? 0.5 * p.initDensity : p.initDensity);
-            acc[0].add(d.x); acc[1].add(d.y); acc[2].add(d.z);
-            acc[3].add(m.x); acc[4].add(m.y); acc[5].add(m.z);
-            acc[6].add(mass);
+        for_slot_batches(b, lane, sh, cen, rad, p, [&](const size_t* idx, const int* xs, const int* ys, const int* zs,
+                                                   const bool* ok) {
+            REC rec[CH];
+#pragma unroll
+            for (int k = 0; k < CH; ++k) {
+                if (ok[k]) rec[k] = hf[idx[k]];
+                else rec[k].own = 0;
+            }
+#pragma unroll
+            for (int k = 0; k < CH; ++k) {
+                // only records written with this coupling step's ownership word count
+                if (!ok[k] || rec[k].own != want) continue;
+                const V3 d((double)rec[k].d[0], (double)rec[k].d[1], (double)rec[k].d[2]);
+                const V3 radius = V3((double)xs[k], (double)ys[k], (double)zs[k]) - x0 / p.unitLength + rv / p.unitLength;
+                const V3 m = radius.cross(d);
+                double mass = p.initDensity;  // without free surface every node keeps rho0 (LB.cpp:147, 935)
+                if (fsMass) {
+                    const bool itf = (link[idx[k]] & LINK_INTERFACE) != 0;
+                    mass = itf ? fsMass[idx[k]] : (double)fsRho[idx[k]];
+                }
+                acc[0].add(d.x); acc[1].add(d.y); acc[2].add(d.z);
+                acc[3].add(m.x); acc[4].add(m.y); acc[5].add(m.z);
+                acc[6].add(mass);
+            }
         });
         for (int off = 32; off > 0; off >>= 1) {
             for (int k = 0; k < 7; ++k) {
@@ -934,26 +893,8 @@ __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ 
                 }
             }
         }
-        if (SG == 64) {
-            if (wl == 0)
-                for (int k = 0; k < 7; ++k) out[s].s[k] = acc[k].s + acc[k].c;
-        } else {
-            // the four wavefront sums, merged in wavefront order
-            if (wl == 0)
-                for (int k = 0; k < 7; ++k) {
-                    sh[threadIdx.x >> 6][k][0] = acc[k].s;
-                    sh[threadIdx.x >> 6][k][1] = acc[k].c;
-                }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                for (int k = 0; k < 7; ++k) {
-                    KSum a{sh[0][k][0], sh[0][k][1]};
-                    for (int w = 1; w < 4; ++w) ksum_merge(a, sh[w][k][0], sh[w][k][1]);
-                    out[s].s[k] = a.s + a.c;
-                }
-            }
-            __syncthreads();
-        }
+        if (wl == 0)
+            for (int k = 0; k < 7; ++k) out[s].s[k] = acc[k].s + acc[k].c;
     }
 }
 
@@ -972,20 +913,17 @@ dim3 lattice_grid(const LBKParams& p) { return dim3((p.X + BX - 1) / BX, (p.Y + 
         if (prec == Prec::F64) {                                                                                   \
             if (fs)                                                                                                \
                 k_fused64<PULL, VV, CO, true><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link, solid, cp,  \
-                                                              (double*)hf, visc, mac, p, *fs);                     \
+                                                              (NodeForce64*)hf, visc, mac, p, *fs);                     \
             else                                                                                                   \
                 k_fused64<PULL, VV, CO, false><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link, solid, cp, \
-                                                               (double*)hf, visc, mac, p, none);                   \
+                                                               (NodeForce64*)hf, visc, mac, p, none);                   \
         } else {                                                                                                   \
             if (fs)                                                                                                \
                 k_fused32<PULL, VV, CO, true><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid, cpf,   \
-                                                              (float*)hf, visc, mac, p, *fs);                      \
-            else if (HYB_FP32_X2 && fout && !mac)                                                                  \
-                k_fused32x2<PULL, VV, CO><<<g, dim3(BX / 2, BY, 1), 0, s>>>((const float*)fin, (float*)fout, link, \
-                                                                            solid, cpf, (float*)hf, visc, p);      \
+                                                              (NodeForce32*)hf, visc, mac, p, *fs);                      \
             else                                                                                                   \
                 k_fused32<PULL, VV, CO, false><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid, cpf,  \
-                                                               (float*)hf, visc, mac, p, none);                    \
+                                                               (NodeForce32*)hf, visc, mac, p, none);                    \
         }                                                                                                          \
         return;                                                                                                    \
     }
@@ -1029,9 +967,9 @@ void launch_lb_init_interface(uint8_t* type, const LBKParams& p, hipStream_t s) 
     (void)hipFree(tmp);
 }
 
-void launch_lb_init_links(const uint8_t* type, uint32_t* link, const LBKParams& p, hipStream_t s) {
+void launch_lb_init_links(const uint8_t* type, uint32_t* link, uint32_t* slipw, const LBKParams& p, hipStream_t s) {
     const dim3 g = lattice_grid(p), b(BX, BY, 1);
-    k_init_links<<<g, b, 0, s>>>(type, link, p);
+    k_init_links<<<g, b, 0, s>>>(type, link, slipw, p);
 }
 
 void launch_lb_max_active(const uint32_t* link, int* mx, const LBKParams& p, hipStream_t s) {
@@ -1053,23 +991,23 @@ void launch_lb_init_pops(Prec prec, void* f, const uint32_t* link, const int* mx
 void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, NodeBox* boxes, int* nboxes,
                          uint32_t* solid, const uint32_t* link, const LBKParams& p, hipStream_t s) {
     if (max_slots <= 0) return;
-    int grid = SG == 64 ? (max_slots + 3) / 4 : max_slots;  // slots per workgroup: 4 or 1
-    if (grid > 8192) grid = 8192;
-    k_classify_set<<<grid, 256, 0, s>>>(nslots, cp, boxes, nboxes, solid, link, p);
+    int grid = (max_slots + SLOT_WAVES - 1) / SLOT_WAVES;  // one wavefront per slot
+    if (grid > 16384) grid = 16384;
+    k_classify_set<<<grid, 64 * SLOT_WAVES, 0, s>>>(nslots, cp, boxes, nboxes, solid, link, p);
 }
 
 void launch_hydro_partials(const int* nslots, int max_slots, const NodeBox* boxes, const CoupleRec* cp,
-                           const uint32_t* solid, const uint32_t* link, Prec prec, const void* hf, const double* fsMass,
-                           const void* fsRho, HydroPartial* out, const LBKParams& p, hipStream_t s) {
+                           const uint32_t* link, Prec prec, const void* hf, const double* fsMass, const void* fsRho,
+                           HydroPartial* out, const LBKParams& p, hipStream_t s) {
     if (max_slots <= 0) return;
-    int grid = SG == 64 ? (max_slots + 3) / 4 : max_slots;  // slots per workgroup: 4 or 1
-    if (grid > 8192) grid = 8192;
+    int grid = (max_slots + SLOT_WAVES - 1) / SLOT_WAVES;  // one wavefront per slot
+    if (grid > 16384) grid = 16384;
     if (prec == Prec::F64)
-        k_hydro_partials<double><<<grid, 256, 0, s>>>(nslots, boxes, cp, solid, link, (const double*)hf, fsMass,
-                                                      (const double*)fsRho, out, p);
+        k_hydro_partials<NodeForce64, double><<<grid, 64 * SLOT_WAVES, 0, s>>>(
+            nslots, boxes, cp, link, (const NodeForce64*)hf, fsMass, (const double*)fsRho, out, p);
     else
-        k_hydro_partials<float><<<grid, 256, 0, s>>>(nslots, boxes, cp, solid, link, (const float*)hf, fsMass,
-                                                     (const float*)fsRho, out, p);
+        k_hydro_partials<NodeForce32, float><<<grid, 64 * SLOT_WAVES, 0, s>>>(
+            nslots, boxes, cp, link, (const NodeForce32*)hf, fsMass, (const float*)fsRho, out, p);
 }
 
 void launch_fill_u32(uint32_t* a, uint32_t v, size_t n, hipStream_t s) {

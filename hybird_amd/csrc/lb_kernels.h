@@ -24,6 +24,13 @@ struct LBKParams {
     double initDensity;
     uint32_t ctag;       // tag of the current coupling step in the node ownership words
     const float* ffield; // optional per-node body force [3][N] added to F (nullptr: none)
+    // slip walls: S1 = slipCoefficient, S2 = 1 - S1 (LB.cpp:1154-1155), per-node slip
+    // directions (bit j) and the link words (neighbour activity)
+    double slipS1, slipS2;
+    const uint32_t* slipw;
+    const uint32_t* linkw;
+    const uint32_t* actPrev;  // free surface with slip walls: activity before the last pass, see FSBuffers
+    uint32_t actTag;          // tag of that pass; 0 = no pass since the last collision
 };
 
 constexpr int SOLID_SLOT_BITS = 20;
@@ -58,6 +65,18 @@ struct NodeBox {
     int hi[3];
 };
 
+// Penalty force of a particle node (LB.cpp:1880-1890) as written by the
+// collision: the force and the node's ownership word of that step, so that the
+// per-slot sums read one record per node and recognise current entries by it.
+struct NodeForce32 {
+    float d[3];
+    uint32_t own;
+};
+struct NodeForce64 {
+    double d[3];
+    uint64_t own;
+};
+
 // Partial hydrodynamic sums of one particle slot: F(3), M(3), fluidVolume.
 struct HydroPartial {
     double s[7];
@@ -75,7 +94,8 @@ void launch_lb_stream(Prec prec, const void* fin, void* fout, const uint32_t* li
 void launch_lb_init_types(uint8_t* type, const int* boundary, int nwalls, const WallDev* walls, int problem,
                           const double* liquidBox, const LBKParams& p, hipStream_t s);
 void launch_lb_init_interface(uint8_t* type, const LBKParams& p, hipStream_t s);
-void launch_lb_init_links(const uint8_t* type, uint32_t* link, const LBKParams& p, hipStream_t s);
+// slipw: per-node slip-wall directions (nullptr when no slip walls exist)
+void launch_lb_init_links(const uint8_t* type, uint32_t* link, uint32_t* slipw, const LBKParams& p, hipStream_t s);
 void launch_lb_max_active(const uint32_t* link, int* maxXYZ, const LBKParams& p, hipStream_t s);
 // With free surface, also the initial density (rho), interface mass and interface velocity.
 void launch_lb_init_pops(Prec prec, void* f, const uint32_t* link, const int* maxXYZ, double* visc, const double* u0,
@@ -83,8 +103,9 @@ void launch_lb_init_pops(Prec prec, void* f, const uint32_t* link, const int* ma
 void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, NodeBox* boxes, int* nboxes,
                          uint32_t* solid, const uint32_t* link, const LBKParams& p, hipStream_t s);
 // fsMass/fsRho: free-surface node masses (interface mass, previous density), nullptr without free surface
+// hf: NodeForce64 (F64) or NodeForce32 (F32S) records written by the collision
 void launch_hydro_partials(const int* nslots, int max_slots, const NodeBox* boxes, const CoupleRec* cp,
-                           const uint32_t* solid, const uint32_t* link, Prec prec, const void* hf, const double* fsMass,
+                           const uint32_t* link, Prec prec, const void* hf, const double* fsMass,
                            const void* fsRho, HydroPartial* out, const LBKParams& p, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_count_active(const uint32_t* link, unsigned long long* count, const LBKParams& p, hipStream_t s);

@@ -13,6 +13,7 @@ class IO;
 
 class DEM {
 public:
+    energy particleEnergy;
     double demInitialRepeat = 0.0;
     double deltat = 1.0;
     double demTime = 0.0;
@@ -41,7 +42,7 @@ public:
     void discreteElementGet(GetPot& lbmCfgFile, GetPot& command_line);
     void discreteElementInit(unsIntList& boundary, unsIntList& lbSize, measureUnits& unit, tVect& lbF);
 
-    // Copies element state (positions, velocities, forces) from the device into elmts.
+    // Copies element state (positions, velocities, forces) and particleEnergy from the device.
     void syncHost();
     std::vector<double> field(const char* name) const;
     hyb_dem* handle() const { return h_; }

@@ -79,6 +79,18 @@ public:
 };
 
 // Unit system (node.cpp:476-488): base units and the composites derived from them.
+// Particle energies (utils.h:165-190); DEM::particleEnergy is refreshed from the
+// device by DEM::syncHost.
+class energy {
+public:
+    double kin = 0.0, rotKin = 0.0, trKin = 0.0, grav = 0.0, elastic = 0.0, total = 0.0;
+    void reset() { kin = rotKin = trKin = grav = elastic = total = 0.0; }
+    void updateTotal() {
+        kin = rotKin + trKin;
+        total = rotKin + trKin + grav + elastic;
+    }
+};
+
 class measureUnits {
 public:
     double Length = 1.0, Time = 1.0, Density = 1.0;

@@ -398,10 +398,17 @@ void DEM::discreteElementStep(IO& io) {  // DEM.cpp:331-376
 }
 
 void DEM::syncHost() {
-    std::vector<double> x, w, fr, misc;
+    std::vector<double> x, w, fr, misc, en;
     if (!view(h_, hyb_dem_get, "e_x", x) || !view(h_, hyb_dem_get, "e_w", w) ||
-        !view(h_, hyb_dem_get, "e_forces", fr) || !view(h_, hyb_dem_get, "e_misc", misc))
+        !view(h_, hyb_dem_get, "e_forces", fr) || !view(h_, hyb_dem_get, "e_misc", misc) ||
+        !view(h_, hyb_dem_get, "energy", en) || en.size() != 6)
         hybird_check(HYB_E_STATE, "DEM::syncHost", __FILE__, __LINE__);
+    particleEnergy.kin = en[0];
+    particleEnergy.rotKin = en[1];
+    particleEnergy.trKin = en[2];
+    particleEnergy.grav = en[3];
+    particleEnergy.elastic = en[4];
+    particleEnergy.total = en[5];
     for (size_t e = 0; e < elmts.size(); ++e) {
         elmt& el = elmts[e];
         const double* X = &x[e * 18];
@@ -443,7 +450,6 @@ void IO::initialize() {  // IO.cpp:25-130 (status file only)
 }
 
 void IO::outputStep(const double& demTime, const LB& lb, const DEM& dem) {  // IO.cpp:132-198
-    (void)dem;
     realTime = demTime;
     const unsigned int screenExpCounter =
         (screenExpTime > 0 ? static_cast<unsigned int>(realTime / screenExpTime) + 1 : 0);
@@ -456,6 +462,15 @@ void IO::outputStep(const double& demTime, const LB& lb, const DEM& dem) {  // I
             const double massTot = lb.totalFluidMass();
             cout << "Volume=" << massTot * lb.unit.Volume << "; Mass = " << massTot * lb.unit.Mass << "\t";
             exportFile << "Volume=" << massTot * lb.unit.Volume << "; Mass = " << massTot * lb.unit.Mass << "\t";
+        }
+        if (dem.elmts.size() && !lbmSolve) {  // exportEnergy (IO.cpp:919-927)
+            const std::vector<double> en = dem.field("energy");
+            cout << "eKin = " << en[0] << "\t";
+            exportFile << "eKin = " << en[0] << "\t";
+            cout << "eGrav = " << en[3] << "\t";
+            exportFile << "eGrav = " << en[3] << "\t";
+            cout << "eTot = " << en[5] << "\t";
+            exportFile << "eTot = " << en[5] << "\t";
         }
         cout << "\n";
         exportFile << "\n";

@@ -29,7 +29,7 @@ DEM_FIELDS = {"e_x": (np.float64, (-1, 6, 3)), "e_xp": (np.float64, (-1, 6, 3)),
               "e_forces": (np.float64, (-1, 7, 3)), "e_misc": (np.float64, (-1, 19)),
               "p_data": (np.float64, (-1, 10)), "p_idx": (np.uint32, (-1, 4)), "neighborTable": (np.uint32, None),
               "nearWallTable": (np.uint32, None), "ghostIndex": (np.uint32, None), "ghostVec": (np.float64, (-1, 3)),
-              "dem_scalars": (np.float64, None), "rebuilds": (np.float64, None)}
+              "dem_scalars": (np.float64, None), "rebuilds": (np.float64, None), "energy": (np.float64, None)}
 
 # every extern "C" entry point declared in include/hybird_amd.h
 EXPORTS = ["hyb_config_defaults", "hyb_last_error", "hyb_device_count", "hyb_sync", "hyb_dem_create", "hyb_dem_init",

@@ -145,7 +145,8 @@ int hyb_cycle(hyb_lb* lb, hyb_dem* dem, int32_t n);
  *       (FHydro, MHydro, FParticle, MParticle, FWall, MWall, FGrav), "e_misc" f64[E][19],
  *       "p_data" f64[P][10] (x0, x1, radiusVec, r), "p_idx" u32[P][4],
  *       "neighborTable" u32[2*pairs], "nearWallTable" u32[2*k], "ghostIndex" u32[G], "ghostVec" f64[G][3],
- *       "dem_scalars" f64[17]
+ *       "dem_scalars" f64[17], "energy" f64[6] (DEM::particleEnergy after the last sub-step: kin, rotKin,
+ *       trKin, grav, elastic, total; DEM.cpp:2226-2276)
  * *bytes_needed receives the size; nothing is copied when capacity is too small.
  * N = lbSize product; E elements; P particles including ghosts. */
 int hyb_lb_get(hyb_lb* lb, const char* name, void* host, size_t capacity, size_t* bytes_needed);

@@ -181,6 +181,8 @@ struct DEM {
     uint32_t stdParticles = 0;
     std::vector<Ghost> ghosts;
     bool newNeighborList = false;
+    // particleEnergy (utils.h:165-190): kin, rotKin, trKin, grav, elastic, total
+    double energy[6] = {0, 0, 0, 0, 0, 0};
     // values the reference freezes in function-local statics at the first call
     double neighListTrigger = 0.0;
     double c1[5], c2[5], coeff1ord[6], coeff2ord[6];
@@ -207,6 +209,7 @@ struct DEM {
     void evaluateForces();
     void updateParticlesPredicted();
     void updateParticlesCorrected();
+    void updateEnergy();
     void particleParticleCollision(const Particle& pi, const Particle& pj, const V3& vectorDistance);
     void wallParticleCollision(Wall& w, const Particle& pj, double overlap);
     double normalContact(double overlap, double vrel, double effRad, double effMass) const;
@@ -268,7 +271,7 @@ struct LB {
     void initializeLists();
     void resetLists();
     void initializeVariables();
-    void initializeWalls();
+    void initializeWalls(const std::vector<Wall>& walls);
     void cleanLists();
     void reconstruction();
     void computeHydroForces(DEM& dem);

@@ -181,6 +181,7 @@ long orc_get(void* h, const char* cname, void* out, long cap) {
                                  dem.mat.dampCoeff, dem.mat.knConst, dem.mat.ksConst, dem.newNeighborList ? 1.0 : 0.0};
         return put(v, out, cap);
     }
+    if (name == "energy") return put(std::vector<double>(dem.energy, dem.energy + 6), out, cap);
     if (name == "walls") {
         std::vector<double> v;
         for (auto& w : dem.walls) { p3(v, w.p); p3(v, w.n); p3(v, w.FParticle); p3(v, w.FHydro); }

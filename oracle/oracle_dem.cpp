@@ -623,7 +623,39 @@ void DEM::step() {  // DEM.cpp:331-376
         evaluateForces();
         corrector();
         updateParticlesCorrected();
+        updateEnergy();
     }
+}
+
+// DEM.cpp:2226-2276 with energy::updateTotal (utils.cpp:324-327): kinetic sums in
+// element order, potential energy above the box corner the gravity points away from.
+void DEM::updateEnergy() {
+    double tKin = 0.0, rKin = 0.0;
+    for (const Element& e : elmts) {
+        tKin += 0.5 * e.m * e.x1.norm2();
+        const V3 w2(e.wLocal.x * e.wLocal.x, e.wLocal.y * e.wLocal.y, e.wLocal.z * e.wLocal.z);
+        rKin += e.I.dot(w2);
+    }
+    double g = 0.0;
+    const double gn = demF.norm();
+    if (gn != 0.0) {
+        const V3 n = (demF * -1.0) / gn;
+        V3 p(0.0, 0.0, 0.0);
+        if (n.dot(V3(1.0, 0.0, 0.0)) < 0) p += V3(demSize[0], 0.0, 0.0);
+        if (n.dot(V3(0.0, 1.0, 0.0)) < 0) p += V3(0.0, demSize[1], 0.0);
+        if (n.dot(V3(0.0, 0.0, 1.0)) < 0) p += V3(0.0, 0.0, demSize[2]);
+        for (const Element& e : elmts) {
+            const double h = n.dot(e.x0 - p);
+            g += e.m * h * gn;
+        }
+    }
+    const double elastic = 0.0;  // not computed by the reference
+    energy[0] = rKin + tKin;
+    energy[1] = rKin;
+    energy[2] = tKin;
+    energy[3] = g;
+    energy[4] = elastic;
+    energy[5] = rKin + tKin + g + elastic;
 }
 
 // ---------------------------------------------------------------- cycle

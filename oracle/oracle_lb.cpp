@@ -19,6 +19,14 @@ const V3 VEC[19] = {V3(0, 0, 0), V3(1, 0, 0), V3(-1, 0, 0), V3(0, 1, 0), V3(0, -
                     V3(1, 1, 0), V3(-1, -1, 0), V3(-1, 1, 0), V3(1, -1, 0), V3(0, 1, 1), V3(0, -1, -1),
                     V3(0, -1, 1), V3(0, 1, -1), V3(1, 0, 1), V3(-1, 0, -1), V3(1, 0, -1), V3(-1, 0, 1)};
 
+// slip-wall reflection tables (lattice.h:87-95): for a diagonal link j, the two
+// axis directions whose neighbours decide the reflection, and the populations
+// that are reflected from them
+const int SLIP1CHECK[19] = {0, 0, 0, 0, 0, 0, 0, 1, 2, 3, 4, 3, 4, 5, 6, 1, 2, 6, 5};
+const int SLIP1[19] = {0, 0, 0, 0, 0, 0, 0, 9, 10, 8, 7, 13, 14, 12, 11, 18, 17, 15, 16};
+const int SLIP2CHECK[19] = {0, 0, 0, 0, 0, 0, 0, 3, 4, 2, 1, 5, 6, 4, 3, 5, 6, 1, 2};
+const int SLIP2[19] = {0, 0, 0, 0, 0, 0, 0, 10, 9, 7, 8, 14, 13, 11, 12, 17, 18, 16, 15};
+
 static const double C1 = 3.0, C2 = 4.5, C3 = 1.5;   // node.cpp:90-93
 static const double minTau = 0.501, maxTau = 1.8;    // lattice.h:29-30
 
@@ -329,7 +337,7 @@ void LB::initializeVariables() {  // LB.cpp:909-944
     }
 }
 
-void LB::initializeWalls() {  // LB.cpp:946-996 (static walls; moving walls are not supported here)
+void LB::initializeWalls(const std::vector<Wall>& walls) {  // LB.cpp:946-996 (plane walls; no cylinders/objects)
     for (uint32_t it = 0; it < totNodes; ++it) {
         if (isWall(it)) continue;
         for (int j = 0; j < 19; ++j) {
@@ -339,9 +347,21 @@ void LB::initializeWalls() {  // LB.cpp:946-996 (static walls; moving walls are 
                     alloc[link] = 1;
                     nodes[link] = Node();
                     nodeInitialize(link, initDensity, V3(), 0.0, 0.0, V3());
-                } else if (t[link] == 8 || t[link] == 6 || t[link] == 9) {
+                } else if (t[link] == 8 || t[link] == 6) {
+                    // LB.cpp:969-990: node velocity from its wall (the node lies inside the wall's plane)
+                    const Wall& w = walls[solidIndex[link]];
+                    if (!positions[link].insidePlane(w.p / unit.Length, w.n)) {
+                        error = HYB_E_UNSUPPORTED;
+                        errmsg = "moving wall node outside its wall plane (cylinders/objects are not supported)";
+                        continue;
+                    }
+                    const V3 solidVelocity = w.getSpeed(positions[link] * unit.Length) / unit.Speed;
+                    alloc[link] = 1;
+                    nodes[link] = Node();
+                    nodeInitialize(link, initDensity, solidVelocity, 0.0, 0.0, V3());
+                } else if (t[link] == 9) {
                     error = HYB_E_UNSUPPORTED;
-                    errmsg = "moving or curved walls are not supported";
+                    errmsg = "curved walls are not supported";
                 }
             }
         }
@@ -363,7 +383,7 @@ void LB::init(DEM& dem) {  // LB.cpp:190-219
     initializeInterface();
     initializeLists();
     initializeVariables();
-    initializeWalls();
+    initializeWalls(dem.walls);
     cleanLists();
     totalMass = 0.0;
     for (uint32_t it = 0; it < totNodes; ++it)
@@ -439,10 +459,13 @@ void LB::collision_() {  // LB.cpp:1072-1142
     }
 }
 
-void LB::streaming(DEM& dem) {  // LB.cpp:1144-1488 (active, gas and static-wall links)
+void LB::streaming(DEM& dem) {  // LB.cpp:1144-1488 (active, gas, plane-wall links; no curved walls)
     const double C2x2 = 9.0, C3x2 = 3.0;
     if (!streamConstsSet) { S1 = slipCoefficient; S2 = 1.0 - slipCoefficient; streamConstsSet = true; }
     double extraMass = 0.0;
+    double staticPres[19];
+    for (int j = 0; j < 19; ++j) staticPres[j] = initDensity * W[j];
+    const double BBCoeff = 2.0 * 3.0 * 1.0 * 1.0;
     for (auto& w : dem.walls) w.FHydro.reset();
     for (uint32_t a : activeNodes)
         for (int j = 0; j < 19; ++j) nodes[a].fs[j] = nodes[a].f[j];
@@ -457,8 +480,44 @@ void LB::streaming(DEM& dem) {  // LB.cpp:1144-1488 (active, gas and static-wall
                 const double usq = nd.u.norm2();
                 const double vuj = nd.u.dot(VEC[j]);
                 nd.f[OPP[j]] = -nd.fs[j] + W[j] * initDensity * (2.0 + C2x2 * (vuj * vuj) - C3x2 * usq);
+            } else if (lt == 8) {  // moving wall (LB.cpp:1321-1341)
+                const V3 vel = nodes[link].u;
+                const double BBi = BBCoeff * nd.n * W[j] * vel.dot(VEC[j]);
+                const V3 BBforce = ((2.0 * (nd.fs[j] - staticPres[j]) - BBi) * 1.0) * VEC[j];  // node.cpp:283-291
+                dem.walls[solidIndex[link]].FHydro += BBforce;
+                nd.f[OPP[j]] = nd.fs[j] - BBi;
+                extraMass += BBi * nd.mass;
             } else if (lt == 7) {
                 nd.f[OPP[j]] = nd.fs[j];
+            } else if (lt == 5) {  // static slip wall (LB.cpp:1358-1400)
+                nd.f[OPP[j]] = nd.fs[j];
+                if (j > 6) {
+                    const uint32_t c1 = nb[(size_t)it * 19 + SLIP1CHECK[j]], c2 = nb[(size_t)it * 19 + SLIP2CHECK[j]];
+                    const bool a1 = isActive(c1), a2 = isActive(c2);
+                    if (a1 && !a2) nd.f[OPP[j]] = S1 * nodes[c1].fs[SLIP1[j]] + S2 * nd.fs[j];
+                    else if (!a1 && a2) nd.f[OPP[j]] = S1 * nodes[c2].fs[SLIP2[j]] + S2 * nd.fs[j];
+                }
+            } else if (lt == 6) {  // moving slip wall (LB.cpp:1402-1456)
+                const V3 vel = nodes[link].u;
+                const double BBi = BBCoeff * nd.n * W[j] * vel.dot(VEC[j]);
+                bool done = false;
+                if (j > 6) {
+                    const uint32_t c1 = nb[(size_t)it * 19 + SLIP1CHECK[j]], c2 = nb[(size_t)it * 19 + SLIP2CHECK[j]];
+                    const bool a1 = isActive(c1), a2 = isActive(c2);
+                    if (a1 && !a2) {
+                        nd.f[OPP[j]] = S1 * nodes[c1].fs[SLIP1[j]] + S2 * (nd.fs[j] - BBi);
+                        extraMass += S2 * nd.mass * BBi;
+                        done = true;
+                    } else if (!a1 && a2) {
+                        nd.f[OPP[j]] = S1 * nodes[c2].fs[SLIP2[j]] + S2 * (nd.fs[j] - BBi);
+                        extraMass += S2 * nd.mass * BBi;
+                        done = true;
+                    }
+                }
+                if (!done) {
+                    nd.f[OPP[j]] = nd.fs[j] - BBi;
+                    extraMass += nd.mass * BBi;
+                }
             } else {
                 error = HYB_E_UNSUPPORTED;
                 errmsg = "TYPE ERROR: unsupported link type in streaming";
@@ -547,7 +606,7 @@ void LB::couplingStep(DEM& dem) {  // LB.cpp:247-280
 }
 
 // ---------------------------------------------------------------- free surface
-void LB::updateMass() {  // LB.cpp:1492-1590 (active, gas and static-wall links)
+void LB::updateMass() {  // LB.cpp:1492-1590 (active, gas, plane-wall links; no curved walls)
     for (uint32_t i : interfaceNodes) nodes[i].newMass = nodes[i].mass;
     for (uint32_t index : interfaceNodes) {
         double deltaMass = 0.0;
@@ -558,6 +617,18 @@ void LB::updateMass() {  // LB.cpp:1492-1590 (active, gas and static-wall links)
             if (t[link] == 3) averageMass = 0.5 * (nodes[link].mass + nd.mass);
             else if (t[link] == 0) averageMass = 1.0;
             else if (t[link] == 2) averageMass = 0.0;
+            else if (t[link] == 8) averageMass = 1.0 * nd.mass;
+            else if (t[link] == 6) {  // LB.cpp:1545-1575
+                if (j > 6) {
+                    const uint32_t c1 = nb[(size_t)index * 19 + SLIP1CHECK[j]], c2 = nb[(size_t)index * 19 + SLIP2CHECK[j]];
+                    const bool a1 = isActive(c1), a2 = isActive(c2);
+                    if (a1 && !a2) averageMass += 1.0 * (1.0 - slipCoefficient) * nd.mass;
+                    else if (!a1 && a2) averageMass += 1.0 * (1.0 - slipCoefficient) * nd.mass;
+                    else averageMass += 1.0 * nd.mass;
+                } else {
+                    averageMass += 1.0 * nd.mass;
+                }
+            }
             deltaMass += 1.0 * averageMass * (nd.f[OPP[j]] - nd.fs[j]);
         }
         nd.newMass += deltaMass;

@@ -195,6 +195,9 @@ void dump(const std::string& path, IO& io, DEM& dem, LB& lb) {
                                   dem.sphereMat.dampCoeff, dem.sphereMat.knConst, dem.sphereMat.ksConst,
                                   dem.newNeighborList ? 1.0 : 0.0};
         w.d("dem_scalars", dm, {dm.size()});
+        const energy& en = dem.particleEnergy;
+        std::vector<double> ev = {en.kin, en.rotKin, en.trKin, en.grav, en.elastic, en.total};
+        w.d("energy", ev, {ev.size()});
         std::vector<double> wl;
         for (size_t k = 0; k < dem.walls.size(); ++k) {
             push3(wl, dem.walls[k].p); push3(wl, dem.walls[k].n); push3(wl, dem.walls[k].FParticle); push3(wl, dem.walls[k].FHydro);

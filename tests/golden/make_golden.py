@@ -58,7 +58,7 @@ def read_bin(path: str) -> dict:
 LATTICE_KEYS = ["types", "pflag", "solidIndex", "f", "n", "u", "hydroForce", "mass", "visc", "activeNodes",
                 "interfaceNodes", "fluidNodes", "particleNodes", "neighbors", "lbSize", "lbF", "totalMass"]
 DEM_KEYS = ["e_x", "e_xp", "e_q", "e_qp", "e_w", "e_wp", "e_misc", "e_forces", "e_comp_off", "e_comp", "p_data",
-            "p_idx", "neighborTable", "nearWallTable", "cellTable", "ghostIndex", "ghostVec", "dem_scalars", "walls"]
+            "p_idx", "neighborTable", "nearWallTable", "cellTable", "ghostIndex", "ghostVec", "dem_scalars", "walls", "energy"]
 
 
 def _cases():
@@ -103,6 +103,19 @@ def _cases():
                                        boundary0=7, boundary1=7, boundary2=4, boundary3=4, initVisc=0.05,
                                        initVelocityX=0.03, initVelocityZ=0.02, lbFX=5.0e-4, lbFZ=-5.0e-4),
                        bed=Bed(np.zeros((0, 3)), np.zeros(0)), cycles=[0, 1, 2, 10, 30, 60], keys=LATTICE_KEYS)
+    # slip (5), moving slip (6), moving no-slip (8) and static (7) walls with two
+    # spheres near the walls; moving walls have the generic zero velocity
+    walls = dict(boundary0=5, boundary1=6, boundary2=4, boundary3=4, boundary4=8, boundary5=7, slipCoefficient=0.3)
+    cases["walls"] = dict(cfg=base_config(lbSizeX=14, lbSizeY=12, lbSizeZ=16, initVisc=0.05, lbFX=3.0e-4,
+                                          lbFY=1.0e-4, lbFZ=-4.0e-4, initVelocityX=0.02, initVelocityY=0.01,
+                                          youngMod=2.0e3, **walls),
+                          bed=Bed(np.array([[3.2, 6.0, 4.0], [9.0, 5.0, 3.1]]), np.array([2.2, 2.0])),
+                          cycles=[0, 1, 3, 10, 25], keys=LATTICE_KEYS + DEM_KEYS)
+    # the same walls around a free surface (SPLASH: gas above z = 20)
+    cases["walls_fs"] = dict(cfg=base_config(problemName="SPLASH", freeSurfaceSolve=1, lbSizeX=12, lbSizeY=10,
+                                             lbSizeZ=26, initVisc=0.05, initVelocityX=0.03, initVelocityZ=0.02,
+                                             lbFX=5.0e-4, lbFZ=-5.0e-4, **walls),
+                             bed=Bed(np.zeros((0, 3)), np.zeros(0)), cycles=[0, 1, 5, 20, 40], keys=LATTICE_KEYS)
     return cases
 
 

@@ -54,6 +54,9 @@ def _compare_dem(gpu, orc, tag):
     for name in ("e_x", "e_q", "e_w"):
         a, b = orc.get(name), gpu.get(name)
         assert rel_l2(b, a) <= ELEM_TOL, f"{tag}: {name} rel L2 {rel_l2(b, a):.3e}"
+    # particleEnergy (kin, rotKin, trKin, grav, elastic, total): sums in element order
+    eo, eg = orc.get("energy"), gpu.get("energy")
+    assert np.all(np.abs(eg - eo) <= ELEM_TOL * np.abs(eo) + 1e-300), f"{tag}: energy {eg} vs {eo}"
     fo, fg = orc.get("e_forces"), gpu.get("e_forces")
     for k, name in enumerate(["FHydro", "MHydro", "FParticle", "MParticle", "FWall", "MWall", "FGrav"]):
         a, b = fo[:, k], fg[:, k]
@@ -69,7 +72,7 @@ def _compare_dem(gpu, orc, tag):
 HORIZON = {"dem60": 100}
 
 
-@pytest.mark.parametrize("name", ["lb16", "chan", "bingham", "couple2", "dense", "dem60"])
+@pytest.mark.parametrize("name", ["lb16", "chan", "bingham", "couple2", "dense", "dem60", "walls"])
 def test_gpu_fp64_matches_oracle(name):
     rt = _runtime()
     cfg, rec, z = load_case(name)
@@ -98,12 +101,14 @@ def _compare_mass(gpu, orc, tag, act):
     assert abs(fg - fo) <= LB_TOL * abs(fo), f"{tag}: fluid mass {fg!r} vs {fo!r}"
 
 
-def test_gpu_free_surface_fp64_matches_oracle():
-    """SPLASH free surface (the fs fixture): mass exchange, filled/emptied nodes,
-    new interface nodes and isolated-node removal over 60 cycles."""
+@pytest.mark.parametrize("name", ["fs", "walls_fs"])
+def test_gpu_free_surface_fp64_matches_oracle(name):
+    """SPLASH free surface: mass exchange, filled/emptied nodes, new interface
+    nodes and isolated-node removal (fs: periodic/no-slip box, 60 cycles;
+    walls_fs: slip, moving slip and moving no-slip faces, 40 cycles)."""
     rt = _runtime()
-    cfg, rec, z = load_case("fs")
-    print("[gpu parity] case=fs", flush=True)
+    cfg, rec, z = load_case(name)
+    print(f"[gpu parity] case={name}", flush=True)
     orc = OracleSim(fixture_config(cfg), rec)
     gpu = rt.Simulation(fixture_config(cfg), rec)
     gpu.set_capture(True)
@@ -112,7 +117,7 @@ def test_gpu_free_surface_fp64_matches_oracle():
         orc.cycle(c - prev)
         gpu.cycle(c - prev)
         prev = c
-        tag = f"fs cycle {c}"
+        tag = f"{name} cycle {c}"
         _compare_lattice(gpu, orc, tag)
         t = orc.get("types")
         _compare_mass(gpu, orc, tag, (t == 0) | (t == 3))
