@@ -76,6 +76,30 @@ void dfree(T*& p) {
     p = nullptr;
 }
 
+// A device buffer that belongs to a handle's persistent state (written by
+// hyb_state_save, restored by hyb_state_load).
+struct Blk {
+    std::string name;
+    void* ptr;
+    size_t bytes;
+};
+
+// dalloc that also records the buffer in the handle's state list.
+template <class T>
+int ralloc(std::vector<Blk>& reg, const char* name, T** p, size_t count) {
+    const int rc = dalloc(p, count);
+    if (rc == HYB_OK && *p) reg.push_back(Blk{name, (void*)*p, count * sizeof(T)});
+    return rc;
+}
+
+void unregister(std::vector<Blk>& reg, const void* ptr) {
+    for (size_t i = 0; i < reg.size(); ++i)
+        if (reg[i].ptr == ptr) {
+            reg.erase(reg.begin() + (long)i);
+            return;
+        }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ handles
@@ -97,6 +121,7 @@ struct hyb_dem {
     bool inited = false;
     hipStream_t stream = nullptr;
     int64_t substeps = 0;
+    std::vector<Blk> saved;  // persistent device state
 };
 
 struct hyb_lb {
@@ -157,6 +182,7 @@ struct hyb_lb {
     size_t evUsed = 0;
     double profMs = 0.0;
     int64_t profLaunches = 0;
+    std::vector<Blk> saved;  // persistent device state
 
     bool varVisc() const { return nonNewtonian || turbulenceOn; }
     bool hasSlip() const {
@@ -433,16 +459,20 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
     }
     // device buffers
     int rc;
-    if ((rc = dalloc(&b.el, el.size())) || (rc = dalloc(&b.st, 1)) || (rc = dalloc(&b.ghostIndex, p.Gcap)) ||
-        (rc = dalloc(&b.ghostVec, 3 * (size_t)p.Gcap)) || (rc = dalloc(&b.pairs, 2 * (size_t)p.pairCap)) ||
-        (rc = dalloc(&b.nearWall, std::max(P, 1))) || (rc = dalloc(&b.cellTable, (size_t)scap + b.ncells)) ||
-        (rc = dalloc(&b.cellOf, scap)) || (rc = dalloc(&b.cellCount, b.ncells + 1)) ||
+    std::vector<Blk>& R = d->saved;
+    R.clear();
+    if ((rc = ralloc(R, "el", &b.el, el.size())) || (rc = ralloc(R, "st", &b.st, 1)) ||
+        (rc = ralloc(R, "ghostIndex", &b.ghostIndex, p.Gcap)) || (rc = ralloc(R, "ghostVec", &b.ghostVec, 3 * (size_t)p.Gcap)) ||
+        (rc = ralloc(R, "pairs", &b.pairs, 2 * (size_t)p.pairCap)) || (rc = ralloc(R, "nearWall", &b.nearWall, std::max(P, 1))) ||
+        (rc = ralloc(R, "cellTable", &b.cellTable, (size_t)scap + b.ncells)) || (rc = ralloc(R, "cellOf", &b.cellOf, scap)) ||
+        (rc = ralloc(R, "cellCount", &b.cellCount, b.ncells + 1)) ||
         (rc = dalloc(&b.work, (size_t)std::max(std::max(b.W, scap), std::max(p.pairCap, b.ncells + 1)) + 1)) ||
-        (rc = dalloc(&b.csrOff, (size_t)P + 1)) || (rc = dalloc(&b.csr, 2 * (size_t)p.pairCap)) ||
-        (rc = dalloc(&b.cursor, scap)) || (rc = dalloc(&b.ghostOfCount, std::max(P, 1))) ||
-        (rc = dalloc(&b.ghostOf, 8 * (size_t)std::max(P, 1))) || (rc = dalloc(&b.side, 3 * (size_t)std::max(P, 1))) ||
-        (rc = dalloc(&b.ghostB, p.Gcap)) || (rc = dalloc(&b.posI, p.pairCap)) || (rc = dalloc(&b.posJ, p.pairCap)) ||
-        (rc = dalloc(&b.prec, 2 * (size_t)p.pairCap)) || (rc = dalloc(&b.vmax, (size_t)(std::max(P, 1) + 63) / 64)) || (rc = dalloc(&b.energy, 6)))
+        (rc = ralloc(R, "csrOff", &b.csrOff, (size_t)P + 1)) || (rc = ralloc(R, "csr", &b.csr, 2 * (size_t)p.pairCap)) ||
+        (rc = dalloc(&b.cursor, scap)) || (rc = ralloc(R, "ghostOfCount", &b.ghostOfCount, std::max(P, 1))) ||
+        (rc = ralloc(R, "ghostOf", &b.ghostOf, 8 * (size_t)std::max(P, 1))) ||
+        (rc = ralloc(R, "side", &b.side, 3 * (size_t)std::max(P, 1))) || (rc = ralloc(R, "ghostB", &b.ghostB, p.Gcap)) ||
+        (rc = ralloc(R, "posI", &b.posI, p.pairCap)) || (rc = ralloc(R, "posJ", &b.posJ, p.pairCap)) ||
+        (rc = dalloc(&b.prec, 2 * (size_t)p.pairCap)) || (rc = ralloc(R, "vmax", &b.vmax, (size_t)(std::max(P, 1) + 63) / 64)) || (rc = dalloc(&b.energy, 6)))
         return rc;
     HIPCHK(hipMemcpyAsync(b.el, el.data(), el.size() * sizeof(double), hipMemcpyHostToDevice, d->stream));
     DEMState st;
@@ -562,25 +592,34 @@ static int lb_alloc(hyb_lb* l, int slotCap) {
     const size_t N = l->N;
     int rc;
     const size_t pb = l->popBytes();
+    std::vector<Blk>& R = l->saved;
+    R.clear();
     HIPCHK(hipMalloc(&l->f[0], pb));
     HIPCHK(hipMalloc(&l->f[1], pb));
-    if ((rc = dalloc(&l->type, N)) || (rc = dalloc(&l->link, N)) || (rc = dalloc(&l->solid, N)) ||
+    R.push_back(Blk{"f0", l->f[0], pb});
+    R.push_back(Blk{"f1", l->f[1], pb});
+    if ((rc = ralloc(R, "type", &l->type, N)) || (rc = ralloc(R, "link", &l->link, N)) ||
+        (rc = ralloc(R, "solid", &l->solid, N)) ||
         (rc = dalloc(&l->maxXYZ, 3 * (size_t)l->Z)) || (rc = dalloc(&l->dcount, (size_t)l->Z)) ||
         (rc = dalloc(&l->nslots, 2)))
         return rc;
-    if (l->varVisc() && (rc = dalloc(&l->visc, N))) return rc;
-    if (l->hasSlip() && (rc = dalloc(&l->slipw, N))) return rc;
+    if (l->varVisc() && (rc = ralloc(R, "visc", &l->visc, N))) return rc;
+    if (l->hasSlip() && (rc = ralloc(R, "slipw", &l->slipw, N))) return rc;
     if (l->freeSurface) {
         FSBuffers& b = l->fsb;
         const size_t eb = l->prec == Prec::F64 ? 8 : 4;
         HIPCHK(hipMalloc(&b.rho[0], eb * N));
         HIPCHK(hipMalloc(&b.rho[1], eb * N));
-        if ((rc = dalloc(&b.mass, N)) || (rc = dalloc(&b.uS, 3 * N)) || (rc = dalloc(&b.hFi, 3 * N)) ||
-            (rc = dalloc(&b.claim, N)) || (rc = dalloc(&b.list[0], N)) || (rc = dalloc(&b.list[1], N)) ||
-            (rc = dalloc(&b.listA, N)) || (rc = dalloc(&b.listB, N)) || (rc = dalloc(&b.mnew, N)) ||
-            (rc = dalloc(&b.state, N)) || (rc = dalloc(&b.ctr, 1)))
+        R.push_back(Blk{"rho0", b.rho[0], eb * N});
+        R.push_back(Blk{"rho1", b.rho[1], eb * N});
+        if ((rc = ralloc(R, "mass", &b.mass, N)) || (rc = ralloc(R, "uS", &b.uS, 3 * N)) ||
+            (rc = ralloc(R, "hFi", &b.hFi, 3 * N)) || (rc = ralloc(R, "claim", &b.claim, N)) ||
+            (rc = ralloc(R, "list0", &b.list[0], N)) || (rc = ralloc(R, "list1", &b.list[1], N)) ||
+            (rc = ralloc(R, "listA", &b.listA, N)) || (rc = ralloc(R, "listB", &b.listB, N)) ||
+            (rc = ralloc(R, "mnew", &b.mnew, N)) || (rc = ralloc(R, "state", &b.state, N)) ||
+            (rc = ralloc(R, "ctr", &b.ctr, 1)))
             return rc;
-        if (l->hasSlip() && (rc = dalloc(&b.actPrev, N))) return rc;
+        if (l->hasSlip() && (rc = ralloc(R, "actPrev", &b.actPrev, N))) return rc;
     }
     l->slotCap = slotCap;
     if (slotCap > 0) {
@@ -777,7 +816,10 @@ int hyb_lb_set_force_field(hyb_lb* l, const float* F) {
     if (!l || !l->inited) return fail(HYB_E_STATE, "hyb_lb_set_force_field before init");
     HIPCHK(hipStreamSynchronize(l->stream));
     if (!F) {
-        if (l->ffield) (void)hipFree(l->ffield);
+        if (l->ffield) {
+            unregister(l->saved, l->ffield);
+            (void)hipFree(l->ffield);
+        }
         l->ffield = nullptr;
         return HYB_OK;
     }
@@ -786,7 +828,7 @@ int hyb_lb_set_force_field(hyb_lb* l, const float* F) {
     for (size_t i = 0; i < N; ++i)
         for (int k = 0; k < 3; ++k) planes[(size_t)k * N + i] = F[3 * i + k];
     int rc;
-    if (!l->ffield && (rc = dalloc(&l->ffield, 3 * N))) return rc;
+    if (!l->ffield && (rc = ralloc(l->saved, "ffield", &l->ffield, 3 * N))) return rc;
     HIPCHK(hipMemcpy(l->ffield, planes.data(), 12 * N, hipMemcpyHostToDevice));
     return HYB_OK;
 }
@@ -986,7 +1028,7 @@ static int ensure_mac(hyb_lb* l, hyb_dem* d) {
     // collision's input with the same kernel code (no population write).
     if (l->macValid) return HYB_OK;
     int rc;
-    if (!l->mac && (rc = dalloc(&l->mac, 7 * l->N))) return rc;
+    if (!l->mac && (rc = ralloc(l->saved, "mac", &l->mac, 7 * l->N))) return rc;
     const LBKParams p = l->params();
     if (l->steps == 0) {
         // state after init: recompute the hydrostatic values the init kernel used
@@ -1292,7 +1334,7 @@ int hyb_lb_set_profiling(hyb_lb* l, int32_t enabled) {
 int hyb_lb_set_capture(hyb_lb* l, int32_t enabled) {
     if (!l) return fail(HYB_E_ARG, "null handle");
     if (enabled && !l->mac) {
-        int rc = dalloc(&l->mac, 7 * l->N);
+        int rc = ralloc(l->saved, "mac", &l->mac, 7 * l->N);
         if (rc) return rc;
     }
     l->capture = enabled != 0;
@@ -1318,6 +1360,201 @@ int hyb_lb_kernel_time(hyb_lb* l, double* total_ms, int64_t* launches) {
 int hyb_lb_active_nodes(hyb_lb* l, int64_t* count) {
     if (!l || !count) return fail(HYB_E_ARG, "null argument");
     *count = (int64_t)l->activeCount;
+    return HYB_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ state file
+// Layout: "HYBSTATE", u32 version, u32 sections (bit 0: lattice, bit 1: DEM);
+// per section its scalar record, u32 block count, then per block u32 name
+// length, name, u64 bytes, raw bytes (the device buffer as it is).
+namespace {
+
+constexpr char STATE_MAGIC[8] = {'H', 'Y', 'B', 'S', 'T', 'A', 'T', 'E'};
+constexpr uint32_t STATE_VERSION = 1;
+constexpr size_t STATE_CHUNK = size_t(64) << 20;
+
+struct LbScalars {
+    int32_t X, Y, Z, prec;
+    uint64_t N, popBytes;
+    int32_t cur, fsCur, rhoCur, fluidMassBuf;
+    uint32_t ctag, fsTag, actTag, time;
+    int64_t steps, fsSteps;
+    uint64_t activeCount;
+    double totalMass;
+    uint8_t capture, macValid, havePart, pad[5];
+};
+
+struct DemScalars {
+    int32_t P, pairCap, Gcap, multiStep;
+    double demTime;
+    uint32_t demTimeStep, pad;
+    int64_t substeps;
+};
+
+struct File {
+    FILE* f = nullptr;
+    ~File() {
+        if (f) std::fclose(f);
+    }
+};
+
+bool wr(FILE* f, const void* p, size_t n) { return std::fwrite(p, 1, n, f) == n; }
+bool rd(FILE* f, void* p, size_t n) { return std::fread(p, 1, n, f) == n; }
+
+int write_blocks(FILE* f, const std::vector<Blk>& reg) {
+    const uint32_t nb = (uint32_t)reg.size();
+    if (!wr(f, &nb, 4)) return fail(HYB_E_ARG, "state file: write error");
+    std::vector<char> buf;
+    for (const Blk& b : reg) {
+        const uint32_t nl = (uint32_t)b.name.size();
+        const uint64_t nbytes = b.bytes;
+        if (!wr(f, &nl, 4) || !wr(f, b.name.data(), nl) || !wr(f, &nbytes, 8))
+            return fail(HYB_E_ARG, "state file: write error");
+        for (size_t off = 0; off < b.bytes; off += STATE_CHUNK) {
+            const size_t n = std::min(STATE_CHUNK, b.bytes - off);
+            buf.resize(n);
+            HIPCHK(hipMemcpy(buf.data(), (const char*)b.ptr + off, n, hipMemcpyDeviceToHost));
+            if (!wr(f, buf.data(), n)) return fail(HYB_E_ARG, "state file: write error");
+        }
+    }
+    return HYB_OK;
+}
+
+// Reads the blocks of one section into the registered buffers; `lazy` may
+// allocate a buffer the handle creates only on demand (returns nullptr if the
+// name is not one of those).
+template <class Lazy>
+int read_blocks(FILE* f, std::vector<Blk>& reg, Lazy&& lazy) {
+    uint32_t nb = 0;
+    if (!rd(f, &nb, 4)) return fail(HYB_E_ARG, "state file: truncated");
+    std::vector<char> buf;
+    std::vector<bool> seen(reg.size(), false);
+    for (uint32_t k = 0; k < nb; ++k) {
+        uint32_t nl = 0;
+        uint64_t nbytes = 0;
+        if (!rd(f, &nl, 4) || nl > 64) return fail(HYB_E_ARG, "state file: corrupt block header");
+        std::string name(nl, '\0');
+        if (!rd(f, &name[0], nl) || !rd(f, &nbytes, 8)) return fail(HYB_E_ARG, "state file: truncated");
+        int at = -1;
+        for (size_t i = 0; i < reg.size(); ++i)
+            if (reg[i].name == name) at = (int)i;
+        if (at < 0) {
+            const int rc = lazy(name);
+            if (rc) return rc;
+            for (size_t i = 0; i < reg.size(); ++i)
+                if (reg[i].name == name) at = (int)i;
+            seen.resize(reg.size(), false);
+        }
+        if (at < 0) return fail(HYB_E_ARG, "state file: buffer '" + name + "' does not exist in this configuration");
+        if (reg[at].bytes != nbytes)
+            return fail(HYB_E_ARG, "state file: buffer '" + name + "' has " + std::to_string(nbytes) + " bytes, this handle " +
+                                       std::to_string(reg[at].bytes));
+        for (size_t off = 0; off < nbytes; off += STATE_CHUNK) {
+            const size_t n = std::min(STATE_CHUNK, (size_t)nbytes - off);
+            buf.resize(n);
+            if (!rd(f, buf.data(), n)) return fail(HYB_E_ARG, "state file: truncated");
+            HIPCHK(hipMemcpy((char*)reg[at].ptr + off, buf.data(), n, hipMemcpyHostToDevice));
+        }
+        seen[at] = true;
+    }
+    for (size_t i = 0; i < reg.size(); ++i)
+        if (!seen[i] && reg[i].name != "mac" && reg[i].name != "ffield")
+            return fail(HYB_E_ARG, "state file: buffer '" + reg[i].name + "' missing");
+    return HYB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hyb_state_save(hyb_lb* l, hyb_dem* d, const char* path) {
+    if (!path || (!l && !d)) return fail(HYB_E_ARG, "hyb_state_save: null argument");
+    if ((l && !l->inited) || (d && !d->inited)) return fail(HYB_E_STATE, "hyb_state_save before init");
+    if (l) HIPCHK(hipStreamSynchronize(l->stream));
+    if (d) HIPCHK(hipStreamSynchronize(d->stream));
+    File fh;
+    fh.f = std::fopen(path, "wb");
+    if (!fh.f) return fail(HYB_E_ARG, std::string("hyb_state_save: cannot open ") + path);
+    const uint32_t sections = (l ? 1u : 0u) | (d ? 2u : 0u);
+    if (!wr(fh.f, STATE_MAGIC, 8) || !wr(fh.f, &STATE_VERSION, 4) || !wr(fh.f, &sections, 4))
+        return fail(HYB_E_ARG, "state file: write error");
+    int rc;
+    if (l) {
+        LbScalars s;
+        std::memset(&s, 0, sizeof(s));
+        s.X = l->X; s.Y = l->Y; s.Z = l->Z; s.prec = (int32_t)l->prec;
+        s.N = l->N; s.popBytes = l->popBytes();
+        s.cur = l->cur; s.fsCur = l->fsCur; s.rhoCur = l->rhoCur; s.fluidMassBuf = l->fluidMassBuf;
+        s.ctag = l->ctag; s.fsTag = l->fsTag; s.actTag = l->actTag; s.time = l->time;
+        s.steps = l->steps; s.fsSteps = l->fsSteps;
+        s.activeCount = l->activeCount; s.totalMass = l->totalMass;
+        s.capture = l->capture; s.macValid = l->macValid; s.havePart = l->havePart;
+        if (!wr(fh.f, &s, sizeof(s))) return fail(HYB_E_ARG, "state file: write error");
+        if ((rc = write_blocks(fh.f, l->saved))) return rc;
+    }
+    if (d) {
+        DemScalars s;
+        std::memset(&s, 0, sizeof(s));
+        s.P = d->P; s.pairCap = d->p.pairCap; s.Gcap = d->p.Gcap; s.multiStep = d->multiStep;
+        s.demTime = d->demTime; s.demTimeStep = d->demTimeStep; s.substeps = d->substeps;
+        if (!wr(fh.f, &s, sizeof(s))) return fail(HYB_E_ARG, "state file: write error");
+        if ((rc = write_blocks(fh.f, d->saved))) return rc;
+    }
+    if (std::fflush(fh.f) != 0) return fail(HYB_E_ARG, "state file: write error");
+    return HYB_OK;
+}
+
+int hyb_state_load(hyb_lb* l, hyb_dem* d, const char* path) {
+    if (!path || (!l && !d)) return fail(HYB_E_ARG, "hyb_state_load: null argument");
+    if ((l && !l->inited) || (d && !d->inited)) return fail(HYB_E_STATE, "hyb_state_load before init");
+    if (l) HIPCHK(hipStreamSynchronize(l->stream));
+    if (d) HIPCHK(hipStreamSynchronize(d->stream));
+    File fh;
+    fh.f = std::fopen(path, "rb");
+    if (!fh.f) return fail(HYB_E_ARG, std::string("hyb_state_load: cannot open ") + path);
+    char magic[8];
+    uint32_t version = 0, sections = 0;
+    if (!rd(fh.f, magic, 8) || std::memcmp(magic, STATE_MAGIC, 8) != 0) return fail(HYB_E_ARG, "not a state file");
+    if (!rd(fh.f, &version, 4) || version != STATE_VERSION || !rd(fh.f, &sections, 4))
+        return fail(HYB_E_ARG, "state file: unsupported version");
+    if ((sections & 1u) != (l ? 1u : 0u) || (sections & 2u) != (d ? 2u : 0u))
+        return fail(HYB_E_ARG, "state file: holds a different set of handles");
+    int rc;
+    if (l) {
+        LbScalars s;
+        if (!rd(fh.f, &s, sizeof(s))) return fail(HYB_E_ARG, "state file: truncated");
+        if (s.X != l->X || s.Y != l->Y || s.Z != l->Z || s.prec != (int32_t)l->prec || s.popBytes != l->popBytes())
+            return fail(HYB_E_ARG, "state file: lattice size or precision differs");
+        auto lazy = [&](const std::string& name) -> int {
+            if (name == "mac" && !l->mac) return ralloc(l->saved, "mac", &l->mac, 7 * l->N);
+            if (name == "ffield" && !l->ffield) return ralloc(l->saved, "ffield", &l->ffield, 3 * l->N);
+            return HYB_OK;
+        };
+        if ((rc = read_blocks(fh.f, l->saved, lazy))) return rc;
+        l->cur = s.cur; l->fsCur = s.fsCur; l->rhoCur = s.rhoCur; l->fluidMassBuf = s.fluidMassBuf;
+        l->ctag = s.ctag; l->fsTag = s.fsTag; l->actTag = s.actTag; l->time = s.time;
+        l->steps = s.steps; l->fsSteps = s.fsSteps;
+        l->activeCount = s.activeCount; l->totalMass = s.totalMass;
+        l->capture = s.capture != 0; l->macValid = s.macValid != 0; l->havePart = s.havePart != 0;
+        // per-step scratch: force records must not match any ownership word yet
+        if (l->hf) {
+            const size_t hb = (l->prec == Prec::F64 ? sizeof(NodeForce64) : sizeof(NodeForce32)) * l->N;
+            HIPCHK(hipMemset(l->hf, 0, hb));
+        }
+        for (void* ck : l->adjCk) (void)hipFree(ck);
+        l->adjCk.clear();
+        l->adjSteps = -1;
+    }
+    if (d) {
+        DemScalars s;
+        if (!rd(fh.f, &s, sizeof(s))) return fail(HYB_E_ARG, "state file: truncated");
+        if (s.P != d->P || s.pairCap != d->p.pairCap || s.Gcap != d->p.Gcap || s.multiStep != d->multiStep)
+            return fail(HYB_E_ARG, "state file: particle set or DEM configuration differs");
+        if ((rc = read_blocks(fh.f, d->saved, [](const std::string&) { return (int)HYB_OK; }))) return rc;
+        d->demTime = s.demTime; d->demTimeStep = s.demTimeStep; d->substeps = s.substeps;
+    }
     return HYB_OK;
 }
 

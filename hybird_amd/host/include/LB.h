@@ -68,6 +68,9 @@ private:
     hyb_config cfg_;
     hyb_lb* h_ = nullptr;
     bool inited_ = false;
+    hyb_dem* dem_ = nullptr;     // DEM handle coupled at init (for the state file)
+    std::string restartFile_;    // extension: state file restored after init
+    std::string saveStateFile_;  // extension: state file written when the run ends
 };
 
 #endif  // HYBIRD_AMD_LB_H

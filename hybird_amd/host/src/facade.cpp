@@ -60,6 +60,12 @@ LB::LB() {
 }
 
 LB::~LB() {
+    if (h_ && inited_ && !saveStateFile_.empty()) {  // extension: checkpoint at the end of the run
+        if (hyb_state_save(h_, dem_, saveStateFile_.c_str()) != HYB_OK)
+            cout << "WARNING: state file not written: " << hyb_last_error() << "\n";
+        else
+            cout << "state written to " << saveStateFile_ << "\n";
+    }
     const char* dir = std::getenv("HYBIRD_AMD_DUMP_DIR");
     if (h_ && inited_ && dir && *dir) {
         std::vector<double> d;
@@ -145,6 +151,9 @@ void LB::latticeBoltzmannGet(GetPot& lbmCfgFile, GetPot& command_line) {  // LB.
     const std::string prec = upper(extension_key<std::string>(lbmCfgFile, command_line, "precision", "FP64"));
     ASSERT(prec == "FP64" || prec == "FP32");
     cfg_.precision = prec == "FP32" ? HYB_FP32_SHIFTED : HYB_FP64;
+    // checkpoint and restart (the reference cannot resume a run)
+    restartFile_ = extension_key<std::string>(lbmCfgFile, command_line, "restartFile", "");
+    saveStateFile_ = extension_key<std::string>(lbmCfgFile, command_line, "saveStateFile", "");
     const char* boxKeys[6] = {"liquidBoxXMin", "liquidBoxXMax", "liquidBoxYMin", "liquidBoxYMax", "liquidBoxZMin",
                               "liquidBoxZMax"};
     bool box = false;
@@ -194,8 +203,13 @@ void LB::latticeBolzmannInit(cylinderList& cylinders, wallList& walls, particleL
         exit(1);
     }
     DEM* dem = particles.owner;
-    HYB_CALL(hyb_lb_init(h_, dem ? dem->handle() : nullptr));
+    dem_ = dem ? dem->handle() : nullptr;
+    HYB_CALL(hyb_lb_init(h_, dem_));
     inited_ = true;
+    if (!restartFile_.empty()) {  // extension: continue from a state file
+        HYB_CALL(hyb_state_load(h_, dem_, restartFile_.c_str()));
+        cout << "state restored from " << restartFile_ << "\n";
+    }
     std::vector<double> tm;
     if (view(h_, hyb_lb_get, "totalMass", tm) && !tm.empty()) totalMass = tm[0];
 }

@@ -36,7 +36,8 @@ EXPORTS = ["hyb_config_defaults", "hyb_last_error", "hyb_device_count", "hyb_syn
            "hyb_dem_step", "hyb_dem_destroy", "hyb_lb_create", "hyb_lb_init", "hyb_lb_fs_step", "hyb_lb_couple",
            "hyb_lb_step", "hyb_lb_destroy", "hyb_cycle", "hyb_lb_get", "hyb_dem_get", "hyb_lb_set_profiling",
            "hyb_lb_kernel_time", "hyb_lb_active_nodes", "hyb_lb_set_capture", "hyb_dem_init_lattice",
-           "hyb_lb_set_force_field", "hyb_lb_set_populations", "hyb_adj_forward", "hyb_adj_backward"]
+           "hyb_lb_set_force_field", "hyb_lb_set_populations", "hyb_adj_forward", "hyb_adj_backward",
+           "hyb_state_save", "hyb_state_load"]
 
 
 class HybError(RuntimeError):
@@ -68,7 +69,8 @@ def load():
                        ("hyb_lb_set_force_field", [vp, vp]), ("hyb_lb_set_populations", [vp, vp]),
                        ("hyb_adj_forward", [vp, vp, i32, i32]), ("hyb_adj_backward", [vp, vp, vp, vp, vp, vp]),
                        ("hyb_lb_kernel_time", [vp, C.POINTER(C.c_double), C.POINTER(i64)]),
-                       ("hyb_lb_active_nodes", [vp, C.POINTER(i64)])]:
+                       ("hyb_lb_active_nodes", [vp, C.POINTER(i64)]),
+                       ("hyb_state_save", [vp, vp, C.c_char_p]), ("hyb_state_load", [vp, vp, C.c_char_p])]:
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
@@ -168,6 +170,14 @@ class Simulation:
         L = load()
         for _ in range(int(n)):
             _check(L.hyb_lb_step(self._lb, self._dem), "hyb_lb_step")
+
+    def save_state(self, path) -> None:
+        """Checkpoint of the lattice and DEM state (full precision)."""
+        _check(load().hyb_state_save(self._lb, self._dem, str(path).encode()), "hyb_state_save")
+
+    def load_state(self, path) -> None:
+        """Restores a checkpoint written from the same configuration and elements."""
+        _check(load().hyb_state_load(self._lb, self._dem, str(path).encode()), "hyb_state_load")
 
     def active_nodes(self) -> int:
         n = C.c_int64(0)

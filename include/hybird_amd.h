@@ -23,6 +23,7 @@
  *   hyb_lb_set_capture   keeps the node fields IO reads (IO.cpp:711-831) while stepping
  *   hyb_lb_set_force_field, hyb_lb_set_populations, hyb_adj_forward, hyb_adj_backward
  *                        discrete adjoint of the LB step (not in the reference)
+ *   hyb_state_save / hyb_state_load   full-precision checkpoint and restart (not in the reference)
  *   hyb_lb_get / hyb_dem_get   host views of LB::nodes/types and DEM::elmts/particles read by IO
  *                        (IO.cpp:711-831, 835-1106)
  * The reference has no foreign-function interface; a C++ caller reaches these
@@ -182,6 +183,19 @@ int hyb_adj_forward(hyb_lb* lb, hyb_dem* dem, int32_t steps, int32_t every);
  * Any output may be NULL. Recomputes the states between checkpoints. */
 int hyb_adj_backward(hyb_lb* lb, hyb_dem* dem, const double* seed, double* grad_force, double* grad_tau,
                      double* grad_f0);
+
+/* ---- checkpoint and restart (no counterpart in the reference, which cannot
+ * resume a run) ----
+ * hyb_state_save writes the complete device state of initialised handles
+ * (either may be NULL) to a binary file: populations, node types and link
+ * words, free-surface masses, densities, lists and counters, the element
+ * state, neighbour tables and every step counter. hyb_state_load restores it
+ * into handles created and initialised from the same configuration and element
+ * records (HYB_E_ARG otherwise); stepping on continues bit-identically to the
+ * run that was saved. Both synchronise the handles' stream. A failed load
+ * may leave the handles partly overwritten. */
+int hyb_state_save(hyb_lb* lb, hyb_dem* dem, const char* path);
+int hyb_state_load(hyb_lb* lb, hyb_dem* dem, const char* path);
 
 #ifdef __cplusplus
 }

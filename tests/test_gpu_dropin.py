@@ -56,3 +56,32 @@ def test_dropin_main_matches_oracle(tmp_path):
     ex = _load(dump, "dem_e_x").reshape(-1, 6, 3)
     assert rel_l2(ex, orc.get("e_x")) <= 1e-11
     assert np.array_equal(_load(dump, "dem_neighborTable", np.uint32), orc.get("neighborTable"))
+
+
+@pytest.mark.skipif(not os.path.exists(BIN), reason="hybird_gpu not built (needs the reference's hybird.cpp)")
+def test_dropin_restart_matches_straight_run(tmp_path):
+    """saveStateFile / restartFile (host-layer keys): 5 cycles, save, 7 cycles
+    from the file equals 12 cycles in one run, bit for bit."""
+    cfg, rec, z = load_case("couple2")
+    tmp = str(tmp_path)
+
+    def run(tag, cycles, **keys):
+        d = os.path.join(tmp, tag)
+        os.makedirs(d)
+        path = _write_case(d, dict(cfg, **keys), z["bed_x"], z["bed_r"], cycles)
+        env = dict(os.environ, HYBIRD_AMD_DUMP_DIR=d)
+        print(f"[gpu dropin] hybird_gpu couple2 {tag} {cycles} cycles", flush=True)
+        r = subprocess.run([BIN, "-c", path, "-d", os.path.join(d, "out")], env=env, capture_output=True, text=True,
+                           timeout=300, cwd=d)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        return d
+
+    state = os.path.join(tmp, "couple2.state")
+    straight = run("straight", 12)
+    run("first", 5, saveStateFile=state)
+    assert os.path.getsize(state) > 0
+    resumed = run("resumed", 7, restartFile=state)
+    for name, dt in (("lb_types", np.uint8), ("lb_f", np.float64), ("dem_e_x", np.float64),
+                     ("dem_e_forces", np.float64), ("dem_neighborTable", np.uint32)):
+        a, b = _load(straight, name, dt), _load(resumed, name, dt)
+        assert np.array_equal(a, b, equal_nan=(dt == np.float64)), name
