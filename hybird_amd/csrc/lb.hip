@@ -820,9 +820,9 @@ struct KSum {
 };
 
 // Per particle slot: sum over the nodes it owns of diffVel, radius x diffVel and
-// the node mass (LB.cpp:1897-1902). One wavefront per slot; each lane keeps
-// Neumaier-compensated FP64 sums, merged by a fixed butterfly (deterministic,
-// no atomics). Only nodes strictly inside the sphere can be owned by the slot,
+// the node mass (LB.cpp:1897-1902). One wavefront per slot; each lane adds its
+// few nodes in FP64, and the lane sums are merged with Neumaier compensation by
+// a fixed butterfly (deterministic, no atomics). Only nodes strictly inside the sphere can be owned by the slot,
 // so the others are skipped before any load.
 __device__ __forceinline__ void ksum_merge(KSum& a, double s2, double c2) {
     a.add(s2);
@@ -848,8 +848,10 @@ __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ 
         const CoupleRec r = cp[s];
         const NodeBox b = boxes[s];
         const uint32_t want = (p.ctag << SOLID_SLOT_BITS) | (uint32_t)(s + 1);
-        KSum acc[7];
-        for (int k = 0; k < 7; ++k) acc[k] = KSum{0.0, 0.0};
+        // each lane sums its few nodes directly; the 64 lane sums are then merged
+        // with compensation
+        double ls[7];
+        for (int k = 0; k < 7; ++k) ls[k] = 0.0;
         const V3 x0(r.x0[0], r.x0[1], r.x0[2]);
         const V3 rv(r.rv[0], r.rv[1], r.rv[2]);
         const V3 cen = x0 / p.unitLength;
@@ -874,11 +876,13 @@ __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ 
                     const bool itf = (link[idx[k]] & LINK_INTERFACE) != 0;
                     mass = itf ? fsMass[idx[k]] : (double)fsRho[idx[k]];
                 }
-                acc[0].add(d.x); acc[1].add(d.y); acc[2].add(d.z);
-                acc[3].add(m.x); acc[4].add(m.y); acc[5].add(m.z);
-                acc[6].add(mass);
+                ls[0] += d.x; ls[1] += d.y; ls[2] += d.z;
+                ls[3] += m.x; ls[4] += m.y; ls[5] += m.z;
+                ls[6] += mass;
             }
         });
+        KSum acc[7];
+        for (int k = 0; k < 7; ++k) acc[k] = KSum{ls[k], 0.0};
         for (int off = 32; off > 0; off >>= 1) {
             for (int k = 0; k < 7; ++k) {
                 const double s2 = __shfl_xor(acc[k].s, off, 64);
