@@ -45,6 +45,7 @@ public:
     // Copies element state (positions, velocities, forces) and particleEnergy from the device.
     void syncHost();
     std::vector<double> field(const char* name) const;
+    std::vector<unsigned int> indexField(const char* name) const;  // u32 views ("p_idx", ...)
     hyb_dem* handle() const { return h_; }
 
 private:

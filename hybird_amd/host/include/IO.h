@@ -1,6 +1,7 @@
 // Input/output side of the C++ host layer: the public part of the reference's
-// IO class (IO.h:36-131). Screen/export-file status lines follow IO.cpp:132-198;
-// the VTK writers of the reference are not part of this library.
+// IO class (IO.h:36-131). Status lines follow IO.cpp:132-198; the fluid (.vti),
+// particle (.vtu) and recycle (.dat) files follow IO.cpp:250-535, 698-831 with
+// the reference's names and number formatting, from views of the device state.
 #ifndef HYBIRD_AMD_IO_H
 #define HYBIRD_AMD_IO_H
 
@@ -31,12 +32,17 @@ public:
     double realTime = 0.0;
     string workDirectory, partDirectory, fluidDirectory;
     string exportFileName;
+    string fluidFileFormat, partFileFormat;
     ofstream exportFile;
     int saveCount = 0;
 
     void initialize();
     void outputStep(const double& demTime, const LB& lb, const DEM& dem);
     void outputFinal();
+    // writers (IO.cpp:289-535, 698-831)
+    void exportParaviewFluid(const LB& lb, const DEM& dem, const string& fluidFile) const;
+    void exportParaviewParticles(const DEM& dem, const string& particleFile) const;
+    void exportRecycleParticles(const DEM& dem) const;
 };
 
 #endif  // HYBIRD_AMD_IO_H

@@ -60,6 +60,8 @@ public:
     // ---- host views of the device state (names and layouts of hyb_lb_get)
     std::vector<double> field(const char* name) const;
     std::vector<unsigned char> nodeTypes() const;
+    std::vector<unsigned char> nodeFlags(const char* name) const;  // u8 views ("pflag")
+    std::vector<unsigned int> solidIndices() const;
     double totalFluidMass() const;
     hyb_lb* handle() const { return h_; }
     const hyb_config& config() const { return cfg_; }
@@ -71,6 +73,7 @@ private:
     hyb_dem* dem_ = nullptr;     // DEM handle coupled at init (for the state file)
     std::string restartFile_;    // extension: state file restored after init
     std::string saveStateFile_;  // extension: state file written when the run ends
+    bool fluidExport_ = false;   // fluid files requested (fluidExpTime > 0)
 };
 
 #endif  // HYBIRD_AMD_LB_H

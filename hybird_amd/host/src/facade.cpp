@@ -4,6 +4,7 @@
 #include <cerrno>
 #include <cstdio>
 #include <fstream>
+#include <sstream>
 
 #include "DEM.h"
 #include "IO.h"
@@ -153,6 +154,8 @@ void LB::latticeBoltzmannGet(GetPot& lbmCfgFile, GetPot& command_line) {  // LB.
     cfg_.precision = prec == "FP32" ? HYB_FP32_SHIFTED : HYB_FP64;
     // checkpoint and restart (the reference cannot resume a run)
     restartFile_ = extension_key<std::string>(lbmCfgFile, command_line, "restartFile", "");
+    // fluid files need n and u of every collision (kept on the device in free-surface mode)
+    fluidExport_ = extension_key<double>(lbmCfgFile, command_line, "fluidExpTime", 0.0) > 0.0;
     saveStateFile_ = extension_key<std::string>(lbmCfgFile, command_line, "saveStateFile", "");
     const char* boxKeys[6] = {"liquidBoxXMin", "liquidBoxXMax", "liquidBoxYMin", "liquidBoxYMax", "liquidBoxZMin",
                               "liquidBoxZMax"};
@@ -206,6 +209,7 @@ void LB::latticeBolzmannInit(cylinderList& cylinders, wallList& walls, particleL
     dem_ = dem ? dem->handle() : nullptr;
     HYB_CALL(hyb_lb_init(h_, dem_));
     inited_ = true;
+    if (freeSurface && fluidExport_) HYB_CALL(hyb_lb_set_capture(h_, 1));
     if (!restartFile_.empty()) {  // extension: continue from a state file
         HYB_CALL(hyb_state_load(h_, dem_, restartFile_.c_str()));
         cout << "state restored from " << restartFile_ << "\n";
@@ -235,6 +239,18 @@ void LB::latticeBoltzmannFreeSurfaceStep() { HYB_CALL(hyb_lb_fs_step(h_)); }  //
 std::vector<double> LB::field(const char* name) const {
     std::vector<double> v;
     if (!view(h_, hyb_lb_get, name, v)) hybird_check(HYB_E_ARG, name, __FILE__, __LINE__);
+    return v;
+}
+
+std::vector<unsigned char> LB::nodeFlags(const char* name) const {
+    std::vector<unsigned char> v;
+    if (!view(h_, hyb_lb_get, name, v)) hybird_check(HYB_E_ARG, name, __FILE__, __LINE__);
+    return v;
+}
+
+std::vector<unsigned int> LB::solidIndices() const {
+    std::vector<unsigned int> v;
+    if (!view(h_, hyb_lb_get, "solidIndex", v)) hybird_check(HYB_E_ARG, "solidIndex", __FILE__, __LINE__);
     return v;
 }
 
@@ -389,6 +405,17 @@ void DEM::discreteElementInit(unsIntList& boundary, unsIntList& lbSize, measureU
         w.slip = bc == 5 || bc == 6;
         walls.push_back(w);
     }
+    // periodic pairs as host records (DEM.cpp:937-988): planes at 0.5 with period N - 2
+    pbcs.clear();
+    for (int k = 0; k < 3; ++k) {
+        if (boundary[2 * k] != 4) continue;
+        pbc b;
+        const double span = (double(lbSize[k]) - 2.0) * unit.Length;
+        b.p1 = tVect(k == 0 ? 0.5 * unit.Length : 0.0, k == 1 ? 0.5 * unit.Length : 0.0, k == 2 ? 0.5 * unit.Length : 0.0);
+        b.v = tVect(k == 0 ? span : 0.0, k == 1 ? span : 0.0, k == 2 ? span : 0.0);
+        b.p2 = b.p1 + b.v;
+        pbcs.push_back(b);
+    }
     particles.clear();
     for (const elmt& e : elmts) {
         particle p;
@@ -446,6 +473,12 @@ void DEM::syncHost() {
     }
 }
 
+std::vector<unsigned int> DEM::indexField(const char* name) const {
+    std::vector<unsigned int> v;
+    if (!view(h_, hyb_dem_get, name, v)) hybird_check(HYB_E_ARG, name, __FILE__, __LINE__);
+    return v;
+}
+
 std::vector<double> DEM::field(const char* name) const {
     std::vector<double> v;
     if (!view(h_, hyb_dem_get, name, v)) hybird_check(HYB_E_ARG, name, __FILE__, __LINE__);
@@ -453,18 +486,224 @@ std::vector<double> DEM::field(const char* name) const {
 }
 
 // ======================================================================== IO
-void IO::initialize() {  // IO.cpp:25-130 (status file only)
+void IO::initialize() {  // IO.cpp:8-130 (status file and the fluid / particle data folders)
     if (!workDirectory.empty()) {
         const int a = mkdir(workDirectory.c_str(), 0777);
         if (a != 0 && errno != EEXIST) cout << "WARNING: cannot create " << workDirectory << "\n";
     }
-    exportFileName = (workDirectory.empty() ? std::string(".") : workDirectory) + "/export.dat";
-    lastScreenExp = 0;
+    const std::string base = workDirectory.empty() ? std::string(".") : workDirectory;
+    partDirectory = base + "/particleData";
+    fluidDirectory = base + "/fluidData";
+    if (demSolve) mkdir(partDirectory.c_str(), 0777);
+    if (lbmSolve) mkdir(fluidDirectory.c_str(), 0777);
+    fluidFileFormat = fluidDirectory + "/fluid%010u.vti";
+    partFileFormat = partDirectory + "/part%010u.vtu";
+    exportFileName = base + "/export.dat";
+    lastScreenExp = lastFluidExp = lastPartExp = lastObjectExp = lastOutputExp = lastRecycleExp = 0;
     realTime = 0.0;
+}
+
+// IO.cpp:698-831 (exportParaviewFluidOld, the writer createParaviewFiles calls):
+// ASCII ImageData with the node type (1 inside particles), velocity and
+// pressure in physical units, viscosity (non-Newtonian), mass (free surface)
+// and solidIndex (with DEM). Nodes without a node object in the reference
+// (inactive ones) print zeros, as do the wall nodes it keeps (density rho0,
+// zero velocity and mass). solidIndex: particle index on particle nodes, on wall
+// nodes the last wall whose plane holds the node (LB.cpp:497-533), 0 elsewhere
+// (the reference keeps the index of the last particle that covered a node).
+void IO::exportParaviewFluid(const LB& lb, const DEM& dem, const string& fluidFile) const {
+    const size_t N = lb.totNodes;
+    const std::vector<unsigned char> t = lb.nodeTypes();
+    const std::vector<unsigned char> pf = lb.nodeFlags("pflag");
+    const std::vector<double> n = lb.field("n"), u = lb.field("u");
+    std::vector<double> visc, mass;
+    if (lb.nonNewtonian) visc = lb.field("visc");
+    if (lb.freeSurface) mass = lb.field("mass");
+    std::vector<unsigned int> sid;
+    if (demSolve) sid = lb.solidIndices();
+    const auto active = [&](size_t i) { return t[i] == 0 || t[i] == 3; };
+    ofstream f(fluidFile.c_str());
+    const double zero = 0.0;
+    f << "<VTKFile type=\"ImageData\" version=\"0.1\">\n";
+    f << " <ImageData WholeExtent=\"0 " << lb.lbSize[0] - 1 << " 0 " << lb.lbSize[1] - 1 << " 0 " << lb.lbSize[2] - 1
+      << "\" " << "Origin=\"0.0 0.0 0.0\" Spacing=\"" << lb.unit.Length << " " << lb.unit.Length << " "
+      << lb.unit.Length << "\">\n";
+    f << "  <Piece Extent=\"0 " << lb.lbSize[0] - 1 << " 0 " << lb.lbSize[1] - 1 << " 0 " << lb.lbSize[2] - 1 << "\">\n";
+    f << "   <PointData>\n";
+    f << "    <DataArray type=\"Int8\" Name=\"type\" format=\"ascii\" RangeMin=\"0\" RangeMax=\"8\">\n";
+    for (size_t i = 0; i < N; ++i) f << ((active(i) && pf[i]) ? 1u : (unsigned int)t[i]) << " ";
+    f << "    </DataArray>\n";
+    f << "    <DataArray type=\"Float64\" Name=\"v\" NumberOfComponents=\"3\" format=\"ascii\">\n";
+    for (size_t i = 0; i < N; ++i) {
+        if (!active(i)) {
+            f << zero << " " << zero << " " << zero << " ";
+        } else {
+            const tVect uPhysic = tVect(u[3 * i], u[3 * i + 1], u[3 * i + 2]) * lb.unit.Speed;
+            f << uPhysic.x << " " << uPhysic.y << " " << uPhysic.z << " ";
+        }
+    }
+    f << "    </DataArray>\n";
+    f << "    <DataArray type=\"Float64\" Name=\"pressure\" format=\"ascii\" RangeMin=\"0\" RangeMax=\"2\">\n";
+    for (size_t i = 0; i < N; ++i) {
+        if (!active(i)) f << zero << " ";
+        else if (n[i] != 0.0) f << 0.3333333 * (n[i] - 1.0) * lb.unit.Pressure << " ";
+        else f << n[i] << " ";
+    }
+    f << "    </DataArray>\n";
+    if (lb.nonNewtonian) {
+        f << "    <DataArray type=\"Float64\" Name=\"dynVisc\" format=\"ascii\" RangeMin=\"0\" RangeMax=\"2\">\n";
+        for (size_t i = 0; i < N; ++i) f << (active(i) ? visc[i] * lb.unit.DynVisc : zero) << " ";
+        f << "    </DataArray>\n";
+    }
+    if (lb.freeSurface) {
+        f << "    <DataArray type=\"Float64\" Name=\"AAAmass\" format=\"ascii\" RangeMin=\"0\" RangeMax=\"2\">\n";
+        for (size_t i = 0; i < N; ++i) f << (active(i) ? mass[i] * lb.unit.Density : zero) << " ";
+        f << "    </DataArray>\n";
+    }
+    if (demSolve) {
+        f << "    <DataArray type=\"Int16\" Name=\"solidIndex\" format=\"ascii\" RangeMin=\"0\" RangeMax=\"8\">\n";
+        for (size_t i = 0; i < N; ++i) {
+            unsigned int s = (active(i) && pf[i]) ? sid[i] : 0u;
+            if (t[i] >= 5 && t[i] <= 8) {
+                const tVect pos(lb.getX(i), lb.getY(i), lb.getZ(i));
+                for (const wall& w : dem.walls) {
+                    const tVect d = pos - w.p / lb.unit.Length;
+                    if (w.n.x * d.x + w.n.y * d.y + w.n.z * d.z < 0.0) s = w.index;
+                }
+            }
+            f << s << " ";
+        }
+        f << "    </DataArray>\n";
+    }
+    f << "   </PointData>\n";
+    f << "   <CellData>\n";
+    f << "   </CellData>\n";
+    f << "  </Piece>\n";
+    f << " </ImageData>\n";
+    f << "</VTKFile>\n";
+}
+
+// IO.cpp:289-419: one point per particle (ghosts included) with its element's
+// mass, velocities and forces.
+void IO::exportParaviewParticles(const DEM& dem, const string& particleFile) const {
+    const std::vector<double> pd = dem.field("p_data");       // [S][10]: x0, x1, radiusVec, r
+    const std::vector<unsigned int> pi = dem.indexField("p_idx");  // [S][4]: particle, cluster, proto, cell
+    const std::vector<double> misc = dem.field("e_misc");     // [E][19]: m, radius, I, wGlobal, ...
+    const std::vector<double> fr = dem.field("e_forces");     // [E][7][3]
+    const size_t S = pd.size() / 10;
+    ofstream f(particleFile.c_str());
+    const auto vec = [&](const double* v) { f << v[0] << " " << v[1] << " " << v[2] << "\n"; };
+    const auto cl = [&](size_t i) { return (size_t)pi[4 * i + 1]; };
+    f << "<?xml version=\"1.0\"?>\n";
+    f << "<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\" compressor=\"vtkZLibDataCompressor\">\n";
+    f << " <UnstructuredGrid GhostLevel=\"0\">\n";
+    f << "  <Piece NumberOfPoints=\"" << S << "\" NumberOfCells=\"" << S << "\">\n";
+    f << "   <PointData>\n";
+    f << "    <DataArray type=\"Float64\" Name=\"radius\"/>\n";
+    for (size_t i = 0; i < S; ++i) f << pd[10 * i + 9] << "\n";
+    f << "    <DataArray type=\"Float64\" Name=\"mass\"/>\n";
+    for (size_t i = 0; i < S; ++i) f << misc[19 * cl(i)] << "\n";
+    f << "    <DataArray type=\"Float64\" Name=\"particleIndex\"/>\n";
+    for (size_t i = 0; i < S; ++i) f << pi[4 * i] << "\n";
+    f << "    <DataArray type=\"Float64\" Name=\"clusterIndex\"/>\n";
+    for (size_t i = 0; i < S; ++i) f << pi[4 * i + 1] << "\n";
+    f << "    <DataArray type=\"Float64\" Name=\"v\" NumberOfComponents=\"3\"/>\n";
+    for (size_t i = 0; i < S; ++i) vec(&pd[10 * i + 3]);
+    f << "    <DataArray type=\"Float64\" Name=\"w\" NumberOfComponents=\"3\"/>\n";
+    for (size_t i = 0; i < S; ++i) vec(&misc[19 * cl(i) + 5]);
+    // e_forces order: FHydro, MHydro, FParticle, MParticle, FWall, MWall, FGrav
+    const int order[6] = {2, 6, 3, 4, 5, 0};
+    const char* names[6] = {"FParticle", "FGrav", "MParticle", "FWall", "MWall", "FHydro"};
+    for (int k = 0; k < 6; ++k) {
+        f << "    <DataArray type=\"Float64\" Name=\"" << names[k] << "\" NumberOfComponents=\"3\"/>\n";
+        for (size_t i = 0; i < S; ++i) vec(&fr[21 * cl(i) + 3 * order[k]]);
+    }
+    f << "   </PointData>\n";
+    f << "   <CellData>\n";
+    f << "   </CellData>\n";
+    f << "   <Points>\n";
+    f << "    <DataArray type=\"Float64\" Name=\"Points\" NumberOfComponents=\"3\"/>\n";
+    for (size_t i = 0; i < S; ++i) vec(&pd[10 * i]);
+    f << "   </Points>\n";
+    f << "   <Cells>\n";
+    f << "    <DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n";
+    for (size_t i = 1; i < S + 1; ++i) f << i - 1 << "\n";
+    f << "    </DataArray>\n";
+    f << "    <DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n";
+    for (size_t i = 1; i < S + 1; ++i) f << i << "\n";
+    f << "    </DataArray>\n";
+    f << "    <DataArray type=\"Int64\" Name=\"types\" format=\"ascii\">\n";
+    for (size_t i = 0; i < S; ++i) f << 1 << "\n";
+    f << "    </DataArray>\n";
+    f << "   </Cells>\n";
+    f << "  </Piece>\n";
+    f << " </UnstructuredGrid>\n";
+    f << "</VTKFile>";
+}
+
+// IO.cpp:486-535: the element file format the code reads back (DEM.cpp:78-127),
+// positions wrapped into the periodic box.
+void IO::exportRecycleParticles(const DEM& dem) const {
+    std::stringstream ss;
+    ss << currentTimeStep;
+    const string path = partDirectory + "/recycle_" + ss.str() + ".dat";
+    const std::vector<double> x = dem.field("e_x"), w = dem.field("e_w"), q = dem.field("e_q"),
+                              misc = dem.field("e_misc");
+    const size_t E = dem.elmts.size();
+    ofstream f(path.c_str());
+    f << E << "\n";
+    const auto v3 = [&](const double* v) { f << v[0] << " " << v[1] << " " << v[2] << " "; };
+    for (size_t i = 0; i < E; ++i) {
+        f << dem.elmts[i].index << " ";
+        f << dem.elmts[i].size << " ";
+        f << misc[19 * i + 1] << " ";
+        const tVect x0(x[18 * i], x[18 * i + 1], x[18 * i + 2]);
+        tVect printPosition = x0;
+        for (size_t b = 0; b < dem.pbcs.size(); ++b) {
+            const pbc& pb = dem.pbcs[b];
+            const tVect n1 = pb.v / pb.v.norm();
+            const tVect d1 = x0 - pb.p1, d2 = x0 - pb.p2;
+            const double leftDist = n1.x * d1.x + n1.y * d1.y + n1.z * d1.z;
+            const tVect n2 = n1 * -1.0;
+            const double rightDist = n2.x * d2.x + n2.y * d2.y + n2.z * d2.z;
+            if (leftDist < 0.0) printPosition = printPosition + pb.v;
+            if (rightDist < 0.0) printPosition = printPosition - pb.v;
+        }
+        f << printPosition.x << " " << printPosition.y << " " << printPosition.z << " ";
+        v3(&x[18 * i + 3]);
+        v3(&w[18 * i]);
+        const double* q0 = &q[24 * i];
+        const double* q1 = &q[24 * i + 4];
+        f << q0[0] << " " << q0[1] << " " << q0[2] << " " << q0[3] << " ";
+        f << q1[0] << " " << q1[1] << " " << q1[2] << " " << q1[3] << "\n";
+    }
 }
 
 void IO::outputStep(const double& demTime, const LB& lb, const DEM& dem) {  // IO.cpp:132-198
     realTime = demTime;
+    // createParaviewFiles (IO.cpp:250-287)
+    char filePathBuffer[1024];
+    if (lbmSolve) {
+        const unsigned int c = (fluidExpTime > 0 ? static_cast<unsigned int>(realTime / fluidExpTime) + 1 : 0);
+        if (c > lastFluidExp) {
+            lastFluidExp = c;
+            std::snprintf(filePathBuffer, sizeof(filePathBuffer), fluidFileFormat.c_str(), currentTimeStep);
+            exportParaviewFluid(lb, dem, filePathBuffer);
+        }
+    }
+    if (demSolve) {
+        const unsigned int c = (partExpTime > 0 ? static_cast<unsigned int>(realTime / partExpTime) + 1 : 0);
+        if (c > lastPartExp) {
+            lastPartExp = c;
+            std::snprintf(filePathBuffer, sizeof(filePathBuffer), partFileFormat.c_str(), currentTimeStep);
+            exportParaviewParticles(dem, filePathBuffer);
+        }
+        const unsigned int r = (recycleExpTime > 0 ? static_cast<unsigned int>(realTime / recycleExpTime) + 1 : 0);
+        if (r > lastRecycleExp) {
+            lastRecycleExp = r;
+            exportRecycleParticles(dem);
+        }
+    }
     const unsigned int screenExpCounter =
         (screenExpTime > 0 ? static_cast<unsigned int>(realTime / screenExpTime) + 1 : 0);
     if (screenExpCounter > lastScreenExp) {

@@ -116,6 +116,20 @@ def _cases():
                                              lbSizeZ=26, initVisc=0.05, initVelocityX=0.03, initVelocityZ=0.02,
                                              lbFX=5.0e-4, lbFZ=-5.0e-4, **walls),
                              bed=Bed(np.zeros((0, 3)), np.zeros(0)), cycles=[0, 1, 5, 20, 40], keys=LATTICE_KEYS)
+    # output files of the reference (IO.cpp:250-535, 698-831): fluid .vti, particle
+    # .vtu and recycle .dat every 4 time units, kept as expected outputs
+    out = dict(fluidExpTime=4.0, partExpTime=4.0, recycleExpTime=4.0)
+    cases["io"] = dict(cfg=base_config(lbSizeX=14, lbSizeY=12, lbSizeZ=16, initVisc=0.05, lbFX=3.0e-4, lbFY=1.0e-4,
+                                       lbFZ=-4.0e-4, initVelocityX=0.02, initVelocityY=0.01, youngMod=2.0e3, **walls,
+                                       **out),
+                       bed=Bed(np.array([[3.2, 6.0, 4.0], [9.0, 5.0, 3.1], [7.0, 6.5, 9.0]]), np.array([2.2, 2.0, 2.5])),
+                       cycles=[0, 10], keys=["types"], files=True)
+    # free surface with a Bingham fluid: adds the viscosity and mass arrays
+    cases["io_fs"] = dict(cfg=base_config(problemName="SPLASH", freeSurfaceSolve=1, nonNewtonianSolve=1, lbSizeX=12,
+                                          lbSizeY=10, lbSizeZ=26, initVisc=0.05, plasticVisc=0.05, yieldStress=1.0e-5,
+                                          initVelocityX=0.03, initVelocityZ=0.02, lbFX=5.0e-4, lbFZ=-5.0e-4,
+                                          boundary0=7, boundary1=7, boundary2=4, boundary3=4, fluidExpTime=4.0),
+                          bed=Bed(np.zeros((0, 3)), np.zeros(0)), cycles=[0, 10], keys=["types"], files=True)
     return cases
 
 
@@ -150,6 +164,13 @@ def run_case(name: str, case: dict, outdir: str) -> None:
                 if k == "f" and c not in full_f:
                     continue
                 arrays[f"c{c}/{k}"] = d[k]
+        if case.get("files"):  # the reference's output files, as bytes
+            for dirpath, _, names in os.walk(work):
+                if os.path.basename(dirpath) not in ("fluidData", "particleData"):
+                    continue
+                for fn in sorted(names):
+                    with open(os.path.join(dirpath, fn), "rb") as fh:
+                        arrays[f"file/{fn}"] = np.frombuffer(fh.read(), dtype=np.uint8)
         arrays["cycles"] = np.array(case["cycles"], dtype=np.int64)
         # inputs, so the fixture is self-contained
         keys = sorted(k for k, v in cfg.items() if not isinstance(v, str))
