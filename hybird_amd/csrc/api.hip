@@ -116,6 +116,8 @@ struct hyb_dem {
     double demSize[3] = {1, 1, 1};
     std::vector<WallDev> walls;
     std::vector<PbcDev> pbcs;
+    std::vector<ObjDev> objects;  // object file entries, then periodic copies (set at init)
+    int stdObjects = 0;
     DEMParams p;
     DEMBuffers b;
     bool inited = false;
@@ -290,6 +292,26 @@ int hyb_dem_create(const hyb_config* cfg, int32_t n, const double* rec, hyb_dem*
     return HYB_OK;
 }
 
+int hyb_dem_set_objects(hyb_dem* d, int32_t n, const double* rec) {  // DEM.cpp:131-173
+    if (!d || n < 0 || (n > 0 && !rec)) return fail(HYB_E_ARG, "hyb_dem_set_objects: bad arguments");
+    if (d->inited) return fail(HYB_E_STATE, "hyb_dem_set_objects after init");
+    d->objects.clear();
+    for (int o = 0; o < n; ++o) {
+        const double* r = rec + (size_t)o * HYB_OBJECT_RECORD;
+        // periodic copies index the list with the object's index (DEM.cpp:1049-1050)
+        if ((int)r[0] != o) return fail(HYB_E_ARG, "object indices must be 0..n-1 in file order");
+        if (!(r[2] > 0.0)) return fail(HYB_E_ARG, "object radius must be > 0");
+        ObjDev ob;
+        ob.index = o;
+        ob.r = r[2];
+        ob.x0 = V3(r[3], r[4], r[5]);
+        ob.x1 = V3(r[6], r[7], r[8]);
+        d->objects.push_back(ob);
+    }
+    d->stdObjects = n;
+    return HYB_OK;
+}
+
 // What DEM::discreteElementInit receives from the lattice (DEM.cpp:185-297):
 // boundary codes, node counts, units and the body force in lattice units.
 struct LatticeInfo {
@@ -414,6 +436,32 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
     p.hertzian = c.contactModel == HYB_CONTACT_HERTZIAN;
     p.numVisc = c.numVisc;
     p.demF[0] = demF.x; p.demF[1] = demF.y; p.demF[2] = demF.z;
+    // objects: periodic copies of those within 2 nebrRange of a periodic plane,
+    // made only when elements exist (DEM.cpp:252-256, 990-1062)
+    d->objects.resize((size_t)d->stdObjects);
+    if (d->P) {
+        struct G { int origin; V3 v; };
+        std::vector<G> side, corner;
+        for (const PbcDev& pb : d->pbcs)
+            for (int o = 0; o < d->stdObjects; ++o) {
+                const double leftDist = pb.n1.dot(d->objects[o].x0 - pb.p1);
+                const double rightDist = pb.n2.dot(d->objects[o].x0 - pb.p2);
+                if (leftDist < 2.0 * p.nebrRange && leftDist > 0.0) side.push_back(G{d->objects[o].index, pb.v});
+                else if (rightDist < 2.0 * p.nebrRange && rightDist > 0.0)
+                    side.push_back(G{d->objects[o].index, -1.0 * pb.v});
+            }
+        for (size_t a = 0; a < side.size(); ++a)
+            for (size_t k = a + 1; k < side.size(); ++k)
+                if (side[a].origin == side[k].origin) corner.push_back(G{side[a].origin, side[a].v + side[k].v});
+        side.insert(side.end(), corner.begin(), corner.end());
+        for (size_t g = 0; g < side.size(); ++g) {
+            ObjDev ob = d->objects[side[g].origin];
+            ob.x0 = ob.x0 + side[g].v;
+            ob.index = d->stdObjects + (int)g;
+            d->objects.push_back(ob);
+        }
+    }
+    p.nobj = d->P ? (int)d->objects.size() : 0;
     // capacities
     const int np = p.npbc;
     const int ghostsPer = np + np * (np - 1) / 2;
@@ -474,6 +522,15 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
         (rc = ralloc(R, "posI", &b.posI, p.pairCap)) || (rc = ralloc(R, "posJ", &b.posJ, p.pairCap)) ||
         (rc = dalloc(&b.prec, 2 * (size_t)p.pairCap)) || (rc = ralloc(R, "vmax", &b.vmax, (size_t)(std::max(P, 1) + 63) / 64)) || (rc = dalloc(&b.energy, 6)))
         return rc;
+    if (p.nobj) {
+        ObjDev* objs = nullptr;
+        if ((rc = ralloc(R, "objs", &objs, (size_t)p.nobj)) ||
+            (rc = ralloc(R, "nearObj", &b.nearObj, (size_t)P * NEAR_OBJ)) ||
+            (rc = ralloc(R, "nearObjCount", &b.nearObjCount, (size_t)P)))
+            return rc;
+        HIPCHK(hipMemcpy(objs, d->objects.data(), sizeof(ObjDev) * (size_t)p.nobj, hipMemcpyHostToDevice));
+        b.objs = objs;
+    }
     HIPCHK(hipMemcpyAsync(b.el, el.data(), el.size() * sizeof(double), hipMemcpyHostToDevice, d->stream));
     DEMState st;
     std::memset(&st, 0, sizeof(st));
@@ -517,7 +574,8 @@ int hyb_dem_destroy(hyb_dem* d) {
     dfree(b.el); dfree(b.st); dfree(b.ghostIndex); dfree(b.ghostVec); dfree(b.pairs); dfree(b.nearWall);
     dfree(b.cellTable); dfree(b.cellOf); dfree(b.cellCount); dfree(b.work); dfree(b.csrOff); dfree(b.csr);
     dfree(b.cursor); dfree(b.ghostOfCount); dfree(b.ghostOf); dfree(b.side); dfree(b.ghostB);
-    dfree(b.posI); dfree(b.posJ); dfree(b.prec); dfree(b.vmax); dfree(b.energy);
+    dfree(b.posI); dfree(b.posJ); dfree(b.prec); dfree(b.vmax); dfree(b.energy); dfree(b.nearObj); dfree(b.nearObjCount);
+    if (b.objs) (void)hipFree((void*)b.objs);
     delete d;
     return HYB_OK;
 }
@@ -657,14 +715,21 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
     if (rc) return rc;
     const LBKParams p = l->params();
     hipStream_t s = l->stream;
-    // node types: box boundaries, DEM plane walls, gas/interface
+    // node types: box boundaries, DEM plane walls, objects (LB.cpp:324-339), gas/interface
     std::vector<WallDev> walls = d ? d->walls : std::vector<WallDev>();
     WallDev* dw = nullptr;
     if (!walls.empty()) {
         HIPCHK(hipMalloc((void**)&dw, walls.size() * sizeof(WallDev)));
         HIPCHK(hipMemcpyAsync(dw, walls.data(), walls.size() * sizeof(WallDev), hipMemcpyHostToDevice, s));
     }
-    launch_lb_init_types(l->type, l->boundary, (int)walls.size(), dw, l->cfg.problem, l->cfg.liquidBox, p, s);
+    std::vector<ObjDev> objs = d ? d->objects : std::vector<ObjDev>();
+    ObjDev* dob = nullptr;
+    if (!objs.empty()) {
+        HIPCHK(hipMalloc((void**)&dob, objs.size() * sizeof(ObjDev)));
+        HIPCHK(hipMemcpyAsync(dob, objs.data(), objs.size() * sizeof(ObjDev), hipMemcpyHostToDevice, s));
+    }
+    launch_lb_init_types(l->type, l->boundary, (int)walls.size(), dw, (int)objs.size(), dob, l->cfg.problem,
+                         l->cfg.liquidBox, p, s);
     launch_lb_init_interface(l->type, p, s);
     launch_lb_init_links(l->type, l->link, l->slipw, p, s);
     HIPCHK(hipMemsetAsync(l->maxXYZ, 0, 3 * sizeof(int) * l->Z, s));
@@ -701,6 +766,7 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
     if (dw) (void)hipFree(dw);
+    if (dob) (void)hipFree(dob);
     {
         std::vector<unsigned long long> cz(l->Z);
         HIPCHK(hipMemcpy(cz.data(), l->dcount, cz.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
@@ -1226,7 +1292,8 @@ int hyb_dem_get(hyb_dem* d, const char* cname, void* host, size_t cap, size_t* n
     std::memset(&st, 0, sizeof(st));
     if (P) HIPCHK(hipMemcpy(&st, d->b.st, sizeof(st), hipMemcpyDeviceToHost));
     if (st.error) return fail(HYB_E_CAPACITY, "DEM device error " + std::to_string(st.error) +
-                                                   " (1/4: ghost or pair capacity, 2: particle outside the cell grid, 3: stencil outside the cell table)");
+                                                   " (1/4: ghost or pair capacity, 2: particle outside the cell grid, 3: stencil outside the cell table,"
+                                                   " 5: more than 16 objects near one particle)");
     std::vector<double> el;
     if (P) {
         el.resize((size_t)EF_COUNT * P);
@@ -1241,6 +1308,25 @@ int hyb_dem_get(hyb_dem* d, const char* cname, void* host, size_t cap, size_t* n
     if (G) {
         HIPCHK(hipMemcpy(gi.data(), d->b.ghostIndex, 4 * (size_t)G, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(gv.data(), d->b.ghostVec, 24 * (size_t)G, hipMemcpyDeviceToHost));
+    }
+    if (name == "nearObjectTable") {  // DEM.cpp:1515-1532: (particle, object) pairs, particle-major
+        std::vector<int> cnt(P ? (size_t)P : 0);
+        std::vector<uint32_t> lst(P ? (size_t)P * NEAR_OBJ : 0);
+        if (P && d->p.nobj) {
+            HIPCHK(hipMemcpy(cnt.data(), d->b.nearObjCount, 4 * (size_t)P, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(lst.data(), d->b.nearObj, 4 * lst.size(), hipMemcpyDeviceToHost));
+            for (int e = 0; e < P; ++e)
+                for (int k = 0; k < cnt[e]; ++k) { u.push_back((uint32_t)e); u.push_back(lst[(size_t)e * NEAR_OBJ + k]); }
+        }
+        return put(u.data(), u.size() * 4, host, cap, need);
+    }
+    if (name == "objects") {  // index, r, x0, x1 of every object (originals, then periodic copies)
+        for (const ObjDev& o : d->objects) {
+            v.push_back((double)o.index); v.push_back(o.r);
+            v.push_back(o.x0.x); v.push_back(o.x0.y); v.push_back(o.x0.z);
+            v.push_back(o.x1.x); v.push_back(o.x1.y); v.push_back(o.x1.z);
+        }
+        return put(v.data(), v.size() * 8, host, cap, need);
     }
     if (name == "energy") {  // DEM::particleEnergy, updated after every sub-step (DEM.cpp:371-372)
         double e6[6] = {0, 0, 0, 0, 0, 0};

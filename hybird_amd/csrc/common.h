@@ -147,6 +147,14 @@ struct WallDev {
     }
 };
 
+// Static sphere of the object file (DEM.cpp:131-173): position, velocity,
+// radius and the object's index (originals, then periodic copies).
+struct ObjDev {
+    V3 x0, x1;
+    double r;
+    int index;
+};
+
 struct PbcDev {
     V3 v;              // period vector
     V3 p1, n1, p2, n2; // the two planes

@@ -329,7 +329,25 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
         nw += wi != 0xFFFFFFFFu;
     }
     red[t] = (double)nw;
+    // ---- near-object lists (DEM.cpp:1515-1532): every object within range, in list order
+    if (p.nobj) {
+        const double nebrRange2 = p.nebrRange * p.nebrRange;
+        for (int e = t; e < P; e += RB) {
+            const V3 x0 = E.v3(EF_X, e);
+            int c = 0;
+            for (int o = 0; o < p.nobj; ++o) {
+                const V3 x0ij = x0 - b.objs[o].x0;
+                if (x0ij.norm2() < nebrRange2) {
+                    if (c < NEAR_OBJ) b.nearObj[(size_t)e * NEAR_OBJ + c] = (uint32_t)o;
+                    ++c;
+                }
+            }
+            if (c > NEAR_OBJ) { sh_err = 5; c = NEAR_OBJ; }
+            b.nearObjCount[e] = c;
+        }
+    }
     __syncthreads();
+    if (sh_err) { if (t == 0) b.st->error = 5; return; }
     // ---- per-particle pair lists, ascending pair index
     for (int e = t; e < P; e += RB) b.cursor[e] = 0;
     __syncthreads();
@@ -641,6 +659,36 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
             const double nt = tangRelVelContact.norm();
             if (nt != 0.0) {
                 const double ft = tangential_contact(p, nt, fn, re, me, p.fricWall);
+                const V3 et = tangRelVelContact / tangRelVelContact.norm();
+                const V3 tangForce = ft * et;
+                MW = MW - centerDistJ.cross(tangForce);
+                FW = FW - tangForce;
+            }
+        }
+    }
+    // object contacts (DEM.cpp:1747-1770, 2060-2136), in list order after the wall
+    if (p.nobj) {
+        const int no = b.nearObjCount[e];
+        for (int k = 0; k < no; ++k) {
+            const ObjDev& ob = b.objs[b.nearObj[(size_t)e * NEAR_OBJ + k]];
+            const V3 vectorDistance = xpe - ob.x0;
+            const double distance = vectorDistance.norm();
+            const double overlap = re + ob.r - distance;
+            if (!(overlap > 0.0)) continue;
+            const V3 en = (1.0 / distance) * vectorDistance;
+            const V3 relVel = vpe - ob.x1;
+            const double normRelVel = relVel.dot(en);
+            const V3 normalRelVel = en * normRelVel;
+            const double fn = normal_contact(p, 2.0 * overlap, normRelVel, re, me);
+            const V3 normalForce = en * fn;
+            const V3 vecRadJ = -re * en;
+            const V3 centerDistJ = vecRadJ + (xpe - xpe);  // particle = element centre (single spheres)
+            FW = FW + normalForce;
+            const V3 relVelContact = relVel + wpg.cross(vecRadJ);
+            const V3 tangRelVelContact = relVelContact - normalRelVel;
+            const double nt = tangRelVelContact.norm();
+            if (nt != 0.0) {
+                const double ft = tangential_contact(p, nt, fn, re, me, p.fricPart);
                 const V3 et = tangRelVelContact / tangRelVelContact.norm();
                 const V3 tangForce = ft * et;
                 MW = MW - centerDistJ.cross(tangForce);

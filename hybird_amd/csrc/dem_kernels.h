@@ -33,6 +33,7 @@ struct DEMParams {
     int Gcap;              // ghost capacity
     int pairCap;           // neighbour-pair capacity
     int npbc, nwalls;
+    int nobj;              // objects (originals and periodic copies), DEMBuffers::objs
     PbcDev pbc[3];
     WallDev walls[6];
     double cellWidth[3];
@@ -44,6 +45,10 @@ struct DEMParams {
     double numVisc;
     double demF[3];
 };
+
+// Objects a particle can have in range at once (a rebuild that finds more
+// reports DEMState::error 5).
+constexpr int NEAR_OBJ = 16;
 
 // Device-side mutable scalars of the DEM.
 struct DEMState {
@@ -81,6 +86,9 @@ struct DEMBuffers {
     struct PairRec* prec;  // [2 * pairCap] per csr slot: signed force/torque increments of one pair
     double* vmax;          // [(P + 63) / 64] per-wavefront max |x1|^2 written by the corrector
     double* energy;        // [6] particleEnergy: kin, rotKin, trKin, grav, elastic, total
+    const ObjDev* objs;    // [nobj]
+    uint32_t* nearObj;     // [P][NEAR_OBJ] objects within nebrRange of each particle, ascending
+    int* nearObjCount;     // [P]
     int scap;
     int ncells;
     int W;                 // ncells * 14

@@ -521,8 +521,8 @@ __global__ void __launch_bounds__(BX * BY) k_stream(const T* __restrict__ fin, T
 // Boundary typing (LB.cpp:394-431), plane walls (LB.cpp:497-533) and the
 // problem's gas region (LB.cpp:611-784: SPLASH plane, or a liquid box).
 __global__ void k_init_types(uint8_t* __restrict__ type, LBKParams p, int b0, int b1, int b2, int b3, int b4, int b5,
-                             int nwalls, const WallDev* __restrict__ walls, int problem, double lx0, double lx1,
-                             double ly0, double ly1, double lz0, double lz1) {
+                             int nwalls, const WallDev* __restrict__ walls, int nobj, const ObjDev* __restrict__ objs,
+                             int problem, double lx0, double lx1, double ly0, double ly1, double lz0, double lz1) {
     Coord c;
     if (!node_coord(p, c)) return;
     uint8_t t = T_FLUID;
@@ -540,6 +540,12 @@ __global__ void k_init_types(uint8_t* __restrict__ type, LBKParams p, int b0, in
             if (walls[w].slip) t = walls[w].moving ? 6 : 5;
             else t = walls[w].moving ? 8 : 7;
         }
+    }
+    // objects are static walls (LB.cpp:535-550)
+    for (int o = 0; o < nobj; ++o) {
+        const V3 d = pos - objs[o].x0 / p.unitLength;
+        const double rr = objs[o].r / p.unitLength;
+        if (d.norm2() < rr * rr) t = T_WALL;
     }
     if (t == T_FLUID) {
         if (problem == HYB_PROBLEM_SPLASH) {
@@ -953,11 +959,11 @@ void launch_lb_stream(Prec prec, const void* fin, void* fout, const uint32_t* li
     else k_stream<float><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, p);
 }
 
-void launch_lb_init_types(uint8_t* type, const int* bnd, int nwalls, const WallDev* walls, int problem,
-                          const double* lbx, const LBKParams& p, hipStream_t s) {
+void launch_lb_init_types(uint8_t* type, const int* bnd, int nwalls, const WallDev* walls, int nobj,
+                          const ObjDev* objs, int problem, const double* lbx, const LBKParams& p, hipStream_t s) {
     const dim3 g = lattice_grid(p), b(BX, BY, 1);
-    k_init_types<<<g, b, 0, s>>>(type, p, bnd[0], bnd[1], bnd[2], bnd[3], bnd[4], bnd[5], nwalls, walls, problem, lbx[0],
-                                 lbx[1], lbx[2], lbx[3], lbx[4], lbx[5]);
+    k_init_types<<<g, b, 0, s>>>(type, p, bnd[0], bnd[1], bnd[2], bnd[3], bnd[4], bnd[5], nwalls, walls, nobj, objs,
+                                 problem, lbx[0], lbx[1], lbx[2], lbx[3], lbx[4], lbx[5]);
 }
 
 void launch_lb_init_interface(uint8_t* type, const LBKParams& p, hipStream_t s) {

@@ -91,8 +91,8 @@ void launch_lb_fused(Prec prec, bool pull, bool varVisc, int collision, const vo
                      const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, const CoupleRecF* cpf, void* hf,
                      double* visc, double* mac, const FSView* fs, const LBKParams& p, hipStream_t s);
 void launch_lb_stream(Prec prec, const void* fin, void* fout, const uint32_t* link, const LBKParams& p, hipStream_t s);
-void launch_lb_init_types(uint8_t* type, const int* boundary, int nwalls, const WallDev* walls, int problem,
-                          const double* liquidBox, const LBKParams& p, hipStream_t s);
+void launch_lb_init_types(uint8_t* type, const int* boundary, int nwalls, const WallDev* walls, int nobj,
+                          const ObjDev* objs, int problem, const double* liquidBox, const LBKParams& p, hipStream_t s);
 void launch_lb_init_interface(uint8_t* type, const LBKParams& p, hipStream_t s);
 // slipw: per-node slip-wall directions (nullptr when no slip walls exist)
 void launch_lb_init_links(const uint8_t* type, uint32_t* link, uint32_t* slipw, const LBKParams& p, hipStream_t s);

@@ -51,6 +51,7 @@ public:
 private:
     hyb_config cfg_;
     std::vector<double> records_;
+    std::vector<double> objectRecords_;  // HYB_OBJECT_RECORD per object
     hyb_dem* h_ = nullptr;
 };
 

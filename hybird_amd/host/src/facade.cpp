@@ -201,8 +201,9 @@ void LB::LBShow() const {  // LB.cpp:2290-2312
 void LB::latticeBolzmannInit(cylinderList& cylinders, wallList& walls, particleList& particles,
                              objectList& objects) {  // LB.cpp:190-219
     (void)walls;
-    if (!cylinders.empty() || !objects.empty()) {
-        cout << INFO << "ERROR: cylinders and objects are not supported by the GPU lattice" << std::endl;
+    (void)objects;  // the DEM handle carries them (hyb_dem_set_objects)
+    if (!cylinders.empty()) {
+        cout << INFO << "ERROR: cylinders are not supported by the GPU lattice" << std::endl;
         exit(1);
     }
     DEM* dem = particles.owner;
@@ -353,16 +354,26 @@ void DEM::discreteElementGet(GetPot& lbmCfgFile, GetPot& command_line) {  // DEM
         e.q1 = tQuat(r[16], r[17], r[18], r[19]);
         elmts.push_back(e);
     }
-    // objects (DEM.cpp:118-161): the file must exist; objects are not supported on the GPU
+    // objects (DEM.cpp:131-173): static spheres; the file must exist. Positions
+    // are taken as written (no scale or translation); the rotation values are skipped.
     string objectFile;
     PARSE_CLASS_MEMBER(lbmCfgFile, objectFile, "objectFile", "objects.dat");
     ifstream objectFileID(objectFile.c_str(), ios::in);
     ASSERT(objectFileID.is_open());
     unsigned int totObjects = 0;
     objectFileID >> totObjects;
-    if (totObjects) {
-        cout << INFO << "ERROR: objects are not supported by the GPU DEM" << std::endl;
-        exit(1);
+    objects.clear();
+    objectRecords_.clear();
+    for (unsigned int n = 0; n < totObjects; ++n) {
+        double v[20];
+        for (int k = 0; k < 20; ++k) objectFileID >> v[k];
+        object o;
+        o.index = o.originalIndex = (unsigned int)v[0];
+        o.r = v[2];
+        o.x0 = tVect(v[3], v[4], v[5]);
+        o.x1 = tVect(v[6], v[7], v[8]);
+        objects.push_back(o);
+        for (int k = 0; k < HYB_OBJECT_RECORD; ++k) objectRecords_.push_back(v[k]);
     }
     double numVisc = 0.0;
     PARSE_CLASS_MEMBER(lbmCfgFile, numVisc, "numVisc", 0.0);
@@ -379,6 +390,8 @@ void DEM::discreteElementInit(unsIntList& boundary, unsIntList& lbSize, measureU
     cfg_.demSolve = 1;
     const int32_t P = (int32_t)(records_.size() / HYB_ELEMENT_RECORD);
     HYB_CALL(hyb_dem_create(&cfg_, P, P ? records_.data() : nullptr, &h_));
+    if (!objectRecords_.empty())
+        HYB_CALL(hyb_dem_set_objects(h_, (int32_t)(objectRecords_.size() / HYB_OBJECT_RECORD), objectRecords_.data()));
     int32_t b[6], s[3];
     for (int k = 0; k < 6; ++k) b[k] = (int32_t)boundary[k];
     for (int k = 0; k < 3; ++k) s[k] = (int32_t)lbSize[k];
@@ -520,7 +533,11 @@ void IO::exportParaviewFluid(const LB& lb, const DEM& dem, const string& fluidFi
     if (lb.nonNewtonian) visc = lb.field("visc");
     if (lb.freeSurface) mass = lb.field("mass");
     std::vector<unsigned int> sid;
-    if (demSolve) sid = lb.solidIndices();
+    std::vector<double> obj;  // index, r, x0, x1 per object (with periodic copies)
+    if (demSolve) {
+        sid = lb.solidIndices();
+        obj = dem.field("objects");
+    }
     const auto active = [&](size_t i) { return t[i] == 0 || t[i] == 3; };
     ofstream f(fluidFile.c_str());
     const double zero = 0.0;
@@ -569,6 +586,11 @@ void IO::exportParaviewFluid(const LB& lb, const DEM& dem, const string& fluidFi
                 for (const wall& w : dem.walls) {
                     const tVect d = pos - w.p / lb.unit.Length;
                     if (w.n.x * d.x + w.n.y * d.y + w.n.z * d.z < 0.0) s = w.index;
+                }
+                for (size_t o = 0; o + 8 <= obj.size(); o += 8) {  // LB.cpp:535-550
+                    const tVect d = pos - tVect(obj[o + 2], obj[o + 3], obj[o + 4]) / lb.unit.Length;
+                    const double rr = obj[o + 1] / lb.unit.Length;
+                    if (d.x * d.x + d.y * d.y + d.z * d.z < rr * rr) s = (unsigned int)obj[o];
                 }
             }
             f << s << " ";

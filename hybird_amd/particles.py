@@ -183,6 +183,28 @@ def read_particle_file(path: str):
     }
 
 
+def object_records(objects) -> np.ndarray:
+    """Objects as (n, 9) records: index, size, r, x, y, z, vx, vy, vz (HYB_OBJECT_RECORD)."""
+    obj = np.asarray(objects, dtype=np.float64).reshape(-1, 7)  # r, x, y, z, vx, vy, vz
+    rec = np.zeros((obj.shape[0], 9))
+    rec[:, 0] = np.arange(obj.shape[0])
+    rec[:, 1] = 1.0
+    rec[:, 2:] = obj
+    return rec
+
+
+def write_object_file(path: str, objects) -> None:
+    """Object file (DEM.cpp:136-173): n, then per object index, size, r, x, y, z,
+    vx, vy, vz and eleven rotation values the reference skips. ``objects`` rows
+    are r, x, y, z, vx, vy, vz."""
+    rec = object_records(objects)
+    with open(path, "w") as fh:
+        fh.write(f"{rec.shape[0]}\n")
+        for r in rec:
+            vals = " ".join(repr(float(v)) for v in r[2:])
+            fh.write(f"{int(r[0])} 1 {vals} 0 0 0 1 0 0 0 0 0 0 0\n")
+
+
 def write_empty_object_file(path: str) -> None:
     """Object file with zero objects (DEM.cpp:131-137 requires the file to exist)."""
     with open(path, "w") as fh:

@@ -29,7 +29,8 @@ DEM_FIELDS = {"e_x": (np.float64, (-1, 6, 3)), "e_xp": (np.float64, (-1, 6, 3)),
               "e_forces": (np.float64, (-1, 7, 3)), "e_misc": (np.float64, (-1, 19)),
               "p_data": (np.float64, (-1, 10)), "p_idx": (np.uint32, (-1, 4)), "neighborTable": (np.uint32, None),
               "nearWallTable": (np.uint32, None), "ghostIndex": (np.uint32, None), "ghostVec": (np.float64, (-1, 3)),
-              "dem_scalars": (np.float64, None), "rebuilds": (np.float64, None), "energy": (np.float64, None)}
+              "dem_scalars": (np.float64, None), "rebuilds": (np.float64, None), "energy": (np.float64, None),
+              "nearObjectTable": (np.uint32, None), "objects": (np.float64, (-1, 8))}
 
 # every extern "C" entry point declared in include/hybird_amd.h
 EXPORTS = ["hyb_config_defaults", "hyb_last_error", "hyb_device_count", "hyb_sync", "hyb_dem_create", "hyb_dem_init",
@@ -37,7 +38,7 @@ EXPORTS = ["hyb_config_defaults", "hyb_last_error", "hyb_device_count", "hyb_syn
            "hyb_lb_step", "hyb_lb_destroy", "hyb_cycle", "hyb_lb_get", "hyb_dem_get", "hyb_lb_set_profiling",
            "hyb_lb_kernel_time", "hyb_lb_active_nodes", "hyb_lb_set_capture", "hyb_dem_init_lattice",
            "hyb_lb_set_force_field", "hyb_lb_set_populations", "hyb_adj_forward", "hyb_adj_backward",
-           "hyb_state_save", "hyb_state_load"]
+           "hyb_state_save", "hyb_state_load", "hyb_dem_set_objects"]
 
 
 class HybError(RuntimeError):
@@ -70,7 +71,8 @@ def load():
                        ("hyb_adj_forward", [vp, vp, i32, i32]), ("hyb_adj_backward", [vp, vp, vp, vp, vp, vp]),
                        ("hyb_lb_kernel_time", [vp, C.POINTER(C.c_double), C.POINTER(i64)]),
                        ("hyb_lb_active_nodes", [vp, C.POINTER(i64)]),
-                       ("hyb_state_save", [vp, vp, C.c_char_p]), ("hyb_state_load", [vp, vp, C.c_char_p])]:
+                       ("hyb_state_save", [vp, vp, C.c_char_p]), ("hyb_state_load", [vp, vp, C.c_char_p]),
+                       ("hyb_dem_set_objects", [vp, i32, vp])]:
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
@@ -92,7 +94,8 @@ def default_config() -> HybConfig:
 class Simulation:
     """Coupled LB + DEM state on the GPU, created as the reference's main() does."""
 
-    def __init__(self, config: HybConfig, records: np.ndarray):
+    def __init__(self, config: HybConfig, records: np.ndarray, objects=None):
+        """objects: optional (n, 9) object records (index, size, r, x0, x1)."""
         L = load()
         rec = np.ascontiguousarray(records, dtype=np.float64).reshape(-1, 20)
         self.config = config
@@ -100,6 +103,9 @@ class Simulation:
         self._lb = C.c_void_p()
         _check(L.hyb_dem_create(C.byref(config), int(rec.shape[0]), rec.ctypes.data if rec.size else None,
                                 C.byref(self._dem)), "hyb_dem_create")
+        if objects is not None and len(objects):
+            obj = np.ascontiguousarray(objects, dtype=np.float64).reshape(-1, 9)
+            _check(L.hyb_dem_set_objects(self._dem, int(obj.shape[0]), obj.ctypes.data), "hyb_dem_set_objects")
         _check(L.hyb_lb_create(C.byref(config), C.byref(self._lb)), "hyb_lb_create")
         _check(L.hyb_dem_init(self._dem, self._lb), "hyb_dem_init")
         _check(L.hyb_lb_init(self._lb, self._dem), "hyb_lb_init")

@@ -11,6 +11,7 @@
  * (reference file:line of the interface each one stands in for):
  *   hyb_lb_create        LB::LB + LB::latticeBoltzmannGet + LB::latticeDefinition  (LB.h:104-155, LB.cpp:42-188)
  *   hyb_dem_create       DEM::DEM + DEM::discreteElementGet                         (DEM.h:98-138, DEM.cpp:13-183)
+ *   hyb_dem_set_objects  the object file part of DEM::discreteElementGet               (DEM.cpp:131-173)
  *   hyb_dem_init         DEM::discreteElementInit                                   (DEM.h:139, DEM.cpp:185-297)
  *   hyb_dem_init_lattice DEM::discreteElementInit with its own arguments
  *                        (boundary, lbSize, unit, lbF) instead of an LB handle   (DEM.h:139, DEM.cpp:185-297)
@@ -100,6 +101,9 @@ void hyb_config_defaults(hyb_config* cfg);
 /* One particle-file record, DEM.cpp:85-120:
  * index size radius x y z vx vy vz wx wy wz q0 q1 q2 q3 dq0 dq1 dq2 dq3 */
 #define HYB_ELEMENT_RECORD 20
+/* One object (static sphere) of the object file (DEM.cpp:136-173): index, size,
+ * r, x, y, z, vx, vy, vz (the line's remaining rotation values are unused). */
+#define HYB_OBJECT_RECORD 9
 
 typedef struct hyb_lb hyb_lb;
 typedef struct hyb_dem hyb_dem;
@@ -110,6 +114,11 @@ int hyb_sync(hyb_lb* lb);
 
 /* ---- DEM ---- */
 int hyb_dem_create(const hyb_config* cfg, int32_t n_elements, const double* records, hyb_dem** out);
+/* The object file's static spheres (DEM::discreteElementGet, DEM.cpp:131-173),
+ * n records of HYB_OBJECT_RECORD, before hyb_dem_init: lattice nodes inside them
+ * become static walls (LB.cpp:535-550) and particles touching them get the
+ * object contact (DEM.cpp:1747-1770, 2060-2136). Indices must be 0..n-1. */
+int hyb_dem_set_objects(hyb_dem* dem, int32_t n_objects, const double* records);
 int hyb_dem_init(hyb_dem* dem, const hyb_lb* lb);
 /* boundary: the six box boundary codes; lbSize: node counts; unit*: base units
  * (after the automatic unitTime of LB.cpp:124-133); lbF: body force in lattice units. */
@@ -146,6 +155,7 @@ int hyb_cycle(hyb_lb* lb, hyb_dem* dem, int32_t n);
  *       (FHydro, MHydro, FParticle, MParticle, FWall, MWall, FGrav), "e_misc" f64[E][19],
  *       "p_data" f64[P][10] (x0, x1, radiusVec, r), "p_idx" u32[P][4],
  *       "neighborTable" u32[2*pairs], "nearWallTable" u32[2*k], "ghostIndex" u32[G], "ghostVec" f64[G][3],
+ *       "nearObjectTable" u32[2*k] (particle, object), "objects" f64[O][8] (index, r, x0, x1),
  *       "dem_scalars" f64[17], "energy" f64[6] (DEM::particleEnergy after the last sub-step: kin, rotKin,
  *       trKin, grav, elastic, total; DEM.cpp:2226-2276)
  * *bytes_needed receives the size; nothing is copied when capacity is too small.

@@ -133,6 +133,12 @@ struct Wall {
     V3 getSpeed(const V3& pt) const;                          // utils.cpp:37-47
 };
 
+struct Object {  // static sphere of the object file (DEM.cpp:131-173; elmt.h object)
+    uint32_t index = 0, originalIndex = 0;
+    double r = 0.0;
+    V3 x0, x1, FParticle;
+};
+
 struct Pbc {
     V3 p, v;
     Wall pl1, pl2;
@@ -176,6 +182,9 @@ struct DEM {
     double criticalRatio = 0.1;
     std::vector<Wall> walls;
     std::vector<Pbc> pbcs;
+    std::vector<Object> objects;  // originals, then periodic copies
+    uint32_t stdObjects = 0;
+    std::vector<uint32_t> nearObjectTable;
     std::vector<Element> elmts;
     std::vector<Particle> particles;
     uint32_t stdParticles = 0;
@@ -202,6 +211,9 @@ struct DEM {
     void evalCellTable();
     void evalNeighborTable();
     void evalNearWallTable();
+    void evalNearObjectTable();
+    void periodicObjects();
+    void objectParticleCollision(Object& o, const Particle& pJ, const V3& vectorDistance);
     void pbcShift();
     void createGhosts();
     void predictor();
@@ -267,6 +279,7 @@ struct LB {
     void initializeLatticeBoundaries();
     void initializeParticleBoundaries(const std::vector<Particle>& particles);
     void initializeWallBoundaries(const std::vector<Wall>& walls);
+    void initializeObjectBoundaries(const std::vector<Object>& objects);
     void initializeInterface();
     void initializeLists();
     void resetLists();

@@ -22,12 +22,12 @@ _DTYPES = {
     "types": np.uint8, "pflag": np.uint8, "solidIndex": np.uint32, "activeNodes": np.uint32, "fluidNodes": np.uint32,
     "interfaceNodes": np.uint32, "particleNodes": np.uint32, "neighbors": np.uint32, "lbSize": np.uint32,
     "p_idx": np.uint32, "neighborTable": np.uint32, "nearWallTable": np.uint32, "cellTable": np.int32,
-    "ghostIndex": np.uint32, "e_comp": np.uint32, "e_comp_off": np.uint32,
+    "ghostIndex": np.uint32, "e_comp": np.uint32, "e_comp_off": np.uint32, "nearObjectTable": np.uint32,
 }
 _SHAPES = {"f": (-1, 19), "u": (-1, 3), "hydroForce": (-1, 3), "neighbors": (-1, 19), "e_x": (-1, 6, 3),
            "e_xp": (-1, 6, 3), "e_q": (-1, 6, 4), "e_qp": (-1, 6, 4), "e_w": (-1, 6, 3), "e_wp": (-1, 6, 3),
            "e_misc": (-1, 19), "e_forces": (-1, 7, 3), "p_data": (-1, 10), "p_idx": (-1, 4), "ghostVec": (-1, 3),
-           "walls": (-1, 12)}
+           "walls": (-1, 12), "objects": (-1, 11)}
 
 
 def build() -> None:
@@ -41,7 +41,7 @@ def lib():
             raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle oracle`")
         _lib = C.CDLL(LIB_PATH)
         _lib.orc_create.restype = C.c_void_p
-        _lib.orc_create.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_char_p, C.c_int]
+        _lib.orc_create.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_char_p, C.c_int]
         _lib.orc_destroy.argtypes = [C.c_void_p]
         _lib.orc_cycle.argtypes = [C.c_void_p, C.c_int]
         _lib.orc_cycle.restype = C.c_int
@@ -57,11 +57,13 @@ def lib():
 class OracleSim:
     """One coupled simulation in the CPU restatement."""
 
-    def __init__(self, config, records: np.ndarray):
+    def __init__(self, config, records: np.ndarray, objects=None):
         L = lib()
         rec = np.ascontiguousarray(records, dtype=np.float64).reshape(-1, 20)
+        obj = np.ascontiguousarray(np.zeros((0, 9)) if objects is None else objects, dtype=np.float64).reshape(-1, 9)
         err = C.create_string_buffer(512)
-        self._h = L.orc_create(C.byref(config), int(rec.shape[0]), rec.ctypes.data if rec.size else None, err, 512)
+        self._h = L.orc_create(C.byref(config), int(rec.shape[0]), rec.ctypes.data if rec.size else None,
+                               int(obj.shape[0]), obj.ctypes.data if obj.size else None, err, 512)
         if not self._h:
             raise RuntimeError("oracle create failed: " + err.value.decode())
 

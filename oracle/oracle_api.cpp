@@ -43,13 +43,24 @@ void p4(std::vector<double>& a, const Q4& q) { a.push_back(q.q0); a.push_back(q.
 
 extern "C" {
 
-void* orc_create(const hyb_config* cfg, int n_elements, const double* records, char* err, int errlen) {
+void* orc_create(const hyb_config* cfg, int n_elements, const double* records, int n_objects, const double* objects,
+                 char* err, int errlen) {
     Sim* s = new Sim();
     s->cfg = *cfg;
     s->demSolve = cfg->demSolve;
     s->lbmSolve = cfg->lbSolve;
     s->lb.get(*cfg);
     s->dem.get(*cfg, n_elements, records);
+    for (int o = 0; o < n_objects; ++o) {  // DEM.cpp:136-173 (positions are not scaled or translated)
+        const double* r = objects + (size_t)o * HYB_OBJECT_RECORD;
+        Object ob;
+        ob.index = (uint32_t)r[0];
+        ob.r = r[2];
+        ob.x0 = V3(r[3], r[4], r[5]);
+        ob.x1 = V3(r[6], r[7], r[8]);
+        ob.originalIndex = ob.index;
+        s->dem.objects.push_back(ob);
+    }
     s->lb.latticeDefinition();
     s->dem.init(s->lb.boundary, s->lb.lbSize, s->lb.unit, s->lb.lbF);
     if (!s->dem.error && s->lbmSolve) s->lb.init(s->dem);
@@ -182,6 +193,12 @@ long orc_get(void* h, const char* cname, void* out, long cap) {
         return put(v, out, cap);
     }
     if (name == "energy") return put(std::vector<double>(dem.energy, dem.energy + 6), out, cap);
+    if (name == "nearObjectTable") return put(dem.nearObjectTable, out, cap);
+    if (name == "objects") {
+        std::vector<double> v;
+        for (auto& o : dem.objects) { v.push_back(o.index); v.push_back(o.r); p3(v, o.x0); p3(v, o.x1); p3(v, o.FParticle); }
+        return put(v, out, cap);
+    }
     if (name == "walls") {
         std::vector<double> v;
         for (auto& w : dem.walls) { p3(v, w.p); p3(v, w.n); p3(v, w.FParticle); p3(v, w.FHydro); }

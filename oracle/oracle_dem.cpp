@@ -204,8 +204,10 @@ void DEM::init(const std::vector<unsigned>& boundary, const std::vector<unsigned
     stdParticles = (uint32_t)particles.size();
     initializeWalls(boundary, lbSize, unit);
     initializePbcs(boundary, lbSize, unit);
+    stdObjects = (uint32_t)objects.size();
     if (elmts.size()) {
         initNeighborParameters();
+        periodicObjects();
         evalNeighborTable();
     }
     // determineTimeStep(unit.Time) repeats the same choice (DEM.cpp:299-329)
@@ -325,6 +327,44 @@ void DEM::evalNeighborTable() {  // DEM.cpp:1369-1494
                     }
                 }
     if (walls.size() != 0) evalNearWallTable();
+    if (objects.size() != 0) evalNearObjectTable();
+}
+
+void DEM::evalNearObjectTable() {  // DEM.cpp:1515-1532: every object within nebrRange
+    nearObjectTable.clear();
+    const double nebrRange2 = nebrRange * nebrRange;
+    for (uint32_t n = 0; n < stdParticles; ++n)
+        for (size_t o = 0; o < objects.size(); ++o) {
+            const V3 x0ij = particles[n].x0 - objects[o].x0;
+            if (x0ij.norm2() < nebrRange2) {
+                nearObjectTable.push_back(n);
+                nearObjectTable.push_back((uint32_t)o);
+            }
+        }
+}
+
+void DEM::periodicObjects() {  // DEM.cpp:990-1062: copies of objects near a periodic plane
+    struct G { uint32_t origin; V3 v; };
+    std::vector<G> side;
+    for (const Pbc& b : pbcs)
+        for (uint32_t o = 0; o < stdObjects; ++o) {
+            const double leftDist = b.pl1.dist(objects[o].x0);
+            const double rightDist = b.pl2.dist(objects[o].x0);
+            if (leftDist < 2.0 * nebrRange && leftDist > 0.0) side.push_back(G{objects[o].index, b.v});
+            else if (rightDist < 2.0 * nebrRange && rightDist > 0.0) side.push_back(G{objects[o].index, -1.0 * b.v});
+        }
+    std::vector<G> corner;
+    for (size_t a = 0; a < side.size(); ++a)
+        for (size_t c = a + 1; c < side.size(); ++c)
+            if (side[a].origin == side[c].origin) corner.push_back(G{side[a].origin, side[a].v + side[c].v});
+    side.insert(side.end(), corner.begin(), corner.end());
+    for (size_t g = 0; g < side.size(); ++g) {
+        Object o = objects[side[g].origin];  // the reference indexes the list with the object's index
+        o.x0 = o.x0 + side[g].v;
+        o.originalIndex = o.index;
+        o.index = stdObjects + (uint32_t)g;
+        objects.push_back(o);
+    }
 }
 
 void DEM::evalNearWallTable() {  // DEM.cpp:1496-1513
@@ -572,9 +612,42 @@ void DEM::wallParticleCollision(Wall& w, const Particle& pJ, double overlap) {  
     }
 }
 
+// DEM.cpp:2060-2136: the object is static (its x1 is the contact-point velocity)
+void DEM::objectParticleCollision(Object& o, const Particle& pJ, const V3& vectorDistance) {
+    Element& eJ = elmts[pJ.clusterIndex];
+    const double radJ = pJ.r;
+    const double distance = vectorDistance.norm();
+    const double overlap = radJ + o.r - distance;
+    const V3 en = (1.0 / distance) * vectorDistance;
+    const V3 contactPointVelocity = o.x1;
+    const V3 relVel = pJ.x1 - contactPointVelocity;
+    const double normRelVel = relVel.dot(en);
+    const V3 normalRelVel = en * normRelVel;
+    const double fn = normalContact(2.0 * overlap, normRelVel, radJ, eJ.m);
+    const V3 normalForce = en * fn;
+    const V3 vecRadJ = -radJ * en;
+    const V3 centerDistJ = vecRadJ + (pJ.x0 - eJ.xp0);
+    eJ.FWall = eJ.FWall + normalForce;
+    o.FParticle = o.FParticle - normalForce;
+    if (eJ.size > 1) eJ.MWall = eJ.MWall + centerDistJ.cross(normalForce);
+    const V3 wJ = eJ.wpGlobal;
+    const V3 relVelContact = relVel + wJ.cross(vecRadJ);
+    const V3 tangRelVelContact = relVelContact - normalRelVel;
+    const double nt = tangRelVelContact.norm();
+    if (nt != 0.0) {
+        const double ft = tangentialContact(nt, fn, radJ, eJ.m, mat.frictionCoefPart);
+        const V3 et = tangRelVelContact / tangRelVelContact.norm();
+        const V3 tangForce = ft * et;
+        eJ.MWall = eJ.MWall - centerDistJ.cross(tangForce);
+        eJ.FWall = eJ.FWall - tangForce;
+        o.FParticle = o.FParticle + tangForce;
+    }
+}
+
 void DEM::evaluateForces() {  // DEM.cpp:1114-1182
     for (auto& e : elmts) { e.FParticle.reset(); e.FWall.reset(); e.MParticle.reset(); e.MWall.reset(); }
     for (auto& w : walls) w.FParticle.reset();
+    for (auto& o : objects) o.FParticle.reset();
     // particleParticleContacts, DEM.cpp:1668-1690
     for (size_t k = 0; k + 1 < neighborTable.size(); k += 2) {
         const Particle& pi = particles[neighborTable[k]];
@@ -591,6 +664,15 @@ void DEM::evaluateForces() {  // DEM.cpp:1114-1182
         const double distance = w.dist(pj.x0);
         const double overlap = pj.r - distance;
         if (overlap > 0.0) wallParticleCollision(w, pj, overlap);
+    }
+    // objectParticleContacts, DEM.cpp:1747-1770
+    for (size_t k = 0; k + 1 < nearObjectTable.size(); k += 2) {
+        const Particle& pj = particles[nearObjectTable[k]];
+        Object& o = objects[nearObjectTable[k + 1]];
+        const V3 vectorDistance = pj.x0 - o.x0;
+        const double distance = vectorDistance.norm();
+        const double overlap = pj.r + o.r - distance;
+        if (overlap > 0.0) objectParticleCollision(o, pj, vectorDistance);
     }
     for (auto& e : elmts) {
         const V3 FVisc = -6.0 * M_PI * numVisc * e.radius * e.xp1;

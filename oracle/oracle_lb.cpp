@@ -261,6 +261,15 @@ void LB::initializeWallBoundaries(const std::vector<Wall>& walls) {  // LB.cpp:4
     }
 }
 
+void LB::initializeObjectBoundaries(const std::vector<Object>& objects) {  // LB.cpp:535-550
+    for (const Object& o : objects)
+        for (uint32_t it = 0; it < totNodes; ++it)
+            if (positions[it].insideSphere(o.x0 / unit.Length, o.r / unit.Length)) {
+                solidIndex[it] = o.index;
+                t[it] = 7;
+            }
+}
+
 void LB::initializeInterface() {  // LB.cpp:605-814 (generic tail + the SPLASH plane + a liquid-box extension)
     if (problem == HYB_PROBLEM_SPLASH) {
         const V3 pp(0.0, 0.0, 20.0), nn(0.0, 0.0, 1.0);
@@ -380,6 +389,7 @@ void LB::init(DEM& dem) {  // LB.cpp:190-219
     initializeLatticeBoundaries();
     initializeParticleBoundaries(dem.particles);
     initializeWallBoundaries(dem.walls);
+    initializeObjectBoundaries(dem.objects);
     initializeInterface();
     initializeLists();
     initializeVariables();

@@ -203,6 +203,14 @@ void dump(const std::string& path, IO& io, DEM& dem, LB& lb) {
             push3(wl, dem.walls[k].p); push3(wl, dem.walls[k].n); push3(wl, dem.walls[k].FParticle); push3(wl, dem.walls[k].FHydro);
         }
         w.d("walls", wl, {dem.walls.size(), 12});
+        std::vector<uint32_t> no(dem.nearObjectTable.begin(), dem.nearObjectTable.end());
+        w.u("nearObjectTable", no);
+        std::vector<double> ob;
+        for (size_t k = 0; k < dem.objects.size(); ++k) {
+            const object& o = dem.objects[k];
+            ob.push_back(o.index); ob.push_back(o.r); push3(ob, o.x0); push3(ob, o.x1); push3(ob, o.FParticle);
+        }
+        w.d("objects", ob, {dem.objects.size(), 11});
     }
     (void)io;
 }

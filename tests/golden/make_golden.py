@@ -23,7 +23,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from hybird_amd.cfg import base_config, write_cfg  # noqa: E402
-from hybird_amd.particles import Bed, rsa_bed, write_empty_object_file, write_particle_file  # noqa: E402
+from hybird_amd.particles import (Bed, rsa_bed, write_empty_object_file, write_object_file,  # noqa: E402
+                                  write_particle_file)
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "hybird_ref_dump")
 
@@ -130,6 +131,16 @@ def _cases():
                                           initVelocityX=0.03, initVelocityZ=0.02, lbFX=5.0e-4, lbFZ=-5.0e-4,
                                           boundary0=7, boundary1=7, boundary2=4, boundary3=4, fluidExpTime=4.0),
                           bed=Bed(np.zeros((0, 3)), np.zeros(0)), cycles=[0, 10], keys=["types"], files=True)
+    # static spheres of the object file (DEM.cpp:131-173): lattice walls (LB.cpp:535-550)
+    # and contacts (DEM.cpp:1747-1770, 2060-2136); one object close to the periodic
+    # x plane gets a ghost copy (DEM.cpp:990-1062); particles are thrown at them
+    objs = np.array([[2.5, 9.0, 9.0, 6.0, 0.0, 0.0, 0.0], [2.0, 2.0, 8.0, 12.0, 0.0, 0.0, 0.0]])
+    bed3 = Bed(np.array([[9.0, 9.0, 10.28], [5.7, 8.0, 12.0], [12.0, 4.0, 5.0]]), np.array([1.8, 1.6, 2.0]))
+    vel3 = np.array([[0.0, 0.0, 0.0], [-0.08, 0.0, 0.0], [0.0, 0.03, 0.0]])
+    cases["objects"] = dict(cfg=base_config(lbSizeX=18, lbSizeY=16, lbSizeZ=18, initVisc=0.05, lbFX=2.0e-4,
+                                            lbFZ=-2.0e-3, youngMod=5.0e3, density=3.0),
+                            bed=bed3, vel=vel3, objects=objs, cycles=[0, 1, 5, 15, 30],
+                            keys=LATTICE_KEYS + DEM_KEYS + ["nearObjectTable", "objects"])
     return cases
 
 
@@ -139,7 +150,10 @@ def run_case(name: str, case: dict, outdir: str) -> None:
         pf = os.path.join(tmp, "particles.dat")
         of = os.path.join(tmp, "objects.dat")
         write_particle_file(pf, case["bed"], case.get("vel"))
-        write_empty_object_file(of)
+        if case.get("objects") is not None:
+            write_object_file(of, case["objects"])
+        else:
+            write_empty_object_file(of)
         cfg["particleFile"] = pf
         cfg["objectFile"] = of
         cfg["maximumTimeSteps"] = max(case["cycles"])
@@ -180,6 +194,7 @@ def run_case(name: str, case: dict, outdir: str) -> None:
         arrays["bed_x"] = case["bed"].x
         arrays["bed_r"] = case["bed"].r
         arrays["bed_v"] = np.zeros_like(case["bed"].x) if case.get("vel") is None else np.asarray(case["vel"])
+        arrays["objects"] = np.zeros((0, 7)) if case.get("objects") is None else np.asarray(case["objects"], float)
         np.savez_compressed(os.path.join(outdir, f"{name}.npz"), **arrays)
 
 

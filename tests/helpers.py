@@ -24,6 +24,14 @@ def load_case(name: str):
     return cfg, rec, z
 
 
+def objects_of(z):
+    """Object records (HYB_OBJECT_RECORD rows) of a fixture, or None."""
+    from hybird_amd.particles import object_records
+    if "objects" not in z.files or z["objects"].size == 0:
+        return None
+    return object_records(z["objects"])
+
+
 def fixture_config(cfg: dict, **kw):
     return config_from_cfg(cfg, **kw)
 

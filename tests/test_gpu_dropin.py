@@ -17,12 +17,15 @@ pytestmark = pytest.mark.gpu
 BIN = os.path.join(ROOT, "hybird_amd", "host", "hybird_gpu")
 
 
-def _write_case(tmp, cfg, bed_x, bed_r, cycles):
+def _write_case(tmp, cfg, bed_x, bed_r, cycles, objects=None, bed_v=None):
     from hybird_amd.cfg import write_cfg
-    from hybird_amd.particles import Bed, write_empty_object_file, write_particle_file
+    from hybird_amd.particles import Bed, write_empty_object_file, write_object_file, write_particle_file
     pf, of = os.path.join(tmp, "p.dat"), os.path.join(tmp, "o.dat")
-    write_particle_file(pf, Bed(np.asarray(bed_x), np.asarray(bed_r)))
-    write_empty_object_file(of)
+    write_particle_file(pf, Bed(np.asarray(bed_x), np.asarray(bed_r)), bed_v)
+    if objects is not None and len(objects):
+        write_object_file(of, objects)
+    else:
+        write_empty_object_file(of)
     cfg = dict(cfg, particleFile=pf, objectFile=of, maximumTimeSteps=cycles, screenExpTime=0)
     path = os.path.join(tmp, "case.cfg")
     write_cfg(path, cfg)
@@ -152,3 +155,30 @@ def test_dropin_output_files_match_reference(tmp_path, name):
                 assert np.array_equal(ar[key], am[key]), (fn, key)
             else:
                 _close(ar[key], am[key], f"{fn}:{key}")
+
+
+@pytest.mark.skipif(not os.path.exists(BIN), reason="hybird_gpu not built (needs the reference's hybird.cpp)")
+def test_dropin_object_file_matches_oracle(tmp_path):
+    """An object file with static spheres (one near a periodic plane, so it gets
+    periodic copies): the reference's main over the GPU against the oracle."""
+    from helpers import objects_of
+    cfg, rec, z = load_case("objects")
+    cycles = int(max(z["cycles"]))
+    tmp = str(tmp_path)
+    path = _write_case(tmp, cfg, z["bed_x"], z["bed_r"], cycles, objects=z["objects"], bed_v=z["bed_v"])
+    dump = os.path.join(tmp, "dump")
+    os.makedirs(dump)
+    env = dict(os.environ, HYBIRD_AMD_DUMP_DIR=dump)
+    print(f"[gpu dropin] hybird_gpu objects {cycles} cycles", flush=True)
+    r = subprocess.run([BIN, "-c", path, "-d", os.path.join(tmp, "out")], env=env, capture_output=True, text=True,
+                       timeout=300, cwd=tmp)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    orc = OracleSim(fixture_config(cfg), rec, objects_of(z))
+    orc.cycle(cycles)
+    t = orc.get("types")
+    assert np.array_equal(_load(dump, "lb_types", np.uint8), t)
+    act = (t == 0) | (t == 3)
+    f = _load(dump, "lb_f").reshape(-1, 19)
+    assert rel_l2(f[act], orc.get("f")[act]) <= 1e-12
+    ex = _load(dump, "dem_e_x").reshape(-1, 6, 3)
+    assert rel_l2(ex, orc.get("e_x")) <= 1e-11

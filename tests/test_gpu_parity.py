@@ -12,7 +12,7 @@ may differ by one ulp in the tangential stiffness).
 import numpy as np
 import pytest
 
-from helpers import fixture_config, load_case, rel_l2
+from helpers import fixture_config, load_case, objects_of, rel_l2
 from oracle.oracle import OracleSim
 
 pytestmark = pytest.mark.gpu
@@ -72,14 +72,15 @@ def _compare_dem(gpu, orc, tag):
 HORIZON = {"dem60": 100}
 
 
-@pytest.mark.parametrize("name", ["lb16", "chan", "bingham", "couple2", "dense", "dem60", "walls"])
+@pytest.mark.parametrize("name", ["lb16", "chan", "bingham", "couple2", "dense", "dem60", "walls", "objects"])
 def test_gpu_fp64_matches_oracle(name):
     rt = _runtime()
     cfg, rec, z = load_case(name)
     conf = fixture_config(cfg)
-    orc = OracleSim(fixture_config(cfg), rec)
+    objs = objects_of(z)
+    orc = OracleSim(fixture_config(cfg), rec, objs)
     print(f"[gpu parity] case={name}", flush=True)
-    gpu = rt.Simulation(conf, rec)
+    gpu = rt.Simulation(conf, rec, objs)
     prev = 0
     lattice = cfg.get("lbSolve", 1) != 0
     for c in (int(c) for c in z["cycles"] if int(c) <= HORIZON.get(name, 10**9)):
@@ -91,6 +92,9 @@ def test_gpu_fp64_matches_oracle(name):
             _compare_lattice(gpu, orc, tag)
         if rec.shape[0]:
             _compare_dem(gpu, orc, tag)
+        if objs is not None:
+            assert np.array_equal(gpu.get("nearObjectTable"), orc.get("nearObjectTable")), f"{tag}: nearObjectTable"
+            assert np.array_equal(gpu.get("objects"), orc.get("objects")[:, :8]), f"{tag}: objects"
     gpu.close()
 
 

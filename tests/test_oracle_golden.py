@@ -8,16 +8,16 @@ populations, velocities, forces, element states (FP64).
 import numpy as np
 import pytest
 
-from helpers import fixture_config, load_case
+from helpers import fixture_config, load_case, objects_of
 from oracle.oracle import OracleSim
 
-CASES = ["lb16", "chan", "bingham", "couple2", "dense", "dem60", "fs", "walls", "walls_fs", "io", "io_fs"]
+CASES = ["lb16", "chan", "bingham", "couple2", "dense", "dem60", "fs", "walls", "walls_fs", "io", "io_fs", "objects"]
 
 
 @pytest.mark.parametrize("name", CASES)
 def test_oracle_matches_reference_bitwise(name):
     cfg, rec, z = load_case(name)
-    sim = OracleSim(fixture_config(cfg), rec)
+    sim = OracleSim(fixture_config(cfg), rec, objects_of(z))
     prev = 0
     checked = 0
     for c in (int(c) for c in z["cycles"]):
