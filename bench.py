@@ -63,8 +63,10 @@ def make_sim(cfg, bed, precision):
 
 
 def timed(sim, steps, warmup):
+    """Seconds for `steps` cycles, mean fused-kernel ms, neighbour rebuilds inside the timed cycles."""
     sim.cycle(warmup)
     sim.sync()
+    r0 = float(sim.get("rebuilds")[0])
     sim.set_profiling(True)
     t0 = time.perf_counter()
     sim.cycle(steps)
@@ -72,7 +74,7 @@ def timed(sim, steps, warmup):
     t1 = time.perf_counter()
     ms, launches = sim.kernel_time()
     sim.set_profiling(False)
-    return t1 - t0, ms / max(launches, 1)
+    return t1 - t0, ms / max(launches, 1), int(float(sim.get("rebuilds")[0]) - r0)
 
 
 def cpu_baseline(size_sub: int, cycles: int):
@@ -204,10 +206,10 @@ def main() -> int:
           file=sys.stderr, flush=True)
     sim32 = make_sim(cfg, bed, "FP32")
     n_act = sim32.active_nodes()
-    sec32, kms32 = timed(sim32, args.steps, args.warmup)
+    sec32, kms32, reb32 = timed(sim32, args.steps, args.warmup)
     print(f"[bench] fp32 done: {sec32:.3f}s", file=sys.stderr, flush=True)
     sim64 = make_sim(cfg, bed, "FP64")
-    sec64, kms64 = timed(sim64, args.steps, args.warmup)
+    sec64, kms64, _ = timed(sim64, args.steps, args.warmup)
     print(f"[bench] fp64 done: {sec64:.3f}s", file=sys.stderr, flush=True)
     # accuracy of the FP32 form against FP64 after the same number of cycles
     t_o = sim64.get("types")
@@ -241,6 +243,7 @@ def main() -> int:
                                "Hertz + Coulomb contacts, 10 DEM sub-steps per LB step",
                    "lattice": [args.size] * 3, "active_nodes": n_act, "particles": bed.count},
         "particle_steps_per_s": round(psteps, 1),
+        "dem_rebuilds_in_timed_steps": reb32,
         "fp64_mlups": round(mlups64, 1),
         "fp64_ms_per_step": round(sec64 / K * 1e3, 4),
         "fp32_over_fp64": round(mlups32 / mlups64, 3),

@@ -516,7 +516,7 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
         (rc = ralloc(R, "cellCount", &b.cellCount, b.ncells + 1)) ||
         (rc = dalloc(&b.work, (size_t)std::max(std::max(b.W, scap), std::max(p.pairCap, b.ncells + 1)) + 1)) ||
         (rc = ralloc(R, "csrOff", &b.csrOff, (size_t)P + 1)) || (rc = ralloc(R, "csr", &b.csr, 2 * (size_t)p.pairCap)) ||
-        (rc = dalloc(&b.cursor, scap)) || (rc = ralloc(R, "ghostOfCount", &b.ghostOfCount, std::max(P, 1))) ||
+        (rc = dalloc(&b.cursor, scap)) || (rc = dalloc(&b.spos, 4 * (size_t)scap)) || (rc = ralloc(R, "ghostOfCount", &b.ghostOfCount, std::max(P, 1))) ||
         (rc = ralloc(R, "ghostOf", &b.ghostOf, 8 * (size_t)std::max(P, 1))) ||
         (rc = ralloc(R, "side", &b.side, 3 * (size_t)std::max(P, 1))) || (rc = ralloc(R, "ghostB", &b.ghostB, p.Gcap)) ||
         (rc = ralloc(R, "posI", &b.posI, p.pairCap)) || (rc = ralloc(R, "posJ", &b.posJ, p.pairCap)) ||
@@ -573,7 +573,7 @@ int hyb_dem_destroy(hyb_dem* d) {
     DEMBuffers& b = d->b;
     dfree(b.el); dfree(b.st); dfree(b.ghostIndex); dfree(b.ghostVec); dfree(b.pairs); dfree(b.nearWall);
     dfree(b.cellTable); dfree(b.cellOf); dfree(b.cellCount); dfree(b.work); dfree(b.csrOff); dfree(b.csr);
-    dfree(b.cursor); dfree(b.ghostOfCount); dfree(b.ghostOf); dfree(b.side); dfree(b.ghostB);
+    dfree(b.cursor); dfree(b.spos); dfree(b.ghostOfCount); dfree(b.ghostOf); dfree(b.side); dfree(b.ghostB);
     dfree(b.posI); dfree(b.posJ); dfree(b.prec); dfree(b.vmax); dfree(b.energy); dfree(b.nearObj); dfree(b.nearObjCount);
     if (b.objs) (void)hipFree((void*)b.objs);
     delete d;

@@ -90,6 +90,9 @@ __device__ __forceinline__ int slot_cluster(const DEMBuffers& b, int P, int s) {
 }
 
 // Walks the chains of the cell pair (cell0, cell1) exactly as DEM.cpp:1455-1481.
+// A chain is the cell's run of the slot array sorted by cell with slots
+// descending (the order of the linked list), so its loads do not wait on each
+// other; positions and clusters come from the packed slot records (spos).
 // With out == nullptr only counts. Returns -1 when the stencil leaves the table.
 __device__ int walk_pairs(const DEMParams& p, const DEMBuffers& b, const El& E, int S, unsigned w, uint32_t* out) {
     static constexpr int sc[14][3] = {{0, 0, 0},  {1, 0, 0},  {1, 1, 0},  {0, 1, 0},  {-1, 1, 0},
@@ -113,28 +116,41 @@ __device__ int walk_pairs(const DEMParams& p, const DEMBuffers& b, const El& E, 
     const uint32_t tsize = Su + (uint32_t)b.ncells;
     const double r2 = p.nebrRange * p.nebrRange;
     int cnt = 0;
-    int n0 = b.cellTable[cell0];
-    while (n0 > -1) {
-        if (cell1 >= tsize) return -1;
-        int n1 = b.cellTable[cell1];
-        const V3 x0 = slot_pos(E, b, p.P, n0);
-        const int c0 = slot_cluster(b, p.P, n0);
-        while (n1 > -1) {
-            if (cell0 != cell1 || n1 < n0) {
-                if (c0 != slot_cluster(b, p.P, n1)) {
-                    const V3 r10 = slot_pos(E, b, p.P, n1) - x0;
-                    if (r10.norm2() < r2) {
-                        if (out) {
-                            out[2 * cnt] = (uint32_t)n0;
-                            out[2 * cnt + 1] = (uint32_t)n1;
-                        }
-                        ++cnt;
+    const int a0 = b.cellCount[cell0 - Su], e0 = b.cellCount[cell0 - Su + 1];  // cell0 - Su < ncells
+    if (a0 >= e0) return 0;
+    if (cell1 >= tsize) return -1;
+    // A stencil step of -1 from index 0 wraps through unsigned arithmetic to the
+    // previous linear cell, and from linear cell 0 below the cell heads: then
+    // cellTable[cell1] is the link of slot cell1 and the reference walks the
+    // rest of that slot's chain (DEM.cpp:1373-1390, 1455-1481). Only that case
+    // follows the links.
+    const bool linked = cell1 < Su;
+    const int a1 = linked ? 0 : b.cellCount[cell1 - Su], e1 = linked ? 0 : b.cellCount[cell1 - Su + 1];
+    const auto visit = [&](int n0, const V3& x0, int c0, int n1) {
+        if (cell0 != cell1 || n1 < n0) {
+            const double* s1 = b.spos + 4 * (size_t)n1;
+            if (c0 != (int)s1[3]) {
+                const V3 r10 = V3(s1[0], s1[1], s1[2]) - x0;
+                if (r10.norm2() < r2) {
+                    if (out) {
+                        out[2 * cnt] = (uint32_t)n0;
+                        out[2 * cnt + 1] = (uint32_t)n1;
                     }
+                    ++cnt;
                 }
             }
-            n1 = b.cellTable[n1];
         }
-        n0 = b.cellTable[n0];
+    };
+    for (int i0 = a0; i0 < e0; ++i0) {
+        const int n0 = b.cursor[i0];
+        const double* s0 = b.spos + 4 * (size_t)n0;
+        const V3 x0(s0[0], s0[1], s0[2]);
+        const int c0 = (int)s0[3];
+        if (linked) {
+            for (int n1 = b.cellTable[cell1]; n1 > -1; n1 = b.cellTable[n1]) visit(n0, x0, c0, n1);
+        } else {
+            for (int i1 = a1; i1 < e1; ++i1) visit(n0, x0, c0, b.cursor[i1]);
+        }
     }
     return cnt;
 }
@@ -306,6 +322,15 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
     }
     __syncthreads();
     // ---- pair list in the reference's order (DEM.cpp:1425-1484): count, scan, fill
+    for (int s = t; s < S; s += RB) {  // packed slot records: position, cluster
+        const V3 x = slot_pos(E, b, P, s);
+        double* r = b.spos + 4 * (size_t)s;
+        r[0] = x.x;
+        r[1] = x.y;
+        r[2] = x.z;
+        r[3] = (double)slot_cluster(b, P, s);
+    }
+    __syncthreads();
     const int W = b.W;
     for (int w = t; w < W; w += RB) {
         const int c = walk_pairs(p, b, E, S, (unsigned)w, nullptr);

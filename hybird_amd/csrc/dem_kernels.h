@@ -76,7 +76,8 @@ struct DEMBuffers {
     int* work;             // [max(W, Scap, pairCap) + 1] scan scratch
     uint32_t* csrOff;      // [P + 1]
     uint32_t* csr;         // [2 * pairCap] pair indices per particle, ascending
-    int* cursor;           // [Scap]
+    int* cursor;           // [Scap] slots sorted by cell (descending within a cell)
+    double* spos;          // [Scap][4] slot position and cluster, packed for the pair walk
     int* ghostOfCount;     // [P]
     int* ghostOf;          // [P][8] ghost slots of each particle, ascending
     int* side;             // [P][3] side-ghost index of each particle per periodic pair, -1 if none
