@@ -124,6 +124,16 @@ struct hyb_dem {
     hipStream_t stream = nullptr;
     int64_t substeps = 0;
     std::vector<Blk> saved;  // persistent device state
+    // MercuryDPM-format saving (DEM.cpp:355-360, 1131-1146): captures of the
+    // saving sub-steps of the last hyb_dem_step, one slot each
+    int saveCount = 0;
+    int saveSlots = 0;
+    double* saveElems = nullptr;    // [slot][P][9]
+    double* saveInfo = nullptr;     // [slot]: pairs of the sub-step
+    StatRec* savePairs = nullptr;   // [slot][pairCap]
+    StatRec* saveWalls = nullptr;   // [slot][P]
+    StatRec* saveObjs = nullptr;    // [slot][P][NEAR_OBJ] (objects only)
+    std::vector<std::pair<double, uint32_t>> saveEvents;  // demTime, demTimeStep of each capture
 };
 
 struct hyb_lb {
@@ -551,15 +561,59 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
     return HYB_OK;
 }
 
+static void save_free(hyb_dem* d) {
+    dfree(d->saveElems); dfree(d->saveInfo); dfree(d->savePairs); dfree(d->saveWalls); dfree(d->saveObjs);
+    d->saveSlots = 0;
+}
+
+int hyb_dem_set_save_count(hyb_dem* d, int32_t saveCount) {
+    if (!d || !d->inited) return fail(HYB_E_STATE, "hyb_dem_set_save_count before hyb_dem_init");
+    if (saveCount < 0) return fail(HYB_E_ARG, "saveCount must be >= 0");
+    if (saveCount == d->saveCount) return HYB_OK;
+    HIPCHK(hipStreamSynchronize(d->stream));
+    save_free(d);
+    d->saveEvents.clear();
+    d->saveCount = saveCount;
+    if (!saveCount) return HYB_OK;
+    // at most ceil(multiStep / saveCount) saving sub-steps in one DEM step
+    const int slots = (d->multiStep + saveCount - 1) / saveCount;
+    const size_t P = (size_t)std::max(d->P, 1);
+    int rc;
+    if ((rc = dalloc(&d->saveElems, (size_t)slots * P * 9)) || (rc = dalloc(&d->saveInfo, (size_t)slots)) ||
+        (rc = dalloc(&d->savePairs, (size_t)slots * d->p.pairCap)) || (rc = dalloc(&d->saveWalls, (size_t)slots * P)) ||
+        (d->p.nobj && (rc = dalloc(&d->saveObjs, (size_t)slots * P * NEAR_OBJ))))
+        return rc;
+    d->saveSlots = slots;
+    return HYB_OK;
+}
+
 int hyb_dem_step(hyb_dem* d) {  // DEM.cpp:331-376
     if (!d || !d->inited) return fail(HYB_E_STATE, "hyb_dem_step before hyb_dem_init");
+    d->saveEvents.clear();
     for (int it = 0; it < d->multiStep; ++it) {
         d->demTimeStep++;
         d->demTime += d->deltat;
+        const bool save = d->saveCount > 0 && d->demTimeStep % (uint32_t)d->saveCount == 0;
+        const int slot = (int)d->saveEvents.size();
+        if (save) {
+            if (slot >= d->saveSlots) return fail(HYB_E_STATE, "hyb_dem_step: more saving sub-steps than slots");
+            d->saveEvents.emplace_back(d->demTime, d->demTimeStep);
+        }
         if (d->P) {
             launch_dem_rebuild(d->p, d->b, 0, d->stream);
-            launch_dem_pairs(d->p, d->b, d->stream);
-            launch_dem_gather(d->p, d->b, d->stream);
+            if (save) {
+                const size_t P = (size_t)d->P;
+                DEMBuffers bs = d->b;
+                bs.pstat = d->savePairs + (size_t)slot * d->p.pairCap;
+                bs.wstat = d->saveWalls + (size_t)slot * P;
+                bs.ostat = d->saveObjs ? d->saveObjs + (size_t)slot * P * NEAR_OBJ : nullptr;
+                launch_dem_save_elements(d->p, d->b, d->saveElems + (size_t)slot * P * 9, d->saveInfo + slot, d->stream);
+                launch_dem_pairs(d->p, bs, d->stream);
+                launch_dem_gather(d->p, bs, d->stream);
+            } else {
+                launch_dem_pairs(d->p, d->b, d->stream);
+                launch_dem_gather(d->p, d->b, d->stream);
+            }
         }
         d->substeps++;
     }
@@ -576,6 +630,7 @@ int hyb_dem_destroy(hyb_dem* d) {
     dfree(b.cursor); dfree(b.spos); dfree(b.ghostOfCount); dfree(b.ghostOf); dfree(b.side); dfree(b.ghostB);
     dfree(b.posI); dfree(b.posJ); dfree(b.prec); dfree(b.vmax); dfree(b.energy); dfree(b.nearObj); dfree(b.nearObjCount);
     if (b.objs) (void)hipFree((void*)b.objs);
+    save_free(d);
     delete d;
     return HYB_OK;
 }
@@ -1280,6 +1335,41 @@ int hyb_lb_get(hyb_lb* l, const char* cname, void* host, size_t cap, size_t* nee
         return put(&m, 8, host, cap, need);
     }
     return fail(HYB_E_ARG, "unknown LB field " + name);
+}
+
+int hyb_dem_saved(hyb_dem* d, int32_t* count) {
+    if (!d || !count) return fail(HYB_E_ARG, "hyb_dem_saved: null argument");
+    *count = (int32_t)d->saveEvents.size();
+    return HYB_OK;
+}
+
+int hyb_dem_saved_get(hyb_dem* d, int32_t event, const char* cpart, void* host, size_t cap, size_t* need) {
+    if (!d || !cpart) return fail(HYB_E_ARG, "hyb_dem_saved_get: null argument");
+    if (event < 0 || event >= (int32_t)d->saveEvents.size()) return fail(HYB_E_ARG, "hyb_dem_saved_get: no such capture");
+    HIPCHK(hipStreamSynchronize(d->stream));
+    const std::string part(cpart);
+    const size_t P = (size_t)d->P;
+    double nPairs = 0.0;
+    if (P) HIPCHK(hipMemcpy(&nPairs, d->saveInfo + event, sizeof(double), hipMemcpyDeviceToHost));
+    if (part == "info") {
+        const double v[10] = {d->saveEvents[event].first, (double)d->saveEvents[event].second, nPairs, (double)d->P,
+                              (double)d->stdObjects, d->dampCoeff, d->deltat, d->demSize[0], d->demSize[1],
+                              d->demSize[2]};
+        return put(v, sizeof(v), host, cap, need);
+    }
+    auto rows = [&](const void* dev, size_t bytes) -> int {
+        if (need) *need = bytes;
+        if (host && cap >= bytes && bytes) HIPCHK(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
+        return HYB_OK;
+    };
+    if (part == "elements") return rows(d->saveElems + (size_t)event * P * 9, P * 9 * sizeof(double));
+    if (part == "pairs")
+        return rows(d->savePairs + (size_t)event * d->p.pairCap, (size_t)nPairs * sizeof(StatRec));
+    if (part == "walls") return rows(d->saveWalls + (size_t)event * P, P * sizeof(StatRec));
+    if (part == "objects")
+        return rows(d->saveObjs ? (const void*)(d->saveObjs + (size_t)event * P * NEAR_OBJ) : nullptr,
+                    d->saveObjs ? P * NEAR_OBJ * sizeof(StatRec) : 0);
+    return fail(HYB_E_ARG, "unknown capture part " + part);
 }
 
 int hyb_dem_get(hyb_dem* d, const char* cname, void* host, size_t cap, size_t* need) {

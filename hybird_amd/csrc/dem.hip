@@ -490,6 +490,13 @@ __device__ __forceinline__ void slot_pred(const El& E, const DEMBuffers& b, int 
 // evaluated at the predicted state. The signed increments for each of the two
 // elements are written into that element's slot of the pair in its csr list, so
 // that k_dem_gather adds them in pair-list order.
+__device__ __forceinline__ void set_stat(StatRec& s, double ov, double fn, double ft, const V3& en, const V3& et) {
+    s.ov = ov; s.fn = fn; s.ft = ft;
+    s.en[0] = en.x; s.en[1] = en.y; s.en[2] = en.z;
+    s.et[0] = et.x; s.et[1] = et.y; s.et[2] = et.z;
+    s.flag = 1.0;
+}
+
 __global__ void __launch_bounds__(256) k_dem_pairs(DEMParams p, DEMBuffers b) {
 #pragma clang fp contract(off)
     const El E{b.el, p.P};
@@ -509,6 +516,7 @@ __global__ void __launch_bounds__(256) k_dem_pairs(DEMParams p, DEMBuffers b) {
         PairRec ri, rj;
         ri.flag = rj.flag = 0;
         ri.pad = rj.pad = 0;
+        StatRec st{(double)n0, (double)n1, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0};
         if (vd.norm2() < sigij2) {
             const double distance = vd.norm();
             const double overlap = rI + rJ - distance;
@@ -546,8 +554,10 @@ __global__ void __launch_bounds__(256) k_dem_pairs(DEMParams p, DEMBuffers b) {
                 rj.ft[0] = mt.x; rj.ft[1] = mt.y; rj.ft[2] = mt.z;
                 rj.m[0] = mJ3.x; rj.m[1] = mJ3.y; rj.m[2] = mJ3.z;
                 ri.flag = rj.flag = 3;
+                if (b.pstat) set_stat(st, overlap, fn, ft, en, et);
             }
         }
+        if (b.pstat) b.pstat[k] = st;
         if (si != 0xFFFFFFFFu) b.prec[si] = ri;
         if (sj != 0xFFFFFFFFu) b.prec[sj] = rj;
     }
@@ -664,6 +674,8 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
         for (uint32_t k = 0; k < GU; ++k) add(r[k]);
     }
     for (; i < hi; ++i) add(b.prec[i]);
+    StatRec wst{wi == 0xFFFFFFFFu ? -1.0 : (double)wi, (double)e, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0},
+                0.0};
     if (wi != 0xFFFFFFFFu) {
         const WallDev& w = p.walls[wi];
         const double distance = w.dist(xpe);
@@ -688,17 +700,26 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
                 const V3 tangForce = ft * et;
                 MW = MW - centerDistJ.cross(tangForce);
                 FW = FW - tangForce;
+                if (b.wstat) set_stat(wst, overlap, fn, ft, en, et);
             }
         }
     }
+    if (b.wstat) b.wstat[e] = wst;
     // object contacts (DEM.cpp:1747-1770, 2060-2136), in list order after the wall
     if (p.nobj) {
         const int no = b.nearObjCount[e];
+        if (b.ostat)
+            for (int k = no; k < NEAR_OBJ; ++k)
+                b.ostat[(size_t)e * NEAR_OBJ + k] = StatRec{-1.0, (double)e, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0},
+                                                            {0.0, 0.0, 0.0}, 0.0};
         for (int k = 0; k < no; ++k) {
-            const ObjDev& ob = b.objs[b.nearObj[(size_t)e * NEAR_OBJ + k]];
+            const uint32_t oslot = b.nearObj[(size_t)e * NEAR_OBJ + k];
+            const ObjDev& ob = b.objs[oslot];
             const V3 vectorDistance = xpe - ob.x0;
             const double distance = vectorDistance.norm();
             const double overlap = re + ob.r - distance;
+            StatRec ost{(double)oslot, (double)e, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0};
+            if (b.ostat && !(overlap > 0.0)) b.ostat[(size_t)e * NEAR_OBJ + k] = ost;
             if (!(overlap > 0.0)) continue;
             const V3 en = (1.0 / distance) * vectorDistance;
             const V3 relVel = vpe - ob.x1;
@@ -718,7 +739,9 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
                 const V3 tangForce = ft * et;
                 MW = MW - centerDistJ.cross(tangForce);
                 FW = FW - tangForce;
+                if (b.ostat) set_stat(ost, overlap, fn, ft, en, et);
             }
+            if (b.ostat) b.ostat[(size_t)e * NEAR_OBJ + k] = ost;
         }
     }
     // Newton-Euler
@@ -903,6 +926,28 @@ __global__ void __launch_bounds__(EN_BLOCK) k_dem_energy(DEMParams p, DEMBuffers
 
 void launch_dem_energy(const DEMParams& p, const DEMBuffers& b, const double demSize[3], double* out, hipStream_t s) {
     k_dem_energy<<<1, EN_BLOCK, 0, s>>>(p, b, demSize[0], demSize[1], demSize[2], out);
+}
+
+namespace {
+// Particle state of a saving sub-step, after the predictor (DEM.cpp:349-359:
+// particles take the predicted element position and velocity; wGlobal is the
+// last corrected angular velocity).
+__global__ void k_dem_save_elements(DEMParams p, DEMBuffers b, double* __restrict__ out, double* __restrict__ info) {
+    const El E{b.el, p.P};
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e == 0) info[0] = (double)b.st->nPairs;
+    if (e >= p.P) return;
+    double* o = out + (size_t)e * 9;
+    for (int k = 0; k < 3; ++k) {
+        o[k] = E.at(EF_XP + k, e);
+        o[3 + k] = E.at(EF_XP + 3 + k, e);
+        o[6 + k] = E.at(EF_WG + k, e);
+    }
+}
+}  // namespace
+
+void launch_dem_save_elements(const DEMParams& p, const DEMBuffers& b, double* out, double* info, hipStream_t s) {
+    k_dem_save_elements<<<(p.P > 0 ? p.P + 63 : 64) / 64, 64, 0, s>>>(p, b, out, info);
 }
 
 void launch_dem_predict(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {

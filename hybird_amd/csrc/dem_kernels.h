@@ -90,6 +90,10 @@ struct DEMBuffers {
     const ObjDev* objs;    // [nobj]
     uint32_t* nearObj;     // [P][NEAR_OBJ] objects within nebrRange of each particle, ascending
     int* nearObjCount;     // [P]
+    // contact statistics of a saving sub-step (nullptr on the others)
+    struct StatRec* pstat;  // [pairCap] per pair
+    struct StatRec* wstat;  // [P] wall contact of each element
+    struct StatRec* ostat;  // [P][NEAR_OBJ] object contacts of each element
     int scap;
     int ncells;
     int W;                 // ncells * 14
@@ -105,6 +109,21 @@ struct PairRec {
     int pad;
 };
 
+// One contact at a saving sub-step, with the fields of the MercuryDPM .fstat
+// line (DEM.cpp:2479-2565): the partners (a, b: the pair's two particle slots;
+// wall index and element; object slot (periodic copies included) and element), overlap, normal and
+// tangential force magnitudes, normal and tangential unit vectors. flag = 1
+// when the contact exists and has tangential motion (the reference writes only
+// those); 0 otherwise.
+struct StatRec {
+    double a, b, ov, fn, ft;
+    double en[3], et[3];
+    double flag;
+};
+
+// The particle state a saving sub-step exports (after the predictor): per
+// element the predicted position and velocity and wGlobal; info[0] = pairs.
+void launch_dem_save_elements(const DEMParams& p, const DEMBuffers& b, double* out, double* info, hipStream_t s);
 void launch_dem_rebuild(const DEMParams& p, const DEMBuffers& b, int force, hipStream_t s);
 void launch_dem_pairs(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
 void launch_dem_gather(const DEMParams& p, const DEMBuffers& b, hipStream_t s);

@@ -49,6 +49,9 @@ public:
     hyb_dem* handle() const { return h_; }
 
 private:
+    // MercuryDPM-format files of the saving sub-steps of the last device step
+    void exportSaved(IO& io);
+    int saveCountSet_ = 0;
     hyb_config cfg_;
     std::vector<double> records_;
     std::vector<double> objectRecords_;  // HYB_OBJECT_RECORD per object

@@ -1,6 +1,7 @@
 // Input/output side of the C++ host layer: the public part of the reference's
 // IO class (IO.h:36-131). Status lines follow IO.cpp:132-198; the fluid (.vti),
-// particle (.vtu) and recycle (.dat) files follow IO.cpp:250-535, 698-831 with
+// particle (.vtu) and recycle (.dat) files follow IO.cpp:250-535, 698-831, and the
+// MercuryDPM files (.data, .restart, .fstat) DEM.cpp:2280-2565, with
 // the reference's names and number formatting, from views of the device state.
 #ifndef HYBIRD_AMD_IO_H
 #define HYBIRD_AMD_IO_H
@@ -35,6 +36,11 @@ public:
     string fluidFileFormat, partFileFormat;
     ofstream exportFile;
     int saveCount = 0;
+    // MercuryDPM-format files (IO.cpp:41-48), written by DEM::discreteElementStep
+    string dataFileName, statisticFileName, restartFileName;
+    // precision the .restart stream starts a file with: the reference's stream
+    // keeps the setprecision(10) of its first file for the later ones
+    int restartPrecision = 6;
 
     void initialize();
     void outputStep(const double& demTime, const LB& lb, const DEM& dem);

@@ -4,6 +4,7 @@
 #include <cerrno>
 #include <cstdio>
 #include <fstream>
+#include <iomanip>
 #include <sstream>
 
 #include "DEM.h"
@@ -49,6 +50,25 @@ bool view(H* h, G get, const char* name, std::vector<T>& out) {
     out.resize(need / sizeof(T));
     if (need && get(h, name, out.data(), need, &need) != HYB_OK) return false;
     return true;
+}
+
+std::vector<double> saved_part(hyb_dem* h, int32_t ev, const char* part) {
+    std::vector<double> out;
+    size_t need = 0;
+    HYB_CALL(hyb_dem_saved_get(h, ev, part, nullptr, 0, &need));
+    out.resize(need / sizeof(double));
+    if (need) HYB_CALL(hyb_dem_saved_get(h, ev, part, out.data(), need, &need));
+    return out;
+}
+
+// One .fstat contact line (DEM.cpp:2479-2565): time, the two indices, contact
+// centre, overlap, tangential spring length (always 0), |Fn|, |Ft|, en, et.
+void stat_line(ofstream& f, double time, long i, long j, const tVect& centre, const double* r) {
+    f << setprecision(3) << time << " " << i << " " << j;
+    const double v[13] = {centre.x, centre.y, centre.z, r[2], 0.0, std::abs(r[3]), std::abs(r[4]),
+                          r[5], r[6], r[7], r[8], r[9], r[10]};
+    for (double x : v) f << " " << setprecision(15) << x;
+    f << endl;
 }
 
 }  // namespace
@@ -362,6 +382,7 @@ void DEM::discreteElementGet(GetPot& lbmCfgFile, GetPot& command_line) {  // DEM
     ASSERT(objectFileID.is_open());
     unsigned int totObjects = 0;
     objectFileID >> totObjects;
+    stdObjects = totObjects;
     objects.clear();
     objectRecords_.clear();
     for (unsigned int n = 0; n < totObjects; ++n) {
@@ -442,12 +463,145 @@ void DEM::discreteElementInit(unsIntList& boundary, unsIntList& lbSize, measureU
 }
 
 void DEM::discreteElementStep(IO& io) {  // DEM.cpp:331-376
-    (void)io;
+    if (io.saveCount != saveCountSet_) {
+        HYB_CALL(hyb_dem_set_save_count(h_, io.saveCount));
+        saveCountSet_ = io.saveCount;
+    }
     HYB_CALL(hyb_dem_step(h_));
     // the same time bookkeeping as the device side, without a device round trip
     for (unsigned it = 0; it < multiStep; ++it) {
         demTimeStep++;
         demTime += deltat;
+    }
+    if (io.saveCount != 0) exportSaved(io);
+}
+
+// The MercuryDPM files of each saving sub-step of the last device step, in
+// sub-step order: exportDataFile (DEM.cpp:2280-2333, appended), exportRestartFile
+// (2337-2434, rewritten), printHeaderStatFile and the contact lines of the force
+// evaluation (1131-1146, 2447-2565, appended): particle-particle contacts in
+// neighbour-table order without ghosts, then wall contacts, then object contacts,
+// each only with tangential motion. Particle state is the predicted one.
+void DEM::exportSaved(IO& io) {
+    int32_t n = 0;
+    HYB_CALL(hyb_dem_saved(h_, &n));
+    if (n == 0) return;
+    const std::vector<double> misc = field("e_misc");  // [E][19]: m, radius, I, ...
+    const size_t E = elmts.size();
+    const double zero = 0.0;
+    for (int32_t ev = 0; ev < n; ++ev) {
+        const std::vector<double> info = saved_part(h_, ev, "info");
+        const std::vector<double> el = saved_part(h_, ev, "elements");
+        const double time = info[0];
+        const unsigned int step = (unsigned int)info[1];
+        const double dampCoeff = info[5], dt = info[6];
+        const double size[3] = {info[7], info[8], info[9]};
+        auto pos = [&](size_t e) { return tVect(el[9 * e], el[9 * e + 1], el[9 * e + 2]); };
+        auto vel = [&](size_t e) { return tVect(el[9 * e + 3], el[9 * e + 4], el[9 * e + 5]); };
+        auto wg = [&](size_t e) { return tVect(el[9 * e + 6], el[9 * e + 7], el[9 * e + 8]); };
+        auto put3 = [](ofstream& f, const tVect& v) { f << v.x << " " << v.y << " " << v.z << " "; };
+        // ---- .data: precision 5 for the time, then 6 + 5 (the stream's default + 5)
+        {
+            ofstream f(io.dataFileName.c_str(), ios::app);
+            f << stdParticles + stdObjects << " " << setprecision(5) << time << " " << setprecision(11) << zero << " "
+              << setprecision(11) << zero << " " << setprecision(11) << zero << " " << setprecision(11) << size[0]
+              << " " << setprecision(11) << size[1] << " " << setprecision(11) << size[2] << "\n";
+            for (unsigned o = 0; o < stdObjects && o < objects.size(); ++o) {
+                put3(f, objects[o].x0);
+                put3(f, objects[o].x1);
+                f << objects[o].r << " " << zero << " " << zero << " " << zero << " " << zero << " " << zero << " "
+                  << zero << " " << zero << " " << "\n";
+            }
+            for (size_t i = 0; i < E; ++i) {
+                put3(f, pos(i));
+                put3(f, vel(i));
+                f << elmts[i].radius << " " << zero << " " << zero << " " << zero << " ";
+                put3(f, wg(i));
+                f << zero << " " << "\n";
+            }
+        }
+        // ---- .restart (rewritten; the first one starts at the default precision)
+        {
+            ofstream f(io.restartFileName.c_str());
+            f << setprecision(io.restartPrecision);
+            const double counter = (double)(step / (unsigned int)io.saveCount);
+            f << "restart_version " << 1 << " name" << " " << io.problemNameString << "\n";
+            const char* files[5] = {"dataFile ", "fStatFile ", "eneFile ", "restartFile ", "statFile "};
+            for (const char* nm : files)
+                f << nm << "fileType " << "ONE_FILE " << "saveCount " << io.saveCount << " " << "counter " << counter
+                  << " nextSavedTimeStep " << 0 << "\n";
+            f << "xMin " << 0 << " xMax " << size[0] << " yMin " << 0 << " yMax " << size[1] << " zMin " << 0
+              << " zMax " << size[2] << "\n"
+              << "timeStep " << setprecision(10) << dt << " time " << time << " ntimeSteps " << step << " timeMax "
+              << time << "\n"
+              << "systemDimensions " << 3 << " particleDimensions " << 3 << " gravity " << demF.x << " " << demF.y
+              << " " << demF.z << "\n"
+              << "Species " << 1 << "\n"
+              << "LinearViscoelasticSlidingFrictionSpecies id " << 0 << " density " << cfg_.density << " stiffness "
+              << cfg_.linearStiff << " dissipation " << dampCoeff << " slidingStiffness " << 0
+              << " slidingDissipation " << 0 << " slidingFrictionCoefficient " << cfg_.frictionCoefPart
+              << " slidingFrictionCoefficientStatic " << 0 << "\n"
+              << "Walls " << 0 << "\n"
+              << "Boundaries " << 0 << "\n"
+              << "Particles " << stdParticles + stdObjects << "\n";
+            io.restartPrecision = 10;
+            for (unsigned o = 0; o < stdObjects && o < objects.size(); ++o) {
+                f << "BaseParticle id " << o << " " << "indSpecies " << 0 << " " << "position ";
+                put3(f, objects[o].x0);
+                f << "orientation " << 0 << " " << 0 << " " << 0 << " " << 0 << " " << "velocity ";
+                put3(f, objects[o].x1);
+                f << "angularVelocity " << zero << " " << zero << " " << zero << " " << 0 << " " << "force " << 0 << " "
+                  << 0 << " " << 0 << " " << "torque " << 0 << " " << 0 << " " << 0 << " " << "radius "
+                  << objects[o].r << " " << "invMass " << zero << " " << "invInertia " << zero << " " << "\n";
+            }
+            for (size_t i = 0; i < E; ++i) {
+                f << "BaseParticle id " << i + stdObjects << " " << "indSpecies " << 0 << " " << "position ";
+                put3(f, pos(i));
+                f << "orientation " << 0 << " " << 0 << " " << 0 << " " << 0 << " " << "velocity ";
+                put3(f, vel(i));
+                f << "angularVelocity ";
+                put3(f, wg(i));
+                f << 0 << " " << "force " << 0 << " " << 0 << " " << 0 << " " << "torque " << 0 << " " << 0 << " " << 0
+                  << " " << "radius " << elmts[i].radius << " " << "invMass " << 1 / misc[19 * i] << " "
+                  << "invInertia " << 1 / misc[19 * i + 2] << " " << "\n";
+            }
+            f << "Interactions " << 0;
+            f << "\n";
+        }
+        // ---- .fstat: header, then the contact lines of this sub-step's force evaluation
+        {
+            ofstream f(io.statisticFileName.c_str(), ios::app);
+            const double r0 = E ? elmts[0].radius : 0.0;  // the reference reads particles[0]
+            f << "#" << " " << setprecision(5) << time << " " << zero << endl;
+            f << "#" << " " << zero << " " << zero << " " << zero << " " << size[0] << " " << size[1] << " " << size[2]
+              << endl;
+            f << "#" << " " << r0 << " " << r0 << " " << zero << " " << zero << " " << zero << " " << zero << endl;
+            const std::vector<double> pr = saved_part(h_, ev, "pairs");
+            for (size_t k = 0; k + 12 <= pr.size(); k += 12) {
+                const double* r = &pr[k];
+                const size_t a = (size_t)r[0], b = (size_t)r[1];
+                if (r[11] == 0.0 || a >= E || b >= E) continue;
+                stat_line(f, time, (long)(a + stdObjects), (long)(b + stdObjects), 0.5 * (pos(a) + pos(b)), r);
+            }
+            const std::vector<double> wl = saved_part(h_, ev, "walls");
+            for (size_t k = 0; k + 12 <= wl.size(); k += 12) {
+                const double* r = &wl[k];
+                if (r[11] == 0.0) continue;
+                const size_t e = (size_t)r[1];
+                const tVect en(r[5], r[6], r[7]);
+                const tVect centre = pos(e) - en * (elmts[e].radius - r[2]);
+                stat_line(f, time, (long)(e + stdObjects), -(long)(r[0] + 1), centre, r);
+            }
+            const std::vector<double> ob = saved_part(h_, ev, "objects");
+            const std::vector<double> orow = ob.empty() ? std::vector<double>() : field("objects");  // [O][8]
+            for (size_t k = 0; k + 12 <= ob.size(); k += 12) {
+                const double* r = &ob[k];
+                if (r[11] == 0.0) continue;
+                const size_t e = (size_t)r[1], o = (size_t)r[0];
+                const tVect ox(orow[8 * o + 2], orow[8 * o + 3], orow[8 * o + 4]);
+                stat_line(f, time, (long)orow[8 * o], (long)(e + stdObjects), 0.5 * (ox + pos(e)), r);
+            }
+        }
     }
 }
 
@@ -512,6 +666,9 @@ void IO::initialize() {  // IO.cpp:8-130 (status file and the fluid / particle d
     fluidFileFormat = fluidDirectory + "/fluid%010u.vti";
     partFileFormat = partDirectory + "/part%010u.vtu";
     exportFileName = base + "/export.dat";
+    restartFileName = base + "/" + problemNameString + ".restart";
+    dataFileName = base + "/" + problemNameString + ".data";
+    statisticFileName = base + "/" + problemNameString + ".fstat";
     lastScreenExp = lastFluidExp = lastPartExp = lastObjectExp = lastOutputExp = lastRecycleExp = 0;
     realTime = 0.0;
 }

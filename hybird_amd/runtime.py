@@ -38,7 +38,8 @@ EXPORTS = ["hyb_config_defaults", "hyb_last_error", "hyb_device_count", "hyb_syn
            "hyb_lb_step", "hyb_lb_destroy", "hyb_cycle", "hyb_lb_get", "hyb_dem_get", "hyb_lb_set_profiling",
            "hyb_lb_kernel_time", "hyb_lb_active_nodes", "hyb_lb_set_capture", "hyb_dem_init_lattice",
            "hyb_lb_set_force_field", "hyb_lb_set_populations", "hyb_adj_forward", "hyb_adj_backward",
-           "hyb_state_save", "hyb_state_load", "hyb_dem_set_objects"]
+           "hyb_state_save", "hyb_state_load", "hyb_dem_set_objects", "hyb_dem_set_save_count", "hyb_dem_saved",
+           "hyb_dem_saved_get"]
 
 
 class HybError(RuntimeError):
@@ -72,7 +73,9 @@ def load():
                        ("hyb_lb_kernel_time", [vp, C.POINTER(C.c_double), C.POINTER(i64)]),
                        ("hyb_lb_active_nodes", [vp, C.POINTER(i64)]),
                        ("hyb_state_save", [vp, vp, C.c_char_p]), ("hyb_state_load", [vp, vp, C.c_char_p]),
-                       ("hyb_dem_set_objects", [vp, i32, vp])]:
+                       ("hyb_dem_set_objects", [vp, i32, vp]), ("hyb_dem_set_save_count", [vp, i32]),
+                       ("hyb_dem_saved", [vp, C.POINTER(i32)]),
+                       ("hyb_dem_saved_get", [vp, i32, C.c_char_p, vp, C.c_size_t, C.POINTER(C.c_size_t)])]:
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
@@ -184,6 +187,31 @@ class Simulation:
     def load_state(self, path) -> None:
         """Restores a checkpoint written from the same configuration and elements."""
         _check(load().hyb_state_load(self._lb, self._dem, str(path).encode()), "hyb_state_load")
+
+    def set_save_count(self, save_count: int) -> None:
+        """MercuryDPM-format saving every save_count DEM sub-steps (0: off)."""
+        _check(load().hyb_dem_set_save_count(self._dem, int(save_count)), "hyb_dem_set_save_count")
+
+    def saved(self):
+        """Captures of the saving sub-steps of the last DEM step: a list of dicts with
+        info (demTime, demTimeStep, pairs, ...), elements [E][9] and the pairs / walls /
+        objects statistics rows [.][12]."""
+        L = load()
+        n = C.c_int32(0)
+        _check(L.hyb_dem_saved(self._dem, C.byref(n)), "hyb_dem_saved")
+        out = []
+        for ev in range(n.value):
+            d = {}
+            for part, cols in (("info", 10), ("elements", 9), ("pairs", 12), ("walls", 12), ("objects", 12)):
+                need = C.c_size_t(0)
+                _check(L.hyb_dem_saved_get(self._dem, ev, part.encode(), None, 0, C.byref(need)), "hyb_dem_saved_get")
+                a = np.empty(need.value // 8, dtype=np.float64)
+                if need.value:
+                    _check(L.hyb_dem_saved_get(self._dem, ev, part.encode(), a.ctypes.data, need.value, C.byref(need)),
+                           "hyb_dem_saved_get")
+                d[part] = a if part == "info" else a.reshape(-1, cols)
+            out.append(d)
+        return out
 
     def active_nodes(self) -> int:
         n = C.c_int64(0)

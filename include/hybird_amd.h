@@ -163,6 +163,29 @@ int hyb_cycle(hyb_lb* lb, hyb_dem* dem, int32_t n);
 int hyb_lb_get(hyb_lb* lb, const char* name, void* host, size_t capacity, size_t* bytes_needed);
 int hyb_dem_get(hyb_dem* dem, const char* name, void* host, size_t capacity, size_t* bytes_needed);
 
+/* MercuryDPM-format saving, the data of DEM::exportDataFile, exportRestartFile
+ * and the .fstat statistics (DEM.cpp:355-360, 1131-1146, 2280-2565), which the
+ * reference writes inside a sub-step when `demTimeStep % saveCount == 0`.
+ * With save_count > 0, each such sub-step of hyb_dem_step (or hyb_cycle)
+ * captures, after the predictor, the particle state and the contact statistics
+ * of its force evaluation; a call keeps only its own captures (call hyb_cycle
+ * one cycle at a time to see every one). 0 turns saving off. */
+int hyb_dem_set_save_count(hyb_dem* dem, int32_t save_count);
+int hyb_dem_saved(hyb_dem* dem, int32_t* count);
+/* Part of capture `event` (0 <= event < count), copied like hyb_dem_get:
+ *   "info"     f64[10]: demTime, demTimeStep, pairs, particles, objects in the
+ *              object file, dampCoeff, deltat, demSize[3]
+ *   "elements" f64[E][9]: predicted x0, predicted x1, wGlobal
+ *   "pairs"    f64[pairs][12] (neighbour-table order), "walls" f64[E][12] (the
+ *              element's near wall), "objects" f64[E][16][12] (its near objects
+ *              in list order; empty without objects): a, b, overlap, |Fn|, |Ft|,
+ *              en[3], et[3], flag, where a, b are the pair's particle indices
+ *              (ghosts >= E), or the wall index (-1: none) / object row of the
+ *              "objects" view (periodic copies included) and the element, and
+ *              flag is 1 for a contact with tangential motion. */
+int hyb_dem_saved_get(hyb_dem* dem, int32_t event, const char* part, void* host, size_t capacity,
+                      size_t* bytes_needed);
+
 /* Profiling of the fused stream-collide kernel: when enabled, every LB step
  * records HIP events around that kernel on the handle's stream. */
 int hyb_lb_set_profiling(hyb_lb* lb, int32_t enabled);

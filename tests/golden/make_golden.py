@@ -141,6 +141,21 @@ def _cases():
                                             lbFZ=-2.0e-3, youngMod=5.0e3, density=3.0),
                             bed=bed3, vel=vel3, objects=objs, cycles=[0, 1, 5, 15, 30],
                             keys=LATTICE_KEYS + DEM_KEYS + ["nearObjectTable", "objects"])
+    # MercuryDPM files (DEM.cpp:2280-2565): .data, .restart and .fstat every 3 DEM
+    # sub-steps (so inside LB steps of 10) with the walls of the walls case: two
+    # spheres on each other, two touching across the periodic y plane (a pair
+    # with a ghost, not written), one on an object, one pushed into the x+ wall
+    # (the first wall within range of it, DEM.cpp:1503-1510); all with tangential
+    # motion. Every particle keeps at least one node from step to step, so the
+    # reference's incremental classification equals the full re-test here.
+    bed4 = Bed(np.array([[4.0, 6.0, 2.4], [7.9, 6.0, 2.4], [10.0, 1.5, 8.0], [10.0, 9.55, 8.0], [10.0, 6.0, 8.6],
+                         [10.6, 2.0, 12.0]]), np.array([2.0, 2.0, 1.0, 1.0, 1.5, 1.95]))
+    vel4 = np.array([[0.05, 0.0, -0.03], [0.0, 0.03, -0.03], [0.02, 0.0, 0.0], [-0.02, 0.0, 0.0], [0.02, 0.0, 0.0],
+                     [0.03, 0.02, 0.0]])
+    cases["mercury"] = dict(cfg=base_config(lbSizeX=14, lbSizeY=12, lbSizeZ=16, initVisc=0.05, lbFX=3.0e-4,
+                                            lbFZ=-4.0e-4, youngMod=2.0e3, density=3.0, saveCount=3, **walls),
+                            bed=bed4, vel=vel4, objects=np.array([[2.0, 10.0, 6.0, 12.0, 0.0, 0.0, 0.0]]),
+                            cycles=[0, 12], keys=["types"], files=True)
     return cases
 
 
@@ -181,7 +196,7 @@ def run_case(name: str, case: dict, outdir: str) -> None:
         if case.get("files"):  # the reference's output files, as bytes
             for dirpath, _, names in os.walk(work):
                 if os.path.basename(dirpath) not in ("fluidData", "particleData"):
-                    continue
+                    names = [n for n in names if n.endswith((".data", ".restart", ".fstat"))]
                 for fn in sorted(names):
                     with open(os.path.join(dirpath, fn), "rb") as fh:
                         arrays[f"file/{fn}"] = np.frombuffer(fh.read(), dtype=np.uint8)
