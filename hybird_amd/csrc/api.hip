@@ -38,6 +38,19 @@ int fail(int code, const std::string& msg) {
         if (e_ != hipSuccess) return fail(HYB_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Value of an exact fixed-point sum (four 32-bit limbs summed in 64-bit words,
+// LSB 2^-96), the host form of the device's exact_value.
+double exact_value_host(const unsigned long long* acc) {
+    long long a[4];
+    for (int i = 0; i < 4; ++i) a[i] = (long long)acc[i];
+    for (int i = 0; i < 3; ++i) {
+        const long long carry = a[i] >> 32;
+        a[i] -= carry * 4294967296LL;
+        a[i + 1] += carry;
+    }
+    return (((double)a[0] * 0x1p-96 + (double)a[1] * 0x1p-64) + (double)a[2] * 0x1p-32) + (double)a[3];
+}
+
 hipStream_t shared_stream() {
     static hipStream_t s = nullptr;
     if (!s) (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
@@ -530,14 +543,18 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
         (rc = ralloc(R, "ghostOf", &b.ghostOf, 8 * (size_t)std::max(P, 1))) ||
         (rc = ralloc(R, "side", &b.side, 3 * (size_t)std::max(P, 1))) || (rc = ralloc(R, "ghostB", &b.ghostB, p.Gcap)) ||
         (rc = ralloc(R, "posI", &b.posI, p.pairCap)) || (rc = ralloc(R, "posJ", &b.posJ, p.pairCap)) ||
-        (rc = dalloc(&b.prec, 2 * (size_t)p.pairCap)) || (rc = ralloc(R, "vmax", &b.vmax, (size_t)(std::max(P, 1) + 63) / 64)) || (rc = dalloc(&b.energy, 6)))
+        (rc = dalloc(&b.prec, 2 * (size_t)p.pairCap)) || (rc = ralloc(R, "vmax", &b.vmax, (size_t)(std::max(P, 1) + 63) / 64)) || (rc = dalloc(&b.energy, 6)) ||
+        (rc = dalloc(&b.wcf, (size_t)std::max(P, 1))))
         return rc;
+    HIPCHK(hipMemset(b.wcf, 0, sizeof(ContactForce) * (size_t)std::max(P, 1)));
     if (p.nobj) {
         ObjDev* objs = nullptr;
         if ((rc = ralloc(R, "objs", &objs, (size_t)p.nobj)) ||
             (rc = ralloc(R, "nearObj", &b.nearObj, (size_t)P * NEAR_OBJ)) ||
-            (rc = ralloc(R, "nearObjCount", &b.nearObjCount, (size_t)P)))
+            (rc = ralloc(R, "nearObjCount", &b.nearObjCount, (size_t)P)) ||
+            (rc = dalloc(&b.ocf, (size_t)std::max(P, 1) * NEAR_OBJ)))
             return rc;
+        HIPCHK(hipMemset(b.ocf, 0, sizeof(ContactForce) * (size_t)std::max(P, 1) * NEAR_OBJ));
         HIPCHK(hipMemcpy(objs, d->objects.data(), sizeof(ObjDev) * (size_t)p.nobj, hipMemcpyHostToDevice));
         b.objs = objs;
     }
@@ -629,6 +646,7 @@ int hyb_dem_destroy(hyb_dem* d) {
     dfree(b.cellTable); dfree(b.cellOf); dfree(b.cellCount); dfree(b.work); dfree(b.csrOff); dfree(b.csr);
     dfree(b.cursor); dfree(b.spos); dfree(b.ghostOfCount); dfree(b.ghostOf); dfree(b.side); dfree(b.ghostB);
     dfree(b.posI); dfree(b.posJ); dfree(b.prec); dfree(b.vmax); dfree(b.energy); dfree(b.nearObj); dfree(b.nearObjCount);
+    dfree(b.wcf); dfree(b.ocf);
     if (b.objs) (void)hipFree((void*)b.objs);
     save_free(d);
     delete d;
@@ -1334,6 +1352,30 @@ int hyb_lb_get(hyb_lb* l, const char* cname, void* host, size_t cap, size_t* nee
         }
         return put(&m, 8, host, cap, need);
     }
+    if (name == "wallFHydro") {  // walls[].FHydro of the last LB step (LB.cpp:1186-1189, 1321-1341, 1485-1487)
+        const hyb_dem* d = l->dem;
+        const size_t W = d ? d->walls.size() : 0;
+        std::vector<double> F(3 * W, 0.0);
+        if (W && l->steps > 0) {
+            WallSet ws;
+            ws.n = (int)std::min<size_t>(W, 6);
+            ws.unitLength = l->unit.Length;
+            for (int k = 0; k < ws.n; ++k) ws.w[k] = d->walls[k];
+            unsigned long long* acc = nullptr;
+            const size_t nacc = 3 * 4 * W;
+            HIPCHK(hipMalloc((void**)&acc, nacc * 8));
+            HIPCHK(hipMemsetAsync(acc, 0, nacc * 8, l->stream));
+            launch_wall_fhydro(l->prec, l->f[l->cur], l->type, ws, acc, p, l->stream);
+            std::vector<unsigned long long> a(nacc);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess) e = hipMemcpyAsync(a.data(), acc, nacc * 8, hipMemcpyDeviceToHost, l->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(l->stream);
+            (void)hipFree(acc);
+            if (e != hipSuccess) return fail(HYB_E_HIP, std::string("wallFHydro: ") + hipGetErrorString(e));
+            for (size_t k = 0; k < 3 * W; ++k) F[k] = exact_value_host(&a[4 * k]) * l->unit.Force;
+        }
+        return put(F.data(), F.size() * 8, host, cap, need);
+    }
     return fail(HYB_E_ARG, "unknown LB field " + name);
 }
 
@@ -1415,6 +1457,48 @@ int hyb_dem_get(hyb_dem* d, const char* cname, void* host, size_t cap, size_t* n
             v.push_back((double)o.index); v.push_back(o.r);
             v.push_back(o.x0.x); v.push_back(o.x0.y); v.push_back(o.x0.z);
             v.push_back(o.x1.x); v.push_back(o.x1.y); v.push_back(o.x1.z);
+        }
+        return put(v.data(), v.size() * 8, host, cap, need);
+    }
+    if (name == "walls" || name == "objectForces") {
+        // walls: p, n, FParticle (DEM.cpp:1123-1125, 1936-1979) and FHydro (0 here; the
+        // lattice's "wallFHydro"); objectForces: FParticle of every object (DEM.cpp:2100-2134).
+        // Contacts of the last sub-step, applied in the reference's list order.
+        const bool wallView = name == "walls";
+        const size_t rows = wallView ? d->walls.size() : d->objects.size();
+        std::vector<V3> F(rows, V3(0.0, 0.0, 0.0));
+        if (P && d->substeps > 0) {
+            const size_t per = wallView ? 1 : NEAR_OBJ;
+            std::vector<ContactForce> cf((size_t)P * per);
+            std::vector<uint32_t> idx((size_t)P * per);
+            std::vector<int> cnt((size_t)P, 1);
+            if (wallView) {
+                HIPCHK(hipMemcpy(cf.data(), d->b.wcf, sizeof(ContactForce) * cf.size(), hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(idx.data(), d->b.nearWall, 4 * idx.size(), hipMemcpyDeviceToHost));
+            } else if (d->p.nobj) {
+                HIPCHK(hipMemcpy(cf.data(), d->b.ocf, sizeof(ContactForce) * cf.size(), hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(idx.data(), d->b.nearObj, 4 * idx.size(), hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(cnt.data(), d->b.nearObjCount, 4 * cnt.size(), hipMemcpyDeviceToHost));
+            } else {
+                cnt.assign((size_t)P, 0);
+            }
+            for (int e = 0; e < P; ++e)
+                for (int k = 0; k < cnt[e]; ++k) {
+                    const uint32_t w = idx[(size_t)e * per + k];
+                    const ContactForce& c = cf[(size_t)e * per + k];
+                    if (w >= rows || c.flag == 0.0) continue;
+                    F[w] = F[w] - V3(c.fn[0], c.fn[1], c.fn[2]);
+                    if (c.flag == 3.0) F[w] = F[w] + V3(c.ft[0], c.ft[1], c.ft[2]);
+                }
+        }
+        for (size_t w = 0; w < rows; ++w) {
+            if (wallView) {
+                const WallDev& wd = d->walls[w];
+                v.push_back(wd.p.x); v.push_back(wd.p.y); v.push_back(wd.p.z);
+                v.push_back(wd.n.x); v.push_back(wd.n.y); v.push_back(wd.n.z);
+            }
+            v.push_back(F[w].x); v.push_back(F[w].y); v.push_back(F[w].z);
+            if (wallView) { v.push_back(0.0); v.push_back(0.0); v.push_back(0.0); }
         }
         return put(v.data(), v.size() * 8, host, cap, need);
     }

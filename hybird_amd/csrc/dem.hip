@@ -497,6 +497,16 @@ __device__ __forceinline__ void set_stat(StatRec& s, double ov, double fn, doubl
     s.flag = 1.0;
 }
 
+__device__ __forceinline__ void set_force(ContactForce& c, const V3& fn, double flag) {
+    c.fn[0] = fn.x; c.fn[1] = fn.y; c.fn[2] = fn.z;
+    c.flag = flag;
+}
+
+__device__ __forceinline__ void set_force_t(ContactForce& c, const V3& ft) {
+    c.ft[0] = ft.x; c.ft[1] = ft.y; c.ft[2] = ft.z;
+    c.flag = 3.0;
+}
+
 __global__ void __launch_bounds__(256) k_dem_pairs(DEMParams p, DEMBuffers b) {
 #pragma clang fp contract(off)
     const El E{b.el, p.P};
@@ -676,6 +686,7 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
     for (; i < hi; ++i) add(b.prec[i]);
     StatRec wst{wi == 0xFFFFFFFFu ? -1.0 : (double)wi, (double)e, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0},
                 0.0};
+    ContactForce wcf{{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0, 0.0};
     if (wi != 0xFFFFFFFFu) {
         const WallDev& w = p.walls[wi];
         const double distance = w.dist(xpe);
@@ -691,6 +702,7 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
             const V3 vecRadJ = -re * en;
             const V3 centerDistJ = vecRadJ;
             FW = FW + normalForce;
+            set_force(wcf, normalForce, 1.0);
             const V3 relVelContact = relVel + wpg.cross(vecRadJ);
             const V3 tangRelVelContact = relVelContact - normalRelVel;
             const double nt = tangRelVelContact.norm();
@@ -700,9 +712,11 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
                 const V3 tangForce = ft * et;
                 MW = MW - centerDistJ.cross(tangForce);
                 FW = FW - tangForce;
+                set_force_t(wcf, tangForce);
                 if (b.wstat) set_stat(wst, overlap, fn, ft, en, et);
             }
         }
+        b.wcf[e] = wcf;
     }
     if (b.wstat) b.wstat[e] = wst;
     // object contacts (DEM.cpp:1747-1770, 2060-2136), in list order after the wall
@@ -719,8 +733,12 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
             const double distance = vectorDistance.norm();
             const double overlap = re + ob.r - distance;
             StatRec ost{(double)oslot, (double)e, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0};
-            if (b.ostat && !(overlap > 0.0)) b.ostat[(size_t)e * NEAR_OBJ + k] = ost;
-            if (!(overlap > 0.0)) continue;
+            ContactForce ocf{{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0, 0.0};
+            if (!(overlap > 0.0)) {
+                if (b.ostat) b.ostat[(size_t)e * NEAR_OBJ + k] = ost;
+                b.ocf[(size_t)e * NEAR_OBJ + k] = ocf;
+                continue;
+            }
             const V3 en = (1.0 / distance) * vectorDistance;
             const V3 relVel = vpe - ob.x1;
             const double normRelVel = relVel.dot(en);
@@ -730,6 +748,7 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
             const V3 vecRadJ = -re * en;
             const V3 centerDistJ = vecRadJ + (xpe - xpe);  // particle = element centre (single spheres)
             FW = FW + normalForce;
+            set_force(ocf, normalForce, 1.0);
             const V3 relVelContact = relVel + wpg.cross(vecRadJ);
             const V3 tangRelVelContact = relVelContact - normalRelVel;
             const double nt = tangRelVelContact.norm();
@@ -739,9 +758,11 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
                 const V3 tangForce = ft * et;
                 MW = MW - centerDistJ.cross(tangForce);
                 FW = FW - tangForce;
+                set_force_t(ocf, tangForce);
                 if (b.ostat) set_stat(ost, overlap, fn, ft, en, et);
             }
             if (b.ostat) b.ostat[(size_t)e * NEAR_OBJ + k] = ost;
+            b.ocf[(size_t)e * NEAR_OBJ + k] = ocf;
         }
     }
     // Newton-Euler

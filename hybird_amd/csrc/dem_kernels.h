@@ -90,6 +90,10 @@ struct DEMBuffers {
     const ObjDev* objs;    // [nobj]
     uint32_t* nearObj;     // [P][NEAR_OBJ] objects within nebrRange of each particle, ascending
     int* nearObjCount;     // [P]
+    // forces of the last sub-step's wall and object contacts (for walls[].FParticle
+    // and objects[].FParticle, summed on request in the reference's order)
+    struct ContactForce* wcf;  // [P] contact of each element with its near wall
+    struct ContactForce* ocf;  // [P][NEAR_OBJ] contacts with its near objects
     // contact statistics of a saving sub-step (nullptr on the others)
     struct StatRec* pstat;  // [pairCap] per pair
     struct StatRec* wstat;  // [P] wall contact of each element
@@ -107,6 +111,14 @@ struct PairRec {
     double m[3];
     int flag;
     int pad;
+};
+
+// What one wall or object contact did to the wall/object: FParticle -= fn, then,
+// with tangential motion, FParticle += ft (DEM.cpp:1936-1979, 2100-2134). flag
+// bit 0: in contact; bit 1: tangential force applied.
+struct ContactForce {
+    double fn[3], ft[3];
+    double flag, pad;
 };
 
 // One contact at a saving sub-step, with the fields of the MercuryDPM .fstat

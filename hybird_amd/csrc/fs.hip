@@ -44,41 +44,6 @@ enum : uint8_t { FS_STAY = 0, FS_FILLED = 1, FS_EMPTIED = 2 };
 #define FS_FOR(i, n) \
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 
-// ---- exact accumulation of the mass surplus
-// A term v is |v| 2^96 rounded toward zero to an integer q < 2^128 (|v| < 2^31);
-// its four 32-bit limbs are added with sign into four 64-bit integer sums
-// (integer addition is associative, so the result is the same in any order).
-__device__ void exact_add(unsigned long long* acc, double v) {
-    if (v == 0.0 || !(v == v)) return;
-    int e = 0;
-    const double m = frexp(fabs(v), &e);  // |v| = m 2^e, m in [0.5, 1)
-    const unsigned long long M = (unsigned long long)ldexp(m, 53);
-    const int k = e + 43;  // |v| 2^96 = M 2^(e - 53 + 96)
-    unsigned long long lo = 0, hi = 0;
-    if (k >= 0) {
-        if (k >= 75) return;  // |v| >= 2^31: not a mass term
-        if (k == 0) lo = M;
-        else if (k < 64) { lo = M << k; hi = M >> (64 - k); }
-        else hi = M << (k - 64);
-    } else if (k > -64) {
-        lo = M >> (-k);
-    }
-    const unsigned long long limb[4] = {lo & 0xFFFFFFFFull, lo >> 32, hi & 0xFFFFFFFFull, hi >> 32};
-    for (int i = 0; i < 4; ++i)
-        if (limb[i]) atomicAdd(&acc[i], v < 0.0 ? (0ull - limb[i]) : limb[i]);
-}
-
-__device__ double exact_value(const unsigned long long* acc) {
-    long long a[4];
-    for (int i = 0; i < 4; ++i) a[i] = (long long)acc[i];
-    for (int i = 0; i < 3; ++i) {  // carry into base-2^32 digits; the top digit keeps the sign
-        const long long carry = a[i] >> 32;
-        a[i] -= carry * 4294967296LL;
-        a[i + 1] += carry;
-    }
-    return (((double)a[0] * 0x1p-96 + (double)a[1] * 0x1p-64) + (double)a[2] * 0x1p-32) + (double)a[3];
-}
-
 template <typename T>
 __device__ __forceinline__ double to_d(T v) {
     return (double)v;

@@ -908,6 +908,42 @@ __global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ 
     }
 }
 
+// Hydrodynamic force on the moving no-slip walls (LB.cpp:1321-1341): every link
+// of an active node to a type-8 wall node adds bounceBackForce (node.cpp:283-291),
+// (2 (fs_j - rho0 w_j) - BBi) c_j with BBi = 0 (box walls have zero velocity), to
+// the wall whose plane holds that node (the last such wall, LB.cpp:497-533).
+// fs is the post-collision state of the last step. Exact fixed-point sums:
+// acc[(wall * 3 + k) * 4 ...].
+template <typename T, bool SHIFT>
+__global__ void __launch_bounds__(BX * BY) k_wall_fhydro(const T* __restrict__ f, const uint8_t* __restrict__ type,
+                                                          WallSet ws, unsigned long long* __restrict__ acc,
+                                                          LBKParams p) {
+#pragma clang fp contract(off)
+    Coord c;
+    if (!node_coord(p, c)) return;
+    const uint8_t t = type[c.idx];
+    if (t != T_FLUID && t != T_INTERFACE) return;
+    for (int j = 1; j < 19; ++j) {
+        const int nx = c.x + cx(j), ny = c.y + cy(j), nz = c.z + cz(j);
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= p.X || ny >= p.Y || nz >= p.Z) continue;
+        const size_t n = (size_t)nx + (size_t)p.X * ((size_t)ny + (size_t)p.Y * (size_t)nz);
+        if (type[n] != T_WALL_MOVING) continue;
+        const V3 pos((double)nx, (double)ny, (double)nz);
+        int wi = -1;
+        for (int k = 0; k < ws.n; ++k) {
+            const V3 wp = ws.w[k].p / ws.unitLength;
+            if (ws.w[k].n.dot(pos - wp) < 0.0) wi = k;
+        }
+        if (wi < 0) continue;
+        const double fj = (double)f[(size_t)j * p.PS + c.idx];
+        const double d = SHIFT ? fj + wgt(j) * (1.0 - p.initDensity) : fj - p.initDensity * wgt(j);
+        const double s = (2.0 * d - 0.0) * 1.0;
+        if (cx(j)) exact_add(&acc[(wi * 3 + 0) * 4], s * (double)cx(j));
+        if (cy(j)) exact_add(&acc[(wi * 3 + 1) * 4], s * (double)cy(j));
+        if (cz(j)) exact_add(&acc[(wi * 3 + 2) * 4], s * (double)cz(j));
+    }
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t v, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
@@ -951,6 +987,13 @@ void launch_lb_fused(Prec prec, bool pull, bool varVisc, int collision, const vo
     HYB_FUSED_CASE(false, false, HYB_COLLISION_TRT)
     HYB_FUSED_CASE(false, true, HYB_COLLISION_BGK)
     HYB_FUSED_CASE(false, true, HYB_COLLISION_TRT)
+}
+
+void launch_wall_fhydro(Prec prec, const void* f, const uint8_t* type, const WallSet& ws, unsigned long long* acc,
+                        const LBKParams& p, hipStream_t s) {
+    const dim3 g = lattice_grid(p), b(BX, BY, 1);
+    if (prec == Prec::F64) k_wall_fhydro<double, false><<<g, b, 0, s>>>((const double*)f, type, ws, acc, p);
+    else k_wall_fhydro<float, true><<<g, b, 0, s>>>((const float*)f, type, ws, acc, p);
 }
 
 void launch_lb_stream(Prec prec, const void* fin, void* fout, const uint32_t* link, const LBKParams& p, hipStream_t s) {

@@ -91,6 +91,16 @@ void launch_lb_fused(Prec prec, bool pull, bool varVisc, int collision, const vo
                      const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, const CoupleRecF* cpf, void* hf,
                      double* visc, double* mac, const FSView* fs, const LBKParams& p, hipStream_t s);
 void launch_lb_stream(Prec prec, const void* fin, void* fout, const uint32_t* link, const LBKParams& p, hipStream_t s);
+// DEM plane walls (positions in physical units) for the lattice-side wall force.
+struct WallSet {
+    int n;
+    double unitLength;
+    WallDev w[6];
+};
+// Exact sums of the hydrodynamic force on moving no-slip walls from the
+// post-collision state f: acc[nwalls][3][4] (zeroed by the caller).
+void launch_wall_fhydro(Prec prec, const void* f, const uint8_t* type, const WallSet& ws, unsigned long long* acc,
+                        const LBKParams& p, hipStream_t s);
 void launch_lb_init_types(uint8_t* type, const int* boundary, int nwalls, const WallDev* walls, int nobj,
                           const ObjDev* objs, int problem, const double* liquidBox, const LBKParams& p, hipStream_t s);
 void launch_lb_init_interface(uint8_t* type, const LBKParams& p, hipStream_t s);

@@ -33,7 +33,7 @@ public:
     double realTime = 0.0;
     string workDirectory, partDirectory, fluidDirectory;
     string exportFileName;
-    string fluidFileFormat, partFileFormat;
+    string fluidFileFormat, partFileFormat, objectFileFormat;
     ofstream exportFile;
     int saveCount = 0;
     // MercuryDPM-format files (IO.cpp:41-48), written by DEM::discreteElementStep
@@ -49,6 +49,7 @@ public:
     void exportParaviewFluid(const LB& lb, const DEM& dem, const string& fluidFile) const;
     void exportParaviewParticles(const DEM& dem, const string& particleFile) const;
     void exportRecycleParticles(const DEM& dem) const;
+    void exportParaviewObjects(const DEM& dem, const string& objectFile) const;
 };
 
 #endif  // HYBIRD_AMD_IO_H

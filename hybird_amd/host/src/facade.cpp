@@ -665,6 +665,7 @@ void IO::initialize() {  // IO.cpp:8-130 (status file and the fluid / particle d
     if (lbmSolve) mkdir(fluidDirectory.c_str(), 0777);
     fluidFileFormat = fluidDirectory + "/fluid%010u.vti";
     partFileFormat = partDirectory + "/part%010u.vtu";
+    objectFileFormat = partDirectory + "/object%010u.vtu";
     exportFileName = base + "/export.dat";
     restartFileName = base + "/" + problemNameString + ".restart";
     dataFileName = base + "/" + problemNameString + ".data";
@@ -764,6 +765,51 @@ void IO::exportParaviewFluid(const LB& lb, const DEM& dem, const string& fluidFi
 
 // IO.cpp:289-419: one point per particle (ghosts included) with its element's
 // mass, velocities and forces.
+// IO.cpp:422-480: one vertex per object (periodic copies included) with radius,
+// index, velocity and the force of the last sub-step's contacts (FParticle).
+void IO::exportParaviewObjects(const DEM& dem, const string& objectFile) const {
+    const std::vector<double> ob = dem.field("objects");       // [O][8]: index, r, x0, x1
+    const std::vector<double> fo = dem.field("objectForces");  // [O][3]
+    const size_t O = ob.size() / 8;
+    auto line = [](ofstream& f, const double* v) { f << v[0] << " " << v[1] << " " << v[2] << "\n"; };
+    ofstream f(objectFile.c_str());
+    f << "<?xml version=\"1.0\"?>\n";
+    f << "<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\" "
+         "compressor=\"vtkZLibDataCompressor\">\n";
+    f << " <UnstructuredGrid GhostLevel=\"0\">\n";
+    f << "  <Piece NumberOfPoints=\"" << O << "\" NumberOfCells=\"" << O << "\">\n";
+    f << "   <PointData>\n";
+    f << "    <DataArray type=\"Float64\" Name=\"radius\"/>\n";
+    for (size_t i = 0; i < O; ++i) f << ob[8 * i + 1] << "\n";
+    f << "    <DataArray type=\"Float64\" Name=\"objectIndex\"/>\n";
+    for (size_t i = 0; i < O; ++i) f << (unsigned int)ob[8 * i] << "\n";
+    f << "    <DataArray type=\"Float64\" Name=\"v\" NumberOfComponents=\"3\"/>\n";
+    for (size_t i = 0; i < O; ++i) line(f, &ob[8 * i + 5]);
+    f << "    <DataArray type=\"Float64\" Name=\"Force\" NumberOfComponents=\"3\"/>\n";
+    for (size_t i = 0; i < O; ++i) line(f, &fo[3 * i]);
+    f << "   </PointData>\n";
+    f << "   <CellData>\n";
+    f << "   </CellData>\n";
+    f << "   <Points>\n";
+    f << "    <DataArray type=\"Float64\" Name=\"Points\" NumberOfComponents=\"3\"/>\n";
+    for (size_t i = 0; i < O; ++i) line(f, &ob[8 * i + 2]);
+    f << "   </Points>\n";
+    f << "   <Cells>\n";
+    f << "    <DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n";
+    for (size_t i = 1; i < O + 1; ++i) f << i - 1 << "\n";
+    f << "    </DataArray>\n";
+    f << "    <DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n";
+    for (size_t i = 1; i < O + 1; ++i) f << i << "\n";
+    f << "    </DataArray>\n";
+    f << "    <DataArray type=\"Int64\" Name=\"types\" format=\"ascii\">\n";
+    for (size_t i = 0; i < O; ++i) f << 1 << "\n";
+    f << "    </DataArray>\n";
+    f << "   </Cells>\n";
+    f << "  </Piece>\n";
+    f << " </UnstructuredGrid>\n";
+    f << "</VTKFile>";
+}
+
 void IO::exportParaviewParticles(const DEM& dem, const string& particleFile) const {
     const std::vector<double> pd = dem.field("p_data");       // [S][10]: x0, x1, radiusVec, r
     const std::vector<unsigned int> pi = dem.indexField("p_idx");  // [S][4]: particle, cluster, proto, cell
@@ -881,6 +927,12 @@ void IO::outputStep(const double& demTime, const LB& lb, const DEM& dem) {  // I
         if (r > lastRecycleExp) {
             lastRecycleExp = r;
             exportRecycleParticles(dem);
+        }
+        const unsigned int o = (objectExpTime > 0 ? static_cast<unsigned int>(realTime / objectExpTime) + 1 : 0);
+        if (o > lastObjectExp) {
+            lastObjectExp = o;
+            std::snprintf(filePathBuffer, sizeof(filePathBuffer), objectFileFormat.c_str(), currentTimeStep);
+            exportParaviewObjects(dem, filePathBuffer);
         }
     }
     const unsigned int screenExpCounter =

@@ -23,8 +23,9 @@ _lib = None
 LB_FIELDS = {"f": (np.float64, (-1, 19)), "n": (np.float64, None), "u": (np.float64, (-1, 3)),
              "hydroForce": (np.float64, (-1, 3)), "mass": (np.float64, None), "visc": (np.float64, None),
              "types": (np.uint8, None), "pflag": (np.uint8, None), "solidIndex": (np.uint32, None),
-             "totalMass": (np.float64, None), "fluidMass": (np.float64, None)}
-DEM_FIELDS = {"e_x": (np.float64, (-1, 6, 3)), "e_xp": (np.float64, (-1, 6, 3)), "e_q": (np.float64, (-1, 6, 4)),
+             "totalMass": (np.float64, None), "fluidMass": (np.float64, None), "wallFHydro": (np.float64, (-1, 3))}
+DEM_FIELDS = {"walls": (np.float64, (-1, 12)), "objectForces": (np.float64, (-1, 3)),
+              "e_x": (np.float64, (-1, 6, 3)), "e_xp": (np.float64, (-1, 6, 3)), "e_q": (np.float64, (-1, 6, 4)),
               "e_qp": (np.float64, (-1, 6, 4)), "e_w": (np.float64, (-1, 6, 3)), "e_wp": (np.float64, (-1, 6, 3)),
               "e_forces": (np.float64, (-1, 7, 3)), "e_misc": (np.float64, (-1, 19)),
               "p_data": (np.float64, (-1, 10)), "p_idx": (np.uint32, (-1, 4)), "neighborTable": (np.uint32, None),

@@ -157,7 +157,11 @@ int hyb_cycle(hyb_lb* lb, hyb_dem* dem, int32_t n);
  *       "neighborTable" u32[2*pairs], "nearWallTable" u32[2*k], "ghostIndex" u32[G], "ghostVec" f64[G][3],
  *       "nearObjectTable" u32[2*k] (particle, object), "objects" f64[O][8] (index, r, x0, x1),
  *       "dem_scalars" f64[17], "energy" f64[6] (DEM::particleEnergy after the last sub-step: kin, rotKin,
- *       trKin, grav, elastic, total; DEM.cpp:2226-2276)
+ *       trKin, grav, elastic, total; DEM.cpp:2226-2276), "walls" f64[W][12] (p, n, FParticle, FHydro = 0:
+ *       walls[] after the last sub-step, DEM.cpp:1936-1979), "objectForces" f64[O][3] (objects[].FParticle,
+ *       DEM.cpp:2100-2134; rows as "objects")
+ *   LB  "wallFHydro" f64[W][3]: walls[].FHydro of the last LB step (moving no-slip walls, LB.cpp:1321-1341,
+ *       summed exactly)
  * *bytes_needed receives the size; nothing is copied when capacity is too small.
  * N = lbSize product; E elements; P particles including ghosts. */
 int hyb_lb_get(hyb_lb* lb, const char* name, void* host, size_t capacity, size_t* bytes_needed);

@@ -153,9 +153,10 @@ def _cases():
     vel4 = np.array([[0.05, 0.0, -0.03], [0.0, 0.03, -0.03], [0.02, 0.0, 0.0], [-0.02, 0.0, 0.0], [0.02, 0.0, 0.0],
                      [0.03, 0.02, 0.0]])
     cases["mercury"] = dict(cfg=base_config(lbSizeX=14, lbSizeY=12, lbSizeZ=16, initVisc=0.05, lbFX=3.0e-4,
-                                            lbFZ=-4.0e-4, youngMod=2.0e3, density=3.0, saveCount=3, **walls),
+                                            lbFZ=-4.0e-4, youngMod=2.0e3, density=3.0, saveCount=3, objectExpTime=4.0,
+                                            **walls),
                             bed=bed4, vel=vel4, objects=np.array([[2.0, 10.0, 6.0, 12.0, 0.0, 0.0, 0.0]]),
-                            cycles=[0, 12], keys=["types"], files=True)
+                            cycles=[0, 4, 5, 12], keys=["types"], files=True)
     return cases
 
 

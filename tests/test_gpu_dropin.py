@@ -179,7 +179,9 @@ def test_dropin_mercury_files_match_reference(tmp_path):
     host layer from the sub-step captures of the GPU DEM step, against the files the
     reference wrote for the same run: saving every 3 sub-steps (inside LB steps of
     10), with particle, wall and object contacts and a pair through a ghost (which
-    the reference does not write). Same lines and words; numbers to 1e-9."""
+    the reference does not write). Same lines and words; numbers to 1e-9. The same
+    run writes object .vtu files (IO.cpp:422-480) with the objects' contact force,
+    printed to six digits."""
     cfg, rec, z = load_case("mercury")
     cycles = int(max(z["cycles"]))
     tmp = str(tmp_path)
@@ -192,13 +194,14 @@ def test_dropin_mercury_files_match_reference(tmp_path):
     mine = {}
     for dirpath, _, names in os.walk(os.path.join(tmp, "out")):  # -n run: files go to out/run
         for fn in names:
-            if fn.endswith((".data", ".restart", ".fstat")):
+            if fn.endswith((".data", ".restart", ".fstat")) or fn.startswith("object"):
                 mine[fn] = open(os.path.join(dirpath, fn)).read()
     ref = {k.split("/", 1)[1]: bytes(z[k]).decode() for k in z.files if k.startswith("file/")}
-    assert sorted(ref) == ["NONE.data", "NONE.fstat", "NONE.restart"]
+    assert {"NONE.data", "NONE.fstat", "NONE.restart"} <= set(ref)
+    assert sum(fn.startswith("object") for fn in ref) == 3
     assert sorted(mine) == sorted(ref)
     for fn, text in ref.items():
-        _tokens_close(text, mine[fn], fn)
+        _tokens_close(text, mine[fn], fn, rtol=2e-5 if fn.endswith(".vtu") else 1e-9)
 
 
 @pytest.mark.skipif(not os.path.exists(BIN), reason="hybird_gpu not built (needs the reference's hybird.cpp)")

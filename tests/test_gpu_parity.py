@@ -63,6 +63,19 @@ def _compare_dem(gpu, orc, tag):
         scale = max(np.linalg.norm(a), 1e-300)
         err = np.linalg.norm(b - a) / scale if np.linalg.norm(a) > 0 else np.max(np.abs(b))
         assert err <= ELEM_TOL or np.max(np.abs(b - a)) < 1e-18, f"{tag}: {name} rel L2 {err:.3e}"
+    # walls[].FParticle (DEM.cpp:1936-1979) and objects[].FParticle (DEM.cpp:2100-2134)
+    wo, wg = orc.get("walls"), gpu.get("walls")
+    assert wo.shape == wg.shape and np.array_equal(wo[:, :6], wg[:, :6]), f"{tag}: walls differ"
+    _close_vec(wg[:, 6:9], wo[:, 6:9], f"{tag}: walls FParticle")
+    if gpu.get("objects").size:
+        _close_vec(gpu.get("objectForces"), orc.get("objects")[:, 8:11], f"{tag}: objects FParticle")
+
+
+def _close_vec(b, a, what, tol=None):
+    tol = ELEM_TOL if tol is None else tol
+    scale = np.linalg.norm(a)
+    err = np.linalg.norm(b - a) / scale if scale > 0 else np.max(np.abs(b), initial=0.0)
+    assert err <= tol or np.max(np.abs(b - a), initial=0.0) < 1e-18, f"{what}: rel L2 {err:.3e}"
 
 
 # Granular dynamics amplify any one-ulp difference (device pow in the tangential
@@ -72,7 +85,8 @@ def _compare_dem(gpu, orc, tag):
 HORIZON = {"dem60": 100}
 
 
-@pytest.mark.parametrize("name", ["lb16", "chan", "bingham", "couple2", "dense", "dem60", "walls", "objects"])
+@pytest.mark.parametrize("name", ["lb16", "chan", "bingham", "couple2", "dense", "dem60", "walls", "objects",
+                                  "mercury"])
 def test_gpu_fp64_matches_oracle(name):
     rt = _runtime()
     cfg, rec, z = load_case(name)
@@ -90,6 +104,8 @@ def test_gpu_fp64_matches_oracle(name):
         tag = f"{name} cycle {c}"
         if lattice:
             _compare_lattice(gpu, orc, tag)
+            # walls[].FHydro of the moving no-slip walls (LB.cpp:1321-1341)
+            _close_vec(gpu.get("wallFHydro"), orc.get("walls")[:, 9:12], f"{tag}: walls FHydro", LB_TOL)
         if rec.shape[0]:
             _compare_dem(gpu, orc, tag)
         if objs is not None:
