@@ -1352,6 +1352,29 @@ int hyb_lb_get(hyb_lb* l, const char* cname, void* host, size_t cap, size_t* nee
         }
         return put(&m, 8, host, cap, need);
     }
+    if (name == "status") {
+        // IO's status-line quantities (IO.cpp:835-850, 969-985): max |u|^2 over active
+        // nodes (u of the last collision), active nodes, active nodes with viscosity
+        // above 0.95 (maxTau - 0.5) / 3 (only with variable viscosity)
+        int rc = ensure_mac(l, l->dem);
+        if (rc) return rc;
+        const double maxVisc = (1.8 - 0.5) / 3 / 1.0;
+        unsigned long long* acc = nullptr;
+        HIPCHK(hipMalloc((void**)&acc, 3 * 8));
+        HIPCHK(hipMemsetAsync(acc, 0, 3 * 8, l->stream));
+        launch_status(l->mac, l->varVisc() ? l->visc : nullptr, l->link, N, 0.95 * maxVisc, acc, l->stream);
+        unsigned long long a[3] = {0, 0, 0};
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(a, acc, 3 * 8, hipMemcpyDeviceToHost, l->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(l->stream);
+        (void)hipFree(acc);
+        if (e != hipSuccess) return fail(HYB_E_HIP, std::string("status: ") + hipGetErrorString(e));
+        double v3[3];
+        std::memcpy(&v3[0], &a[0], 8);
+        v3[1] = (double)a[1];
+        v3[2] = (double)a[2];
+        return put(v3, sizeof(v3), host, cap, need);
+    }
     if (name == "wallFHydro") {  // walls[].FHydro of the last LB step (LB.cpp:1186-1189, 1321-1341, 1485-1487)
         const hyb_dem* d = l->dem;
         const size_t W = d ? d->walls.size() : 0;

@@ -944,6 +944,39 @@ __global__ void __launch_bounds__(BX * BY) k_wall_fhydro(const T* __restrict__ f
     }
 }
 
+// Status-line reductions over the active nodes (IO.cpp:835-850, 969-985):
+// out[0] = max |u|^2 as the bits of a non-negative double (NaN skipped, as
+// std::max keeps its first argument), out[1] = active nodes, out[2] = active
+// nodes with viscosity > viscMin. Order-free, so exact.
+__global__ void __launch_bounds__(256) k_status(const double* __restrict__ mac, const double* __restrict__ visc,
+                                                const uint32_t* __restrict__ link, size_t N, double viscMin,
+                                                unsigned long long* __restrict__ out) {
+#pragma clang fp contract(off)
+    double m = 0.0;
+    unsigned long long act = 0, hi = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+        if (!(link[i] & LINK_ACTIVE)) continue;
+        ++act;
+        if (mac) {
+            const double ux = mac[N + i], uy = mac[2 * N + i], uz = mac[3 * N + i];
+            const double u2 = ux * ux + uy * uy + uz * uz;
+            if (u2 > m) m = u2;
+        }
+        if (visc && visc[i] > viscMin) ++hi;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+        act += __shfl_xor(act, off, 64);
+        hi += __shfl_xor(hi, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&out[0], (unsigned long long)__double_as_longlong(m));
+        atomicAdd(&out[1], act);
+        atomicAdd(&out[2], hi);
+    }
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t v, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
@@ -994,6 +1027,11 @@ void launch_wall_fhydro(Prec prec, const void* f, const uint8_t* type, const Wal
     const dim3 g = lattice_grid(p), b(BX, BY, 1);
     if (prec == Prec::F64) k_wall_fhydro<double, false><<<g, b, 0, s>>>((const double*)f, type, ws, acc, p);
     else k_wall_fhydro<float, true><<<g, b, 0, s>>>((const float*)f, type, ws, acc, p);
+}
+
+void launch_status(const double* mac, const double* visc, const uint32_t* link, size_t N, double viscMin,
+                   unsigned long long* out, hipStream_t s) {
+    k_status<<<1024, 256, 0, s>>>(mac, visc, link, N, viscMin, out);
 }
 
 void launch_lb_stream(Prec prec, const void* fin, void* fout, const uint32_t* link, const LBKParams& p, hipStream_t s) {

@@ -91,6 +91,10 @@ void launch_lb_fused(Prec prec, bool pull, bool varVisc, int collision, const vo
                      const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, const CoupleRecF* cpf, void* hf,
                      double* visc, double* mac, const FSView* fs, const LBKParams& p, hipStream_t s);
 void launch_lb_stream(Prec prec, const void* fin, void* fout, const uint32_t* link, const LBKParams& p, hipStream_t s);
+// out[3] (zeroed by the caller): max |u|^2 bits from mac (may be nullptr), active
+// nodes, active nodes with visc > viscMin (visc may be nullptr).
+void launch_status(const double* mac, const double* visc, const uint32_t* link, size_t N, double viscMin,
+                   unsigned long long* out, hipStream_t s);
 // DEM plane walls (positions in physical units) for the lattice-side wall force.
 struct WallSet {
     int n;

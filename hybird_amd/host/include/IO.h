@@ -38,6 +38,8 @@ public:
     int saveCount = 0;
     // MercuryDPM-format files (IO.cpp:41-48), written by DEM::discreteElementStep
     string dataFileName, statisticFileName, restartFileName;
+    // status-line side files (IO.cpp:23-39, 105-123)
+    string energyFileName, maxSpeedFileName, forceFileName, plasticityFileName;
     // precision the .restart stream starts a file with: the reference's stream
     // keeps the setprecision(10) of its first file for the later ones
     int restartPrecision = 6;

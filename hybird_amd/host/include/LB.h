@@ -63,6 +63,8 @@ public:
     std::vector<unsigned char> nodeFlags(const char* name) const;  // u8 views ("pflag")
     std::vector<unsigned int> solidIndices() const;
     double totalFluidMass() const;
+    // max |u|^2 over active nodes, active nodes, active nodes near the maximum viscosity
+    std::vector<double> statusValues() const;
     hyb_lb* handle() const { return h_; }
     const hyb_config& config() const { return cfg_; }
 

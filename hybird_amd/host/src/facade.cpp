@@ -4,6 +4,7 @@
 #include <cerrno>
 #include <cstdio>
 #include <fstream>
+#include <algorithm>
 #include <iomanip>
 #include <sstream>
 
@@ -174,8 +175,10 @@ void LB::latticeBoltzmannGet(GetPot& lbmCfgFile, GetPot& command_line) {  // LB.
     cfg_.precision = prec == "FP32" ? HYB_FP32_SHIFTED : HYB_FP64;
     // checkpoint and restart (the reference cannot resume a run)
     restartFile_ = extension_key<std::string>(lbmCfgFile, command_line, "restartFile", "");
-    // fluid files need n and u of every collision (kept on the device in free-surface mode)
-    fluidExport_ = extension_key<double>(lbmCfgFile, command_line, "fluidExpTime", 0.0) > 0.0;
+    // fluid files and the status line need u of every collision (kept on the device in
+    // free-surface mode)
+    fluidExport_ = extension_key<double>(lbmCfgFile, command_line, "fluidExpTime", 0.0) > 0.0 ||
+                   extension_key<double>(lbmCfgFile, command_line, "screenExpTime", 0.0) > 0.0;
     saveStateFile_ = extension_key<std::string>(lbmCfgFile, command_line, "saveStateFile", "");
     const char* boxKeys[6] = {"liquidBoxXMin", "liquidBoxXMax", "liquidBoxYMin", "liquidBoxYMax", "liquidBoxZMin",
                               "liquidBoxZMax"};
@@ -285,6 +288,8 @@ double LB::totalFluidMass() const {
     const std::vector<double> m = field("fluidMass");
     return m.empty() ? 0.0 : m[0];
 }
+
+std::vector<double> LB::statusValues() const { return field("status"); }
 
 // ======================================================================== DEM
 DEM::DEM() {
@@ -667,6 +672,22 @@ void IO::initialize() {  // IO.cpp:8-130 (status file and the fluid / particle d
     partFileFormat = partDirectory + "/part%010u.vtu";
     objectFileFormat = partDirectory + "/object%010u.vtu";
     exportFileName = base + "/export.dat";
+    // energy.dat header; maxVel.dat, force.dat and plasticity.dat headers (IO.cpp:23-39, 105-123)
+    energyFileName = base + "/energy.dat";
+    {
+        ofstream f(energyFileName.c_str(), ios::app);
+        const int width = (int)f.precision() + 5;
+        f << "\t" << " time";
+        for (const char* h : {"ene_grav", "ene_kin", "ene_rot", "ene_ela", "X_com", "Y_com", "Z_com"})
+            f << "\t" << setw(width) << h;
+        f << "\n";
+    }
+    maxSpeedFileName = base + "/maxVel.dat";
+    forceFileName = base + "/force.dat";
+    plasticityFileName = base + "/plasticity.dat";
+    { ofstream f(maxSpeedFileName.c_str(), ios::app); f << "time maxFluidSpeed\n"; }
+    { ofstream f(forceFileName.c_str(), ios::app); f << "time collisionForces hydraulicForces\n"; }
+    { ofstream f(plasticityFileName.c_str(), ios::app); f << "time percPlastic\n"; }
     restartFileName = base + "/" + problemNameString + ".restart";
     dataFileName = base + "/" + problemNameString + ".data";
     statisticFileName = base + "/" + problemNameString + ".fstat";
@@ -942,10 +963,55 @@ void IO::outputStep(const double& demTime, const LB& lb, const DEM& dem) {  // I
         exportFile.open(exportFileName.c_str(), ios::app);
         cout << currentTimeStep << "; time=" << realTime << "\t";
         exportFile << currentTimeStep << "; time=" << realTime << "\t";
-        if (lbmSolve) {  // exportTotalMass (IO.cpp:878-883)
+        std::vector<double> st;
+        if (lbmSolve) {
+            st = lb.statusValues();
+            // exportMaxSpeedFluid (IO.cpp:835-850)
+            const double maxFluidSpeed = sqrt(st[0]);
+            cout << "MaxFSpeed= " << maxFluidSpeed * lb.unit.Speed << "("
+                 << int(1.0 * 1.0 * maxFluidSpeed * 100.0 / 0.01) << "%)\t";
+            exportFile << "MaxFSpeed= " << maxFluidSpeed * lb.unit.Speed << "("
+                       << int(1.0 * 1.0 * maxFluidSpeed * 100.0 / 0.01) << "%)\t";
+            {
+                ofstream f(maxSpeedFileName.c_str(), ios::app);
+                f << realTime << " " << maxFluidSpeed * lb.unit.Speed << "\n";
+            }
+            // exportTotalMass (IO.cpp:878-883)
             const double massTot = lb.totalFluidMass();
             cout << "Volume=" << massTot * lb.unit.Volume << "; Mass = " << massTot * lb.unit.Mass << "\t";
             exportFile << "Volume=" << massTot * lb.unit.Volume << "; Mass = " << massTot * lb.unit.Mass << "\t";
+        }
+        if (lbmSolve & lb.nonNewtonian) {  // exportPlasticity (IO.cpp:885-895, 969-985)
+            const double perc = 100.0 * st[2] / st[1];
+            cout << "Plastic =" << int(perc) << "%;\t";
+            exportFile << "Plastic =" << int(perc) << "%;\t";
+            ofstream f(plasticityFileName.c_str(), ios::app);
+            f << realTime << " " << perc << "\n";
+        }
+        if (dem.elmts.size()) {
+            // exportMaxSpeedParticles (IO.cpp:852-876) and exportForces (IO.cpp:930-938)
+            const std::vector<double> x = dem.field("e_x"), misc = dem.field("e_misc"), fr = dem.field("e_forces");
+            const size_t E = dem.elmts.size();
+            double maxPartSpeed = 0.0, maxPartSpin = 0.0, collisionForce = 0.0, hydraulicForce = 0.0;
+            for (size_t e = 0; e < E; ++e)
+                maxPartSpeed = std::max(maxPartSpeed, tVect(x[18 * e + 3], x[18 * e + 4], x[18 * e + 5]).norm2());
+            maxPartSpeed = sqrt(maxPartSpeed);
+            cout << "MaxPSpeed= " << maxPartSpeed << "\t";
+            exportFile << "MaxPSpeed= " << maxPartSpeed << "\t";
+            for (size_t e = 0; e < E; ++e)
+                maxPartSpin = std::max(maxPartSpin, tVect(misc[19 * e + 5], misc[19 * e + 6], misc[19 * e + 7]).norm2());
+            maxPartSpin = sqrt(maxPartSpin);
+            cout << "MaxPSpin= " << maxPartSpin << "\t";
+            exportFile << "MaxPSpin= " << maxPartSpin << "\t";
+            cout << "\t";
+            for (size_t e = 0; e < E; ++e) {
+                const double* f = &fr[21 * e];
+                collisionForce += tVect(f[6], f[7], f[8]).norm();
+                collisionForce += tVect(f[12], f[13], f[14]).norm();
+            }
+            for (size_t e = 0; e < E; ++e) hydraulicForce += tVect(fr[21 * e], fr[21 * e + 1], fr[21 * e + 2]).norm();
+            ofstream f(forceFileName.c_str(), ios::app);
+            f << realTime << " " << collisionForce << " " << hydraulicForce << "\n";
         }
         if (dem.elmts.size() && !lbmSolve) {  // exportEnergy (IO.cpp:919-927)
             const std::vector<double> en = dem.field("energy");

@@ -161,7 +161,8 @@ int hyb_cycle(hyb_lb* lb, hyb_dem* dem, int32_t n);
  *       walls[] after the last sub-step, DEM.cpp:1936-1979), "objectForces" f64[O][3] (objects[].FParticle,
  *       DEM.cpp:2100-2134; rows as "objects")
  *   LB  "wallFHydro" f64[W][3]: walls[].FHydro of the last LB step (moving no-slip walls, LB.cpp:1321-1341,
- *       summed exactly)
+ *       summed exactly); "status" f64[3]: max |u|^2 over active nodes, active nodes, active nodes with
+ *       viscosity > 0.95 (maxTau - 0.5) / 3 (IO.cpp:835-850, 969-985; free surface: needs capture)
  * *bytes_needed receives the size; nothing is copied when capacity is too small.
  * N = lbSize product; E elements; P particles including ghosts. */
 int hyb_lb_get(hyb_lb* lb, const char* name, void* host, size_t capacity, size_t* bytes_needed);

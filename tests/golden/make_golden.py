@@ -118,8 +118,9 @@ def _cases():
                                              lbFX=5.0e-4, lbFZ=-5.0e-4, **walls),
                              bed=Bed(np.zeros((0, 3)), np.zeros(0)), cycles=[0, 1, 5, 20, 40], keys=LATTICE_KEYS)
     # output files of the reference (IO.cpp:250-535, 698-831): fluid .vti, particle
-    # .vtu and recycle .dat every 4 time units, kept as expected outputs
-    out = dict(fluidExpTime=4.0, partExpTime=4.0, recycleExpTime=4.0)
+    # .vtu and recycle .dat every 4 time units, and the status line (export.dat and
+    # its side files, IO.cpp:132-198, 835-938) every 2, kept as expected outputs
+    out = dict(fluidExpTime=4.0, partExpTime=4.0, recycleExpTime=4.0, screenExpTime=2.0)
     cases["io"] = dict(cfg=base_config(lbSizeX=14, lbSizeY=12, lbSizeZ=16, initVisc=0.05, lbFX=3.0e-4, lbFY=1.0e-4,
                                        lbFZ=-4.0e-4, initVelocityX=0.02, initVelocityY=0.01, youngMod=2.0e3, **walls,
                                        **out),
@@ -129,7 +130,8 @@ def _cases():
     cases["io_fs"] = dict(cfg=base_config(problemName="SPLASH", freeSurfaceSolve=1, nonNewtonianSolve=1, lbSizeX=12,
                                           lbSizeY=10, lbSizeZ=26, initVisc=0.05, plasticVisc=0.05, yieldStress=1.0e-5,
                                           initVelocityX=0.03, initVelocityZ=0.02, lbFX=5.0e-4, lbFZ=-5.0e-4,
-                                          boundary0=7, boundary1=7, boundary2=4, boundary3=4, fluidExpTime=4.0),
+                                          boundary0=7, boundary1=7, boundary2=4, boundary3=4, fluidExpTime=4.0,
+                                          screenExpTime=2.0),
                           bed=Bed(np.zeros((0, 3)), np.zeros(0)), cycles=[0, 10], keys=["types"], files=True)
     # static spheres of the object file (DEM.cpp:131-173): lattice walls (LB.cpp:535-550)
     # and contacts (DEM.cpp:1747-1770, 2060-2136); one object close to the periodic
@@ -197,7 +199,7 @@ def run_case(name: str, case: dict, outdir: str) -> None:
         if case.get("files"):  # the reference's output files, as bytes
             for dirpath, _, names in os.walk(work):
                 if os.path.basename(dirpath) not in ("fluidData", "particleData"):
-                    names = [n for n in names if n.endswith((".data", ".restart", ".fstat"))]
+                    names = [n for n in names if n.endswith((".data", ".restart", ".fstat", ".dat"))]
                 for fn in sorted(names):
                     with open(os.path.join(dirpath, fn), "rb") as fh:
                         arrays[f"file/{fn}"] = np.frombuffer(fh.read(), dtype=np.uint8)
