@@ -10,6 +10,7 @@ import ctypes as C
 
 HYB_PROBLEM_NONE = 0
 HYB_PROBLEM_SPLASH = 2
+HYB_PROBLEM_NET = 6
 HYB_PROBLEM_LIQUID_BOX = 100
 HYB_CONTACT_LINEAR = 0
 HYB_CONTACT_HERTZIAN = 1
@@ -20,7 +21,8 @@ HYB_FP32_SHIFTED = 1
 
 ELEMENT_RECORD = 20
 
-_PROBLEMS = {"NONE": HYB_PROBLEM_NONE, "SPLASH": HYB_PROBLEM_SPLASH, "LIQUID_BOX": HYB_PROBLEM_LIQUID_BOX}
+_PROBLEMS = {"NONE": HYB_PROBLEM_NONE, "SPLASH": HYB_PROBLEM_SPLASH, "NET": HYB_PROBLEM_NET,
+             "LIQUID_BOX": HYB_PROBLEM_LIQUID_BOX}
 
 
 class HybConfig(C.Structure):
@@ -63,6 +65,7 @@ class HybConfig(C.Structure):
         ("trtMagic", C.c_double),
         ("precision", C.c_int32),
         ("liquidBox", C.c_double * 6),
+        ("avalanchePosit", C.c_double),
         ("reserved", C.c_int32 * 8),
     ]
 
@@ -113,6 +116,7 @@ def config_from_cfg(cfg: dict, collision: str = "BGK", precision: str = "FP64", 
     c.precision = HYB_FP32_SHIFTED if precision.upper() in ("FP32", "FP32_SHIFTED") else HYB_FP64
     if liquid_box is not None:
         c.liquidBox[:] = [float(v) for v in liquid_box]
+    c.avalanchePosit = float(g("avalanchePosit", 0.0))
     return c
 
 

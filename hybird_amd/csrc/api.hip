@@ -130,6 +130,7 @@ struct hyb_dem {
     std::vector<WallDev> walls;
     std::vector<PbcDev> pbcs;
     std::vector<ObjDev> objects;  // object file entries, then periodic copies (set at init)
+    std::vector<CylDev> cylinders;  // problem cylinders (DEM.cpp:870-935)
     int stdObjects = 0;
     DEMParams p;
     DEMBuffers b;
@@ -166,6 +167,12 @@ struct hyb_lb {
     uint8_t* type = nullptr;
     uint32_t* link = nullptr;
     uint32_t* slipw = nullptr;  // slip-wall directions per node (only when a box face is a slip wall)
+    // cylinders (curved walls): curved-link directions per node, n and u of the last
+    // two collisions ([4][N] each, ping-pong: macBuf[macCur] holds the last one)
+    uint32_t* curvw = nullptr;
+    double* macBuf[2] = {nullptr, nullptr};
+    int macCur = 0;
+    CylDev cyl;
     uint32_t* solid = nullptr;
     void* hf = nullptr;
     double* visc = nullptr;
@@ -237,6 +244,14 @@ struct hyb_lb {
         p.linkw = link;
         p.actPrev = fsb.actPrev;
         p.actTag = actTag;
+        // curved walls: the pull reads the last collision's n and u, the collision
+        // writes its own into the other buffer
+        p.curvw = curvw;
+        p.macPrev = macBuf[macCur];
+        p.macNext = macBuf[macCur ^ 1];
+        p.typew = type;
+        p.viscw = varVisc() ? visc : nullptr;
+        p.cyl = cyl;
         return p;
     }
     // planes are padded so that successive planes do not start on the same
@@ -407,6 +422,23 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
             d->walls.push_back(w);
         }
     }
+    // cylinders (DEM.cpp:870-935): the NET problem's static pipe along x; DRUM and
+    // AVALANCHE (a rotating drum, a free-surface obstacle) are not supported
+    d->cylinders.clear();
+    if (c.problem == HYB_PROBLEM_NET) {
+        CylDev cy;
+        cy.p1 = V3(0.0, 4.0, 4.0);
+        cy.p2 = V3(1.0, 4.0, 4.0);
+        cy.R = 4.0;
+        cy.omega = V3();
+        cy.moving = 0;
+        cy.slip = 0;
+        const V3 axes = cy.p2 - cy.p1;  // initAxes (utils.cpp:154-161)
+        cy.naxes = axes / axes.norm();
+        d->cylinders.push_back(cy);
+    }
+    p.ncyl = (int)d->cylinders.size();
+    for (int k = 0; k < p.ncyl; ++k) p.cyl[k] = d->cylinders[k];
     // periodic pairs (DEM.cpp:937-988)
     d->pbcs.clear();
     for (int k = 0; k < 3; ++k) {
@@ -547,6 +579,7 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
         (rc = dalloc(&b.wcf, (size_t)std::max(P, 1))))
         return rc;
     HIPCHK(hipMemset(b.wcf, 0, sizeof(ContactForce) * (size_t)std::max(P, 1)));
+    if (p.ncyl && (rc = ralloc(R, "nearCyl", &b.nearCyl, (size_t)std::max(P, 1)))) return rc;
     if (p.nobj) {
         ObjDev* objs = nullptr;
         if ((rc = ralloc(R, "objs", &objs, (size_t)p.nobj)) ||
@@ -565,6 +598,7 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
     st.nSlots = P;
     HIPCHK(hipMemcpyAsync(b.st, &st, sizeof(st), hipMemcpyHostToDevice, d->stream));
     HIPCHK(hipMemsetAsync(b.nearWall, 0xFF, sizeof(uint32_t) * std::max(P, 1), d->stream));
+    if (b.nearCyl) HIPCHK(hipMemsetAsync(b.nearCyl, 0xFF, sizeof(uint32_t) * std::max(P, 1), d->stream));
     HIPCHK(hipMemsetAsync(b.csrOff, 0, sizeof(uint32_t) * ((size_t)P + 1), d->stream));
     HIPCHK(hipMemsetAsync(b.ghostOfCount, 0, sizeof(int) * std::max(P, 1), d->stream));
     if (P) {
@@ -646,7 +680,7 @@ int hyb_dem_destroy(hyb_dem* d) {
     dfree(b.cellTable); dfree(b.cellOf); dfree(b.cellCount); dfree(b.work); dfree(b.csrOff); dfree(b.csr);
     dfree(b.cursor); dfree(b.spos); dfree(b.ghostOfCount); dfree(b.ghostOf); dfree(b.side); dfree(b.ghostB);
     dfree(b.posI); dfree(b.posJ); dfree(b.prec); dfree(b.vmax); dfree(b.energy); dfree(b.nearObj); dfree(b.nearObjCount);
-    dfree(b.wcf); dfree(b.ocf);
+    dfree(b.wcf); dfree(b.ocf); dfree(b.nearCyl);
     if (b.objs) (void)hipFree((void*)b.objs);
     save_free(d);
     delete d;
@@ -695,7 +729,8 @@ int hyb_lb_create(const hyb_config* cfg, hyb_lb** out) {  // LB.cpp:85-188 + LB.
         if ((c.boundary[2 * k] == 4) != (c.boundary[2 * k + 1] == 4)) { delete l; return fail(HYB_E_ARG, "Periodic boundaries error!"); }
     l->slipCoefficient = c.slipCoefficient;
     l->turbConst = c.turbConst;
-    if (c.problem != HYB_PROBLEM_NONE && c.problem != HYB_PROBLEM_SPLASH && c.problem != HYB_PROBLEM_LIQUID_BOX) {
+    if (c.problem != HYB_PROBLEM_NONE && c.problem != HYB_PROBLEM_SPLASH && c.problem != HYB_PROBLEM_NET &&
+        c.problem != HYB_PROBLEM_LIQUID_BOX) {
         delete l;
         return fail(HYB_E_UNSUPPORTED, "problemName not supported");
     }
@@ -784,8 +819,25 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
     if (d && !d->inited) return fail(HYB_E_STATE, "hyb_lb_init: DEM handle not initialised");
     const int slotCap = d ? d->P + d->p.Gcap : 0;
     if (slotCap >= (int)SOLID_SLOT_MASK) return fail(HYB_E_CAPACITY, "too many particle slots for the ownership word");
+    const bool cylinders = d && !d->cylinders.empty();
     int rc = lb_alloc(l, d && d->P ? slotCap : 0);
     if (rc) return rc;
+    // the curved-wall rule tests the type of the node behind as it was when streamed
+    if (cylinders && l->freeSurface && !l->fsb.actPrev && (rc = ralloc(l->saved, "actPrev", &l->fsb.actPrev, l->N)))
+        return rc;
+    CylSet cs;
+    cs.n = 0;
+    if (cylinders) {
+        cs.n = (int)std::min<size_t>(d->cylinders.size(), 2);
+        for (int k = 0; k < cs.n; ++k) cs.c[k] = d->cylinders[k];
+        l->cyl = d->cylinders[0];  // the curved-wall distances use the first cylinder (LB.cpp:597)
+        if ((rc = ralloc(l->saved, "curvw", &l->curvw, l->N)) || (rc = ralloc(l->saved, "mac0", &l->macBuf[0], 4 * l->N)) ||
+            (rc = ralloc(l->saved, "mac1", &l->macBuf[1], 4 * l->N)))
+            return rc;
+        HIPCHK(hipMemset(l->macBuf[0], 0, 32 * l->N));
+        HIPCHK(hipMemset(l->macBuf[1], 0, 32 * l->N));
+        l->macCur = 0;
+    }
     const LBKParams p = l->params();
     hipStream_t s = l->stream;
     // node types: box boundaries, DEM plane walls, objects (LB.cpp:324-339), gas/interface
@@ -801,10 +853,12 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
         HIPCHK(hipMalloc((void**)&dob, objs.size() * sizeof(ObjDev)));
         HIPCHK(hipMemcpyAsync(dob, objs.data(), objs.size() * sizeof(ObjDev), hipMemcpyHostToDevice, s));
     }
-    launch_lb_init_types(l->type, l->boundary, (int)walls.size(), dw, (int)objs.size(), dob, l->cfg.problem,
-                         l->cfg.liquidBox, p, s);
+    const double region[7] = {l->cfg.liquidBox[0], l->cfg.liquidBox[1], l->cfg.liquidBox[2], l->cfg.liquidBox[3],
+                              l->cfg.liquidBox[4], l->cfg.liquidBox[5], l->cfg.avalanchePosit};
+    launch_lb_init_types(l->type, l->boundary, (int)walls.size(), dw, (int)objs.size(), dob, cs, l->cfg.problem,
+                         region, p, s);
     launch_lb_init_interface(l->type, p, s);
-    launch_lb_init_links(l->type, l->link, l->slipw, p, s);
+    launch_lb_init_links(l->type, l->link, l->slipw, l->curvw, p, s);
     HIPCHK(hipMemsetAsync(l->maxXYZ, 0, 3 * sizeof(int) * l->Z, s));
     launch_lb_max_active(l->link, l->maxXYZ, p, s);
     {
@@ -874,7 +928,7 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
 int hyb_lb_fs_step(hyb_lb* l) {  // LB.cpp:235-245
     if (!l || !l->inited) return fail(HYB_E_STATE, "hyb_lb_fs_step before init");
     if (!l->freeSurface) return HYB_OK;
-    if (++l->fsTag >= 0x80000000u) {  // tag wrap-around (31-bit tags): clear the tagged words once
+    if (++l->fsTag >= 0x40000000u) {  // tag wrap-around (30-bit tags): clear the tagged words once
         launch_fill_u32((uint32_t*)l->fsb.claim, 0u, 2 * l->N, l->stream);
         if (l->fsb.actPrev) launch_fill_u32(l->fsb.actPrev, 0u, l->N, l->stream);
         l->fsTag = 1;
@@ -944,9 +998,18 @@ int hyb_lb_step(hyb_lb* l, hyb_dem* d) {  // LB.cpp:221-233 fused into one kerne
     }
     HIPCHK(hipGetLastError());
     l->cur ^= 1;
+    if (l->curvw) l->macCur ^= 1;
     if (l->freeSurface) l->rhoCur ^= 1;
     l->actTag = 0;  // the next pull streams populations of this collision
     l->steps++;
+    // the streaming that the next pull completes adds the curved-wall rule's extra
+    // mass to the interface at the end of this step (LB.cpp:1315-1317, 1477)
+    if (l->curvw && l->freeSurface) {
+        const LBKParams q = l->params();
+        launch_fs_curved_extra(l->prec, l->f[l->cur], l->link, l->fsb, l->fsCur, l->fsb.rho[l->fluidMassBuf],
+                               l->fsSteps > 0, q, l->stream);
+        HIPCHK(hipGetLastError());
+    }
     l->macValid = l->capture;
     return HYB_OK;
 }
@@ -1002,6 +1065,7 @@ static int adj_check(hyb_lb* l) {
     if (l->freeSurface) return fail(HYB_E_UNSUPPORTED, "the adjoint does not differentiate the free surface");
     if (l->varVisc()) return fail(HYB_E_UNSUPPORTED, "the adjoint needs a constant viscosity");
     if (l->hasSlip()) return fail(HYB_E_UNSUPPORTED, "the adjoint supports periodic and no-slip boundaries only");
+    if (l->curvw) return fail(HYB_E_UNSUPPORTED, "the adjoint does not differentiate curved walls");
     return HYB_OK;
 }
 
@@ -1117,6 +1181,7 @@ int hyb_lb_destroy(hyb_lb* l) {
     if (l->hf) (void)hipFree(l->hf);
     if (l->scratch) (void)hipFree(l->scratch);
     dfree(l->type); dfree(l->link); dfree(l->slipw); dfree(l->solid); dfree(l->visc); dfree(l->mac); dfree(l->maxXYZ);
+    dfree(l->curvw); dfree(l->macBuf[0]); dfree(l->macBuf[1]);
     dfree(l->dcount); dfree(l->cp); dfree(l->cpf); dfree(l->boxes); dfree(l->nslots); dfree(l->part);
     for (void* ck : l->adjCk) (void)hipFree(ck);
     if (l->ffield) (void)hipFree(l->ffield);
@@ -1168,7 +1233,10 @@ static int ensure_mac(hyb_lb* l, hyb_dem* d) {
     if (l->macValid) return HYB_OK;
     int rc;
     if (!l->mac && (rc = ralloc(l->saved, "mac", &l->mac, 7 * l->N))) return rc;
-    const LBKParams p = l->params();
+    LBKParams p = l->params();
+    // the recomputed collision pulls with the n, u of the collision before it
+    p.macPrev = l->macBuf[l->macCur ^ 1];
+    p.macNext = nullptr;
     if (l->steps == 0) {
         // state after init: recompute the hydrostatic values the init kernel used
         std::vector<double> h(7 * l->N, NAN);

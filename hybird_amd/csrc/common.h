@@ -69,6 +69,7 @@ __host__ __device__ constexpr bool is_slip_wall(uint8_t t) { return t == 5 || t 
 constexpr uint32_t LINK_ACTIVE = 1u << 31;
 constexpr uint32_t LINK_INTERFACE = 1u << 30;
 constexpr uint32_t LINK_PRESTREAMED = 1u << 29;
+constexpr uint32_t LINK_CURVED = 1u << 28;  // some wall links end on curved (cylinder) wall nodes
 constexpr uint32_t LINK_SLIP = 1u;
 constexpr uint32_t LINK_WALLS = 0x7FFFEu;
 
@@ -158,6 +159,63 @@ struct ObjDev {
 struct PbcDev {
     V3 v;              // period vector
     V3 p1, n1, p2, n2; // the two planes
+};
+
+// Infinite cylinder through p1 and p2 with radius R, fluid inside
+// (utils.h:84-130, utils.cpp:56-232); physical units. The functions keep the
+// reference's operation order.
+struct CylDev {
+    V3 p1, p2, naxes, omega;
+    double R;
+    int moving, slip;
+    // R minus the distance of pt from the axis (utils.cpp:56-67)
+    __host__ __device__ double dist(const V3& pt) const {
+        const V3 p1dist = pt - p1;
+        const V3 p1distax = (p1dist.dot(naxes)) * naxes;
+        const V3 p1distcylinder = p1distax - p1dist;
+        return R - p1distcylinder.norm();
+    }
+    // pt minus its nearest point on the surface (utils.cpp:69-82)
+    __host__ __device__ V3 vecDist(const V3& pt) const {
+        const V3 pt1 = pt - p1;
+        const V3 projectedPoint = p1 + (naxes.dot(pt1)) * naxes;
+        const V3 distFromAxes = pt - projectedPoint;
+        return pt - (projectedPoint + distFromAxes / distFromAxes.norm() * R);
+    }
+    // surface velocity of a rotating cylinder at pt (utils.cpp:106-120)
+    __host__ __device__ V3 getSpeed(const V3& pt) const {
+        if (moving) {
+            const V3 pt1 = pt - p1;
+            const V3 projectedPoint = p1 + (naxes.dot(pt1)) * naxes;
+            const V3 distFromAxes = pt - projectedPoint;
+            return omega.cross(distFromAxes);
+        }
+        return V3(0.0, 0.0, 0.0);
+    }
+    // lattice position inside, with axis point and radius in lattice units (vector.cpp:160-170)
+    __host__ __device__ static bool inside(const V3& pos, const V3& p1L, const V3& naxes, double RL) {
+        const V3 p1dist = pos - p1L;
+        const V3 p1distax = (p1dist.dot(naxes)) * naxes;
+        const V3 p1distcylinder = p1dist - p1distax;
+        return p1distcylinder.norm2() < RL * RL;
+    }
+    // time at which start + t dir meets the surface, 0 when the quadratic has no
+    // admissible root (utils.cpp:177-232; both roots use the minus sign there)
+    __host__ __device__ double segmentIntercept(const V3& start, const V3& dir) const {
+        const V3 AB = p2 - p1;
+        const V3 AO = start - p1;
+        const V3 AOxAB = AO.cross(AB);
+        const V3 VxAB = dir.cross(AB);
+        const double ab2 = AB.dot(AB);
+        const double a = VxAB.dot(VxAB);
+        const double b = 2 * VxAB.dot(AOxAB);
+        const double c = AOxAB.dot(AOxAB) - (R * R * ab2);
+        const double delta = b * b - 4 * a * c;
+        if (delta < 0) return 0.0;
+        const double t = (-b - sqrt(delta)) / (2 * a);
+        if (t < 0) return 0.0;
+        return t;
+    }
 };
 
 }  // namespace hyb

@@ -352,6 +352,13 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
             if (p.walls[k].dist(x0) < p.nebrRange) { wi = (uint32_t)k; break; }
         b.nearWall[e] = wi;
         nw += wi != 0xFFFFFFFFu;
+        // near-cylinder list (DEM.cpp:1535-1550): the first cylinder within range
+        if (p.ncyl) {
+            uint32_t ci = 0xFFFFFFFFu;
+            for (int k = 0; k < p.ncyl; ++k)
+                if (p.cyl[k].dist(x0) < p.nebrRange) { ci = (uint32_t)k; break; }
+            b.nearCyl[e] = ci;
+        }
     }
     red[t] = (double)nw;
     // ---- near-object lists (DEM.cpp:1515-1532): every object within range, in list order
@@ -763,6 +770,39 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
             }
             if (b.ostat) b.ostat[(size_t)e * NEAR_OBJ + k] = ost;
             b.ocf[(size_t)e * NEAR_OBJ + k] = ocf;
+        }
+    }
+    // cylinder contact (DEM.cpp:1719-1744, 1984-2058), after the objects
+    if (p.ncyl) {
+        const uint32_t ci = b.nearCyl[e];
+        if (ci != 0xFFFFFFFFu) {
+            const CylDev& cyl = p.cyl[ci];
+            const double distance = cyl.dist(xpe);
+            const double overlap = re - distance;
+            if (overlap > 0.0) {
+                const V3 vecDistance = cyl.vecDist(xpe);
+                const V3 contactPoint = xpe - vecDistance;
+                const V3 en = vecDistance / (re - overlap);
+                const V3 contactPointVelocity = cyl.getSpeed(contactPoint);
+                const V3 relVel = vpe - contactPointVelocity;
+                const double normRelVel = relVel.dot(en);
+                const V3 normalRelVel = en * normRelVel;
+                const double fn = normal_contact(p, 2.0 * overlap, normRelVel, re, me);
+                const V3 normalForce = en * fn;
+                const V3 vecRadJ = -re * en;
+                const V3 centerDistJ = vecRadJ + (xpe - xpe);  // particle = element centre (single spheres)
+                FW = FW + normalForce;
+                const V3 relVelContact = relVel + wpg.cross(vecRadJ);
+                const V3 tangRelVelContact = relVelContact - normalRelVel;
+                const double nt = tangRelVelContact.norm();
+                if (nt != 0.0) {
+                    const double ft = tangential_contact(p, nt, fn, re, me, p.fricWall);
+                    const V3 et = tangRelVelContact / tangRelVelContact.norm();
+                    const V3 tangForce = ft * et;
+                    MW = MW - centerDistJ.cross(tangForce);
+                    FW = FW - tangForce;
+                }
+            }
         }
     }
     // Newton-Euler

@@ -44,6 +44,8 @@ struct DEMParams {
     int hertzian;
     double numVisc;
     double demF[3];
+    int ncyl;              // cylinders (DEM.cpp:870-935)
+    CylDev cyl[2];
 };
 
 // Objects a particle can have in range at once (a rebuild that finds more
@@ -90,6 +92,7 @@ struct DEMBuffers {
     const ObjDev* objs;    // [nobj]
     uint32_t* nearObj;     // [P][NEAR_OBJ] objects within nebrRange of each particle, ascending
     int* nearObjCount;     // [P]
+    uint32_t* nearCyl;     // [P] first cylinder within nebrRange or 0xFFFFFFFF
     // forces of the last sub-step's wall and object contacts (for walls[].FParticle
     // and objects[].FParticle, summed on request in the reference's order)
     struct ContactForce* wcf;  // [P] contact of each element with its near wall

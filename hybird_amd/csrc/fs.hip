@@ -76,12 +76,12 @@ __global__ void __launch_bounds__(FS_BLOCK)
         for (int j = 0; j < 19; ++j) fs[j] = fpost[(size_t)j * p.PS + idx];
         const double mi = mass[idx];
         const V3 u(uS[idx], uS[N + idx], uS[2 * N + idx]);
-        if (p.actPrev) {  // activity before this pass of the nodes it may change
+        if (p.actPrev) {  // type class before this pass of the nodes it may change
             uint32_t* ap = const_cast<uint32_t*>(p.actPrev);
-            ap[idx] = (p.actTag << 1) | 1u;
+            ap[idx] = (p.actTag << 2) | 2u;
             for (int j = 1; j < 19; ++j) {
                 const size_t L = nb_index(p, c, j);
-                if (type[L] == T_GAS) ap[L] = p.actTag << 1;
+                if (type[L] == T_GAS) ap[L] = p.actTag << 2;
             }
         }
         const double usq = u.norm2();
@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(FS_BLOCK)
                 v = fs[o];  // before the first collision f = fs
                 if (tl == T_INTERFACE) a = 0.5 * (mass[L] + mi);
                 else if (tl == T_FLUID) a = 1.0;
-                else if (tl == T_WALL_MOVING) a = 1.0 * mi;
+                else if (tl == T_WALL_MOVING || tl == T_CURVED) a = 1.0 * mi;
                 else if (tl == T_SLIP_MOVING) {
                     const auto active = [&](size_t n) { const uint8_t t = type[n]; return t == T_FLUID || t == T_INTERFACE; };
                     bool half = false;
@@ -124,7 +124,10 @@ __global__ void __launch_bounds__(FS_BLOCK)
                 v = fs[j];
                 const auto active = [&](size_t n) { const uint8_t t = type[n]; return t == T_FLUID || t == T_INTERFACE; };
                 if (is_slip_wall(tl)) v = wall_link<T>(fpost, p.PS, p, c, true, j, fs[j], active);
-                if (tl == T_WALL_MOVING) {
+                if (tl == T_CURVED) {  // Mei-Luo-Shyy (LB.cpp:1278-1319), a = m (LB.cpp:1539-1545)
+                    v = curved_link<T, SHIFT>(p, c, j, fs[j]);
+                    a = 1.0 * mi;
+                } else if (tl == T_WALL_MOVING) {
                     a = 1.0 * mi;
                 } else if (tl == T_SLIP_MOVING) {
                     bool half = false;
@@ -202,7 +205,7 @@ __global__ void __launch_bounds__(FS_BLOCK)
                 const unsigned long long old = atomicMax(&claim[L], tagv | 1ull);
                 if (old >= tagv) continue;
                 listB[atomicAdd(&ctr->nB, 1u)] = (uint32_t)L;
-                if (p.actPrev) const_cast<uint32_t*>(p.actPrev)[L] = (p.actTag << 1) | 1u;  // was fluid
+                if (p.actPrev) const_cast<uint32_t*>(p.actPrev)[L] = (p.actTag << 2) | 1u;  // was fluid
                 type[L] = T_INTERFACE;
                 link[L] |= LINK_INTERFACE;
                 const double n = to_d(rho[L]);
@@ -244,14 +247,17 @@ __global__ void __launch_bounds__(FS_BLOCK)
                 fout[(size_t)j * p.PS + L] = SHIFT ? (T)(feq - wgt(j)) : (T)feq;
             }
             const Coord c = coord_of(p, L);
-            uint32_t lw = LINK_ACTIVE | LINK_INTERFACE | LINK_PRESTREAMED, sw = 0;
+            uint32_t lw = LINK_ACTIVE | LINK_INTERFACE | LINK_PRESTREAMED, sw = 0, cw = 0;
             for (int j = 1; j < 19; ++j) {
                 const uint8_t tl = type[nb_index(p, c, j)];
-                if (is_plane_wall(tl)) lw |= 1u << j;
+                if (is_plane_wall(tl) || tl == T_CURVED) lw |= 1u << j;
                 if (is_slip_wall(tl)) sw |= 1u << j;
+                if (tl == T_CURVED) cw |= 1u << j;
             }
             if (sw) lw |= LINK_SLIP;
+            if (cw) lw |= LINK_CURVED;
             if (slipw) slipw[L] = sw;
+            if (p.curvw) const_cast<uint32_t*>(p.curvw)[L] = cw;
             link[L] = lw;
             type[L] = T_INTERFACE;
             mass[L] = 0.01 * rho0;
@@ -395,7 +401,9 @@ __global__ void k_fs_stream_view(const T* __restrict__ fin, T* __restrict__ fout
                 v = SHIFT ? (T)(-fsj + (g - 2.0 * wgt(j))) : (T)(-fsj + g);
             } else {
                 v = fin[(size_t)j * p.PS + idx];
-                if (is_slip_wall(tl))
+                if (tl == T_CURVED)
+                    v = curved_link<T, SHIFT>(p, c, j, v);
+                else if (is_slip_wall(tl))
                     v = wall_link<T>(fin, p.PS, p, c, true, j, v,
                                      [&](size_t n) { const uint8_t t = type[n]; return t == T_FLUID || t == T_INTERFACE; });
             }
@@ -405,6 +413,63 @@ __global__ void k_fs_stream_view(const T* __restrict__ fin, T* __restrict__ fout
 }
 
 }  // namespace
+
+// Extra mass of the curved-wall rule in the streaming of the last LB step
+// (LB.cpp:1315-1317): every active node adds m (chi fs_j - chi fStar + BBi) per
+// curved link, m its mass (interface mass, or the fluid node's last density).
+// Summed exactly into ctr->extra.
+template <typename T, bool SHIFT>
+__global__ void __launch_bounds__(FS_BLOCK) k_fs_curved_extra(const T* __restrict__ fpost,
+                                                              const uint32_t* __restrict__ link,
+                                                              const double* __restrict__ mass,
+                                                              const T* __restrict__ rhoFluid, FSCounters* __restrict__ ctr,
+                                                              double rho0, int massKnown, LBKParams p) {
+#pragma clang fp contract(off)
+    const size_t N = (size_t)p.X * p.Y * p.Z;
+    FS_FOR(i, (uint32_t)N) {
+        const uint32_t lw = link[i];
+        if (!(lw & LINK_ACTIVE) || !(lw & LINK_CURVED)) continue;
+        const uint32_t cw = p.curvw[i];
+        const Coord c = coord_of(p, i);
+        double m;
+        if (lw & LINK_INTERFACE) m = mass[i];
+        else m = massKnown ? to_d(rhoFluid[i]) : rho0;
+        for (int j = 1; j < 19; ++j) {
+            if (!(cw & (1u << j))) continue;
+            const CurvedParts q = curved_parts(p, c, j);
+            const double fs = SHIFT ? to_d(fpost[(size_t)j * p.PS + i]) + wgt(j) : to_d(fpost[(size_t)j * p.PS + i]);
+            exact_add(ctr->extra, m * (q.chi * fs - q.chi * q.fStar + q.BBi));
+        }
+    }
+}
+
+// redistributeMass(extraMass) at the end of the streaming (LB.cpp:1477,
+// 1796-1804): every interface node of the current list gains extra / count.
+// One workgroup; the accumulator is cleared for the next step.
+__global__ void __launch_bounds__(FS_BLOCK) k_fs_add_extra(const uint32_t* __restrict__ list,
+                                                           FSCounters* __restrict__ ctr, int cur,
+                                                           double* __restrict__ mass) {
+#pragma clang fp contract(off)
+    const uint32_t nI = ctr->n[cur];
+    const double add = exact_value(ctr->extra) / (double)nI;
+    for (uint32_t pos = threadIdx.x; pos < nI; pos += blockDim.x) mass[list[pos]] += add;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 0; i < 4; ++i) ctr->extra[i] = 0ull;
+}
+
+void launch_fs_curved_extra(Prec prec, const void* fpost, const uint32_t* link, FSBuffers& b, int cur,
+                            const void* rhoFluid, bool massKnown, const LBKParams& p, hipStream_t s) {
+    if (prec == Prec::F64)
+        k_fs_curved_extra<double, false><<<FS_GRID, FS_BLOCK, 0, s>>>((const double*)fpost, link, b.mass,
+                                                                      (const double*)rhoFluid, b.ctr, p.initDensity,
+                                                                      massKnown ? 1 : 0, p);
+    else
+        k_fs_curved_extra<float, true><<<FS_GRID, FS_BLOCK, 0, s>>>((const float*)fpost, link, b.mass,
+                                                                    (const float*)rhoFluid, b.ctr, p.initDensity,
+                                                                    massKnown ? 1 : 0, p);
+    k_fs_add_extra<<<1, FS_BLOCK, 0, s>>>(b.list[cur], b.ctr, cur, b.mass);
+}
 
 void launch_fs_build_list(const uint8_t* type, FSBuffers& b, int cur, const LBKParams& p, hipStream_t s) {
     const size_t N = (size_t)p.X * p.Y * p.Z;

@@ -15,6 +15,7 @@ struct FSCounters {
     uint32_t n[2];              // interface list lengths (current / next, ping-pong)
     uint32_t nA, nB;            // gas -> interface and fluid -> interface lists of this step
     unsigned long long acc[4];  // exact fixed-point mass-surplus accumulator (32-bit limbs, LSB 2^-96)
+    unsigned long long extra[4];  // extra mass of the curved-wall rule in the last streaming (same format)
 };
 
 // Device state of the free-surface mode. Densities use the population type
@@ -31,9 +32,11 @@ struct FSBuffers {
     double* mnew = nullptr;                // [N] new mass per interface-list position
     uint8_t* state = nullptr;              // [N] filled / emptied / unchanged per interface-list position
     FSCounters* ctr = nullptr;
-    // [N] (step tag << 1) | active, written for every node whose activity the
-    // pass may change (only allocated with slip walls): the slip rule of the next
-    // collision's streaming tests the neighbours' activity before the pass
+    // [N] (step tag << 2) | class (0 inactive, 1 fluid, 2 interface), written for
+    // every node whose type the pass may change (only allocated with slip walls or
+    // cylinders): the slip rule of the next collision's streaming tests the
+    // neighbours' activity before the pass, the curved-wall rule whether the node
+    // behind was fluid
     uint32_t* actPrev = nullptr;
 };
 
@@ -49,6 +52,11 @@ struct FSView {
 };
 
 // Builds the interface list of the initial lattice.
+// Before a free-surface pass with cylinders: the curved-wall rule's extra mass of
+// the last streaming, spread over the interface list `cur` (fpost: post-collision
+// state; rhoFluid: fluid-node masses, or initDensity when !massKnown).
+void launch_fs_curved_extra(Prec prec, const void* fpost, const uint32_t* link, FSBuffers& b, int cur,
+                            const void* rhoFluid, bool massKnown, const LBKParams& p, hipStream_t s);
 void launch_fs_build_list(const uint8_t* type, FSBuffers& b, int cur, const LBKParams& p, hipStream_t s);
 // One free-surface step (updateMass + updateInterface, LB.cpp:1492-1804).
 // cur: which interface list is current; rcur: which density buffer holds the

@@ -108,6 +108,77 @@ __device__ __forceinline__ T wall_link(const T* __restrict__ fs, size_t PS, cons
     return own;
 }
 
+// Population f_opp(j) streamed back into a node from a curved wall node w in
+// direction j (Mei-Luo-Shyy, LB.cpp:1278-1319, node.cpp:458-472): with delta the
+// wall distance fraction of w towards the node (1 - the segment intercept from w,
+// LB.cpp:589-602; 0 for w in a boundary layer), chi = (2 delta - 1) / tau (delta >= 1/2) or / (tau - 2), the
+// reference velocity ubf = m1 u + m2 uw (delta >= 1/2) or the velocity of the
+// fluid node behind (else; uw when there is none), fStar = n w_j (1 + 3 c_j.ubf +
+// 4.5 (u.c_j)^2 + 1.5 u^2) and f = (1 - chi) fs_j + chi fStar - BBi. The cylinder
+// is static: uw = 0, BBi = 0. n, u, visc: the node's own from its last collision.
+// SHIFT: populations hold f - w (rho0 = 1).
+struct CurvedParts {
+    double chi, fStar, BBi;
+};
+
+// Whether node n was a fluid node (not interface) when the reference streamed the
+// populations being pulled: a free-surface pass in between records the type class
+// of the nodes it may change as (pass tag << 2) | class (0 inactive, 1 fluid,
+// 2 interface).
+__device__ __forceinline__ bool fluid_at_stream(const LBKParams& p, size_t n) {
+    if (p.actPrev && p.actTag) {
+        const uint32_t w = p.actPrev[n];
+        if ((w >> 2) == p.actTag) return (w & 3u) == 1u;
+    }
+    return p.typew[n] == T_FLUID;
+}
+
+__device__ __forceinline__ CurvedParts curved_parts(const LBKParams& p, const Coord& c, int j) {
+#pragma clang fp contract(off)
+    const size_t N = (size_t)p.X * p.Y * p.Z;
+    const int i = opp(j);
+    const size_t w = nb_index(p, c, j), back = nb_index(p, c, i);
+    const int wx = (int)(w % (size_t)p.X), wy = (int)((w / (size_t)p.X) % (size_t)p.Y), wz = (int)(w / ((size_t)p.X * p.Y));
+    const double L = p.unitLength;
+    const V3 vi((double)cx(i), (double)cy(i), (double)cz(i)), vj((double)cx(j), (double)cy(j), (double)cz(j));
+    // boundary-layer nodes have themselves as every neighbour (LB.cpp:394-431), a
+    // wall, so the reference never computes their distances: delta stays 0
+    const bool layer = wx == 0 || wy == 0 || wz == 0 || wx == p.X - 1 || wy == p.Y - 1 || wz == p.Z - 1;
+    const double delta =
+        layer ? 0.0 : 1.0 - p.cyl.segmentIntercept(L * V3((double)wx, (double)wy, (double)wz), L * vi);
+    const double n = p.macPrev[c.idx];
+    const V3 u(p.macPrev[N + c.idx], p.macPrev[2 * N + c.idx], p.macPrev[3 * N + c.idx]);
+    const double visc = p.viscw ? p.viscw[c.idx] : p.visc0;
+    const double tau = 0.5 + 3.0 * visc;
+    const double chi = delta >= 0.5 ? (2.0 * delta - 1.0) / tau : (2.0 * delta - 1.0) / (tau - 2.0);
+    const V3 vel(0.0, 0.0, 0.0);
+    V3 ubf(0.0, 0.0, 0.0);
+    if (delta >= 0.5) {
+        const double m1 = (delta - 1) / delta, m2 = 1 / delta;
+        ubf = m1 * u + m2 * vel;
+    } else if (fluid_at_stream(p, back)) {
+        ubf = V3(p.macPrev[N + back], p.macPrev[2 * N + back], p.macPrev[3 * N + back]);
+    } else {
+        ubf = vel;
+    }
+    const double usq = u.norm2();
+    const double vu = u.dot(vj);
+    const double wj = wgt(j);
+    const double fStar = n * wj * (1 + 3.0 * vj.dot(ubf) + 4.5 * vu * vu + 1.5 * usq);
+    const double BBi = 6.0 * n * wj * vel.dot(vj);
+    return CurvedParts{chi, fStar, BBi};
+}
+
+template <typename T, bool SHIFT>
+__device__ __forceinline__ T curved_link(const LBKParams& p, const Coord& c, int j, T own) {
+#pragma clang fp contract(off)
+    const CurvedParts q = curved_parts(p, c, j);
+    const double wj = wgt(j);
+    const double fs = SHIFT ? (double)own + wj : (double)own;
+    const double f = (1.0 - q.chi) * fs + q.chi * q.fStar - q.BBi;
+    return (T)(SHIFT ? f - wj : f);
+}
+
 __device__ __forceinline__ bool interior(const LBKParams& p, const Coord& c) {
     return c.x > 0 && c.x < p.X - 1 && c.y > 0 && c.y < p.Y - 1 && c.z > 0 && c.z < p.Z - 1;
 }

@@ -76,7 +76,7 @@ __device__ __forceinline__ bool node_coord(const LBKParams& p, Coord& c) {
 __device__ __forceinline__ bool active_at_stream(const LBKParams& p, size_t n) {
     if (p.actPrev && p.actTag) {
         const uint32_t w = p.actPrev[n];
-        if ((w >> 1) == p.actTag) return (w & 1u) != 0u;
+        if ((w >> 2) == p.actTag) return (w & 3u) != 0u;
     }
     return (p.linkw[n] & LINK_ACTIVE) != 0u;
 }
@@ -86,7 +86,7 @@ __device__ __forceinline__ bool active_at_stream(const LBKParams& p, size_t n) {
 // walls bounce back the node's own f_opp(i) instead; they only occur next to
 // walls, so that fix-up runs as a separate, rarely taken branch and the common
 // path issues its 19 loads back to back.
-template <typename T>
+template <typename T, bool CURVED = false>
 __device__ __forceinline__ void pull19(const T* __restrict__ fin, size_t PS, const LBKParams& p, const Coord& c,
                                        uint32_t lw, T f[19]) {
     const Nb3 nx = nb3(c.x, p.X, p.per[0]);
@@ -103,12 +103,15 @@ __device__ __forceinline__ void pull19(const T* __restrict__ fin, size_t PS, con
     }
     if (lw & LINK_WALLS) {
         const uint32_t sw = (lw & LINK_SLIP) ? p.slipw[self] : 0u;
+        const uint32_t cw = (CURVED && (lw & LINK_CURVED)) ? p.curvw[self] : 0u;
 #pragma unroll
         for (int i = 1; i < 19; ++i) {
             const int j = opp(i);
             if (!(lw & (1u << j))) continue;
             const T own = ld_pop(&fin[(size_t)j * PS + self]);
-            if (sw & (1u << j))
+            if (CURVED && (cw & (1u << j)))
+                f[i] = curved_link<T, sizeof(T) == 4>(p, c, j, own);
+            else if (sw & (1u << j))
                 f[i] = wall_link<T>(fin, PS, p, c, true, j, own, [&](size_t n) { return active_at_stream(p, n); });
             else
                 f[i] = own;
@@ -219,7 +222,7 @@ __device__ __forceinline__ void load_prepared(const FSView& fsv, size_t PS, size
     fsv.linkw[idx] = lw & ~LINK_PRESTREAMED;
 }
 
-template <bool PULL, bool VARVISC, int COLL, bool FS>
+template <bool PULL, bool VARVISC, int COLL, bool FS, bool CURVED = false>
 __global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ fin, double* __restrict__ fout,
                                                      const uint32_t* __restrict__ link,
                                                      const uint32_t* __restrict__ solid,
@@ -234,7 +237,7 @@ __global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ 
     if (!(lw & LINK_ACTIVE)) return;
     double f[19];
     if (FS && (lw & LINK_PRESTREAMED)) load_prepared<double>(fsv, p.PS, c.idx, lw, f);
-    else if (PULL) pull19<double>(fin, p.PS, p, c, lw, f);
+    else if (PULL) pull19<double, CURVED>(fin, p.PS, p, c, lw, f);
     else {
 #pragma unroll
         for (int j = 0; j < 19; ++j) f[j] = ld_pop(&fin[(size_t)j * p.PS + c.idx]);
@@ -255,6 +258,12 @@ __global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ 
     if (p.ffield) Fb = Fb + V3((double)p.ffield[c.idx], (double)p.ffield[N + c.idx], (double)p.ffield[2 * N + c.idx]);
     const V3 T = Fb + hF;
     u += 0.5 * T / n;  // node.cpp:85-88
+    if (CURVED && p.macNext) {
+        p.macNext[c.idx] = n;
+        p.macNext[N + c.idx] = u.x;
+        p.macNext[2 * N + c.idx] = u.y;
+        p.macNext[3 * N + c.idx] = u.z;
+    }
     if (mac) {
         mac[c.idx] = n;
         mac[N + c.idx] = u.x;
@@ -347,7 +356,7 @@ __device__ __forceinline__ float cdotf(int j, float ux, float uy, float uz) {
 // outputs (node force, macroscopic fields, viscosity, free-surface state) and,
 // when out is given, the post-collision populations into out[]; returns whether
 // out was written.
-template <bool VARVISC, int COLL, bool FS>
+template <bool VARVISC, int COLL, bool FS, bool CURVED = false>
 __device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uint32_t lw, size_t N,
                                           const uint32_t* __restrict__ solid, const CoupleRecF* __restrict__ cp,
                                           NodeForce32* __restrict__ hf, double* __restrict__ visc, double* __restrict__ mac,
@@ -393,6 +402,12 @@ __device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uin
     ux += 0.5f * Tx * irho;
     uy += 0.5f * Ty * irho;
     uz += 0.5f * Tz * irho;
+    if (CURVED && p.macNext) {
+        p.macNext[c.idx] = (double)rho;
+        p.macNext[N + c.idx] = ux;
+        p.macNext[2 * N + c.idx] = uy;
+        p.macNext[3 * N + c.idx] = uz;
+    }
     if (mac) {
         mac[c.idx] = (double)rho;
         mac[N + c.idx] = ux;
@@ -476,7 +491,7 @@ __device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uin
 }
 
 
-template <bool PULL, bool VARVISC, int COLL, bool FS>
+template <bool PULL, bool VARVISC, int COLL, bool FS, bool CURVED = false>
 __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ fin, float* __restrict__ fout,
                                                      const uint32_t* __restrict__ link,
                                                      const uint32_t* __restrict__ solid,
@@ -490,20 +505,21 @@ __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ f
     if (!(lw & LINK_ACTIVE)) return;
     float g[19];
     if (FS && (lw & LINK_PRESTREAMED)) load_prepared<float>(fsv, p.PS, c.idx, lw, g);
-    else if (PULL) pull19<float>(fin, p.PS, p, c, lw, g);
+    else if (PULL) pull19<float, CURVED>(fin, p.PS, p, c, lw, g);
     else {
 #pragma unroll
         for (int j = 0; j < 19; ++j) g[j] = ld_pop(&fin[(size_t)j * p.PS + c.idx]);
     }
     float out[19];
-    if (!collide32<VARVISC, COLL, FS>(g, c, lw, N, solid, cp, hf, visc, mac, p, fsv, fout != nullptr, out)) return;
+    if (!collide32<VARVISC, COLL, FS, CURVED>(g, c, lw, N, solid, cp, hf, visc, mac, p, fsv, fout != nullptr, out))
+        return;
 #pragma unroll
     for (int j = 0; j < 19; ++j) st_pop(&fout[(size_t)j * p.PS + c.idx], out[j]);
 }
 
 // Standalone pull streaming: turns the post-collision state into the
 // reference's step-end (post-stream) populations, for host views only.
-template <typename T>
+template <typename T, bool CURVED = false>
 __global__ void __launch_bounds__(BX * BY) k_stream(const T* __restrict__ fin, T* __restrict__ fout,
                                                     const uint32_t* __restrict__ link, LBKParams p) {
     Coord c;
@@ -512,7 +528,7 @@ __global__ void __launch_bounds__(BX * BY) k_stream(const T* __restrict__ fin, T
     const uint32_t lw = link[c.idx];
     if (!(lw & LINK_ACTIVE)) return;
     T f[19];
-    pull19<T>(fin, p.PS, p, c, lw, f);
+    pull19<T, CURVED>(fin, p.PS, p, c, lw, f);
 #pragma unroll
     for (int j = 0; j < 19; ++j) fout[(size_t)j * p.PS + c.idx] = f[j];
 }
@@ -522,7 +538,8 @@ __global__ void __launch_bounds__(BX * BY) k_stream(const T* __restrict__ fin, T
 // problem's gas region (LB.cpp:611-784: SPLASH plane, or a liquid box).
 __global__ void k_init_types(uint8_t* __restrict__ type, LBKParams p, int b0, int b1, int b2, int b3, int b4, int b5,
                              int nwalls, const WallDev* __restrict__ walls, int nobj, const ObjDev* __restrict__ objs,
-                             int problem, double lx0, double lx1, double ly0, double ly1, double lz0, double lz1) {
+                             CylSet cs, int problem, double lx0, double lx1, double ly0, double ly1, double lz0,
+                             double lz1, double avalanchePosit) {
     Coord c;
     if (!node_coord(p, c)) return;
     uint8_t t = T_FLUID;
@@ -541,6 +558,9 @@ __global__ void k_init_types(uint8_t* __restrict__ type, LBKParams p, int b0, in
             else t = walls[w].moving ? 8 : 7;
         }
     }
+    // nodes outside a cylinder are curved walls (LB.cpp:552-587)
+    for (int k = 0; k < cs.n; ++k)
+        if (!CylDev::inside(pos, cs.c[k].p1 / p.unitLength, cs.c[k].naxes, cs.c[k].R / p.unitLength)) t = T_CURVED;
     // objects are static walls (LB.cpp:535-550)
     for (int o = 0; o < nobj; ++o) {
         const V3 d = pos - objs[o].x0 / p.unitLength;
@@ -552,6 +572,17 @@ __global__ void k_init_types(uint8_t* __restrict__ type, LBKParams p, int b0, in
             if ((pos - V3(0.0, 0.0, 20.0)).dot(V3(0.0, 0.0, 1.0)) > 0.0) t = T_GAS;
         } else if (problem == HYB_PROBLEM_LIQUID_BOX) {
             if (pos.x < lx0 || pos.x > lx1 || pos.y < ly0 || pos.y > ly1 || pos.z < lz0 || pos.z > lz1) t = T_GAS;
+        } else if (problem == HYB_PROBLEM_NET) {
+            // box before the net, then the plane through (xMin, 0, 0) with normal
+            // (-1, 0, 1) (LB.cpp:662-690); the bounds are unsigned lattice indices
+            const unsigned xMin = (unsigned)int(avalanchePosit / p.unitLength), xMax = (unsigned)p.X;
+            const unsigned yMin = 0, yMax = (unsigned)p.Y, zMin = 0, zMax = (unsigned)int(4.0 / p.unitLength);
+            const unsigned gx = (unsigned)c.x, gy = (unsigned)c.y, gz = (unsigned)c.z;
+            if (gx < xMin || gx > xMax || gy < yMin || gy > yMax || gz < zMin || gz > zMax) t = T_GAS;
+            if (t == T_FLUID) {
+                const V3 pp((double)int(avalanchePosit / p.unitLength), 0.0, 0.0), nn(-1.0, 0.0, 1.0);
+                if ((pos - pp).dot(nn) > 0.0) t = T_GAS;
+            }
         }
     }
     type[c.idx] = t;
@@ -586,23 +617,26 @@ __global__ void k_remove_isolated(const uint8_t* __restrict__ tin, uint8_t* __re
 }
 
 __global__ void k_init_links(const uint8_t* __restrict__ type, uint32_t* __restrict__ link, uint32_t* __restrict__ slipw,
-                             LBKParams p) {
+                             uint32_t* __restrict__ curvw, LBKParams p) {
     Coord c;
     if (!node_coord(p, c)) return;
     const uint8_t t = type[c.idx];
-    uint32_t lw = 0, sw = 0;
+    uint32_t lw = 0, sw = 0, cw = 0;
     if ((t == T_FLUID || t == T_INTERFACE) && interior(p, c)) {
         lw |= LINK_ACTIVE;
         if (t == T_INTERFACE) lw |= LINK_INTERFACE;
         for (int j = 1; j < 19; ++j) {
             const uint8_t tl = type[nb_index(p, c, j)];
-            if (is_plane_wall(tl)) lw |= (1u << j);
+            if (is_plane_wall(tl) || tl == T_CURVED) lw |= (1u << j);
             if (is_slip_wall(tl)) sw |= (1u << j);
+            if (tl == T_CURVED) cw |= (1u << j);
         }
         if (sw) lw |= LINK_SLIP;
+        if (cw) lw |= LINK_CURVED;
     }
     link[c.idx] = lw;
     if (slipw) slipw[c.idx] = sw;
+    if (curvw) curvw[c.idx] = cw;
 }
 
 // Largest coordinates of active nodes (LB.cpp:913-921). Wavefront maxima are
@@ -1007,11 +1041,45 @@ dim3 lattice_grid(const LBKParams& p) { return dim3((p.X + BX - 1) / BX, (p.Y + 
         return;                                                                                                    \
     }
 
+// Cylinders (p.curvw set): the curved-wall pull and the capture of every active
+// node's density and velocity for the next step's curved links.
+#define HYB_CURVED_CASE(PULL, VV, CO)                                                                              \
+    if (pull == PULL && varVisc == VV && collision == CO) {                                                        \
+        if (prec == Prec::F64) {                                                                                   \
+            if (fs)                                                                                                \
+                k_fused64<PULL, VV, CO, true, true><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link,       \
+                                                                    solid, cp, (NodeForce64*)hf, visc, mac, p,     \
+                                                                    *fs);                                          \
+            else                                                                                                   \
+                k_fused64<PULL, VV, CO, false, true><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link,      \
+                                                                     solid, cp, (NodeForce64*)hf, visc, mac, p,    \
+                                                                     none);                                        \
+        } else {                                                                                                   \
+            if (fs)                                                                                                \
+                k_fused32<PULL, VV, CO, true, true><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid,  \
+                                                                    cpf, (NodeForce32*)hf, visc, mac, p, *fs);     \
+            else                                                                                                   \
+                k_fused32<PULL, VV, CO, false, true><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid, \
+                                                                     cpf, (NodeForce32*)hf, visc, mac, p, none);   \
+        }                                                                                                          \
+        return;                                                                                                    \
+    }
+
 void launch_lb_fused(Prec prec, bool pull, bool varVisc, int collision, const void* fin, void* fout,
                      const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, const CoupleRecF* cpf, void* hf,
                      double* visc, double* mac, const FSView* fs, const LBKParams& p, hipStream_t s) {
     const dim3 g = lattice_grid(p), b(BX, BY, 1);
     const FSView none{};
+    if (p.curvw) {
+        HYB_CURVED_CASE(true, false, HYB_COLLISION_BGK)
+        HYB_CURVED_CASE(true, false, HYB_COLLISION_TRT)
+        HYB_CURVED_CASE(true, true, HYB_COLLISION_BGK)
+        HYB_CURVED_CASE(true, true, HYB_COLLISION_TRT)
+        HYB_CURVED_CASE(false, false, HYB_COLLISION_BGK)
+        HYB_CURVED_CASE(false, false, HYB_COLLISION_TRT)
+        HYB_CURVED_CASE(false, true, HYB_COLLISION_BGK)
+        HYB_CURVED_CASE(false, true, HYB_COLLISION_TRT)
+    }
     HYB_FUSED_CASE(true, false, HYB_COLLISION_BGK)
     HYB_FUSED_CASE(true, false, HYB_COLLISION_TRT)
     HYB_FUSED_CASE(true, true, HYB_COLLISION_BGK)
@@ -1036,15 +1104,21 @@ void launch_status(const double* mac, const double* visc, const uint32_t* link, 
 
 void launch_lb_stream(Prec prec, const void* fin, void* fout, const uint32_t* link, const LBKParams& p, hipStream_t s) {
     const dim3 g = lattice_grid(p), b(BX, BY, 1);
+    if (p.curvw) {
+        if (prec == Prec::F64) k_stream<double, true><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link, p);
+        else k_stream<float, true><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, p);
+        return;
+    }
     if (prec == Prec::F64) k_stream<double><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link, p);
     else k_stream<float><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, p);
 }
 
 void launch_lb_init_types(uint8_t* type, const int* bnd, int nwalls, const WallDev* walls, int nobj,
-                          const ObjDev* objs, int problem, const double* lbx, const LBKParams& p, hipStream_t s) {
+                          const ObjDev* objs, const CylSet& cs, int problem, const double* lbx, const LBKParams& p,
+                          hipStream_t s) {
     const dim3 g = lattice_grid(p), b(BX, BY, 1);
     k_init_types<<<g, b, 0, s>>>(type, p, bnd[0], bnd[1], bnd[2], bnd[3], bnd[4], bnd[5], nwalls, walls, nobj, objs,
-                                 problem, lbx[0], lbx[1], lbx[2], lbx[3], lbx[4], lbx[5]);
+                                 cs, problem, lbx[0], lbx[1], lbx[2], lbx[3], lbx[4], lbx[5], lbx[6]);
 }
 
 void launch_lb_init_interface(uint8_t* type, const LBKParams& p, hipStream_t s) {
@@ -1058,9 +1132,10 @@ void launch_lb_init_interface(uint8_t* type, const LBKParams& p, hipStream_t s) 
     (void)hipFree(tmp);
 }
 
-void launch_lb_init_links(const uint8_t* type, uint32_t* link, uint32_t* slipw, const LBKParams& p, hipStream_t s) {
+void launch_lb_init_links(const uint8_t* type, uint32_t* link, uint32_t* slipw, uint32_t* curvw, const LBKParams& p,
+                          hipStream_t s) {
     const dim3 g = lattice_grid(p), b(BX, BY, 1);
-    k_init_links<<<g, b, 0, s>>>(type, link, slipw, p);
+    k_init_links<<<g, b, 0, s>>>(type, link, slipw, curvw, p);
 }
 
 void launch_lb_max_active(const uint32_t* link, int* mx, const LBKParams& p, hipStream_t s) {

@@ -31,6 +31,16 @@ struct LBKParams {
     const uint32_t* linkw;
     const uint32_t* actPrev;  // free surface with slip walls: activity before the last pass, see FSBuffers
     uint32_t actTag;          // tag of that pass; 0 = no pass since the last collision
+    // curved walls of a static cylinder (LB.cpp:1278-1319): per-node curved-link
+    // directions (bit j), density and Guo-shifted velocity of the previous collision
+    // (macPrev, [4][N]: n, ux, uy, uz; read by the pull) and of this one (macNext,
+    // written for every active node), node types, per-node viscosity (nullptr: visc0)
+    const uint32_t* curvw;
+    const double* macPrev;
+    double* macNext;
+    const uint8_t* typew;
+    const double* viscw;
+    CylDev cyl;
 };
 
 constexpr int SOLID_SLOT_BITS = 20;
@@ -105,11 +115,20 @@ struct WallSet {
 // post-collision state f: acc[nwalls][3][4] (zeroed by the caller).
 void launch_wall_fhydro(Prec prec, const void* f, const uint8_t* type, const WallSet& ws, unsigned long long* acc,
                         const LBKParams& p, hipStream_t s);
+// Cylinders of the lattice (nodes outside become curved walls).
+struct CylSet {
+    int n;
+    CylDev c[2];
+};
+// region: liquid box (6 values), then avalanchePosit (NET)
 void launch_lb_init_types(uint8_t* type, const int* boundary, int nwalls, const WallDev* walls, int nobj,
-                          const ObjDev* objs, int problem, const double* liquidBox, const LBKParams& p, hipStream_t s);
+                          const ObjDev* objs, const CylSet& cs, int problem, const double* region, const LBKParams& p,
+                          hipStream_t s);
 void launch_lb_init_interface(uint8_t* type, const LBKParams& p, hipStream_t s);
 // slipw: per-node slip-wall directions (nullptr when no slip walls exist)
-void launch_lb_init_links(const uint8_t* type, uint32_t* link, uint32_t* slipw, const LBKParams& p, hipStream_t s);
+// curvw: per-node curved-link directions (nullptr without cylinders)
+void launch_lb_init_links(const uint8_t* type, uint32_t* link, uint32_t* slipw, uint32_t* curvw, const LBKParams& p,
+                          hipStream_t s);
 void launch_lb_max_active(const uint32_t* link, int* maxXYZ, const LBKParams& p, hipStream_t s);
 // With free surface, also the initial density (rho), interface mass and interface velocity.
 void launch_lb_init_pops(Prec prec, void* f, const uint32_t* link, const int* maxXYZ, double* visc, const double* u0,

@@ -196,6 +196,7 @@ void LB::latticeBoltzmannGet(GetPot& lbmCfgFile, GetPot& command_line) {  // LB.
 
 void LB::latticeDefinition() {  // LB.cpp:42-83 (the lattice itself lives on the device)
     if (cfg_.problem != HYB_PROBLEM_LIQUID_BOX) cfg_.problem = (int32_t)problemName;
+    cfg_.avalanchePosit = avalanchePosit;  // parsed by the driver for NET and BARRIER (hybird.cpp:216-219)
     cfg_.freeSurface = freeSurface;
     cfg_.forceField = forceField;
     cfg_.nonNewtonian = nonNewtonian;
@@ -411,6 +412,8 @@ void DEM::discreteElementGet(GetPot& lbmCfgFile, GetPot& command_line) {  // DEM
 void DEM::discreteElementInit(unsIntList& boundary, unsIntList& lbSize, measureUnits& unit,
                               tVect& lbF) {  // DEM.cpp:185-297
     cfg_.multiStep = (int32_t)multiStep;
+    // the problem decides the cylinders (DEM.cpp:870-935)
+    cfg_.problem = (int32_t)problemName;
     cfg_.criticalRatio = criticalRatio;
     cfg_.demInitialRepeat = demInitialRepeat;
     cfg_.demSolve = 1;

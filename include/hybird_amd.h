@@ -53,6 +53,9 @@ extern "C" {
 enum hyb_problem {
     HYB_PROBLEM_NONE = 0,
     HYB_PROBLEM_SPLASH = 2,     /* gas above z = 20 (LB.cpp:628-640) */
+    HYB_PROBLEM_NET = 6,        /* static cylinder through (0,4,4)-(1,4,4), R = 4 (physical units), fluid
+                                   inside with curved walls (DEM.cpp:916-927, LB.cpp:552-602, 1278-1319);
+                                   without free surface */
     HYB_PROBLEM_LIQUID_BOX = 100 /* liquid inside liquidBox[], gas elsewhere (not in the reference) */
 };
 
@@ -91,6 +94,7 @@ typedef struct hyb_config {
     double trtMagic;        /* TRT magic parameter Lambda (default 1/4) */
     int32_t precision;      /* HYB_FP64 or HYB_FP32_SHIFTED */
     double liquidBox[6];    /* HYB_PROBLEM_LIQUID_BOX: xmin,xmax,ymin,ymax,zmin,zmax (node coordinates, inclusive) */
+    double avalanchePosit;  /* HYB_PROBLEM_NET: front of the liquid (physical units, LB.h avalanchePosit) */
     int32_t reserved[8];
 } hyb_config;
 

@@ -159,6 +159,16 @@ def _cases():
                                             **walls),
                             bed=bed4, vel=vel4, objects=np.array([[2.0, 10.0, 6.0, 12.0, 0.0, 0.0, 0.0]]),
                             cycles=[0, 4, 5, 12], keys=["types"], files=True)
+    # the NET problem (DEM.cpp:916-927, LB.cpp:662-690): a static cylinder along x
+    # through (y, z) = (4, 4) with R = 4, outside of which every node is a curved
+    # wall (Mei-Luo-Shyy, LB.cpp:1278-1319), liquid below z = 4 and z = x, free
+    # surface; spheres in the liquid, two thrown at the cylinder
+    bed5 = Bed(np.array([[7.0, 4.0, 2.2], [10.0, 2.5, 1.6], [5.5, 6.2, 1.9]]), np.array([1.0, 1.0, 0.9]))
+    vel5 = np.array([[0.0, 0.0, 0.0], [0.02, -0.1, -0.1], [0.0, 0.1, -0.06]])
+    cases["net"] = dict(cfg=base_config(problemName="NET", freeSurfaceSolve=1, lbSizeX=14, lbSizeY=10, lbSizeZ=10,
+                                        boundary0=7, boundary1=7, boundary2=7, boundary3=7, initVisc=0.05,
+                                        lbFX=2.0e-4, lbFZ=-5.0e-4, youngMod=2.0e3, density=3.0),
+                        bed=bed5, vel=vel5, cycles=[0, 1, 2, 5, 15, 30], keys=LATTICE_KEYS + DEM_KEYS)
     return cases
 
 
