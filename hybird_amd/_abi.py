@@ -10,6 +10,7 @@ import ctypes as C
 
 HYB_PROBLEM_NONE = 0
 HYB_PROBLEM_SPLASH = 2
+HYB_PROBLEM_DRUM = 4
 HYB_PROBLEM_NET = 6
 HYB_PROBLEM_LIQUID_BOX = 100
 HYB_CONTACT_LINEAR = 0
@@ -21,7 +22,7 @@ HYB_FP32_SHIFTED = 1
 
 ELEMENT_RECORD = 20
 
-_PROBLEMS = {"NONE": HYB_PROBLEM_NONE, "SPLASH": HYB_PROBLEM_SPLASH, "NET": HYB_PROBLEM_NET,
+_PROBLEMS = {"NONE": HYB_PROBLEM_NONE, "SPLASH": HYB_PROBLEM_SPLASH, "DRUM": HYB_PROBLEM_DRUM, "NET": HYB_PROBLEM_NET,
              "LIQUID_BOX": HYB_PROBLEM_LIQUID_BOX}
 
 
@@ -66,7 +67,9 @@ class HybConfig(C.Structure):
         ("precision", C.c_int32),
         ("liquidBox", C.c_double * 6),
         ("avalanchePosit", C.c_double),
-        ("reserved", C.c_int32 * 8),
+        ("drumSpeed", C.c_double),
+        ("fluidMass", C.c_double),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -117,6 +120,8 @@ def config_from_cfg(cfg: dict, collision: str = "BGK", precision: str = "FP64", 
     if liquid_box is not None:
         c.liquidBox[:] = [float(v) for v in liquid_box]
     c.avalanchePosit = float(g("avalanchePosit", 0.0))
+    c.drumSpeed = float(g("drumSpeed", 0.0))
+    c.fluidMass = float(g("fluidMass", 0.0))
     return c
 
 

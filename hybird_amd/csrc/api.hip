@@ -235,6 +235,9 @@ struct hyb_lb {
         p.collision = cfg.collision;
         p.trtMagic = cfg.trtMagic;
         p.unitLength = unit.Length; p.unitSpeed = unit.Speed; p.unitAngVel = unit.AngVel;
+        p.wallSides = 0;
+        for (int k = 0; k < 6; ++k)
+            if (boundary[k] >= 5 && boundary[k] <= 8) p.wallSides |= 1 << k;
         p.initDensity = initDensity;
         p.ctag = ctag;
         p.ffield = ffield;
@@ -422,9 +425,41 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
             d->walls.push_back(w);
         }
     }
-    // cylinders (DEM.cpp:870-935): the NET problem's static pipe along x; DRUM and
-    // AVALANCHE (a rotating drum, a free-surface obstacle) are not supported
+    // DRUM's two extra y walls (DEM.cpp:783-868), after the box walls; only their
+    // static forms (boundary 5, 7): the rotating ones (6, 8) are not supported
+    if (c.problem == HYB_PROBLEM_DRUM) {
+        for (int k = 2; k < 4; ++k) {
+            const int bc = lb->boundary[k];
+            if (bc == 6 || bc == 8)
+                return fail(HYB_E_UNSUPPORTED, "DRUM: rotating y walls (boundary 6, 8) are not supported");
+            if (bc != 5 && bc != 7) continue;
+            WallDev w;
+            w.p = k == 2 ? V3(0.0 + 0.6 + 0.35, 0.50001 * U.Length, 0.0)
+                         : V3(0.0 + 0.6 + 0.35, (double(lb->Y) - 1.50001) * U.Length, 0.0);
+            w.n = k == 2 ? V3(0.0, 1.0, 0.0) : V3(0.0, -1.0, 0.0);
+            w.rotCenter = V3();
+            w.omega = V3();
+            w.vel = V3();
+            w.moving = 0;
+            w.slip = bc == 5;
+            d->walls.push_back(w);
+        }
+    }
+    // cylinders (DEM.cpp:870-935): the NET problem's static pipe along x and DRUM's
+    // rotating drum along y; AVALANCHE (a free-surface obstacle) is not supported
     d->cylinders.clear();
+    if (c.problem == HYB_PROBLEM_DRUM) {
+        CylDev cy;
+        cy.p1 = V3(0.0 + 0.6 + 0.35, 0.0, 1.26);
+        cy.p2 = V3(0.0 + 0.6 + 0.35, 1.0, 1.26);
+        cy.R = 1.243;
+        cy.omega = V3(0.0, c.drumSpeed, 0.0);
+        cy.moving = 1;
+        cy.slip = 0;
+        const V3 axes = cy.p2 - cy.p1;  // initAxes (utils.cpp:154-161)
+        cy.naxes = axes / axes.norm();
+        d->cylinders.push_back(cy);
+    }
     if (c.problem == HYB_PROBLEM_NET) {
         CylDev cy;
         cy.p1 = V3(0.0, 4.0, 4.0);
@@ -730,7 +765,7 @@ int hyb_lb_create(const hyb_config* cfg, hyb_lb** out) {  // LB.cpp:85-188 + LB.
     l->slipCoefficient = c.slipCoefficient;
     l->turbConst = c.turbConst;
     if (c.problem != HYB_PROBLEM_NONE && c.problem != HYB_PROBLEM_SPLASH && c.problem != HYB_PROBLEM_NET &&
-        c.problem != HYB_PROBLEM_LIQUID_BOX) {
+        c.problem != HYB_PROBLEM_DRUM && c.problem != HYB_PROBLEM_LIQUID_BOX) {
         delete l;
         return fail(HYB_E_UNSUPPORTED, "problemName not supported");
     }
@@ -820,6 +855,8 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
     const int slotCap = d ? d->P + d->p.Gcap : 0;
     if (slotCap >= (int)SOLID_SLOT_MASK) return fail(HYB_E_CAPACITY, "too many particle slots for the ownership word");
     const bool cylinders = d && !d->cylinders.empty();
+    if (l->cfg.problem == HYB_PROBLEM_DRUM && !cylinders)
+        return fail(HYB_E_ARG, "hyb_lb_init: DRUM needs the DEM handle that holds the drum");
     int rc = lb_alloc(l, d && d->P ? slotCap : 0);
     if (rc) return rc;
     // the curved-wall rule tests the type of the node behind as it was when streamed
@@ -853,8 +890,17 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
         HIPCHK(hipMalloc((void**)&dob, objs.size() * sizeof(ObjDev)));
         HIPCHK(hipMemcpyAsync(dob, objs.data(), objs.size() * sizeof(ObjDev), hipMemcpyHostToDevice, s));
     }
-    const double region[7] = {l->cfg.liquidBox[0], l->cfg.liquidBox[1], l->cfg.liquidBox[2], l->cfg.liquidBox[3],
-                              l->cfg.liquidBox[4], l->cfg.liquidBox[5], l->cfg.avalanchePosit};
+    double region[7] = {l->cfg.liquidBox[0], l->cfg.liquidBox[1], l->cfg.liquidBox[2], l->cfg.liquidBox[3],
+                        l->cfg.liquidBox[4], l->cfg.liquidBox[5], l->cfg.avalanchePosit};
+    if (l->cfg.problem == HYB_PROBLEM_DRUM) {
+        // the liquid cylinder of LB.cpp:734-741: axis point (x, z) and radius, lattice units
+        const double fluidMass = l->cfg.fluidMass;
+        const double fluidRadius = 1.01 + 0.00035298 * sqrt(7891604 - 5713 * (1389.9 - fluidMass));
+        region[0] = 0.45 / l->unit.Length;
+        region[1] = 0.0 / l->unit.Length;
+        region[2] = (-1.0 + 0.0 + 0.04) / l->unit.Length;
+        region[3] = fluidRadius / l->unit.Length;
+    }
     launch_lb_init_types(l->type, l->boundary, (int)walls.size(), dw, (int)objs.size(), dob, cs, l->cfg.problem,
                          region, p, s);
     launch_lb_init_interface(l->type, p, s);
@@ -910,7 +956,8 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
             const bool owned = (so[i] >> SOLID_SLOT_BITS) == l->ctag && (so[i] & SOLID_SLOT_MASK);
             if ((lk[i] & LINK_ACTIVE) && !owned) tm += (lk[i] & LINK_INTERFACE) ? 0.5 * l->initDensity : l->initDensity;
         }
-        l->totalMass = tm;
+        // DRUM: the target liquid mass instead (LB.cpp:206-208)
+        l->totalMass = l->cfg.problem == HYB_PROBLEM_DRUM ? l->cfg.fluidMass / l->unit.Mass : tm;
     }
     l->steps = 0;
     l->cur = 0;
@@ -941,6 +988,9 @@ int hyb_lb_fs_step(hyb_lb* l) {  // LB.cpp:235-245
     for (int k = 0; k < 3; ++k) Fnew[k] = (l->forceField || l->steps == 0) ? l->lbF[k] : 0.0;
     launch_fs_step(l->prec, l->f[l->cur], l->f[l->cur ^ 1], l->type, l->link, l->slipw, l->visc, l->fsb, l->fsCur, l->rhoCur,
                    l->fsTag, Fnew, l->steps > 0, p, l->stream);
+    if (l->cfg.problem == HYB_PROBLEM_DRUM)  // LB.cpp:239-243
+        launch_fs_mass_fix(l->prec, l->type, l->dem && l->dem->P && l->slotCap > 0 ? l->solid : nullptr, l->ctag, l->fsb, l->fsCur,
+                           l->rhoCur, l->totalMass, p, l->stream);
     HIPCHK(hipGetLastError());
     l->fsCur ^= 1;
     l->fluidMassBuf = l->rhoCur;

@@ -458,6 +458,55 @@ __global__ void __launch_bounds__(FS_BLOCK) k_fs_add_extra(const uint32_t* __res
         for (int i = 0; i < 4; ++i) ctr->extra[i] = 0ull;
 }
 
+// Mass of the active nodes outside particles after the pass (LB.cpp:1807-1813):
+// interface nodes their mass, fluid nodes their last density; summed exactly.
+template <typename T>
+__global__ void __launch_bounds__(FS_BLOCK) k_fs_total_mass(const uint8_t* __restrict__ type,
+                                                            const uint32_t* __restrict__ solid, uint32_t ctag,
+                                                            const double* __restrict__ mass, const T* __restrict__ rho,
+                                                            FSCounters* __restrict__ ctr, size_t N) {
+    long long acc[4] = {0, 0, 0, 0};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (size_t)gridDim.x * blockDim.x) {
+        const uint8_t t = type[i];
+        if (t != T_FLUID && t != T_INTERFACE) continue;
+        if (solid && owner_slot(solid[i], ctag)) continue;
+        exact_local(acc, t == T_INTERFACE ? mass[i] : to_d(rho[i]));
+    }
+    exact_flush(ctr->tot, acc);
+}
+
+// redistributeMass(-0.01 * (mass - totalMass)) over the new interface list
+// (LB.cpp:1815-1821, 1796-1804). One workgroup; the sum is cleared for the next pass.
+__global__ void __launch_bounds__(FS_BLOCK) k_fs_mass_fix(const uint32_t* __restrict__ next,
+                                                          FSCounters* __restrict__ ctr, int cur,
+                                                          double* __restrict__ mass, double totalMass) {
+#pragma clang fp contract(off)
+    const uint32_t n = ctr->n[cur ^ 1];
+    const double massDeficit = exact_value(ctr->tot) - totalMass;
+    const double massSurplus = -0.01 * massDeficit;
+    if (n) {
+        const double add = massSurplus / (double)n;
+        for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) mass[next[q]] += add;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 0; i < 4; ++i) ctr->tot[i] = 0ull;
+}
+
+void launch_fs_mass_fix(Prec prec, const uint8_t* type, const uint32_t* solid, uint32_t ctag, FSBuffers& b, int cur,
+                        int rcur, double totalMass, const LBKParams& p, hipStream_t s) {
+    const size_t N = (size_t)p.X * p.Y * p.Z;
+    size_t g = (N + FS_BLOCK - 1) / FS_BLOCK;
+    if (g > FS_GRID) g = FS_GRID;
+    if (prec == Prec::F64)
+        k_fs_total_mass<double><<<(unsigned)g, FS_BLOCK, 0, s>>>(type, solid, ctag, b.mass,
+                                                                 (const double*)b.rho[rcur], b.ctr, N);
+    else
+        k_fs_total_mass<float><<<(unsigned)g, FS_BLOCK, 0, s>>>(type, solid, ctag, b.mass, (const float*)b.rho[rcur],
+                                                                b.ctr, N);
+    k_fs_mass_fix<<<1, FS_BLOCK, 0, s>>>(b.list[cur ^ 1], b.ctr, cur, b.mass, totalMass);
+}
+
 void launch_fs_curved_extra(Prec prec, const void* fpost, const uint32_t* link, FSBuffers& b, int cur,
                             const void* rhoFluid, bool massKnown, const LBKParams& p, hipStream_t s) {
     if (prec == Prec::F64)

@@ -16,6 +16,7 @@ struct FSCounters {
     uint32_t nA, nB;            // gas -> interface and fluid -> interface lists of this step
     unsigned long long acc[4];  // exact fixed-point mass-surplus accumulator (32-bit limbs, LSB 2^-96)
     unsigned long long extra[4];  // extra mass of the curved-wall rule in the last streaming (same format)
+    unsigned long long tot[4];    // mass of the active nodes outside particles (DRUM mass fix, same format)
 };
 
 // Device state of the free-surface mode. Densities use the population type
@@ -57,6 +58,12 @@ struct FSView {
 // state; rhoFluid: fluid-node masses, or initDensity when !massKnown).
 void launch_fs_curved_extra(Prec prec, const void* fpost, const uint32_t* link, FSBuffers& b, int cur,
                             const void* rhoFluid, bool massKnown, const LBKParams& p, hipStream_t s);
+// DRUM's enforceMassConservation (LB.cpp:1806-1822) after a pass started with
+// interface list `cur`: the mass of the active nodes outside particles (solid:
+// ownership words of coupling step ctag, nullptr without particles) against
+// totalMass; the interface nodes of the new list gain -0.01 (mass - totalMass) / count.
+void launch_fs_mass_fix(Prec prec, const uint8_t* type, const uint32_t* solid, uint32_t ctag, FSBuffers& b, int cur,
+                        int rcur, double totalMass, const LBKParams& p, hipStream_t s);
 void launch_fs_build_list(const uint8_t* type, FSBuffers& b, int cur, const LBKParams& p, hipStream_t s);
 // One free-surface step (updateMass + updateInterface, LB.cpp:1492-1804).
 // cur: which interface list is current; rcur: which density buffer holds the

@@ -60,24 +60,47 @@ __device__ __forceinline__ uint32_t owner_slot(uint32_t w, uint32_t tag) {
 // A term v is |v| 2^96 rounded toward zero to an integer q < 2^128 (|v| < 2^31);
 // its four 32-bit limbs are added with sign into four 64-bit integer sums
 // (integer addition is associative, so the result is the same in any order).
-__device__ inline void exact_add(unsigned long long* acc, double v) {
-    if (v == 0.0 || !(v == v)) return;
+// The four limbs of |v| 2^96 (false: v is zero, NaN or outside the range).
+__device__ inline bool exact_split(double v, unsigned long long limb[4]) {
+    if (v == 0.0 || !(v == v)) return false;
     int e = 0;
     const double m = frexp(fabs(v), &e);  // |v| = m 2^e, m in [0.5, 1)
     const unsigned long long M = (unsigned long long)ldexp(m, 53);
     const int k = e + 43;  // |v| 2^96 = M 2^(e - 53 + 96)
     unsigned long long lo = 0, hi = 0;
     if (k >= 0) {
-        if (k >= 75) return;  // |v| >= 2^31: outside the accumulator's range
+        if (k >= 75) return false;  // |v| >= 2^31: outside the accumulator's range
         if (k == 0) lo = M;
         else if (k < 64) { lo = M << k; hi = M >> (64 - k); }
         else hi = M << (k - 64);
     } else if (k > -64) {
         lo = M >> (-k);
     }
-    const unsigned long long limb[4] = {lo & 0xFFFFFFFFull, lo >> 32, hi & 0xFFFFFFFFull, hi >> 32};
+    limb[0] = lo & 0xFFFFFFFFull; limb[1] = lo >> 32; limb[2] = hi & 0xFFFFFFFFull; limb[3] = hi >> 32;
+    return true;
+}
+
+__device__ inline void exact_add(unsigned long long* acc, double v) {
+    unsigned long long limb[4];
+    if (!exact_split(v, limb)) return;
     for (int i = 0; i < 4; ++i)
         if (limb[i]) atomicAdd(&acc[i], v < 0.0 ? (0ull - limb[i]) : limb[i]);
+}
+
+// The same sum kept per thread in signed 64-bit limbs (each below 2^32, so up to
+// 2^31 terms), then folded over the wavefront and added once per wavefront.
+__device__ inline void exact_local(long long s[4], double v) {
+    unsigned long long limb[4];
+    if (!exact_split(v, limb)) return;
+    for (int i = 0; i < 4; ++i) s[i] += v < 0.0 ? -(long long)limb[i] : (long long)limb[i];
+}
+
+__device__ inline void exact_flush(unsigned long long* acc, long long s[4]) {
+    for (int i = 0; i < 4; ++i) {
+        long long t = s[i];
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        if ((threadIdx.x & 63) == 0 && t) atomicAdd(&acc[i], (unsigned long long)t);
+    }
 }
 
 __device__ inline double exact_value(const unsigned long long* acc) {
@@ -114,9 +137,12 @@ __device__ __forceinline__ T wall_link(const T* __restrict__ fs, size_t PS, cons
 // LB.cpp:589-602; 0 for w in a boundary layer), chi = (2 delta - 1) / tau (delta >= 1/2) or / (tau - 2), the
 // reference velocity ubf = m1 u + m2 uw (delta >= 1/2) or the velocity of the
 // fluid node behind (else; uw when there is none), fStar = n w_j (1 + 3 c_j.ubf +
-// 4.5 (u.c_j)^2 + 1.5 u^2) and f = (1 - chi) fs_j + chi fStar - BBi. The cylinder
-// is static: uw = 0, BBi = 0. n, u, visc: the node's own from its last collision.
-// SHIFT: populations hold f - w (rho0 = 1).
+// 4.5 (u.c_j)^2 + 1.5 u^2) and f = (1 - chi) fs_j + chi fStar - BBi with
+// BBi = 6 n w_j uw.c_j. uw is the velocity the reference gives the wall node at set-up
+// (LB.cpp:946-993): a plane wall's (zero for the box walls) for a node in a wall's
+// boundary layer, else the cylinder's surface speed at the node (utils.cpp:106-120,
+// zero for a static cylinder) in lattice units. n, u, visc: the node's own from its
+// last collision. SHIFT: populations hold f - w (rho0 = 1).
 struct CurvedParts {
     double chi, fStar, BBi;
 };
@@ -151,7 +177,13 @@ __device__ __forceinline__ CurvedParts curved_parts(const LBKParams& p, const Co
     const double visc = p.viscw ? p.viscw[c.idx] : p.visc0;
     const double tau = 0.5 + 3.0 * visc;
     const double chi = delta >= 0.5 ? (2.0 * delta - 1.0) / tau : (2.0 * delta - 1.0) / (tau - 2.0);
-    const V3 vel(0.0, 0.0, 0.0);
+    V3 vel(0.0, 0.0, 0.0);
+    if (p.cyl.moving) {
+        const bool inWall = (wx == 0 && (p.wallSides & 1)) || (wx == p.X - 1 && (p.wallSides & 2)) ||
+                            (wy == 0 && (p.wallSides & 4)) || (wy == p.Y - 1 && (p.wallSides & 8)) ||
+                            (wz == 0 && (p.wallSides & 16)) || (wz == p.Z - 1 && (p.wallSides & 32));
+        if (!inWall) vel = p.cyl.getSpeed(V3((double)wx, (double)wy, (double)wz) * L) / p.unitSpeed;
+    }
     V3 ubf(0.0, 0.0, 0.0);
     if (delta >= 0.5) {
         const double m1 = (delta - 1) / delta, m2 = 1 / delta;

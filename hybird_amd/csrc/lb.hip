@@ -535,7 +535,8 @@ __global__ void __launch_bounds__(BX * BY) k_stream(const T* __restrict__ fin, T
 
 // ------------------------------------------------------------------ initialisation
 // Boundary typing (LB.cpp:394-431), plane walls (LB.cpp:497-533) and the
-// problem's gas region (LB.cpp:611-784: SPLASH plane, or a liquid box).
+// problem's gas region (LB.cpp:611-784: SPLASH plane, DRUM and NET shapes, or a
+// liquid box).
 __global__ void k_init_types(uint8_t* __restrict__ type, LBKParams p, int b0, int b1, int b2, int b3, int b4, int b5,
                              int nwalls, const WallDev* __restrict__ walls, int nobj, const ObjDev* __restrict__ objs,
                              CylSet cs, int problem, double lx0, double lx1, double ly0, double ly1, double lz0,
@@ -572,6 +573,10 @@ __global__ void k_init_types(uint8_t* __restrict__ type, LBKParams p, int b0, in
             if ((pos - V3(0.0, 0.0, 20.0)).dot(V3(0.0, 0.0, 1.0)) > 0.0) t = T_GAS;
         } else if (problem == HYB_PROBLEM_LIQUID_BOX) {
             if (pos.x < lx0 || pos.x > lx1 || pos.y < ly0 || pos.y > ly1 || pos.z < lz0 || pos.z > lz1) t = T_GAS;
+        } else if (problem == HYB_PROBLEM_DRUM) {
+            // gas outside the cylinder through (lx0, lx1, ly0) along y with radius ly1
+            // (all in lattice units, LB.cpp:734-747)
+            if (!CylDev::inside(pos, V3(lx0, lx1, ly0), V3(0.0, 1.0, 0.0), ly1)) t = T_GAS;
         } else if (problem == HYB_PROBLEM_NET) {
             // box before the net, then the plane through (xMin, 0, 0) with normal
             // (-1, 0, 1) (LB.cpp:662-690); the bounds are unsigned lattice indices

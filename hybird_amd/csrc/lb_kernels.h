@@ -31,7 +31,7 @@ struct LBKParams {
     const uint32_t* linkw;
     const uint32_t* actPrev;  // free surface with slip walls: activity before the last pass, see FSBuffers
     uint32_t actTag;          // tag of that pass; 0 = no pass since the last collision
-    // curved walls of a static cylinder (LB.cpp:1278-1319): per-node curved-link
+    // curved walls of a cylinder (LB.cpp:1278-1319): per-node curved-link
     // directions (bit j), density and Guo-shifted velocity of the previous collision
     // (macPrev, [4][N]: n, ux, uy, uz; read by the pull) and of this one (macNext,
     // written for every active node), node types, per-node viscosity (nullptr: visc0)
@@ -41,6 +41,9 @@ struct LBKParams {
     const uint8_t* typew;
     const double* viscw;
     CylDev cyl;
+    // box sides (bit 2 * axis + high) whose boundary layer is a plane wall: a curved
+    // wall node there takes that wall's (zero) velocity, every other one the cylinder's
+    int wallSides;
 };
 
 constexpr int SOLID_SLOT_BITS = 20;

@@ -197,6 +197,7 @@ void LB::latticeBoltzmannGet(GetPot& lbmCfgFile, GetPot& command_line) {  // LB.
 void LB::latticeDefinition() {  // LB.cpp:42-83 (the lattice itself lives on the device)
     if (cfg_.problem != HYB_PROBLEM_LIQUID_BOX) cfg_.problem = (int32_t)problemName;
     cfg_.avalanchePosit = avalanchePosit;  // parsed by the driver for NET and BARRIER (hybird.cpp:216-219)
+    cfg_.fluidMass = fluidMass;            // parsed by the driver for DRUM (hybird.cpp:199-205)
     cfg_.freeSurface = freeSurface;
     cfg_.forceField = forceField;
     cfg_.nonNewtonian = nonNewtonian;
@@ -414,6 +415,7 @@ void DEM::discreteElementInit(unsIntList& boundary, unsIntList& lbSize, measureU
     cfg_.multiStep = (int32_t)multiStep;
     // the problem decides the cylinders (DEM.cpp:870-935)
     cfg_.problem = (int32_t)problemName;
+    cfg_.drumSpeed = drumSpeed;  // parsed by the driver for DRUM (hybird.cpp:199-205)
     cfg_.criticalRatio = criticalRatio;
     cfg_.demInitialRepeat = demInitialRepeat;
     cfg_.demSolve = 1;

@@ -53,6 +53,11 @@ extern "C" {
 enum hyb_problem {
     HYB_PROBLEM_NONE = 0,
     HYB_PROBLEM_SPLASH = 2,     /* gas above z = 20 (LB.cpp:628-640) */
+    HYB_PROBLEM_DRUM = 4,       /* rotating drum: cylinder through (0.95,0,1.26)-(0.95,1,1.26), R = 1.243,
+                                   angular velocity (0, drumSpeed, 0) (DEM.cpp:876-887); liquid inside the
+                                   cylinder through (0.45,0,-0.96) with the radius of LB.cpp:734-741; the
+                                   global mass fix towards fluidMass after every free-surface pass
+                                   (LB.cpp:1806-1822); y walls only static (boundary 5 or 7) */
     HYB_PROBLEM_NET = 6,        /* static cylinder through (0,4,4)-(1,4,4), R = 4 (physical units), fluid
                                    inside with curved walls (DEM.cpp:916-927, LB.cpp:552-602, 1278-1319);
                                    without free surface */
@@ -95,7 +100,9 @@ typedef struct hyb_config {
     int32_t precision;      /* HYB_FP64 or HYB_FP32_SHIFTED */
     double liquidBox[6];    /* HYB_PROBLEM_LIQUID_BOX: xmin,xmax,ymin,ymax,zmin,zmax (node coordinates, inclusive) */
     double avalanchePosit;  /* HYB_PROBLEM_NET: front of the liquid (physical units, LB.h avalanchePosit) */
-    int32_t reserved[8];
+    double drumSpeed;       /* HYB_PROBLEM_DRUM: drum angular velocity (rad/s, cfg key drumSpeed) */
+    double fluidMass;       /* HYB_PROBLEM_DRUM: target liquid mass (physical units, cfg key fluidMass) */
+    int32_t reserved[4];
 } hyb_config;
 
 /* Fills *cfg with the reference's parse defaults (LB.cpp:93-185, DEM.cpp:15-181)

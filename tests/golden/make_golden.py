@@ -169,6 +169,18 @@ def _cases():
                                         boundary0=7, boundary1=7, boundary2=7, boundary3=7, initVisc=0.05,
                                         lbFX=2.0e-4, lbFZ=-5.0e-4, youngMod=2.0e3, density=3.0),
                         bed=bed5, vel=vel5, cycles=[0, 1, 2, 5, 15, 30], keys=LATTICE_KEYS + DEM_KEYS)
+    # the DRUM problem (DEM.cpp:783-887, LB.cpp:206-208, 734-747, 1806-1822): the
+    # drum (R = 1.243 m around (0.95, 1.26)) turning at 0.3 rad/s, so its curved
+    # wall nodes move; liquid pool inside the circle of the fluidMass formula; the
+    # global mass fix pulls the liquid mass towards fluidMass after every pass;
+    # extra static y walls; one sphere resting near the drum bottom, one thrown at it
+    bed6 = Bed(np.array([[0.6, 0.35, 0.35], [1.0, 0.35, 0.2], [0.4, 0.35, 0.5]]), np.array([0.12, 0.12, 0.11]))
+    vel6 = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, -0.05], [-0.2, 0.0, -0.3]])
+    cases["drum"] = dict(cfg=base_config(problemName="DRUM", freeSurfaceSolve=1, unitLength=0.1, unitTime=0.01,
+                                         unitDensity=1000.0, lbSizeX=2.4, lbSizeY=0.8, lbSizeZ=2.7, boundary0=7,
+                                         boundary1=7, boundary2=7, boundary3=7, initVisc=50.0, lbFZ=-5.0e-4,
+                                         youngMod=1.0e6, density=3000.0, drumSpeed=0.3, fluidMass=800.0),
+                         bed=bed6, vel=vel6, cycles=[0, 1, 2, 5, 15, 30], keys=LATTICE_KEYS + DEM_KEYS)
     return cases
 
 
