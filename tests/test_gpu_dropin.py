@@ -248,3 +248,28 @@ def test_dropin_object_file_matches_oracle(tmp_path):
     assert rel_l2(f[act], orc.get("f")[act]) <= 1e-12
     ex = _load(dump, "dem_e_x").reshape(-1, 6, 3)
     assert rel_l2(ex, orc.get("e_x")) <= 1e-11
+
+
+def test_dropin_drum_matches_reference(tmp_path):
+    """problemName DRUM through the reference's own main (drumSpeed and fluidMass
+    parsed by its driver, hybird.cpp:199-205) against the reference's dump of the
+    same run (tests/golden/drum.npz) at the last fixture cycle."""
+    cfg, rec, z = load_case("drum")
+    cycles = int(max(z["cycles"]))
+    tmp = str(tmp_path)
+    path = _write_case(tmp, cfg, z["bed_x"], z["bed_r"], cycles, bed_v=z["bed_v"])
+    dump = os.path.join(tmp, "dump")
+    os.makedirs(dump)
+    env = dict(os.environ, HYBIRD_AMD_DUMP_DIR=dump)
+    print(f"[gpu dropin] hybird_gpu drum {cycles} cycles", flush=True)
+    r = subprocess.run([BIN, "-c", path, "-d", os.path.join(tmp, "out")], env=env, capture_output=True, text=True,
+                       timeout=300, cwd=tmp)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    t = z[f"c{cycles}/types"].ravel()
+    assert np.array_equal(_load(dump, "lb_types", np.uint8).ravel(), t)
+    act = (t == 0) | (t == 3)
+    f = _load(dump, "lb_f").reshape(-1, 19)
+    assert rel_l2(f[act], z[f"c{cycles}/f"].reshape(-1, 19)[act]) <= 1e-12
+    # positions and velocities (the higher Gear slots carry 1/dt^k times rounding)
+    ex = _load(dump, "dem_e_x").reshape(-1, 6, 3)
+    assert rel_l2(ex[:, :2], z[f"c{cycles}/e_x"][:, :2]) <= 1e-11
