@@ -77,6 +77,24 @@ def _cases():
     cases["bingham"] = dict(cfg=base_config(lbSizeX=12, lbSizeY=10, lbSizeZ=14, nonNewtonianSolve=1,
                                             plasticVisc=0.05, yieldStress=1.0e-5, initVisc=0.05, lbFX=1.0e-4),
                             bed=Bed(np.zeros((0, 3)), np.zeros(0)), cycles=[0, 1, 3, 6], keys=LATTICE_KEYS)
+    # Smagorinsky viscosity (node.cpp:103-145, LB.cpp:1104-1107): a sheared channel
+    # (uniform initial velocity between no-slip z walls, body force along x) with a
+    # sphere; with turbulence alone the reference adds turbConst * shearRate to the
+    # node's previous viscosity every collision, so the wall layers reach the
+    # upper clamp (maxTau = 1.8) within the run while the core stays below it
+    cases["turb"] = dict(cfg=base_config(lbSizeX=14, lbSizeY=12, lbSizeZ=16, turbulenceSolve=1, turbConst=2.0,
+                                         initVisc=0.02, initVelocityX=0.06, lbFX=2.0e-4, lbFZ=-1.0e-4,
+                                         youngMod=2.0e3),
+                         bed=Bed(np.array([[7.0, 6.0, 8.0]]), np.array([2.5])), cycles=[0, 1, 3, 8, 20],
+                         keys=LATTICE_KEYS + DEM_KEYS)
+    # Smagorinsky and Bingham together: nuApp = plasticVisc + yieldStress / (2 n shearRate) plus turbConst *
+    # shearRate, clamped
+    cases["turb_bingham"] = dict(cfg=base_config(lbSizeX=12, lbSizeY=10, lbSizeZ=14, turbulenceSolve=1,
+                                                 turbConst=0.3, nonNewtonianSolve=1, plasticVisc=0.03,
+                                                 yieldStress=2.0e-5, initVisc=0.03, initVelocityX=0.04,
+                                                 lbFX=1.0e-4),
+                                 bed=Bed(np.zeros((0, 3)), np.zeros(0)), cycles=[0, 1, 3, 6, 15],
+                                 keys=LATTICE_KEYS)
     # two spheres, one straddling the periodic x plane (ghosts), settling under gravity
     cases["couple2"] = dict(cfg=base_config(lbSizeX=24, lbSizeY=24, lbSizeZ=24, initVisc=0.1, lbFZ=-1.0e-5),
                             bed=Bed(np.array([[2.5, 12.25, 9.5], [15.75, 17.5, 13.0]]), np.array([4.0, 4.0])),

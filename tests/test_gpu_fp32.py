@@ -26,7 +26,8 @@ def _pair(cfg, rec, **kw):
     return a, b
 
 
-@pytest.mark.parametrize("name,cycles", [("couple2", 12), ("dense", 40), ("bingham", 6), ("walls", 25)])
+@pytest.mark.parametrize("name,cycles", [("couple2", 12), ("dense", 40), ("bingham", 6), ("turb", 20),
+                                         ("turb_bingham", 15), ("walls", 25)])
 def test_gpu_fp32_matches_fp64(name, cycles):
     cfg, rec, _ = load_case(name)
     print(f"[gpu fp32] case={name}", flush=True)

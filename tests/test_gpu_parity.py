@@ -26,7 +26,7 @@ def _runtime():
     return runtime
 
 
-def _compare_lattice(gpu, orc, tag):
+def _compare_lattice(gpu, orc, tag, var_visc=False):
     t_o = orc.get("types")
     t_g = gpu.get("types")
     assert np.array_equal(t_o, t_g), f"{tag}: node types differ"
@@ -43,6 +43,10 @@ def _compare_lattice(gpu, orc, tag):
         b = gpu.get(name)[act]
         err = rel_l2(b, a) if np.linalg.norm(a) > 0 else float(np.max(np.abs(b - a)))
         assert err <= LB_TOL, f"{tag}: {name} rel L2 {err:.3e}"
+    if var_visc:
+        # Bingham / Smagorinsky viscosity of the last collision (node.cpp:103-145)
+        a, b = orc.get("visc")[act], gpu.get("visc")[act]
+        assert rel_l2(b, a) <= LB_TOL, f"{tag}: visc rel L2 {rel_l2(b, a):.3e}"
 
 
 def _compare_dem(gpu, orc, tag):
@@ -85,8 +89,8 @@ def _close_vec(b, a, what, tol=None):
 HORIZON = {"dem60": 100}
 
 
-@pytest.mark.parametrize("name", ["lb16", "chan", "bingham", "couple2", "dense", "dem60", "walls", "objects",
-                                  "mercury"])
+@pytest.mark.parametrize("name", ["lb16", "chan", "bingham", "turb", "turb_bingham", "couple2", "dense", "dem60",
+                                  "walls", "objects", "mercury"])
 def test_gpu_fp64_matches_oracle(name):
     rt = _runtime()
     cfg, rec, z = load_case(name)
@@ -103,7 +107,7 @@ def test_gpu_fp64_matches_oracle(name):
         prev = c
         tag = f"{name} cycle {c}"
         if lattice:
-            _compare_lattice(gpu, orc, tag)
+            _compare_lattice(gpu, orc, tag, var_visc=bool(cfg.get("nonNewtonianSolve") or cfg.get("turbulenceSolve")))
             # walls[].FHydro of the moving no-slip walls (LB.cpp:1321-1341)
             _close_vec(gpu.get("wallFHydro"), orc.get("walls")[:, 9:12], f"{tag}: walls FHydro", LB_TOL)
         if rec.shape[0]:
