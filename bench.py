@@ -20,9 +20,7 @@ import argparse
 import json
 import math
 import os
-import subprocess
 import sys
-import tempfile
 import time
 
 import numpy as np
@@ -32,8 +30,8 @@ sys.path.insert(0, ROOT)
 
 from hybird_amd import runtime  # noqa: E402
 from hybird_amd._abi import config_from_cfg, element_records  # noqa: E402
-from hybird_amd.cfg import base_config, write_cfg  # noqa: E402
-from hybird_amd.particles import rsa_bed, write_empty_object_file, write_particle_file  # noqa: E402
+from hybird_amd.cfg import base_config  # noqa: E402
+from hybird_amd.particles import rsa_bed  # noqa: E402
 
 METRIC = "MLUPS + particle-steps/s, FP32-shift vs FP64 plain at <=1e-4 velocity L2 error"
 HBM_PEAK_GBS = 8000.0
@@ -77,42 +75,48 @@ def timed(sim, steps, warmup):
     return t1 - t0, ms / max(launches, 1), int(float(sim.get("rebuilds")[0]) - r0)
 
 
-def cpu_baseline(size_sub: int, cycles: int):
-    """The reference itself (oracle/_ref, built from its sources) on a bounded sub-box."""
-    harness = os.path.join(ROOT, "oracle", "_ref", "hybird_ref_dump")
-    threads = min(16, os.cpu_count() or 1)
-    cfg, bed = workload(size_sub, seed=1)
-    with tempfile.TemporaryDirectory() as tmp:
-        pf, of = os.path.join(tmp, "p.dat"), os.path.join(tmp, "o.dat")
-        write_particle_file(pf, bed)
-        write_empty_object_file(of)
-        cfg = dict(cfg, particleFile=pf, objectFile=of, maximumTimeSteps=cycles)
-        cp = os.path.join(tmp, "c.cfg")
-        write_cfg(cp, cfg)
-        work = os.path.join(tmp, "w")
-        os.makedirs(work)
-        sample = (f"{size_sub}^3 sub-box of the same workload ({bed.count} spheres, same radii and solid fraction), "
-                  f"BGK tau=0.6 (the reference has no TRT), FP64, {cycles} cycles, steady-state (T(n)-T(1))/(n-1)")
-        if os.path.exists(harness):
-            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-            out = subprocess.run([harness, cp, work, "-", str(cycles)], env=env, capture_output=True, text=True,
-                                 timeout=600, check=True).stdout.strip().splitlines()[-1]
-            r = json.loads(out)
-            mlups = r["active_nodes"] / r["seconds_per_cycle"] / 1e6
-            return {"value": round(mlups, 3), "unit": "MLUPS", "cores": threads, "kind": "reference",
-                    "sample": sample}
-        # restatement fallback: one thread
-        sys.path.insert(0, ROOT)
-        from oracle.oracle import OracleSim
+def host_cpu():
+    """Model name, physical cores and logical CPUs of this host (from /proc/cpuinfo)."""
+    model, cores, logical = "unknown", set(), 0
+    try:
+        with open("/proc/cpuinfo") as fh:
+            phys = core = None
+            for line in fh:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    logical += 1
+                elif k == "model name":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                    cores.add((phys, core))
+    except OSError:
+        pass
+    return {"model": model, "physical_cores": len(cores) or None, "logical_cpus": logical or os.cpu_count()}
 
-        conf = config_from_cfg(cfg)
-        o = OracleSim(conf, element_records(bed.x, bed.r))
-        o.cycle(1)
-        t0 = time.perf_counter()
-        o.cycle(cycles - 1)
-        dt = (time.perf_counter() - t0) / (cycles - 1)
-        act = int(np.sum((o.get("types") == 0) | (o.get("types") == 3)))
-        return {"value": round(act / dt / 1e6, 3), "unit": "MLUPS", "cores": 1, "kind": "port", "sample": sample}
+
+def cpu_baseline(size_sub: int, cycles: int):
+    """The FP64 CPU restatement of the reference's cycle (oracle/, one thread) on a bounded
+    sub-box of the same workload: same radii, solid fraction, TRT collision and contacts."""
+    from oracle.oracle import OracleSim
+
+    cfg, bed = workload(size_sub, seed=1)
+    conf = config_from_cfg(cfg, collision="TRT", precision="FP64")
+    o = OracleSim(conf, element_records(bed.x, bed.r))
+    o.cycle(1)
+    t0 = time.perf_counter()
+    o.cycle(cycles - 1)
+    dt = (time.perf_counter() - t0) / (cycles - 1)
+    t = o.get("types")
+    act = int(np.sum((t == 0) | (t == 3)))
+    o.close()
+    sample = (f"{size_sub}^3 sub-box of the same workload ({bed.count} spheres, same radii and solid fraction, TRT "
+              f"tau 0.6, 10 DEM sub-steps), FP64, {cycles} cycles, steady-state (T(n)-T(1))/(n-1), one thread")
+    return {"value": round(act / dt / 1e6, 3), "unit": "MLUPS", "cores": 1, "kind": "port", "sample": sample,
+            "particle_steps_per_s": round(bed.count * 10 / dt, 1), "host": host_cpu()}
 
 
 def adjoint_cost(size: int = 128, steps: int = 1000, every: int = 50, count: int = 2000, seed: int = 3):
@@ -167,6 +171,155 @@ def adjoint_cost(size: int = 128, steps: int = 1000, every: int = 50, count: int
             "grad_tau": gtau, "grad_force_norm": float(np.linalg.norm(gF)), "grad_f0_norm": float(np.linalg.norm(g0))}
 
 
+def copy_ceiling(nbytes: int = 1 << 31):
+    """GB/s of a plain non-temporal device copy (in the library) with 4-, 8- and 16-byte lanes: what a
+    streaming kernel can reach on this HBM at each access width."""
+    return {f"{w}B_per_lane": round(runtime.copy_bandwidth(nbytes, w, 20), 1) for w in (4, 8, 16)}
+
+
+def _rel(a, b):
+    n = float(np.linalg.norm(b))
+    return float(np.linalg.norm(a - b) / n) if n > 0 else float(np.linalg.norm(a))
+
+
+def horizon(sim32, sim64, done: int, steps: int, every: int):
+    """FP32 against FP64 on the same workload, continued to `steps` cycles in total:
+    velocity-field and drag relative L2 and the largest particle-position difference
+    every `every` cycles."""
+    t = sim64.get("types")
+    act = (t == 0) | (t == 3)
+    rows = []
+    while done < steps:
+        n = min(every - done % every, steps - done)
+        sim32.cycle(n)
+        sim64.cycle(n)
+        done += n
+        u32, u64 = sim32.get("u")[act], sim64.get("u")[act]
+        f32, f64 = sim32.get("e_forces")[:, 0], sim64.get("e_forces")[:, 0]
+        x32, x64 = sim32.get("e_x")[:, 0], sim64.get("e_x")[:, 0]
+        rows.append({"step": done, "velocity_rel_l2": _rel(u32, u64), "drag_rel_l2": _rel(f32, f64),
+                     "position_max_abs": float(np.max(np.abs(x32 - x64)))})
+        print(f"[bench] horizon step {done}: velocity {rows[-1]['velocity_rel_l2']:.3e}", file=sys.stderr,
+              flush=True)
+    return rows
+
+
+# Young's modulus (lattice units) of the two free-surface workloads, which state no contact stiffness: at 10
+# DEM sub-steps per LB step (dt = 0.1) a Hertz wall contact of a radius-3 sphere of density 2.5 has omega dt
+# of about 1.2 at E = 1e5 (the Gear integration then pumps energy into resting contacts and particles sink
+# through the floor within ~2000 steps of the dam break) and about 0.4 at E = 1e4.
+YOUNG_FS = 1.0e4
+
+
+def dam_break(steps: int = 10000, every: int = 2000, phi: float = 0.25, seed: int = 2):
+    """Free-surface dam break with Bingham rheology and particles: 384x128x192 D3Q19 closed no-slip tank, liquid
+    block x < 128, z < 120, plastic viscosity tau 0.55 (nu = 0.05/3), yield stress 1e-5, gravity -2e-5 (lattice
+    units); spheres r ~ U[3,5] filling the block at solid fraction `phi` by RSA (seed 2), E = YOUNG_FS; BGK; FP32 populations
+    (FP64 masses) against the plain FP64 path: liquid-mass drift and front position every `every` steps."""
+    X, Y, Z = 384, 128, 192
+    visc = 0.05 / 3.0
+    cfg = base_config(problemName="LIQUID_BOX", freeSurfaceSolve=1, lbSizeX=X, lbSizeY=Y, lbSizeZ=Z,
+                      boundary0=7, boundary1=7, boundary2=7, boundary3=7, boundary4=7, boundary5=7,
+                      nonNewtonianSolve=1, plasticVisc=visc, yieldStress=1.0e-5, initVisc=visc, lbFZ=-2.0e-5,
+                      density=2.5, youngMod=YOUNG_FS, multiStep=10)
+    box = (1.0, 127.0, 1.0, Y - 2.0, 1.0, 119.0)
+    vol = 127.0 * (Y - 2.0) * 119.0
+    count = int(phi * vol / (4.0 / 3.0 * math.pi * (5.0 ** 4 - 3.0 ** 4) / 8.0))
+    bed = rsa_bed(seed, (0.5, 0.5, 0.5), (127.5, Y - 1.5, 119.5), lambda r: r.uniform(3.0, 5.0), count,
+                  max_attempts=400000)
+    rec = element_records(bed.x, bed.r)
+    sims = {k: runtime.Simulation(config_from_cfg(cfg, precision=k, liquid_box=box), rec) for k in ("FP64", "FP32")}
+
+    def front(sim):
+        t = sim.get("types").reshape(Z, Y, X)
+        liq = (t[1] == 0) | (t[1] == 3)  # the bottom liquid layer
+        cols = np.nonzero(liq.any(axis=0))[0]
+        return int(cols.max()) if cols.size else -1
+
+    m0 = {k: float(s.get("fluidMass")[0]) for k, s in sims.items()}
+    rows = []
+    done = 0
+    t0 = time.perf_counter()
+    while done < steps:
+        n = min(every, steps - done)
+        for s in sims.values():
+            s.cycle(n)
+        done += n
+        m = {k: float(s.get("fluidMass")[0]) for k, s in sims.items()}
+        rows.append({"step": done, "mass_drift_fp64": m["FP64"] / m0["FP64"] - 1.0,
+                     "mass_drift_fp32": m["FP32"] / m0["FP32"] - 1.0,
+                     "mass_rel_fp32_vs_fp64": m["FP32"] / m["FP64"] - 1.0,
+                     "front_x_fp64": front(sims["FP64"]), "front_x_fp32": front(sims["FP32"])})
+        print(f"[bench] dam break step {done}: front {rows[-1]['front_x_fp64']} / {rows[-1]['front_x_fp32']}",
+              file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    for s in sims.values():
+        s.close()
+    return {"workload": f"dam break {X}x{Y}x{Z}, liquid block x<128 z<120, Bingham tau 0.55 yield 1e-5, gravity "
+                        f"-2e-5, {bed.count} spheres r~U[3,5] (phi {phi}, seed {seed}), closed no-slip tank, BGK",
+            "steps": steps, "seconds_both_precisions": round(wall, 2), "rows": rows}
+
+
+def collapse(steps: int, warmup: int, err_steps: int = 2000, phi: float = 0.2, seed: int = 4):
+    """Granular collapse into fluid: 512x512x256 D3Q19 TRT tau 0.6 closed no-slip tank, free surface at z = 160,
+    lognormal spheres (mean radius 4, sigma 0.2) packed by RSA at solid fraction `phi` in the column x, y < 160,
+    z < 220 (seed 4), density ratio 2.6, Hertz (E = YOUNG_FS) + Coulomb (mu 0.6), gravity -1e-5. Timed FP32 and
+    FP64 cycles, then
+    FP32 against FP64 after `err_steps` cycles."""
+    X, Y, Z = 512, 512, 256
+    tau = 0.6
+    cfg = base_config(problemName="LIQUID_BOX", freeSurfaceSolve=1, lbSizeX=X, lbSizeY=Y, lbSizeZ=Z,
+                      boundary0=7, boundary1=7, boundary2=7, boundary3=7, boundary4=7, boundary5=7,
+                      initVisc=(tau - 0.5) / 3.0, lbFZ=-1.0e-5, density=2.6, youngMod=YOUNG_FS, poisson=0.3,
+                      restitution=0.88, frictionCoefPart=0.6, frictionCoefWall=0.6, viscTang=0.2, multiStep=10)
+    box = (1.0, X - 2.0, 1.0, Y - 2.0, 1.0, 159.0)
+    sig = 0.2
+    mu = math.log(4.0) - 0.5 * sig * sig
+    ev = 4.0 / 3.0 * math.pi * math.exp(3.0 * mu + 4.5 * sig * sig)
+    count = int(phi * 160.0 * 160.0 * 219.5 / ev)
+    bed = rsa_bed(seed, (0.5, 0.5, 0.5), (160.5, 160.5, 220.0), lambda r: math.exp(mu + sig * r.normal()), count,
+                  max_attempts=400000)
+    rec = element_records(bed.x, bed.r)
+    out = {"workload": f"granular collapse {X}x{Y}x{Z} TRT tau 0.6, free surface z=160, {bed.count} lognormal spheres "
+                       f"(mean r 4, sigma 0.2; phi {phi} in x,y<160 z<220, seed {seed}), rho 2.6, mu 0.6, gravity -1e-5, "
+                       f"closed no-slip tank"}
+    sims = {}
+    for prec in ("FP32", "FP64"):
+        sim = runtime.Simulation(config_from_cfg(cfg, collision="TRT", precision=prec, liquid_box=box), rec)
+        n_act = sim.active_nodes()
+        sec, kms, reb = timed(sim, steps, warmup)
+        key = prec.lower()
+        out[f"{key}_mlups"] = round(n_act * steps / sec / 1e6, 1)
+        out[f"{key}_ms_per_step"] = round(sec / steps * 1e3, 4)
+        out[f"{key}_lb_kernel_ms"] = round(kms, 4)
+        if prec == "FP32":
+            out["active_nodes_initial"] = n_act
+            out["particle_steps_per_s"] = round(bed.count * 10 * steps / sec, 1)
+            out["dem_rebuilds_in_timed_steps"] = reb
+            gbs = BYTES_PER_LUP["fp32"] * n_act / (kms * 1e-3) / 1e9
+            out["lb_kernel_hbm_frac"] = round(gbs / HBM_PEAK_GBS, 4)
+        sims[prec] = sim
+        print(f"[bench] collapse {prec} timed: {sec:.3f}s", file=sys.stderr, flush=True)
+    done = warmup + steps
+    if err_steps > done:
+        for s in sims.values():
+            s.cycle(err_steps - done - 1)
+            s.set_capture(True)
+            s.cycle(1)
+        a, b = sims["FP64"], sims["FP32"]
+        ta, tb = a.get("types"), b.get("types")
+        act = ((ta == 0) | (ta == 3)) & (ta == tb)
+        ma, mb = float(a.get("fluidMass")[0]), float(b.get("fluidMass")[0])
+        out["errors_at_step"] = err_steps
+        out["velocity_rel_l2_fp32_vs_fp64"] = _rel(b.get("u")[act], a.get("u")[act])
+        out["node_types_differing"] = int(np.sum(ta != tb))
+        out["mass_rel_fp32_vs_fp64"] = mb / ma - 1.0
+        out["position_max_abs_fp32_vs_fp64"] = float(np.max(np.abs(b.get("e_x")[:, 0] - a.get("e_x")[:, 0])))
+    for s in sims.values():
+        s.close()
+    return out
+
+
 def copy_probe(nbytes: int = 1 << 31, reps: int = 5) -> int:
     """A plain device-to-device copy of nbytes, reps times: the reference point that
     scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -189,8 +342,13 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-size", type=int, default=64)
-    ap.add_argument("--cpu-cycles", type=int, default=150)
+    ap.add_argument("--cpu-cycles", type=int, default=60)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
+    ap.add_argument("--horizon", type=int, default=5000,
+                    help="cycles over which FP32 is compared with FP64 on the headline workload (0: skip)")
+    ap.add_argument("--dam-break", type=int, default=10000, help="dam-break steps (0: skip)")
+    ap.add_argument("--collapse", type=int, default=2000,
+                    help="cycles after which the granular collapse compares FP32 with FP64 (0: skip the leg)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of known size (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -218,8 +376,25 @@ def main() -> int:
     vel_err = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
     f32, f64 = sim32.get("e_forces")[:, 0], sim64.get("e_forces")[:, 0]
     drag_err = float(np.linalg.norm(f32 - f64) / max(np.linalg.norm(f64), 1e-300))
+    extra = {}
+    if args.horizon > args.warmup + args.steps:
+        try:
+            extra["precision_horizon"] = {"steps": args.horizon, "rows": horizon(sim32, sim64, args.warmup + args.steps,
+                                                                                  args.horizon, 1000)}
+        except Exception as e:  # the headline line is printed in any case
+            extra["precision_horizon"] = {"error": str(e)}
     sim32.close()
     sim64.close()
+    if args.collapse:
+        try:
+            extra["granular_collapse"] = collapse(20, 3, args.collapse)
+        except Exception as e:
+            extra["granular_collapse"] = {"error": str(e)}
+    if args.dam_break:
+        try:
+            extra["dam_break"] = dam_break(args.dam_break)
+        except Exception as e:
+            extra["dam_break"] = {"error": str(e)}
     K = args.steps
     mlups32 = n_act * K / sec32 / 1e6
     mlups64 = n_act * K / sec64 / 1e6
@@ -253,8 +428,10 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": TRAFFIC_FUSED32_256 if args.size == 256 else None,
-                     "kernel": "k_fused32 (TRT, pull)", "bytes_per_lup": BYTES_PER_LUP["fp32"]},
+                     "kernel": "k_fused32 (TRT, pull)", "bytes_per_lup": BYTES_PER_LUP["fp32"],
+                     "copy_ceiling": copy_ceiling()},
     }
+    line.update(extra)
     if not args.no_adjoint:
         print("[bench] adjoint ...", file=sys.stderr, flush=True)
         line["adjoint"] = adjoint_cost()

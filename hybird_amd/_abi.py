@@ -69,7 +69,8 @@ class HybConfig(C.Structure):
         ("avalanchePosit", C.c_double),
         ("drumSpeed", C.c_double),
         ("fluidMass", C.c_double),
-        ("reserved", C.c_int32 * 4),
+        ("pairCapacity", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 

@@ -105,6 +105,41 @@ int ralloc(std::vector<Blk>& reg, const char* name, T** p, size_t count) {
     return rc;
 }
 
+// Error word of a handle: pinned host memory mapped into the device, so that a
+// kernel that finds a failure can store its code where the host reads it
+// without synchronising (see the device-side failures in hybird_amd.h).
+struct ErrWord {
+    int* host = nullptr;  // host address
+    int* dev = nullptr;   // device address of the same word
+};
+
+int errword_alloc(ErrWord& w) {
+    if (w.host) return HYB_OK;
+    void* h = nullptr;
+    HIPCHK(hipHostMalloc(&h, 64, hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(h, 0, 64);
+    void* dp = nullptr;
+    const hipError_t e = hipHostGetDevicePointer(&dp, h, 0);
+    if (e != hipSuccess) {
+        (void)hipHostFree(h);
+        return fail(HYB_E_HIP, std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
+    }
+    w.host = (int*)h;
+    w.dev = (int*)dp;
+    return HYB_OK;
+}
+
+void errword_free(ErrWord& w) {
+    if (w.host) (void)hipHostFree(w.host);
+    w.host = w.dev = nullptr;
+}
+
+int errword_code(const ErrWord& w) { return w.host ? __atomic_load_n(w.host, __ATOMIC_ACQUIRE) : 0; }
+
+void errword_clear(ErrWord& w) {
+    if (w.host) __atomic_store_n(w.host, 0, __ATOMIC_RELEASE);
+}
+
 void unregister(std::vector<Blk>& reg, const void* ptr) {
     for (size_t i = 0; i < reg.size(); ++i)
         if (reg[i].ptr == ptr) {
@@ -137,6 +172,7 @@ struct hyb_dem {
     bool inited = false;
     hipStream_t stream = nullptr;
     int64_t substeps = 0;
+    ErrWord err;             // device-side failures of the DEM kernels
     std::vector<Blk> saved;  // persistent device state
     // MercuryDPM-format saving (DEM.cpp:355-360, 1131-1146): captures of the
     // saving sub-steps of the last hyb_dem_step, one slot each
@@ -174,7 +210,10 @@ struct hyb_lb {
     int macCur = 0;
     CylDev cyl;
     uint32_t* solid = nullptr;
-    void* hf = nullptr;
+    // per-slot hydrodynamic run sums appended by the collision (segCap per slot)
+    HydroSeg* segs = nullptr;
+    uint32_t* segCount = nullptr;
+    uint32_t segCap = 0;
     double* visc = nullptr;
     double* mac = nullptr;
     void* scratch = nullptr;
@@ -184,7 +223,6 @@ struct hyb_lb {
     CoupleRecF* cpf = nullptr;
     NodeBox* boxes = nullptr;
     int* nslots = nullptr;  // [0] slots this step, [1] boxes marked last step
-    HydroPartial* part = nullptr;
     int slotCap = 0;
     uint32_t ctag = 0;       // coupling-step tag of the node ownership words
     hyb_dem* dem = nullptr;  // DEM handle coupled at init (read by the host views)
@@ -214,6 +252,7 @@ struct hyb_lb {
     size_t evUsed = 0;
     double profMs = 0.0;
     int64_t profLaunches = 0;
+    ErrWord err;             // device-side failures of the LB kernels
     std::vector<Blk> saved;  // persistent device state
 
     bool varVisc() const { return nonNewtonian || turbulenceOn; }
@@ -255,6 +294,7 @@ struct hyb_lb {
         p.typew = type;
         p.viscw = varVisc() ? visc : nullptr;
         p.cyl = cyl;
+        p.herr = err.dev;
         return p;
     }
     // planes are padded so that successive planes do not start on the same
@@ -267,6 +307,26 @@ struct hyb_lb {
 extern "C" {
 
 const char* hyb_last_error(void) { return g_err.c_str(); }
+
+// The pending device-side failure of a handle, if any, as a return code.
+static int dem_pending(const hyb_dem* d) {
+    const int e = d ? errword_code(d->err) : 0;
+    if (!e) return HYB_OK;
+    if (e == DEM_ERR_NUMERIC)
+        return fail(HYB_E_NUMERIC, "DEM: non-finite element velocity; the DEM stopped at that sub-step");
+    return fail(HYB_E_CAPACITY, "DEM device error " + std::to_string(e) +
+                                    " (1/4: ghost or pair capacity, 2: particle outside the cell grid, 3: stencil outside"
+                                    " the cell table, 5: more than 16 objects near one particle); the DEM stopped at"
+                                    " that sub-step");
+}
+
+static int lb_pending(const hyb_lb* l) {
+    const int e = l ? errword_code(l->err) : 0;
+    if (!e) return HYB_OK;
+    if (e == LB_ERR_SEGMENTS)
+        return fail(HYB_E_CAPACITY, "LB: a particle slot's hydrodynamic run list overflowed its capacity");
+    return fail(HYB_E_NUMERIC, "LB: non-finite density in a collision");
+}
 
 int hyb_device_count(int* count) {
     if (!count) return fail(HYB_E_ARG, "null count");
@@ -557,7 +617,7 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
     const int ghostsPer = np + np * (np - 1) / 2;
     p.Gcap = std::max(1, d->P * std::max(ghostsPer, 1));
     const int scap = d->P + p.Gcap;
-    p.pairCap = std::max(1024, scap * 16 + 1024);
+    p.pairCap = c.pairCapacity > 0 ? c.pairCapacity : std::max(1024, scap * 16 + 1024);
     DEMBuffers& b = d->b;
     std::memset(&b, 0, sizeof(b));
     b.scap = scap;
@@ -597,6 +657,9 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
     }
     // device buffers
     int rc;
+    if ((rc = errword_alloc(d->err))) return rc;
+    errword_clear(d->err);
+    p.herr = d->err.dev;
     std::vector<Blk>& R = d->saved;
     R.clear();
     if ((rc = ralloc(R, "el", &b.el, el.size())) || (rc = ralloc(R, "st", &b.st, 1)) ||
@@ -675,6 +738,7 @@ int hyb_dem_set_save_count(hyb_dem* d, int32_t saveCount) {
 
 int hyb_dem_step(hyb_dem* d) {  // DEM.cpp:331-376
     if (!d || !d->inited) return fail(HYB_E_STATE, "hyb_dem_step before hyb_dem_init");
+    if (int rc = dem_pending(d)) return rc;
     d->saveEvents.clear();
     for (int it = 0; it < d->multiStep; ++it) {
         d->demTimeStep++;
@@ -718,6 +782,7 @@ int hyb_dem_destroy(hyb_dem* d) {
     dfree(b.wcf); dfree(b.ocf); dfree(b.nearCyl);
     if (b.objs) (void)hipFree((void*)b.objs);
     save_free(d);
+    errword_free(d->err);
     delete d;
     return HYB_OK;
 }
@@ -792,6 +857,8 @@ int hyb_lb_create(const hyb_config* cfg, hyb_lb** out) {  // LB.cpp:85-188 + LB.
 static int lb_alloc(hyb_lb* l, int slotCap) {
     const size_t N = l->N;
     int rc;
+    if ((rc = errword_alloc(l->err))) return rc;
+    errword_clear(l->err);
     const size_t pb = l->popBytes();
     std::vector<Blk>& R = l->saved;
     R.clear();
@@ -824,12 +891,10 @@ static int lb_alloc(hyb_lb* l, int slotCap) {
     }
     l->slotCap = slotCap;
     if (slotCap > 0) {
-        const size_t hb = (l->prec == Prec::F64 ? sizeof(NodeForce64) : sizeof(NodeForce32)) * N;
-        HIPCHK(hipMalloc(&l->hf, hb));
-        HIPCHK(hipMemset(l->hf, 0, hb));  // no record carries a valid ownership word yet
         if ((rc = dalloc(&l->cp, slotCap)) || (rc = dalloc(&l->cpf, slotCap)) || (rc = dalloc(&l->boxes, slotCap)) ||
-            (rc = dalloc(&l->part, slotCap)))
+            (rc = dalloc(&l->segs, (size_t)slotCap * l->segCap)) || (rc = dalloc(&l->segCount, slotCap)))
             return rc;
+        HIPCHK(hipMemset(l->segCount, 0, sizeof(uint32_t) * (size_t)slotCap));
     }
     return HYB_OK;
 }
@@ -837,11 +902,8 @@ static int lb_alloc(hyb_lb* l, int slotCap) {
 // Particle-node classification of one coupling step: every slot marks the
 // active nodes strictly inside it with this step's tag (LB.cpp:475-495).
 static void lb_couple_launch(hyb_lb* l, hyb_dem* d) {
-    if (++l->ctag > SOLID_TAG_MAX) {  // tag wrap-around: clear ownership words and force records once
+    if (++l->ctag > SOLID_TAG_MAX) {  // tag wrap-around: clear the ownership words once
         launch_fill_u32(l->solid, 0u, l->N, l->stream);
-        if (l->hf)
-            launch_fill_u32((uint32_t*)l->hf,
-                            0u, (l->prec == Prec::F64 ? sizeof(NodeForce64) : sizeof(NodeForce32)) / 4 * l->N, l->stream);
         l->ctag = 1;
     }
     const LBKParams p = l->params();
@@ -857,6 +919,17 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
     const bool cylinders = d && !d->cylinders.empty();
     if (l->cfg.problem == HYB_PROBLEM_DRUM && !cylinders)
         return fail(HYB_E_ARG, "hyb_lb_init: DRUM needs the DEM handle that holds the drum");
+    // run-sum list capacity per slot: two runs per lattice row of the largest
+    // sphere's bounding box (a run splits at most once at a wavefront boundary)
+    // plus margin for rows cut by overlapping slots
+    if (d && d->P) {
+        double rmax = 0.0;
+        for (int e = 0; e < d->P; ++e) rmax = std::max(rmax, d->rec[(size_t)e * HYB_ELEMENT_RECORD + 2]);
+        rmax *= d->cfg.scale / l->unit.Length;
+        const double side = 2.0 * std::ceil(rmax) + 3.0;
+        if (!(side < 4096.0)) return fail(HYB_E_ARG, "particle radius too large for the lattice");
+        l->segCap = (uint32_t)(2.0 * side * side + 64.0);
+    }
     int rc = lb_alloc(l, d && d->P ? slotCap : 0);
     if (rc) return rc;
     // the curved-wall rule tests the type of the node behind as it was when streamed
@@ -880,12 +953,22 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
     // node types: box boundaries, DEM plane walls, objects (LB.cpp:324-339), gas/interface
     std::vector<WallDev> walls = d ? d->walls : std::vector<WallDev>();
     WallDev* dw = nullptr;
+    ObjDev* dob = nullptr;
+    struct Release {  // the two upload buffers, on every return path
+        WallDev*& w;
+        ObjDev*& o;
+        hipStream_t s;
+        ~Release() {
+            if (w || o) (void)hipStreamSynchronize(s);
+            if (w) (void)hipFree(w);
+            if (o) (void)hipFree(o);
+        }
+    } release{dw, dob, s};
     if (!walls.empty()) {
         HIPCHK(hipMalloc((void**)&dw, walls.size() * sizeof(WallDev)));
         HIPCHK(hipMemcpyAsync(dw, walls.data(), walls.size() * sizeof(WallDev), hipMemcpyHostToDevice, s));
     }
     std::vector<ObjDev> objs = d ? d->objects : std::vector<ObjDev>();
-    ObjDev* dob = nullptr;
     if (!objs.empty()) {
         HIPCHK(hipMalloc((void**)&dob, objs.size() * sizeof(ObjDev)));
         HIPCHK(hipMemcpyAsync(dob, objs.data(), objs.size() * sizeof(ObjDev), hipMemcpyHostToDevice, s));
@@ -903,7 +986,8 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
     }
     launch_lb_init_types(l->type, l->boundary, (int)walls.size(), dw, (int)objs.size(), dob, cs, l->cfg.problem,
                          region, p, s);
-    launch_lb_init_interface(l->type, p, s);
+    // the ownership words are cleared below, so their memory is the tagging scratch
+    launch_lb_init_interface(l->type, (uint8_t*)l->solid, p, s);
     launch_lb_init_links(l->type, l->link, l->slipw, l->curvw, p, s);
     HIPCHK(hipMemsetAsync(l->maxXYZ, 0, 3 * sizeof(int) * l->Z, s));
     launch_lb_max_active(l->link, l->maxXYZ, p, s);
@@ -938,8 +1022,6 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
     if (d && d->P) lb_couple_launch(l, d);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
-    if (dw) (void)hipFree(dw);
-    if (dob) (void)hipFree(dob);
     {
         std::vector<unsigned long long> cz(l->Z);
         HIPCHK(hipMemcpy(cz.data(), l->dcount, cz.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
@@ -974,6 +1056,7 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
 
 int hyb_lb_fs_step(hyb_lb* l) {  // LB.cpp:235-245
     if (!l || !l->inited) return fail(HYB_E_STATE, "hyb_lb_fs_step before init");
+    if (int rc = lb_pending(l)) return rc;
     if (!l->freeSurface) return HYB_OK;
     if (++l->fsTag >= 0x40000000u) {  // tag wrap-around (30-bit tags): clear the tagged words once
         launch_fill_u32((uint32_t*)l->fsb.claim, 0u, 2 * l->N, l->stream);
@@ -1000,6 +1083,8 @@ int hyb_lb_fs_step(hyb_lb* l) {  // LB.cpp:235-245
 
 int hyb_lb_couple(hyb_lb* l, hyb_dem* d, int32_t* newNeighborList) {  // LB.cpp:247-280
     if (!l || !l->inited) return fail(HYB_E_STATE, "hyb_lb_couple before init");
+    if (int rc = lb_pending(l)) return rc;
+    if (int rc = dem_pending(d)) return rc;
     // The GPU re-tests every node against every particle slot at each coupling
     // step, so the rebuild handshake flag is consumed unconditionally.
     if (newNeighborList) *newNeighborList = 0;
@@ -1011,6 +1096,7 @@ int hyb_lb_couple(hyb_lb* l, hyb_dem* d, int32_t* newNeighborList) {  // LB.cpp:
 
 int hyb_lb_step(hyb_lb* l, hyb_dem* d) {  // LB.cpp:221-233 fused into one kernel
     if (!l || !l->inited) return fail(HYB_E_STATE, "hyb_lb_step before init");
+    if (int rc = lb_pending(l)) return rc;
     const LBKParams p = l->params();
     const bool withParticles = d && d->P && l->slotCap > 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1036,16 +1122,12 @@ int hyb_lb_step(hyb_lb* l, hyb_dem* d) {  // LB.cpp:221-233 fused into one kerne
         fv.hFi = l->fsb.hFi;
         fv.linkw = l->link;
     }
+    const HydroSink sink{l->segs, l->segCount, l->segCap};
     launch_lb_fused(l->prec, l->steps > 0, l->varVisc(), l->cfg.collision, l->f[l->cur], l->f[l->cur ^ 1], l->link,
-                    withParticles ? l->solid : nullptr, l->cp, l->cpf, withParticles ? l->hf : nullptr, l->visc,
+                    withParticles ? l->solid : nullptr, l->cp, l->cpf, withParticles ? &sink : nullptr, l->visc,
                     l->capture ? l->mac : nullptr, l->freeSurface ? &fv : nullptr, p, l->stream);
     if (l->profiling) HIPCHK(hipEventRecord(e1, l->stream));
-    if (withParticles) {
-        launch_hydro_partials(l->nslots, l->slotCap, l->boxes, l->cp, l->link, l->prec, l->hf,
-                              l->freeSurface ? l->fsb.mass : nullptr, l->freeSurface ? fv.rhoPrev : nullptr, l->part,
-                              p, l->stream);
-        launch_dem_hydro_combine(d->p, d->b, l->part, l->unit, l->stream);
-    }
+    if (withParticles) launch_dem_hydro_combine(d->p, d->b, sink, l->unit, l->stream);
     HIPCHK(hipGetLastError());
     l->cur ^= 1;
     if (l->curvw) l->macCur ^= 1;
@@ -1228,13 +1310,14 @@ int hyb_lb_destroy(hyb_lb* l) {
     for (auto& ev : l->evs) (void)hipEventDestroy(ev);
     if (l->f[0]) (void)hipFree(l->f[0]);
     if (l->f[1]) (void)hipFree(l->f[1]);
-    if (l->hf) (void)hipFree(l->hf);
     if (l->scratch) (void)hipFree(l->scratch);
     dfree(l->type); dfree(l->link); dfree(l->slipw); dfree(l->solid); dfree(l->visc); dfree(l->mac); dfree(l->maxXYZ);
     dfree(l->curvw); dfree(l->macBuf[0]); dfree(l->macBuf[1]);
-    dfree(l->dcount); dfree(l->cp); dfree(l->cpf); dfree(l->boxes); dfree(l->nslots); dfree(l->part);
+    dfree(l->dcount); dfree(l->cp); dfree(l->cpf); dfree(l->boxes); dfree(l->nslots); dfree(l->segs);
+    dfree(l->segCount);
     for (void* ck : l->adjCk) (void)hipFree(ck);
     if (l->ffield) (void)hipFree(l->ffield);
+    errword_free(l->err);
     FSBuffers& b = l->fsb;
     if (b.rho[0]) (void)hipFree(b.rho[0]);
     if (b.rho[1]) (void)hipFree(b.rho[1]);
@@ -1246,6 +1329,8 @@ int hyb_lb_destroy(hyb_lb* l) {
 
 int hyb_cycle(hyb_lb* l, hyb_dem* d, int32_t n) {  // hybird.cpp:35-66 without IO
     if (!l || !d) return fail(HYB_E_ARG, "hyb_cycle: null handle");
+    if (int rc = dem_pending(d)) return rc;
+    if (int rc = lb_pending(l)) return rc;
     const bool demSolve = l->cfg.demSolve != 0, lbSolve = l->cfg.lbSolve != 0;
     for (int k = 0; k < n; ++k) {
         ++l->time;
@@ -1267,6 +1352,10 @@ int hyb_cycle(hyb_lb* l, hyb_dem* d, int32_t n) {  // hybird.cpp:35-66 without I
 int hyb_sync(hyb_lb* l) {
     hipStream_t s = l ? l->stream : shared_stream();
     HIPCHK(hipStreamSynchronize(s));
+    if (l) {
+        if (int rc = dem_pending(l->dem)) return rc;
+        if (int rc = lb_pending(l)) return rc;
+    }
     return HYB_OK;
 }
 
@@ -1369,6 +1458,7 @@ int hyb_lb_get(hyb_lb* l, const char* cname, void* host, size_t cap, size_t* nee
     if (!l || !cname) return fail(HYB_E_ARG, "hyb_lb_get: null argument");
     if (!l->inited) return fail(HYB_E_STATE, "hyb_lb_get before init");
     HIPCHK(hipStreamSynchronize(l->stream));
+    if (int rc = lb_pending(l)) return rc;
     const std::string name(cname);
     const size_t N = l->N;
     const LBKParams p = l->params();
@@ -1564,9 +1654,7 @@ int hyb_dem_get(hyb_dem* d, const char* cname, void* host, size_t cap, size_t* n
     DEMState st;
     std::memset(&st, 0, sizeof(st));
     if (P) HIPCHK(hipMemcpy(&st, d->b.st, sizeof(st), hipMemcpyDeviceToHost));
-    if (st.error) return fail(HYB_E_CAPACITY, "DEM device error " + std::to_string(st.error) +
-                                                   " (1/4: ghost or pair capacity, 2: particle outside the cell grid, 3: stencil outside the cell table,"
-                                                   " 5: more than 16 objects near one particle)");
+    if (int rc = dem_pending(d)) return rc;
     std::vector<double> el;
     if (P) {
         el.resize((size_t)EF_COUNT * P);
@@ -1723,6 +1811,41 @@ int hyb_dem_get(hyb_dem* d, const char* cname, void* host, size_t cap, size_t* n
     return fail(HYB_E_ARG, "unknown DEM field " + name);
 }
 
+int hyb_copy_bandwidth(size_t bytes, int32_t width, int32_t reps, double* gbps) {
+    if (!gbps || reps < 1 || bytes < 4096 || bytes % 16) return fail(HYB_E_ARG, "hyb_copy_bandwidth: bad arguments");
+    if (width != 4 && width != 8 && width != 16) return fail(HYB_E_ARG, "hyb_copy_bandwidth: width must be 4, 8 or 16");
+    hipStream_t s = shared_stream();
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto release = [&]() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (a) (void)hipFree(a);
+        if (b) (void)hipFree(b);
+    };
+    hipError_t e = hipMalloc(&a, bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, bytes, s);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        release();
+        return fail(HYB_E_HIP, std::string("hyb_copy_bandwidth: ") + hipGetErrorString(e));
+    }
+    launch_copy_probe(width, a, b, bytes, s);  // warm-up
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps; ++r) launch_copy_probe(width, a, b, bytes, s);
+    (void)hipEventRecord(e1, s);
+    e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess) e = hipGetLastError();
+    release();
+    if (e != hipSuccess) return fail(HYB_E_HIP, std::string("hyb_copy_bandwidth: ") + hipGetErrorString(e));
+    *gbps = 2.0 * (double)bytes * reps / ((double)ms * 1e-3) / 1e9;
+    return HYB_OK;
+}
+
 int hyb_lb_set_profiling(hyb_lb* l, int32_t enabled) {
     if (!l) return fail(HYB_E_ARG, "null handle");
     l->profiling = enabled != 0;
@@ -1784,7 +1907,7 @@ struct LbScalars {
     int64_t steps, fsSteps;
     uint64_t activeCount;
     double totalMass;
-    uint8_t capture, macValid, havePart, pad[5];
+    uint8_t capture, macValid, havePart, macCur, pad[4];
 };
 
 struct DemScalars {
@@ -1891,7 +2014,7 @@ int hyb_state_save(hyb_lb* l, hyb_dem* d, const char* path) {
         s.ctag = l->ctag; s.fsTag = l->fsTag; s.actTag = l->actTag; s.time = l->time;
         s.steps = l->steps; s.fsSteps = l->fsSteps;
         s.activeCount = l->activeCount; s.totalMass = l->totalMass;
-        s.capture = l->capture; s.macValid = l->macValid; s.havePart = l->havePart;
+        s.capture = l->capture; s.macValid = l->macValid; s.havePart = l->havePart; s.macCur = (uint8_t)l->macCur;
         if (!wr(fh.f, &s, sizeof(s))) return fail(HYB_E_ARG, "state file: write error");
         if ((rc = write_blocks(fh.f, l->saved))) return rc;
     }
@@ -1928,22 +2051,27 @@ int hyb_state_load(hyb_lb* l, hyb_dem* d, const char* path) {
         if (!rd(fh.f, &s, sizeof(s))) return fail(HYB_E_ARG, "state file: truncated");
         if (s.X != l->X || s.Y != l->Y || s.Z != l->Z || s.prec != (int32_t)l->prec || s.popBytes != l->popBytes())
             return fail(HYB_E_ARG, "state file: lattice size or precision differs");
+        // buffer selectors index two-element arrays
+        if ((s.cur | s.fsCur | s.rhoCur | s.fluidMassBuf) & ~1 || s.macCur > 1)
+            return fail(HYB_E_ARG, "state file: corrupt buffer selector");
         auto lazy = [&](const std::string& name) -> int {
             if (name == "mac" && !l->mac) return ralloc(l->saved, "mac", &l->mac, 7 * l->N);
             if (name == "ffield" && !l->ffield) return ralloc(l->saved, "ffield", &l->ffield, 3 * l->N);
             return HYB_OK;
         };
-        if ((rc = read_blocks(fh.f, l->saved, lazy))) return rc;
+        if ((rc = read_blocks(fh.f, l->saved, lazy))) {
+            l->inited = false;  // partly overwritten: unusable until destroyed
+            return rc;
+        }
+        l->macCur = s.macCur;
+        errword_clear(l->err);
         l->cur = s.cur; l->fsCur = s.fsCur; l->rhoCur = s.rhoCur; l->fluidMassBuf = s.fluidMassBuf;
         l->ctag = s.ctag; l->fsTag = s.fsTag; l->actTag = s.actTag; l->time = s.time;
         l->steps = s.steps; l->fsSteps = s.fsSteps;
         l->activeCount = s.activeCount; l->totalMass = s.totalMass;
         l->capture = s.capture != 0; l->macValid = s.macValid != 0; l->havePart = s.havePart != 0;
-        // per-step scratch: force records must not match any ownership word yet
-        if (l->hf) {
-            const size_t hb = (l->prec == Prec::F64 ? sizeof(NodeForce64) : sizeof(NodeForce32)) * l->N;
-            HIPCHK(hipMemset(l->hf, 0, hb));
-        }
+        // per-step scratch: the run-sum lists start empty
+        if (l->segCount) HIPCHK(hipMemset(l->segCount, 0, sizeof(uint32_t) * (size_t)l->slotCap));
         for (void* ck : l->adjCk) (void)hipFree(ck);
         l->adjCk.clear();
         l->adjSteps = -1;
@@ -1953,7 +2081,11 @@ int hyb_state_load(hyb_lb* l, hyb_dem* d, const char* path) {
         if (!rd(fh.f, &s, sizeof(s))) return fail(HYB_E_ARG, "state file: truncated");
         if (s.P != d->P || s.pairCap != d->p.pairCap || s.Gcap != d->p.Gcap || s.multiStep != d->multiStep)
             return fail(HYB_E_ARG, "state file: particle set or DEM configuration differs");
-        if ((rc = read_blocks(fh.f, d->saved, [](const std::string&) { return (int)HYB_OK; }))) return rc;
+        if ((rc = read_blocks(fh.f, d->saved, [](const std::string&) { return (int)HYB_OK; }))) {
+            d->inited = false;  // partly overwritten: unusable until destroyed
+            return rc;
+        }
+        errword_clear(d->err);
         d->demTime = s.demTime; d->demTimeStep = s.demTimeStep; d->substeps = s.substeps;
     }
     return HYB_OK;

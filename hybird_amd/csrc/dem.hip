@@ -9,7 +9,7 @@
 //     per-particle list of the pairs each particle takes part in.
 //  2. k_dem_pairs: one thread per pair evaluates the contact (DEM.cpp:1801-1900,
 //     2138-2222) and writes each element's signed increments into that
-//     element's slot of the pair.
+//     element's slot of the pair (and, on saving sub-steps, its statistics).
 //  3. k_dem_gather: per element, the increments in pair-list order (the same
 //     order as the reference's sequential loop, so no atomics and bit-identical
 //     sums), the wall contact, Newton-Euler (DEM.cpp:1149-1181), the Gear
@@ -23,6 +23,7 @@
 #include "common.h"
 #include "dem_kernels.h"
 #include "hybird_amd.h"
+#include "lattice.h"
 
 namespace hyb {
 
@@ -41,6 +42,13 @@ struct El {
 };
 
 constexpr int RB = 1024;  // rebuild workgroup size
+
+// Records a device error: the sticky DEMState word (which stops every later DEM
+// kernel) and the handle's host-visible error word.
+__device__ __forceinline__ void dem_error(const DEMParams& p, const DEMBuffers& b, int code) {
+    b.st->error = code;
+    if (p.herr) *p.herr = code;
+}
 
 // In-place exclusive scan of a[0..n) by the whole (single) workgroup; returns the total.
 __device__ int block_exscan(int* a, int n) {
@@ -163,18 +171,23 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
     __shared__ double red[RB];
     __shared__ int doit;
     __shared__ int sh_err;
+    if (b.st->error) return;
     if (t == 0) sh_err = 0;
+    __syncthreads();
     // ---- evalMaxDisp (DEM.cpp:1314-1324) and the trigger (DEM.cpp:342-347)
     if (!force) {
-        // max |x1|^2 over elements, from the per-wavefront maxima the corrector left
+        // max |x1|^2 over elements, from the per-wavefront maxima the corrector
+        // left; a NaN or infinite maximum stops the DEM (error 6)
         double m = 0.0;
         const int nw = (P + 63) / 64;
         for (int k = t; k < nw; k += RB) {
             const double v = b.vmax[k];
+            if (!(v <= 1.79769313486231570e308)) sh_err = DEM_ERR_NUMERIC;
             if (v > m) m = v;
         }
         red[t] = m;
         __syncthreads();
+        if (sh_err) { if (t == 0) dem_error(p, b, DEM_ERR_NUMERIC); return; }
         for (int w = RB / 2; w > 0; w >>= 1) {
             if (t < w && red[t + w] > red[t]) red[t] = red[t + w];
             __syncthreads();
@@ -252,7 +265,7 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
         }
         if (nSide > p.Gcap) { if (t == 0) sh_err = 1; }
         __syncthreads();
-        if (sh_err) { if (t == 0) b.st->error = 1; return; }
+        if (sh_err) { if (t == 0) dem_error(p, b, 1); return; }
         // ---- corner ghosts: every pair g1 < g2 of side ghosts of one particle (DEM.cpp:1620-1635)
         for (int g = t; g < nSide; g += RB) {
             const int e = (int)b.ghostIndex[g];
@@ -263,7 +276,7 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
         __syncthreads();
         const int nCorner = block_exscan(b.work, nSide);
         G = nSide + nCorner;
-        if (G > p.Gcap) { if (t == 0) b.st->error = 1; return; }
+        if (G > p.Gcap) { if (t == 0) dem_error(p, b, 1); return; }
         for (int g = t; g < nSide; g += RB) {
             const int e = (int)b.ghostIndex[g];
             int slot = nSide + b.work[g];
@@ -300,7 +313,7 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
         }
     }
     __syncthreads();
-    if (sh_err) { if (t == 0) b.st->error = 2; return; }
+    if (sh_err) { if (t == 0) dem_error(p, b, 2); return; }
     block_exscan(b.cellCount, nc + 1);
     for (int c = t; c < nc; c += RB) b.work[c] = b.cellCount[c];
     __syncthreads();
@@ -338,9 +351,9 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
         else b.work[w] = c;
     }
     __syncthreads();
-    if (sh_err) { if (t == 0) b.st->error = 3; return; }
+    if (sh_err) { if (t == 0) dem_error(p, b, 3); return; }
     const int T = block_exscan(b.work, W);
-    if (T > p.pairCap) { if (t == 0) b.st->error = 4; return; }
+    if (T > p.pairCap) { if (t == 0) dem_error(p, b, 4); return; }
     for (int w = t; w < W; w += RB) walk_pairs(p, b, E, S, (unsigned)w, b.pairs + 2 * (size_t)b.work[w]);
     if (t == 0) b.st->nPairs = T;
     // ---- near-wall list (DEM.cpp:1496-1513): the first wall within range
@@ -379,7 +392,7 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
         }
     }
     __syncthreads();
-    if (sh_err) { if (t == 0) b.st->error = 5; return; }
+    if (sh_err) { if (t == 0) dem_error(p, b, 5); return; }
     // ---- per-particle pair lists, ascending pair index
     for (int e = t; e < P; e += RB) b.cursor[e] = 0;
     __syncthreads();
@@ -493,10 +506,7 @@ __device__ __forceinline__ void slot_pred(const El& E, const DEMBuffers& b, int 
     }
 }
 
-// One thread per neighbour pair (grid-stride): the contact of DEM.cpp:1801-1900
-// evaluated at the predicted state. The signed increments for each of the two
-// elements are written into that element's slot of the pair in its csr list, so
-// that k_dem_gather adds them in pair-list order.
+// Contact statistics and wall/object force records written by the contact code.
 __device__ __forceinline__ void set_stat(StatRec& s, double ov, double fn, double ft, const V3& en, const V3& et) {
     s.ov = ov; s.fn = fn; s.ft = ft;
     s.en[0] = en.x; s.en[1] = en.y; s.en[2] = en.z;
@@ -514,66 +524,79 @@ __device__ __forceinline__ void set_force_t(ContactForce& c, const V3& ft) {
     c.flag = 3.0;
 }
 
-__global__ void __launch_bounds__(256) k_dem_pairs(DEMParams p, DEMBuffers b) {
+// The contact of neighbour pair k (DEM.cpp:1801-1900) at the predicted state:
+// the signed increments for the pair's first (ri) and second (rj) element, and
+// its statistics record st.
+__device__ __forceinline__ void pair_contact(const DEMParams& p, const DEMBuffers& b, const El& E, int k, PairRec& ri,
+                                             PairRec& rj, StatRec& st) {
 #pragma clang fp contract(off)
-    const El E{b.el, p.P};
     const int P = p.P;
+    const int n0 = (int)b.pairs[2 * k], n1 = (int)b.pairs[2 * k + 1];
+    V3 xI, vI, xJ, vJ;
+    int eI, eJ;
+    slot_pred(E, b, P, n0, xI, vI, eI);
+    slot_pred(E, b, P, n1, xJ, vJ, eJ);
+    const double rI = E.at(EF_RAD, eI), rJ = E.at(EF_RAD, eJ);
+    const double sigij = rI + rJ;
+    const double sigij2 = sigij * sigij;
+    const V3 vd = xJ - xI;
+    ri.flag = rj.flag = 0;
+    ri.pad = rj.pad = 0;
+    st = StatRec{(double)n0, (double)n1, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0};
+    if (!(vd.norm2() < sigij2)) return;
+    const double distance = vd.norm();
+    const double overlap = rI + rJ - distance;
+    const V3 relVel = vJ - vI;
+    const V3 en = vd / distance;
+    const double normRelVel = relVel.dot(en);
+    const V3 normalRelVel = en * normRelVel;
+    const double mI = E.at(EF_M, eI), mJ = E.at(EF_M, eJ);
+    const double effMass = mI * mJ / (mI + mJ);
+    const double effRad = rI * rJ / (rI + rJ);
+    const double fn = normal_contact(p, overlap, normRelVel, effRad, effMass);
+    const V3 normalForce = en * fn;
+    const V3 vecRadI = rI * en;
+    const V3 vecRadJ = -rJ * en;
+    // radiusVec is zero for single-sphere elements
+    const V3 centerDistI = vecRadI + V3(0.0, 0.0, 0.0);
+    const V3 centerDistJ = vecRadJ + V3(0.0, 0.0, 0.0);
+    const V3 mn = -1.0 * normalForce;
+    ri.fn[0] = mn.x; ri.fn[1] = mn.y; ri.fn[2] = mn.z;  // F - n == F + (-n)
+    rj.fn[0] = normalForce.x; rj.fn[1] = normalForce.y; rj.fn[2] = normalForce.z;
+    ri.flag = rj.flag = 1;
+    const V3 wI = E.v3(EF_WPG, eI), wJ = E.v3(EF_WPG, eJ);
+    const V3 relVelContact = relVel - wI.cross(vecRadI) + wJ.cross(vecRadJ);
+    const V3 tangRelVelContact = relVelContact - normalRelVel;
+    const double nt = tangRelVelContact.norm();
+    if (nt != 0.0) {
+        const double ft = tangential_contact(p, nt, fn, effRad, effMass, p.fricPart);
+        const V3 et = tangRelVelContact / nt;
+        const V3 tangForce = ft * et;
+        const V3 mI3 = centerDistI.cross(tangForce);
+        const V3 mJ3 = -1.0 * centerDistJ.cross(tangForce);
+        const V3 mt = -1.0 * tangForce;
+        ri.ft[0] = tangForce.x; ri.ft[1] = tangForce.y; ri.ft[2] = tangForce.z;
+        ri.m[0] = mI3.x; ri.m[1] = mI3.y; ri.m[2] = mI3.z;
+        rj.ft[0] = mt.x; rj.ft[1] = mt.y; rj.ft[2] = mt.z;
+        rj.m[0] = mJ3.x; rj.m[1] = mJ3.y; rj.m[2] = mJ3.z;
+        ri.flag = rj.flag = 3;
+        set_stat(st, overlap, fn, ft, en, et);
+    }
+}
+
+// One thread per neighbour pair (grid-stride): the contact at the predicted
+// state; the signed increments for each of the two elements go into that
+// element's slot of the pair in its csr list, so that k_dem_gather adds them in
+// pair-list order.
+__global__ void __launch_bounds__(256) k_dem_pairs(DEMParams p, DEMBuffers b) {
+    const El E{b.el, p.P};
+    if (b.st->error) return;
     const int T = b.st->nPairs;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < T; k += gridDim.x * blockDim.x) {
         const uint32_t si = b.posI[k], sj = b.posJ[k];
-        const int n0 = (int)b.pairs[2 * k], n1 = (int)b.pairs[2 * k + 1];
-        V3 xI, vI, xJ, vJ;
-        int eI, eJ;
-        slot_pred(E, b, P, n0, xI, vI, eI);
-        slot_pred(E, b, P, n1, xJ, vJ, eJ);
-        const double rI = E.at(EF_RAD, eI), rJ = E.at(EF_RAD, eJ);
-        const double sigij = rI + rJ;
-        const double sigij2 = sigij * sigij;
-        const V3 vd = xJ - xI;
         PairRec ri, rj;
-        ri.flag = rj.flag = 0;
-        ri.pad = rj.pad = 0;
-        StatRec st{(double)n0, (double)n1, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0};
-        if (vd.norm2() < sigij2) {
-            const double distance = vd.norm();
-            const double overlap = rI + rJ - distance;
-            const V3 relVel = vJ - vI;
-            const V3 en = vd / distance;
-            const double normRelVel = relVel.dot(en);
-            const V3 normalRelVel = en * normRelVel;
-            const double mI = E.at(EF_M, eI), mJ = E.at(EF_M, eJ);
-            const double effMass = mI * mJ / (mI + mJ);
-            const double effRad = rI * rJ / (rI + rJ);
-            const double fn = normal_contact(p, overlap, normRelVel, effRad, effMass);
-            const V3 normalForce = en * fn;
-            const V3 vecRadI = rI * en;
-            const V3 vecRadJ = -rJ * en;
-            // radiusVec is zero for single-sphere elements
-            const V3 centerDistI = vecRadI + V3(0.0, 0.0, 0.0);
-            const V3 centerDistJ = vecRadJ + V3(0.0, 0.0, 0.0);
-            const V3 mn = -1.0 * normalForce;
-            ri.fn[0] = mn.x; ri.fn[1] = mn.y; ri.fn[2] = mn.z;  // F - n == F + (-n)
-            rj.fn[0] = normalForce.x; rj.fn[1] = normalForce.y; rj.fn[2] = normalForce.z;
-            ri.flag = rj.flag = 1;
-            const V3 wI = E.v3(EF_WPG, eI), wJ = E.v3(EF_WPG, eJ);
-            const V3 relVelContact = relVel - wI.cross(vecRadI) + wJ.cross(vecRadJ);
-            const V3 tangRelVelContact = relVelContact - normalRelVel;
-            const double nt = tangRelVelContact.norm();
-            if (nt != 0.0) {
-                const double ft = tangential_contact(p, nt, fn, effRad, effMass, p.fricPart);
-                const V3 et = tangRelVelContact / nt;
-                const V3 tangForce = ft * et;
-                const V3 mI3 = centerDistI.cross(tangForce);
-                const V3 mJ3 = -1.0 * centerDistJ.cross(tangForce);
-                const V3 mt = -1.0 * tangForce;
-                ri.ft[0] = tangForce.x; ri.ft[1] = tangForce.y; ri.ft[2] = tangForce.z;
-                ri.m[0] = mI3.x; ri.m[1] = mI3.y; ri.m[2] = mI3.z;
-                rj.ft[0] = mt.x; rj.ft[1] = mt.y; rj.ft[2] = mt.z;
-                rj.m[0] = mJ3.x; rj.m[1] = mJ3.y; rj.m[2] = mJ3.z;
-                ri.flag = rj.flag = 3;
-                if (b.pstat) set_stat(st, overlap, fn, ft, en, et);
-            }
-        }
+        StatRec st;
+        pair_contact(p, b, E, k, ri, rj, st);
         if (b.pstat) b.pstat[k] = st;
         if (si != 0xFFFFFFFFu) b.prec[si] = ri;
         if (sj != 0xFFFFFFFFu) b.prec[sj] = rj;
@@ -645,6 +668,7 @@ __global__ void __launch_bounds__(64) k_dem_predict(DEMParams p, DEMBuffers b) {
 __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffers& b, int e);
 
 __global__ void __launch_bounds__(64) k_dem_gather(DEMParams p, DEMBuffers b) {
+    if (b.st->error) return;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     double v2 = e < p.P ? gather_elem(p, b, e) : 0.0;
     v2 = wave_max(v2);
@@ -896,30 +920,68 @@ __global__ void k_dem_couple_table(DEMParams p, DEMBuffers b, CoupleRec* __restr
     cpf[s] = q;
 }
 
-// Element hydrodynamic force and torque: own slot first, then its ghosts in
-// slot order, compensated FP64; then scaled to physical units (LB.cpp:1907-1918).
-__global__ void k_dem_hydro_combine(DEMParams p, DEMBuffers b, const HydroPartial* __restrict__ part, Units u) {
+// Element hydrodynamic force and torque (LB.cpp:1897-1918): the run sums the
+// collision appended for the element's own slot and for its ghosts' slots,
+// added exactly (fixed point, so the result does not depend on the order of the
+// appends), then scaled to physical units. One wavefront per element; lane l
+// accumulates component l % 7 of every ninth entry (l / 7 < 9), so that each
+// entry component is split once; the nine partial sums per component are then
+// added through LDS. The lists are emptied for the next LB step.
+constexpr int COMBINE_BLOCK = 256;
+constexpr int COMBINE_ROWS = 9;  // entries per pass (9 x 7 components = 63 lanes)
+__global__ void __launch_bounds__(COMBINE_BLOCK) k_dem_hydro_combine(DEMParams p, DEMBuffers b, HydroSink hs, Units u) {
 #pragma clang fp contract(off)
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= p.P) return;
+    __shared__ long long red[COMBINE_BLOCK / 64][64][4];
+    __shared__ double val[COMBINE_BLOCK / 64][7];
+    const int lane = (int)(threadIdx.x & 63);
+    const int w = (int)(threadIdx.x >> 6);
+    const int e = (int)((blockIdx.x * (unsigned)COMBINE_BLOCK + threadIdx.x) >> 6);
+    if (e >= p.P) return;  // whole wavefronts
     const El E{b.el, p.P};
-    double s[7], c[7];
-    for (int k = 0; k < 7; ++k) { s[k] = part[e].s[k]; c[k] = 0.0; }
-    const int n = b.ghostOfCount[e] < 8 ? b.ghostOfCount[e] : 8;
-    for (int i = 0; i < n; ++i) {
-        const HydroPartial& q = part[b.ghostOf[e * 8 + i]];
-        for (int k = 0; k < 7; ++k) {
-            const double v = q.s[k];
-            const double t = s[k] + v;
-            if (fabs(s[k]) >= fabs(v)) c[k] += (s[k] - t) + v;
-            else c[k] += (v - t) + s[k];
-            s[k] = t;
+    const int ng = b.ghostOfCount[e] < 8 ? b.ghostOfCount[e] : 8;
+    const int q = lane % 7, j = lane / 7;
+    long long acc[4] = {0, 0, 0, 0};
+    for (int k = -1; k < ng; ++k) {
+        const uint32_t slot = k < 0 ? (uint32_t)e : (uint32_t)b.ghostOf[e * 8 + k];
+        const uint32_t cnt = hs.count[slot];
+        const uint32_t n = cnt < hs.cap ? cnt : hs.cap;
+        const HydroSeg* L = hs.segs + (size_t)slot * hs.cap;
+        if (j < COMBINE_ROWS) {
+            constexpr uint32_t AHEAD = 4;  // loads issued before the first split
+            for (uint32_t i = (uint32_t)j; i < n; i += AHEAD * COMBINE_ROWS) {
+                double v[AHEAD];
+#pragma unroll
+                for (uint32_t a = 0; a < AHEAD; ++a) {
+                    const uint32_t ia = i + a * COMBINE_ROWS;
+                    v[a] = ia < n ? L[ia].s[q] : 0.0;
+                }
+#pragma unroll
+                for (uint32_t a = 0; a < AHEAD; ++a) exact_local(acc, v[a]);
+            }
         }
     }
-    for (int k = 0; k < 7; ++k) s[k] += c[k];
-    E.set3(EF_FH, e, V3(s[0], s[1], s[2]) * u.Force);
-    E.set3(EF_MH, e, V3(s[3], s[4], s[5]) * u.Torque);
-    E.at(EF_FVOL, e) = s[6] * u.Volume;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[w][lane][i] = acc[i];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // every lane has read the counts above; empty the lists for the next step
+    if (lane <= ng) hs.count[lane == 0 ? (uint32_t)e : (uint32_t)b.ghostOf[e * 8 + lane - 1]] = 0u;
+    if (lane < 7) {
+        long long t[4] = {0, 0, 0, 0};
+        for (int r = 0; r < COMBINE_ROWS; ++r)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) t[i] += red[w][lane + 7 * r][i];
+        val[w][lane] = exact_value((const unsigned long long*)t);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane != 0) return;
+    const double* v = val[w];
+    E.set3(EF_FH, e, V3(v[0], v[1], v[2]) * u.Force);
+    E.set3(EF_MH, e, V3(v[3], v[4], v[5]) * u.Torque);
+    E.at(EF_FVOL, e) = v[6] * u.Volume;
 }
 
 }  // namespace
@@ -1028,16 +1090,19 @@ void launch_dem_gather(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {
     k_dem_gather<<<(p.P + 63) / 64, 64, 0, s>>>(p, b);
 }
 
+
+
 void launch_dem_couple_table(const DEMParams& p, const DEMBuffers& b, CoupleRec* cp, CoupleRecF* cpf, int* nslots,
                              const Units& u, hipStream_t s) {
     const int cap = p.P + p.Gcap;
     k_dem_couple_table<<<(cap + 255) / 256 > 0 ? (cap + 255) / 256 : 1, 256, 0, s>>>(p, b, cp, cpf, nslots, u);
 }
 
-void launch_dem_hydro_combine(const DEMParams& p, const DEMBuffers& b, const HydroPartial* part, const Units& u,
+void launch_dem_hydro_combine(const DEMParams& p, const DEMBuffers& b, const HydroSink& hs, const Units& u,
                               hipStream_t s) {
     if (p.P <= 0) return;
-    k_dem_hydro_combine<<<(p.P + 255) / 256, 256, 0, s>>>(p, b, part, u);
+    const int waves_per_block = COMBINE_BLOCK / 64;
+    k_dem_hydro_combine<<<(p.P + waves_per_block - 1) / waves_per_block, COMBINE_BLOCK, 0, s>>>(p, b, hs, u);
 }
 
 }  // namespace hyb

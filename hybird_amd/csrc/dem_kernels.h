@@ -46,7 +46,13 @@ struct DEMParams {
     double demF[3];
     int ncyl;              // cylinders (DEM.cpp:870-935)
     CylDev cyl[2];
+    int* herr;             // the handle's error word (mapped pinned host memory), nullptr: none
 };
+
+// DEMState::error codes: 1 ghost capacity, 2 particle outside the cell grid,
+// 3 stencil outside the cell table, 4 pair capacity, 5 more than NEAR_OBJ objects
+// near one particle, 6 non-finite element velocity.
+constexpr int DEM_ERR_NUMERIC = 6;
 
 // Objects a particle can have in range at once (a rebuild that finds more
 // reports DEMState::error 5).
@@ -60,7 +66,7 @@ struct DEMState {
     int nPairs;
     int nNearWall;
     int newNeighborList;
-    int error;             // nonzero: capacity overflow or particle outside the cell grid
+    int error;             // nonzero: see the DEMState::error codes; every DEM kernel then does nothing
     int rebuilds;
 };
 
@@ -139,14 +145,20 @@ struct StatRec {
 // The particle state a saving sub-step exports (after the predictor): per
 // element the predicted position and velocity and wGlobal; info[0] = pairs.
 void launch_dem_save_elements(const DEMParams& p, const DEMBuffers& b, double* out, double* info, hipStream_t s);
+// force = 1: rebuild unconditionally (initialisation); 0: displacement check
+// (DEM.cpp:1314-1324, 342-347), rebuilding when it fires.
 void launch_dem_rebuild(const DEMParams& p, const DEMBuffers& b, int force, hipStream_t s);
+// A sub-step's contacts and corrector: per-pair contact records, then the
+// per-element gather (corrector, next predictor).
 void launch_dem_pairs(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
 void launch_dem_gather(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
 void launch_dem_predict(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
 void launch_dem_energy(const DEMParams& p, const DEMBuffers& b, const double demSize[3], double* out, hipStream_t s);
 void launch_dem_couple_table(const DEMParams& p, const DEMBuffers& b, CoupleRec* cp, CoupleRecF* cpf, int* nslots,
                              const Units& u, hipStream_t s);
-void launch_dem_hydro_combine(const DEMParams& p, const DEMBuffers& b, const HydroPartial* part, const Units& u,
+// FHydro, MHydro and fluidVolume of every element from the run sums of its own
+// and its ghosts' slots (hs: filled by the fused collision, emptied here).
+void launch_dem_hydro_combine(const DEMParams& p, const DEMBuffers& b, const HydroSink& hs, const Units& u,
                               hipStream_t s);
 
 }  // namespace hyb

@@ -24,6 +24,10 @@ namespace hyb {
 
 namespace {
 
+// Occupancy of the fused kernels: the plain instances (no variable viscosity,
+// no curved walls) are compiled for 8 (FP32) and 5 (FP64) wavefronts per SIMD,
+// which they reach without spilling (more loads in flight for a bandwidth-bound
+// kernel); the others keep the compiler's register choice.
 // Workgroup tile of the lattice kernels: a full 256-node x-row per workgroup
 // keeps the +-1 x-shifted loads of a row inside one workgroup's cache.
 #ifndef HYB_LB_BX
@@ -222,13 +226,99 @@ __device__ __forceinline__ void load_prepared(const FSView& fsv, size_t PS, size
     fsv.linkw[idx] = lw & ~LINK_PRESTREAMED;
 }
 
+// ------------------------------------------------------------------ hydrodynamic run sums
+// Per-slot hydrodynamic sums (LB.cpp:1897-1902) folded into the collision: the
+// workgroup's nodes are 256 consecutive x nodes of one row, so the nodes one
+// slot owns there form runs. Each particle node stages its penalty increment
+// diffVel and node mass in LDS as the collision computes them (every node first
+// marks its entry empty); at the end the first node of each run adds the run's
+// diffVel, radius x diffVel (radius from the slot's FP64 centre and radius
+// vector, LB.cpp:1875) and masses in x order in FP64 and appends one HydroSeg to
+// the slot's list. The per-element combine adds the lists exactly (order-free),
+// so the result does not depend on the order of the appends. Runs end at
+// wavefront boundaries, so only lanes of one wavefront exchange data.
+constexpr int RUN_NODES = 256;  // nodes of one workgroup (one row segment)
+static_assert(BX == RUN_NODES && BY == 1, "run sums need the workgroup on 256 consecutive x nodes of one row");
+
+template <typename DT>
+struct RunLDS {
+    uint32_t sid[RUN_NODES];
+    DT d[3][RUN_NODES];
+    double m[RUN_NODES];
+};
+
+// What a collision needs to stage its node: the LDS block and the node's
+// position in it (nullptr: no sums this launch).
+template <typename DT>
+struct RunSlot {
+    RunLDS<DT>* lds;
+    int n;
+    __device__ void clear() const {
+        if (lds) lds->sid[n] = 0u;
+    }
+    __device__ void stage(uint32_t sid, DT dx, DT dy, DT dz, double mass) const {
+        if (!lds) return;
+        lds->sid[n] = sid;
+        lds->d[0][n] = dx;
+        lds->d[1][n] = dy;
+        lds->d[2][n] = dz;
+        lds->m[n] = mass;
+    }
+};
+
+// The run sums of a workgroup whose lanes hold NPL consecutive nodes each
+// (node n = NPL * lane + k, x = blockIdx.x * RUN_NODES + n of row (y, z)).
+template <typename DT, int NPL>
+__device__ __forceinline__ void hydro_runs(const RunLDS<DT>& L, const HydroSink& hs, const CoupleRec* __restrict__ cp,
+                                           const LBKParams& p) {
+#pragma clang fp contract(off)
+    static_assert(RUN_NODES % (64 * NPL) == 0, "workgroup rows of whole wavefronts");
+    constexpr int WAVE_NODES = 64 * NPL;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int y = (int)blockIdx.y, z = (int)blockIdx.z;
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) {
+        const int n = NPL * (int)threadIdx.x + k;
+        const uint32_t sid = L.sid[n];
+        if (!sid) continue;
+        if (n % WAVE_NODES != 0 && L.sid[n - 1] == sid) continue;  // not the first node of its run
+        const uint32_t slot = sid - 1;
+        const CoupleRec& r = cp[slot];
+        const V3 x0(r.x0[0], r.x0[1], r.x0[2]);
+        const V3 rv(r.rv[0], r.rv[1], r.rv[2]);
+        double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        const int end = (n / WAVE_NODES + 1) * WAVE_NODES;
+        const int xb = (int)blockIdx.x * RUN_NODES;
+        for (int m = n; m < end && L.sid[m] == sid; ++m) {
+            const V3 d((double)L.d[0][m], (double)L.d[1][m], (double)L.d[2][m]);
+            const V3 radius = V3((double)(xb + m), (double)y, (double)z) - x0 / p.unitLength + rv / p.unitLength;
+            const V3 mo = radius.cross(d);
+            acc[0] += d.x; acc[1] += d.y; acc[2] += d.z;
+            acc[3] += mo.x; acc[4] += mo.y; acc[5] += mo.z;
+            acc[6] += L.m[m];
+        }
+        const uint32_t i = atomicAdd(&hs.count[slot], 1u);
+        if (i < hs.cap) {
+            HydroSeg* o = hs.segs + (size_t)slot * hs.cap + i;
+#pragma unroll
+            for (int q = 0; q < 7; ++q) o->s[q] = acc[q];
+            o->slot = slot;
+            o->pad = 0;
+        } else if (p.herr) {
+            *p.herr = LB_ERR_SEGMENTS;
+        }
+    }
+}
+
 template <bool PULL, bool VARVISC, int COLL, bool FS, bool CURVED = false>
-__global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ fin, double* __restrict__ fout,
-                                                     const uint32_t* __restrict__ link,
-                                                     const uint32_t* __restrict__ solid,
-                                                     const CoupleRec* __restrict__ cp, NodeForce64* __restrict__ hf,
+__device__ __forceinline__ void fused64_node(const double* __restrict__ fin, double* __restrict__ fout,
+                                             const uint32_t* __restrict__ link,
+                                             const uint32_t* __restrict__ solid,
+                                                     const CoupleRec* __restrict__ cp, const RunSlot<double>& rs,
                                                      double* __restrict__ visc, double* __restrict__ mac,
-                                                     LBKParams p, FSView fsv) {
+                                                     const LBKParams& p, const FSView& fsv) {
 #pragma clang fp contract(off)
     Coord c;
     if (!node_coord(p, c)) return;
@@ -245,6 +335,7 @@ __global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ 
     double n;
     V3 u;
     moments64(f, n, u);
+    if (p.herr && !isfinite(n)) *p.herr = LB_ERR_NUMERIC;
     // penalty coupling force
     V3 hF(0.0, 0.0, 0.0);
     const uint32_t sw = solid ? solid[c.idx] : 0u;
@@ -252,7 +343,7 @@ __global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ 
     if (sid) {
         const double mass = node_mass<FS, double>(lw, c.idx, fsv, p);
         const V3 d = penalty64(cp[sid - 1], c, mass, n, u, p, hF);
-        if (hf) hf[c.idx] = NodeForce64{{d.x, d.y, d.z}, (uint64_t)sw};
+        rs.stage(sid, d.x, d.y, d.z, mass);
     }
     V3 Fb(p.F[0], p.F[1], p.F[2]);
     if (p.ffield) Fb = Fb + V3((double)p.ffield[c.idx], (double)p.ffield[N + c.idx], (double)p.ffield[2 * N + c.idx]);
@@ -340,6 +431,21 @@ __global__ void __launch_bounds__(BX * BY) k_fused64(const double* __restrict__ 
     }
 }
 
+template <bool PULL, bool VARVISC, int COLL, bool FS, bool CURVED = false>
+__global__ void __launch_bounds__(BX * BY, (VARVISC || CURVED) ? 1 : 5) k_fused64(const double* __restrict__ fin, double* __restrict__ fout,
+                                                     const uint32_t* __restrict__ link,
+                                                     const uint32_t* __restrict__ solid,
+                                                     const CoupleRec* __restrict__ cp, HydroSink hs,
+                                                     double* __restrict__ visc, double* __restrict__ mac,
+                                                     LBKParams p, FSView fsv) {
+    __shared__ RunLDS<double> lds;
+    const bool sums = hs.segs != nullptr;
+    const RunSlot<double> rs{sums ? &lds : nullptr, (int)threadIdx.x};
+    rs.clear();
+    fused64_node<PULL, VARVISC, COLL, FS, CURVED>(fin, fout, link, solid, cp, rs, visc, mac, p, fsv);
+    if (sums) hydro_runs<double, 1>(lds, hs, cp, p);
+}
+
 // ------------------------------------------------------------------ FP32 deviation path
 // g_j = f_j - w_j (rho0 = 1). Moments: rho = 1 + sum g, j = sum g c (sum w c = 0).
 // geq_j = w_j (drho + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u^2)).
@@ -359,7 +465,7 @@ __device__ __forceinline__ float cdotf(int j, float ux, float uy, float uz) {
 template <bool VARVISC, int COLL, bool FS, bool CURVED = false>
 __device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uint32_t lw, size_t N,
                                           const uint32_t* __restrict__ solid, const CoupleRecF* __restrict__ cp,
-                                          NodeForce32* __restrict__ hf, double* __restrict__ visc, double* __restrict__ mac,
+                                          const RunSlot<float>& rs, double* __restrict__ visc, double* __restrict__ mac,
                                           const LBKParams& p, const FSView& fsv, bool doOut, float out[19]) {
 #pragma clang fp contract(fast)
     float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
@@ -374,6 +480,7 @@ __device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uin
         if (cz(j) < 0) jz -= g[j];
     }
     const float rho = 1.0f + drho;
+    if (p.herr && !isfinite(rho)) *p.herr = LB_ERR_NUMERIC;
     const float irho = 1.0f / rho;
     float ux = jx * irho, uy = jy * irho, uz = jz * irho;
     float Tx = (float)p.F[0], Ty = (float)p.F[1], Tz = (float)p.F[2];
@@ -387,7 +494,8 @@ __device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uin
     const uint32_t sid = owner_slot(sw, p.ctag);
     if (sid) {
         const CoupleRecF& r = cp[sid - 1];
-        const float mass = (float)node_mass<FS, float>(lw, c.idx, fsv, p);
+        const double massD = node_mass<FS, float>(lw, c.idx, fsv, p);
+        const float mass = (float)massD;
         // rigid-body velocity at the node: v + w x (node - c)
         const float rx = (float)c.x - r.c[0], ry = (float)c.y - r.c[1], rz = (float)c.z - r.c[2];
         const float lvx = r.v[0] + (r.w[1] * rz - r.w[2] * ry);
@@ -396,7 +504,7 @@ __device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uin
         const float lf = mass * irho;
         const float dx = lf * (ux - lvx), dy = lf * (uy - lvy), dz = lf * (uz - lvz);
         hx = -dx; hy = -dy; hz = -dz;
-        if (hf) hf[c.idx] = NodeForce32{{dx, dy, dz}, sw};
+        rs.stage(sid, dx, dy, dz, massD);
         Tx += hx; Ty += hy; Tz += hz;
     }
     ux += 0.5f * Tx * irho;
@@ -492,12 +600,11 @@ __device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uin
 
 
 template <bool PULL, bool VARVISC, int COLL, bool FS, bool CURVED = false>
-__global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ fin, float* __restrict__ fout,
-                                                     const uint32_t* __restrict__ link,
-                                                     const uint32_t* __restrict__ solid,
-                                                     const CoupleRecF* __restrict__ cp, NodeForce32* __restrict__ hf,
-                                                     double* __restrict__ visc, double* __restrict__ mac,
-                                                     LBKParams p, FSView fsv) {
+__device__ __forceinline__ void fused32_node(const float* __restrict__ fin, float* __restrict__ fout,
+                                             const uint32_t* __restrict__ link, const uint32_t* __restrict__ solid,
+                                             const CoupleRecF* __restrict__ cp, const RunSlot<float>& rs,
+                                             double* __restrict__ visc, double* __restrict__ mac, const LBKParams& p,
+                                             const FSView& fsv) {
     Coord c;
     if (!node_coord(p, c)) return;
     const size_t N = (size_t)p.X * p.Y * p.Z;
@@ -511,10 +618,25 @@ __global__ void __launch_bounds__(BX * BY) k_fused32(const float* __restrict__ f
         for (int j = 0; j < 19; ++j) g[j] = ld_pop(&fin[(size_t)j * p.PS + c.idx]);
     }
     float out[19];
-    if (!collide32<VARVISC, COLL, FS, CURVED>(g, c, lw, N, solid, cp, hf, visc, mac, p, fsv, fout != nullptr, out))
+    if (!collide32<VARVISC, COLL, FS, CURVED>(g, c, lw, N, solid, cp, rs, visc, mac, p, fsv, fout != nullptr, out))
         return;
 #pragma unroll
     for (int j = 0; j < 19; ++j) st_pop(&fout[(size_t)j * p.PS + c.idx], out[j]);
+}
+
+template <bool PULL, bool VARVISC, int COLL, bool FS, bool CURVED = false>
+__global__ void __launch_bounds__(BX * BY, (VARVISC || CURVED) ? 1 : 8) k_fused32(const float* __restrict__ fin, float* __restrict__ fout,
+                                                     const uint32_t* __restrict__ link,
+                                                     const uint32_t* __restrict__ solid,
+                                                     const CoupleRecF* __restrict__ cp, const CoupleRec* __restrict__ cp64,
+                                                     HydroSink hs, double* __restrict__ visc,
+                                                     double* __restrict__ mac, LBKParams p, FSView fsv) {
+    __shared__ RunLDS<float> lds;
+    const bool sums = hs.segs != nullptr;
+    const RunSlot<float> rs{sums ? &lds : nullptr, (int)threadIdx.x};
+    rs.clear();
+    fused32_node<PULL, VARVISC, COLL, FS, CURVED>(fin, fout, link, solid, cp, rs, visc, mac, p, fsv);
+    if (sums) hydro_runs<float, 1>(lds, hs, cp64, p);
 }
 
 // Standalone pull streaming: turns the post-collision state into the
@@ -852,101 +974,6 @@ __global__ void __launch_bounds__(256) k_classify_set(const int* __restrict__ ns
     }
 }
 
-// Neumaier-compensated FP64 accumulator.
-struct KSum {
-    double s, c;
-    __device__ void add(double v) {
-        const double t = s + v;
-        if (fabs(s) >= fabs(v)) c += (s - t) + v;
-        else c += (v - t) + s;
-        s = t;
-    }
-    __device__ double value() const { return s + c; }
-};
-
-// Per particle slot: sum over the nodes it owns of diffVel, radius x diffVel and
-// the node mass (LB.cpp:1897-1902). One wavefront per slot; each lane adds its
-// few nodes in FP64, and the lane sums are merged with Neumaier compensation by
-// a fixed butterfly (deterministic, no atomics). Only nodes strictly inside the sphere can be owned by the slot,
-// so the others are skipped before any load.
-__device__ __forceinline__ void ksum_merge(KSum& a, double s2, double c2) {
-    a.add(s2);
-    a.c += c2;
-}
-
-template <typename REC, typename HT>
-__global__ void __launch_bounds__(256) k_hydro_partials(const int* __restrict__ nslots_p, const NodeBox* __restrict__ boxes,
-                                                        const CoupleRec* __restrict__ cp,
-                                                        const uint32_t* __restrict__ link, const REC* __restrict__ hf,
-                                                        const double* __restrict__ fsMass,
-                                                        const HT* __restrict__ fsRho,
-                                                        HydroPartial* __restrict__ out, LBKParams p) {
-#pragma clang fp contract(off)
-    __shared__ SlotRows shr[SLOT_WAVES];
-    const int ns = *nslots_p;
-    const int wl = threadIdx.x & 63;
-    const int lane = wl;
-    const int unit = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int nunits = (int)((gridDim.x * blockDim.x) >> 6);
-    SlotRows& sh = shr[threadIdx.x >> 6];
-    for (int s = unit; s < ns; s += nunits) {
-        const CoupleRec r = cp[s];
-        const NodeBox b = boxes[s];
-        const uint32_t want = (p.ctag << SOLID_SLOT_BITS) | (uint32_t)(s + 1);
-        // each lane sums its few nodes directly; the 64 lane sums are then merged
-        // with compensation
-        double ls[7];
-        for (int k = 0; k < 7; ++k) ls[k] = 0.0;
-        const V3 x0(r.x0[0], r.x0[1], r.x0[2]);
-        const V3 rv(r.rv[0], r.rv[1], r.rv[2]);
-        const V3 cen = x0 / p.unitLength;
-        const double rad = r.r / p.unitLength;
-        for_slot_batches(b, lane, sh, cen, rad, p, [&](const size_t* idx, const int* xs, const int* ys, const int* zs,
-                                                   const bool* ok) {
-            REC rec[CH];
-#pragma unroll
-            for (int k = 0; k < CH; ++k) {
-                if (ok[k]) rec[k] = hf[idx[k]];
-                else rec[k].own = 0;
-            }
-#pragma unroll
-            for (int k = 0; k < CH; ++k) {
-                // only records written with this coupling step's ownership word count
-                if (!ok[k] || rec[k].own != want) continue;
-                const V3 d((double)rec[k].d[0], (double)rec[k].d[1], (double)rec[k].d[2]);
-                const V3 radius = V3((double)xs[k], (double)ys[k], (double)zs[k]) - x0 / p.unitLength + rv / p.unitLength;
-                const V3 m = radius.cross(d);
-                double mass = p.initDensity;  // without free surface every node keeps rho0 (LB.cpp:147, 935)
-                if (fsMass) {
-                    const bool itf = (link[idx[k]] & LINK_INTERFACE) != 0;
-                    mass = itf ? fsMass[idx[k]] : (double)fsRho[idx[k]];
-                }
-                ls[0] += d.x; ls[1] += d.y; ls[2] += d.z;
-                ls[3] += m.x; ls[4] += m.y; ls[5] += m.z;
-                ls[6] += mass;
-            }
-        });
-        KSum acc[7];
-        for (int k = 0; k < 7; ++k) acc[k] = KSum{ls[k], 0.0};
-        for (int off = 32; off > 0; off >>= 1) {
-            for (int k = 0; k < 7; ++k) {
-                const double s2 = __shfl_xor(acc[k].s, off, 64);
-                const double c2 = __shfl_xor(acc[k].c, off, 64);
-                // both partners combine in the same (lower lane first) order
-                if (wl & off) {
-                    KSum a{s2, c2};
-                    ksum_merge(a, acc[k].s, acc[k].c);
-                    acc[k] = a;
-                } else {
-                    ksum_merge(acc[k], s2, c2);
-                }
-            }
-        }
-        if (wl == 0)
-            for (int k = 0; k < 7; ++k) out[s].s[k] = acc[k].s + acc[k].c;
-    }
-}
-
 // Hydrodynamic force on the moving no-slip walls (LB.cpp:1321-1341): every link
 // of an active node to a type-8 wall node adds bounceBackForce (node.cpp:283-291),
 // (2 (fs_j - rho0 w_j) - BBi) c_j with BBi = 0 (box walls have zero velocity), to
@@ -1026,55 +1053,38 @@ dim3 lattice_grid(const LBKParams& p) { return dim3((p.X + BX - 1) / BX, (p.Y + 
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
-#define HYB_FUSED_CASE(PULL, VV, CO)                                                                               \
+#define HYB_FUSED_LAUNCH(PULL, VV, CO, CUR)                                                                        \
     if (pull == PULL && varVisc == VV && collision == CO) {                                                        \
         if (prec == Prec::F64) {                                                                                   \
             if (fs)                                                                                                \
-                k_fused64<PULL, VV, CO, true><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link, solid, cp,  \
-                                                              (NodeForce64*)hf, visc, mac, p, *fs);                     \
+                k_fused64<PULL, VV, CO, true, CUR><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link, solid, \
+                                                                   cp, sink, visc, mac, p, *fs);                   \
             else                                                                                                   \
-                k_fused64<PULL, VV, CO, false><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link, solid, cp, \
-                                                               (NodeForce64*)hf, visc, mac, p, none);                   \
+                k_fused64<PULL, VV, CO, false, CUR><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link,       \
+                                                                    solid, cp, sink, visc, mac, p, none);          \
         } else {                                                                                                   \
             if (fs)                                                                                                \
-                k_fused32<PULL, VV, CO, true><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid, cpf,   \
-                                                              (NodeForce32*)hf, visc, mac, p, *fs);                      \
+                k_fused32<PULL, VV, CO, true, CUR><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid,   \
+                                                                   cpf, cp, sink, visc, mac, p, *fs);              \
             else                                                                                                   \
-                k_fused32<PULL, VV, CO, false><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid, cpf,  \
-                                                               (NodeForce32*)hf, visc, mac, p, none);                    \
+                k_fused32<PULL, VV, CO, false, CUR><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid,  \
+                                                                    cpf, cp, sink, visc, mac, p, none);            \
         }                                                                                                          \
         return;                                                                                                    \
     }
-
-// Cylinders (p.curvw set): the curved-wall pull and the capture of every active
-// node's density and velocity for the next step's curved links.
-#define HYB_CURVED_CASE(PULL, VV, CO)                                                                              \
-    if (pull == PULL && varVisc == VV && collision == CO) {                                                        \
-        if (prec == Prec::F64) {                                                                                   \
-            if (fs)                                                                                                \
-                k_fused64<PULL, VV, CO, true, true><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link,       \
-                                                                    solid, cp, (NodeForce64*)hf, visc, mac, p,     \
-                                                                    *fs);                                          \
-            else                                                                                                   \
-                k_fused64<PULL, VV, CO, false, true><<<g, b, 0, s>>>((const double*)fin, (double*)fout, link,      \
-                                                                     solid, cp, (NodeForce64*)hf, visc, mac, p,    \
-                                                                     none);                                        \
-        } else {                                                                                                   \
-            if (fs)                                                                                                \
-                k_fused32<PULL, VV, CO, true, true><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid,  \
-                                                                    cpf, (NodeForce32*)hf, visc, mac, p, *fs);     \
-            else                                                                                                   \
-                k_fused32<PULL, VV, CO, false, true><<<g, b, 0, s>>>((const float*)fin, (float*)fout, link, solid, \
-                                                                     cpf, (NodeForce32*)hf, visc, mac, p, none);   \
-        }                                                                                                          \
-        return;                                                                                                    \
-    }
+// Cylinders (p.curvw set, CUR = true): the curved-wall pull and the capture of
+// every active node's density and velocity for the next step's curved links.
+#define HYB_FUSED_CASE(PULL, VV, CO) HYB_FUSED_LAUNCH(PULL, VV, CO, false)
+#define HYB_CURVED_CASE(PULL, VV, CO) HYB_FUSED_LAUNCH(PULL, VV, CO, true)
 
 void launch_lb_fused(Prec prec, bool pull, bool varVisc, int collision, const void* fin, void* fout,
-                     const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, const CoupleRecF* cpf, void* hf,
-                     double* visc, double* mac, const FSView* fs, const LBKParams& p, hipStream_t s) {
+                     const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, const CoupleRecF* cpf,
+                     const HydroSink* hs, double* visc, double* mac, const FSView* fs, const LBKParams& p,
+                     hipStream_t s) {
     const dim3 g = lattice_grid(p), b(BX, BY, 1);
     const FSView none{};
+    const HydroSink sink = (hs && solid) ? *hs : HydroSink{nullptr, nullptr, 0u};
+
     if (p.curvw) {
         HYB_CURVED_CASE(true, false, HYB_COLLISION_BGK)
         HYB_CURVED_CASE(true, false, HYB_COLLISION_TRT)
@@ -1102,6 +1112,42 @@ void launch_wall_fhydro(Prec prec, const void* f, const uint8_t* type, const Wal
     else k_wall_fhydro<float, true><<<g, b, 0, s>>>((const float*)f, type, ws, acc, p);
 }
 
+namespace {
+typedef float F4 __attribute__((ext_vector_type(4)));
+// Plain non-temporal copy with `T`-sized accesses per lane: the reference point
+// for what HBM delivers to a streaming kernel at that width. Each lane issues
+// COPY_UNROLL loads (coalesced across the workgroup) before their stores, so
+// that enough bytes are in flight to cover the memory latency.
+constexpr int COPY_UNROLL = 8;
+template <typename T>
+__global__ void __launch_bounds__(256) k_copy_nt(const T* __restrict__ src, T* __restrict__ dst, size_t n) {
+    const size_t tile = (size_t)blockDim.x * COPY_UNROLL;
+    for (size_t base = (size_t)blockIdx.x * tile; base < n; base += (size_t)gridDim.x * tile) {
+        T v[COPY_UNROLL];
+#pragma unroll
+        for (int k = 0; k < COPY_UNROLL; ++k) {
+            const size_t i = base + (size_t)k * blockDim.x + threadIdx.x;
+            if (i < n) v[k] = __builtin_nontemporal_load(&src[i]);
+        }
+#pragma unroll
+        for (int k = 0; k < COPY_UNROLL; ++k) {
+            const size_t i = base + (size_t)k * blockDim.x + threadIdx.x;
+            if (i < n) __builtin_nontemporal_store(v[k], &dst[i]);
+        }
+    }
+}
+}  // namespace
+
+bool launch_copy_probe(int width, const void* src, void* dst, size_t bytes, hipStream_t s) {
+    const unsigned grid = 256 * 8;  // 8 workgroups of 256 lanes per CU
+    switch (width) {
+        case 4: k_copy_nt<float><<<grid, 256, 0, s>>>((const float*)src, (float*)dst, bytes / 4); return true;
+        case 8: k_copy_nt<double><<<grid, 256, 0, s>>>((const double*)src, (double*)dst, bytes / 8); return true;
+        case 16: k_copy_nt<F4><<<grid, 256, 0, s>>>((const F4*)src, (F4*)dst, bytes / 16); return true;
+        default: return false;
+    }
+}
+
 void launch_status(const double* mac, const double* visc, const uint32_t* link, size_t N, double viscMin,
                    unsigned long long* out, hipStream_t s) {
     k_status<<<1024, 256, 0, s>>>(mac, visc, link, N, viscMin, out);
@@ -1126,15 +1172,10 @@ void launch_lb_init_types(uint8_t* type, const int* bnd, int nwalls, const WallD
                                  cs, problem, lbx[0], lbx[1], lbx[2], lbx[3], lbx[4], lbx[5], lbx[6]);
 }
 
-void launch_lb_init_interface(uint8_t* type, const LBKParams& p, hipStream_t s) {
+void launch_lb_init_interface(uint8_t* type, uint8_t* tmp, const LBKParams& p, hipStream_t s) {
     const dim3 g = lattice_grid(p), b(BX, BY, 1);
-    const size_t N = (size_t)p.X * p.Y * p.Z;
-    uint8_t* tmp = nullptr;
-    if (hipMalloc((void**)&tmp, N) != hipSuccess) return;
     k_tag_interface<<<g, b, 0, s>>>(type, tmp, p);
     k_remove_isolated<<<g, b, 0, s>>>(tmp, type, p);
-    (void)hipStreamSynchronize(s);
-    (void)hipFree(tmp);
 }
 
 void launch_lb_init_links(const uint8_t* type, uint32_t* link, uint32_t* slipw, uint32_t* curvw, const LBKParams& p,
@@ -1167,19 +1208,6 @@ void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, 
     k_classify_set<<<grid, 64 * SLOT_WAVES, 0, s>>>(nslots, cp, boxes, nboxes, solid, link, p);
 }
 
-void launch_hydro_partials(const int* nslots, int max_slots, const NodeBox* boxes, const CoupleRec* cp,
-                           const uint32_t* link, Prec prec, const void* hf, const double* fsMass, const void* fsRho,
-                           HydroPartial* out, const LBKParams& p, hipStream_t s) {
-    if (max_slots <= 0) return;
-    int grid = (max_slots + SLOT_WAVES - 1) / SLOT_WAVES;  // one wavefront per slot
-    if (grid > 16384) grid = 16384;
-    if (prec == Prec::F64)
-        k_hydro_partials<NodeForce64, double><<<grid, 64 * SLOT_WAVES, 0, s>>>(
-            nslots, boxes, cp, link, (const NodeForce64*)hf, fsMass, (const double*)fsRho, out, p);
-    else
-        k_hydro_partials<NodeForce32, float><<<grid, 64 * SLOT_WAVES, 0, s>>>(
-            nslots, boxes, cp, link, (const NodeForce32*)hf, fsMass, (const float*)fsRho, out, p);
-}
 
 void launch_fill_u32(uint32_t* a, uint32_t v, size_t n, hipStream_t s) {
     if (!n) return;

@@ -44,7 +44,12 @@ struct LBKParams {
     // box sides (bit 2 * axis + high) whose boundary layer is a plane wall: a curved
     // wall node there takes that wall's (zero) velocity, every other one the cylinder's
     int wallSides;
+    // the handle's error word (mapped pinned host memory; nullptr: none): a
+    // collision that finds a non-finite density stores LB_ERR_NUMERIC there
+    int* herr;
 };
+
+constexpr int LB_ERR_NUMERIC = 6;
 
 constexpr int SOLID_SLOT_BITS = 20;
 constexpr uint32_t SOLID_SLOT_MASK = (1u << SOLID_SLOT_BITS) - 1u;
@@ -78,31 +83,37 @@ struct NodeBox {
     int hi[3];
 };
 
-// Penalty force of a particle node (LB.cpp:1880-1890) as written by the
-// collision: the force and the node's ownership word of that step, so that the
-// per-slot sums read one record per node and recognise current entries by it.
-struct NodeForce32 {
-    float d[3];
-    uint32_t own;
-};
-struct NodeForce64 {
-    double d[3];
-    uint64_t own;
+// Hydrodynamic sums of one particle slot over one run of its nodes: a
+// wavefront's consecutive x nodes that the slot owns (LB.cpp:1897-1902):
+// diffVel (3), radius x diffVel (3), node mass (fluidVolume), added in x order.
+struct HydroSeg {
+    double s[7];
+    uint32_t slot;
+    uint32_t pad;
 };
 
-// Partial hydrodynamic sums of one particle slot: F(3), M(3), fluidVolume.
-struct HydroPartial {
-    double s[7];
+// Where the collision appends the run sums: slot k's runs go to
+// segs[k * cap + i] for i = atomicAdd(&count[k], 1) < cap. The consumer (the
+// per-element combine) zeroes count[k] after reading. segs == nullptr: no sums.
+struct HydroSink {
+    HydroSeg* segs;
+    uint32_t* count;
+    uint32_t cap;
 };
+
+// LB error-word code of a run that found its slot's segment list full.
+constexpr int LB_ERR_SEGMENTS = 4;
 
 enum class Prec { F64, F32S };
 
 struct FSView;
 // f planes: FP64 -> double[19][PS]; FP32S -> float[19][PS] holding f - w (PS >= N).
 // fs: free-surface state (nullptr without free surface), see fs_kernels.h
+// hs: per-slot hydrodynamic run sums of the particle nodes (nullptr: none)
 void launch_lb_fused(Prec prec, bool pull, bool varVisc, int collision, const void* fin, void* fout,
-                     const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, const CoupleRecF* cpf, void* hf,
-                     double* visc, double* mac, const FSView* fs, const LBKParams& p, hipStream_t s);
+                     const uint32_t* link, const uint32_t* solid, const CoupleRec* cp, const CoupleRecF* cpf,
+                     const HydroSink* hs, double* visc, double* mac, const FSView* fs, const LBKParams& p,
+                     hipStream_t s);
 void launch_lb_stream(Prec prec, const void* fin, void* fout, const uint32_t* link, const LBKParams& p, hipStream_t s);
 // out[3] (zeroed by the caller): max |u|^2 bits from mac (may be nullptr), active
 // nodes, active nodes with visc > viscMin (visc may be nullptr).
@@ -127,7 +138,8 @@ struct CylSet {
 void launch_lb_init_types(uint8_t* type, const int* boundary, int nwalls, const WallDev* walls, int nobj,
                           const ObjDev* objs, const CylSet& cs, int problem, const double* region, const LBKParams& p,
                           hipStream_t s);
-void launch_lb_init_interface(uint8_t* type, const LBKParams& p, hipStream_t s);
+// tmp: N bytes of device scratch (its contents are overwritten)
+void launch_lb_init_interface(uint8_t* type, uint8_t* tmp, const LBKParams& p, hipStream_t s);
 // slipw: per-node slip-wall directions (nullptr when no slip walls exist)
 // curvw: per-node curved-link directions (nullptr without cylinders)
 void launch_lb_init_links(const uint8_t* type, uint32_t* link, uint32_t* slipw, uint32_t* curvw, const LBKParams& p,
@@ -138,11 +150,9 @@ void launch_lb_init_pops(Prec prec, void* f, const uint32_t* link, const int* ma
                          const double* Finit, void* rho, double* mass, double* uS, const LBKParams& p, hipStream_t s);
 void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, NodeBox* boxes, int* nboxes,
                          uint32_t* solid, const uint32_t* link, const LBKParams& p, hipStream_t s);
-// fsMass/fsRho: free-surface node masses (interface mass, previous density), nullptr without free surface
-// hf: NodeForce64 (F64) or NodeForce32 (F32S) records written by the collision
-void launch_hydro_partials(const int* nslots, int max_slots, const NodeBox* boxes, const CoupleRec* cp,
-                           const uint32_t* link, Prec prec, const void* hf, const double* fsMass,
-                           const void* fsRho, HydroPartial* out, const LBKParams& p, hipStream_t s);
+// Non-temporal device copy of `bytes` with 4-, 8- or 16-byte accesses per lane
+// (false: unsupported width).
+bool launch_copy_probe(int width, const void* src, void* dst, size_t bytes, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_count_active(const uint32_t* link, unsigned long long* count, const LBKParams& p, hipStream_t s);
 

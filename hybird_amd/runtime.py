@@ -40,11 +40,15 @@ EXPORTS = ["hyb_config_defaults", "hyb_last_error", "hyb_device_count", "hyb_syn
            "hyb_lb_kernel_time", "hyb_lb_active_nodes", "hyb_lb_set_capture", "hyb_dem_init_lattice",
            "hyb_lb_set_force_field", "hyb_lb_set_populations", "hyb_adj_forward", "hyb_adj_backward",
            "hyb_state_save", "hyb_state_load", "hyb_dem_set_objects", "hyb_dem_set_save_count", "hyb_dem_saved",
-           "hyb_dem_saved_get"]
+           "hyb_dem_saved_get", "hyb_copy_bandwidth"]
 
 
 class HybError(RuntimeError):
-    pass
+    """A failed C-ABI call; ``code`` is its HYB_E_* return value."""
+
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
 
 
 def load():
@@ -76,7 +80,8 @@ def load():
                        ("hyb_state_save", [vp, vp, C.c_char_p]), ("hyb_state_load", [vp, vp, C.c_char_p]),
                        ("hyb_dem_set_objects", [vp, i32, vp]), ("hyb_dem_set_save_count", [vp, i32]),
                        ("hyb_dem_saved", [vp, C.POINTER(i32)]),
-                       ("hyb_dem_saved_get", [vp, i32, C.c_char_p, vp, C.c_size_t, C.POINTER(C.c_size_t)])]:
+                       ("hyb_dem_saved_get", [vp, i32, C.c_char_p, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
+                       ("hyb_copy_bandwidth", [C.c_size_t, i32, i32, C.POINTER(C.c_double)])]:
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
@@ -86,7 +91,14 @@ def load():
 
 def _check(rc: int, what: str) -> None:
     if rc != 0:
-        raise HybError(f"{what} failed ({rc}): {load().hyb_last_error().decode()}")
+        raise HybError(f"{what} failed ({rc}): {load().hyb_last_error().decode()}", rc)
+
+
+def copy_bandwidth(nbytes: int, width: int, reps: int = 20) -> float:
+    """GB/s (read + written) of a plain non-temporal device copy with `width`-byte lanes."""
+    g = C.c_double(0.0)
+    _check(load().hyb_copy_bandwidth(int(nbytes), int(width), int(reps), C.byref(g)), "hyb_copy_bandwidth")
+    return g.value
 
 
 def default_config() -> HybConfig:

@@ -48,6 +48,14 @@ extern "C" {
 #define HYB_E_CAPACITY (-4) /* a device list overflowed its capacity */
 #define HYB_E_UNSUPPORTED (-5)
 #define HYB_E_NUMERIC (-6) /* non-finite value detected */
+/* Device-side failures (a DEM list overflowing its capacity, a particle leaving
+ * the cell grid, a non-finite density or element velocity) are detected by the
+ * kernels while the stream runs ahead of the host. The kernel that finds one
+ * records it in the handle's error word (pinned host memory) and every later
+ * DEM kernel of that handle does nothing, so the state stays that of the
+ * failing sub-step. The error is returned by the first call on the handle (or
+ * on hyb_cycle/hyb_sync with it) made after the device has run that kernel; a
+ * call that synchronises (hyb_sync, the views) always returns it. */
 
 /* problemName values (vector.h:21), plus an extension for a generic liquid box */
 enum hyb_problem {
@@ -102,7 +110,8 @@ typedef struct hyb_config {
     double avalanchePosit;  /* HYB_PROBLEM_NET: front of the liquid (physical units, LB.h avalanchePosit) */
     double drumSpeed;       /* HYB_PROBLEM_DRUM: drum angular velocity (rad/s, cfg key drumSpeed) */
     double fluidMass;       /* HYB_PROBLEM_DRUM: target liquid mass (physical units, cfg key fluidMass) */
-    int32_t reserved[4];
+    int32_t pairCapacity;   /* DEM neighbour-pair capacity (0: sized from the particle count) */
+    int32_t reserved[3];
 } hyb_config;
 
 /* Fills *cfg with the reference's parse defaults (LB.cpp:93-185, DEM.cpp:15-181)
@@ -212,6 +221,11 @@ int hyb_lb_kernel_time(hyb_lb* lb, double* total_ms, int64_t* launches);
  * surface they must be captured while stepping. Costs 56 bytes per node and
  * step of extra HBM writes while enabled. */
 int hyb_lb_set_capture(hyb_lb* lb, int32_t enabled);
+/* HBM reference point (not in the reference): bytes read + written per second
+ * (GB/s) of a plain non-temporal device copy of `bytes` with `width` (4, 8 or
+ * 16) bytes per lane, averaged over `reps` launches timed with HIP events.
+ * Allocates and frees its two buffers; synchronises. */
+int hyb_copy_bandwidth(size_t bytes, int32_t width, int32_t reps, double* gbps);
 /* Active (fluid + interface) node count, the MLUPS denominator. */
 int hyb_lb_active_nodes(hyb_lb* lb, int64_t* count);
 
