@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1608,6 +1609,62 @@ int hyb_lb_get(hyb_lb* l, const char* cname, void* host, size_t cap, size_t* nee
         return put(F.data(), F.size() * 8, host, cap, need);
     }
     return fail(HYB_E_ARG, "unknown LB field " + name);
+}
+
+// ------------------------------------------------------------------ no-allocation entry point
+// walls[].FHydro into caller-provided device memory, for callers that cannot
+// allocate (code under graph capture, serving loops). The exact-sum limbs live
+// in `scratch`, a fixed device array (one copy per GPU, kept by the process):
+// its device address is looked up once per device with hipGetSymbolAddress and
+// only that address is handed to kernels and fills. Calls on one GPU share the
+// array, so calls that overlap on one GPU race.
+static __device__ __align__(256) unsigned long long scratch[(size_t)HYB_WALL_FHYDRO_MAX_WALLS * 3 * 4];
+
+static int scratch_address(unsigned long long** out) {
+    constexpr int MAX_DEVICES = 64;
+    static std::mutex lock;
+    static void* addr[MAX_DEVICES] = {};
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(HYB_E_ARG, "device index out of range");
+    std::lock_guard<std::mutex> guard(lock);
+    if (!addr[dev]) {
+        void* q = nullptr;
+        HIPCHK(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+        addr[dev] = q;
+    }
+    *out = (unsigned long long*)addr[dev];
+    return HYB_OK;
+}
+
+int hyb_lb_wall_fhydro_into(hyb_lb* l, double* d_out, int32_t n_out) {
+    if (!l || !d_out) return fail(HYB_E_ARG, "hyb_lb_wall_fhydro_into: null argument");
+    if (!l->inited) return fail(HYB_E_STATE, "hyb_lb_wall_fhydro_into before init");
+    if (int rc = lb_pending(l)) return rc;
+    const hyb_dem* d = l->dem;
+    const int W = d ? (int)d->walls.size() : 0;
+    if (W > HYB_WALL_FHYDRO_MAX_WALLS)
+        return fail(HYB_E_CAPACITY, "hyb_lb_wall_fhydro_into: " + std::to_string(W) + " walls, at most " +
+                                        std::to_string(HYB_WALL_FHYDRO_MAX_WALLS));
+    if (n_out < 3 * W) return fail(HYB_E_ARG, "hyb_lb_wall_fhydro_into: output holds fewer than 3 values per wall");
+    if (!W) return HYB_OK;
+    if (l->steps == 0) {  // no LB step yet: zero force, as the view
+        launch_exact_finalize(nullptr, 3 * W, 0.0, d_out, l->stream);
+        HIPCHK(hipGetLastError());
+        return HYB_OK;
+    }
+    unsigned long long* acc = nullptr;
+    if (int rc = scratch_address(&acc)) return rc;
+    WallSet ws;
+    ws.n = W;
+    ws.unitLength = l->unit.Length;
+    for (int k = 0; k < W; ++k) ws.w[k] = d->walls[k];
+    // a fill kernel, not a memset node, so the sequence can be captured into a graph
+    launch_fill_u32((uint32_t*)acc, 0u, (size_t)W * 3 * 4 * 2, l->stream);
+    launch_wall_fhydro(l->prec, l->f[l->cur], l->type, ws, acc, l->params(), l->stream);
+    launch_exact_finalize(acc, 3 * W, l->unit.Force, d_out, l->stream);
+    HIPCHK(hipGetLastError());
+    return HYB_OK;
 }
 
 int hyb_dem_saved(hyb_dem* d, int32_t* count) {

@@ -1148,6 +1148,20 @@ bool launch_copy_probe(int width, const void* src, void* dst, size_t bytes, hipS
     }
 }
 
+namespace {
+// out[k] = value of the exact sum acc[4k..4k+3] times scale, k < n (acc == nullptr: zeros)
+__global__ void k_exact_finalize(const unsigned long long* __restrict__ acc, int n, double scale,
+                                 double* __restrict__ out) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k < n) out[k] = acc ? exact_value(acc + 4 * (size_t)k) * scale : 0.0;
+}
+}  // namespace
+
+void launch_exact_finalize(const unsigned long long* acc, int n, double scale, double* out, hipStream_t s) {
+    if (n <= 0) return;
+    k_exact_finalize<<<(n + 63) / 64, 64, 0, s>>>(acc, n, scale, out);
+}
+
 void launch_status(const double* mac, const double* visc, const uint32_t* link, size_t N, double viscMin,
                    unsigned long long* out, hipStream_t s) {
     k_status<<<1024, 256, 0, s>>>(mac, visc, link, N, viscMin, out);

@@ -129,6 +129,9 @@ struct WallSet {
 // post-collision state f: acc[nwalls][3][4] (zeroed by the caller).
 void launch_wall_fhydro(Prec prec, const void* f, const uint8_t* type, const WallSet& ws, unsigned long long* acc,
                         const LBKParams& p, hipStream_t s);
+// out[k] = (value of the exact sum acc[4k..4k+3]) * scale for k < n, on the device
+// (acc == nullptr writes zeros).
+void launch_exact_finalize(const unsigned long long* acc, int n, double scale, double* out, hipStream_t s);
 // Cylinders of the lattice (nodes outside become curved walls).
 struct CylSet {
     int n;

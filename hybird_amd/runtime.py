@@ -40,7 +40,7 @@ EXPORTS = ["hyb_config_defaults", "hyb_last_error", "hyb_device_count", "hyb_syn
            "hyb_lb_kernel_time", "hyb_lb_active_nodes", "hyb_lb_set_capture", "hyb_dem_init_lattice",
            "hyb_lb_set_force_field", "hyb_lb_set_populations", "hyb_adj_forward", "hyb_adj_backward",
            "hyb_state_save", "hyb_state_load", "hyb_dem_set_objects", "hyb_dem_set_save_count", "hyb_dem_saved",
-           "hyb_dem_saved_get", "hyb_copy_bandwidth"]
+           "hyb_dem_saved_get", "hyb_copy_bandwidth", "hyb_lb_wall_fhydro_into"]
 
 
 class HybError(RuntimeError):
@@ -81,7 +81,8 @@ def load():
                        ("hyb_dem_set_objects", [vp, i32, vp]), ("hyb_dem_set_save_count", [vp, i32]),
                        ("hyb_dem_saved", [vp, C.POINTER(i32)]),
                        ("hyb_dem_saved_get", [vp, i32, C.c_char_p, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
-                       ("hyb_copy_bandwidth", [C.c_size_t, i32, i32, C.POINTER(C.c_double)])]:
+                       ("hyb_copy_bandwidth", [C.c_size_t, i32, i32, C.POINTER(C.c_double)]),
+                       ("hyb_lb_wall_fhydro_into", [vp, vp, i32])]:
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int

@@ -229,6 +229,17 @@ int hyb_copy_bandwidth(size_t bytes, int32_t width, int32_t reps, double* gbps);
 /* Active (fluid + interface) node count, the MLUPS denominator. */
 int hyb_lb_active_nodes(hyb_lb* lb, int64_t* count);
 
+/* walls[].FHydro of the last LB step (the "wallFHydro" view: moving no-slip box
+ * walls, LB.cpp:1321-1341, exact sums) written to DEVICE memory d_out[3 * walls]
+ * (physical units) on the handle's stream, without synchronising and without
+ * allocating. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * The sums use one fixed device array per GPU, sized for at most
+ * HYB_WALL_FHYDRO_MAX_WALLS walls (more: HYB_E_CAPACITY); calls that overlap on
+ * one GPU race on it. n_out: capacity of d_out in doubles (at least 3 * walls). */
+#define HYB_WALL_FHYDRO_MAX_WALLS 6
+int hyb_lb_wall_fhydro_into(hyb_lb* lb, double* d_out, int32_t n_out);
+
 /* ---- extensions for the adjoint (no counterpart in the reference) ----
  * Per-node body force F[N][3] in lattice units, added to lbF in every
  * collision (NULL removes it). */
