@@ -19,6 +19,10 @@ HYB_COLLISION_BGK = 0
 HYB_COLLISION_TRT = 1
 HYB_FP64 = 0
 HYB_FP32_SHIFTED = 1
+HYB_RHEOLOGY_BINGHAM = 0
+HYB_RHEOLOGY_FRICTIONAL = 1
+HYB_TANGENTIAL_VISCOUS = 0
+HYB_TANGENTIAL_SPRING = 1
 
 ELEMENT_RECORD = 20
 
@@ -70,6 +74,9 @@ class HybConfig(C.Structure):
         ("drumSpeed", C.c_double),
         ("fluidMass", C.c_double),
         ("pairCapacity", C.c_int32),
+        ("rheology", C.c_int32),
+        ("frictionCoefFluid", C.c_double),
+        ("tangentialModel", C.c_int32),
         ("reserved", C.c_int32 * 3),
     ]
 
@@ -123,11 +130,17 @@ def config_from_cfg(cfg: dict, collision: str = "BGK", precision: str = "FP64", 
     c.avalanchePosit = float(g("avalanchePosit", 0.0))
     c.drumSpeed = float(g("drumSpeed", 0.0))
     c.fluidMass = float(g("fluidMass", 0.0))
+    c.rheology = HYB_RHEOLOGY_FRICTIONAL if str(g("rheology", "BINGHAM")).upper() == "FRICTIONAL" else HYB_RHEOLOGY_BINGHAM
+    c.frictionCoefFluid = float(g("frictionCoefFluid", 0.0))
+    c.tangentialModel = (HYB_TANGENTIAL_SPRING if str(g("tangentialModel", "VISCOUS")).upper() == "SPRING"
+                         else HYB_TANGENTIAL_VISCOUS)
     return c
 
 
-def element_records(x, r, v=None):
-    """Particle-file records (DEM.cpp:85-120) for single-sphere elements."""
+def element_records(x, r, v=None, size=None, w=None, q=None):
+    """Particle-file records (DEM.cpp:85-120): centres x, radii r and optional
+    velocities v, sphere counts size (1-4), angular velocities w and
+    orientation quaternions q (defaults: at rest, single spheres, identity)."""
     import numpy as np
 
     x = np.asarray(x, dtype=np.float64).reshape(-1, 3)
@@ -135,10 +148,14 @@ def element_records(x, r, v=None):
     n = x.shape[0]
     rec = np.zeros((n, ELEMENT_RECORD), dtype=np.float64)
     rec[:, 0] = np.arange(n)
-    rec[:, 1] = 1
+    rec[:, 1] = 1 if size is None else np.asarray(size, dtype=np.float64).reshape(-1)
     rec[:, 2] = r
     rec[:, 3:6] = x
     if v is not None:
         rec[:, 6:9] = np.asarray(v, dtype=np.float64).reshape(-1, 3)
+    if w is not None:
+        rec[:, 9:12] = np.asarray(w, dtype=np.float64).reshape(-1, 3)
     rec[:, 12] = 1.0
+    if q is not None:
+        rec[:, 12:16] = np.asarray(q, dtype=np.float64).reshape(-1, 4)
     return np.ascontiguousarray(rec)

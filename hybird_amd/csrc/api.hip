@@ -154,7 +154,8 @@ void unregister(std::vector<Blk>& reg, const void* ptr) {
 // ------------------------------------------------------------------ handles
 struct hyb_dem {
     hyb_config cfg;
-    int P = 0;
+    int E = 0;  // elements (particle-file records)
+    int P = 0;  // standard particles: the spheres of all elements
     std::vector<double> rec;
     // material (DEM.cpp:15-61)
     double knConst = 0, ksConst = 0, dampCoeff = 0;
@@ -196,6 +197,8 @@ struct hyb_lb {
     double initVelocity[3] = {0, 0, 0}, lbF[3] = {0, 0, 0};
     int boundary[6] = {1, 1, 1, 1, 1, 1};
     bool freeSurface = false, forceField = false, nonNewtonian = false, turbulenceOn = false;
+    bool frictional = false;      // frictional apparent viscosity (extension)
+    double frictionFluid = 0.0;
     Prec prec = Prec::F64;
     hipStream_t stream = nullptr;
     // device state
@@ -256,7 +259,7 @@ struct hyb_lb {
     ErrWord err;             // device-side failures of the LB kernels
     std::vector<Blk> saved;  // persistent device state
 
-    bool varVisc() const { return nonNewtonian || turbulenceOn; }
+    bool varVisc() const { return nonNewtonian || turbulenceOn || frictional; }
     bool hasSlip() const {
         for (int k = 0; k < 6; ++k)
             if (boundary[k] == 5 || boundary[k] == 6) return true;
@@ -272,6 +275,7 @@ struct hyb_lb {
         p.visc0 = initDynVisc;
         p.plasticVisc = plasticVisc; p.yieldStress = yieldStress; p.turbConst = turbConst;
         p.nonNewtonian = nonNewtonian; p.turbulence = turbulenceOn;
+        p.frictional = frictional; p.frictionFluid = frictionFluid;
         p.collision = cfg.collision;
         p.trtMagic = cfg.trtMagic;
         p.unitLength = unit.Length; p.unitSpeed = unit.Speed; p.unitAngVel = unit.AngVel;
@@ -377,15 +381,18 @@ int hyb_dem_create(const hyb_config* cfg, int32_t n, const double* rec, hyb_dem*
     const double le = std::log(cfg->restitution);
     d->dampCoeff = -1.0 * std::sqrt(2.0) * le / std::sqrt((le * le + M_PI));
     if (!(d->dampCoeff < 1.0)) { delete d; return fail(HYB_E_ARG, "dampCoeff >= 1"); }
-    d->P = n;
+    d->E = n;
+    d->P = 0;
     d->rec.assign(rec, rec + (size_t)n * HYB_ELEMENT_RECORD);
     for (int e = 0; e < n; ++e) {
         const double* r = rec + (size_t)e * HYB_ELEMENT_RECORD;
         if ((int)r[0] != e) { delete d; return fail(HYB_E_ARG, "element indices must be 0..n-1 in file order"); }
-        if ((int)r[1] != 1) {
+        // the clump prototypes exist for 1 to 4 spheres (DEM.cpp:404-433)
+        if (!(r[1] >= 1.0 && r[1] <= 4.0) || r[1] != (double)(int)r[1]) {
             delete d;
-            return fail(HYB_E_UNSUPPORTED, "multi-sphere elements (size > 1) are not supported by the GPU path");
+            return fail(HYB_E_ARG, "element size must be 1, 2, 3 or 4 spheres");
         }
+        d->P += (int)r[1];
     }
     d->multiStep = cfg->multiStep;
     d->demInitialRepeat = cfg->demInitialRepeat;
@@ -466,6 +473,8 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
     DEMParams& p = d->p;
     std::memset(&p, 0, sizeof(p));
     p.P = d->P;
+    p.E = d->E;
+    p.clumps = d->P != d->E;
     // walls (DEM.cpp:435-638, the six box planes)
     d->walls.clear();
     const V3 ps[6] = {V3(0.5 * U.Length, 0, 0), V3((double(lb->X) - 1.5) * U.Length, 0, 0), V3(0, 0.5 * U.Length, 0),
@@ -560,7 +569,7 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
     for (int k = 0; k < p.nwalls; ++k) p.walls[k] = d->walls[k];
     // neighbour-cell sizing (DEM.cpp:1270-1312)
     double maxRad = 0.0;
-    for (int e = 0; e < d->P; ++e) maxRad = std::max(maxRad, d->rec[(size_t)e * HYB_ELEMENT_RECORD + 2] * c.scale);
+    for (int e = 0; e < d->E; ++e) maxRad = std::max(maxRad, d->rec[(size_t)e * HYB_ELEMENT_RECORD + 2] * c.scale);
     for (int k = 0; k < 3; ++k) {
         p.cellWidth[k] = d->P ? std::min(maxRad * 5.0, d->demSize[k]) : d->demSize[k];
         p.nCells[k] = (unsigned)int(std::ceil(d->demSize[k] / p.cellWidth[k]));
@@ -585,6 +594,9 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
     p.viscTang = c.viscTang; p.fricPart = c.frictionCoefPart; p.fricWall = c.frictionCoefWall;
     p.linearStiff = c.linearStiff;
     p.hertzian = c.contactModel == HYB_CONTACT_HERTZIAN;
+    if (c.tangentialModel != HYB_TANGENTIAL_VISCOUS && c.tangentialModel != HYB_TANGENTIAL_SPRING)
+        return fail(HYB_E_ARG, "unknown tangential contact model");
+    p.spring = c.tangentialModel == HYB_TANGENTIAL_SPRING;
     p.numVisc = c.numVisc;
     p.demF[0] = demF.x; p.demF[1] = demF.y; p.demF[2] = demF.z;
     // objects: periodic copies of those within 2 nebrRange of a periodic plane,
@@ -624,15 +636,30 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
     b.scap = scap;
     b.ncells = (int)(p.nCells[0] * p.nCells[1] * p.nCells[2]);
     b.W = b.ncells * 14;
-    // element state, host side (elmt.cpp:17-83 with wSolver = true, single spheres)
+    // element state, host side (elmt.cpp:17-83 with wSolver = true), and the
+    // particles of each element in prototype order (elmt.cpp:122-137)
     const int P = d->P;
-    std::vector<double> el((size_t)EF_COUNT * std::max(P, 1), 0.0);
-    auto at = [&](int f, int e) -> double& { return el[(size_t)f * P + e]; };
+    const int NE = d->E;
+    // compositeProperties (DEM.cpp:404-433); -1/2 is integer division there, so
+    // the second coordinate of the three-sphere prototype's last two vectors is 0
+    const double s3 = std::sqrt(3), s2 = std::sqrt(2), s6 = std::sqrt(6);
+    const V3 proto[5][4] = {{},
+                            {V3()},
+                            {V3(0.5, 0.0, 0.0), V3(-0.5, 0.0, 0.0)},
+                            {V3(0.0, 1.0, 0.0), V3(-s3 / 2, -1 / 2, 0.0), V3(s3 / 2, -1 / 2, 0.0)},
+                            {V3(0.0, 0.0, 1.0), V3(0.0, 2.0 * s2 / 3.0, -1.0 / 3.0),
+                             V3(2.0 * s6 / 6.0, -2.0 * s2 / 6.0, -1.0 / 3.0),
+                             V3(-2.0 * s6 / 6.0, -2.0 * s2 / 6.0, -1.0 / 3.0)}};
+    std::vector<double> el((size_t)EF_COUNT * std::max(NE, 1), 0.0);
+    std::vector<int> pelem, efirst;
+    std::vector<double> pproto;
+    auto at = [&](int f, int e) -> double& { return el[(size_t)f * NE + e]; };
     auto set3 = [&](int f, int e, const V3& v) { at(f, e) = v.x; at(f + 1, e) = v.y; at(f + 2, e) = v.z; };
     auto set4 = [&](int f, int e, const Q4& q) { at(f, e) = q.q0; at(f + 1, e) = q.q1; at(f + 2, e) = q.q2; at(f + 3, e) = q.q3; };
     const V3 translate(c.translate[0], c.translate[1], c.translate[2]);
-    for (int e = 0; e < P; ++e) {
+    for (int e = 0; e < NE; ++e) {
         const double* r = &d->rec[(size_t)e * HYB_ELEMENT_RECORD];
+        const int size = (int)r[1];
         const double radius = r[2] * c.scale;
         const V3 x0 = V3(r[3] * c.scale, r[4] * c.scale, r[5] * c.scale) + translate;
         const V3 x1(r[6], r[7], r[8]);
@@ -646,16 +673,24 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
         set3(EF_W, e, w0);
         set3(EF_WG, e, w0); set3(EF_WL, e, w0); set3(EF_WPG, e, w0); set3(EF_WPL, e, w0);
         const double singleMass = 4.0 / 3.0 * c.density * M_PI * radius * radius * radius;
-        const double m = 1 * singleMass;
-        V3 I = 1 * 2.0 / 5.0 * singleMass * radius * radius * V3(1.0, 1.0, 1.0);
-        const V3 proto(0.0, 0.0, 0.0);
-        I += singleMass * radius * radius * V3(proto.y * proto.y + proto.z * proto.z, proto.z * proto.z + proto.x * proto.x,
-                                               proto.x * proto.x + proto.y * proto.y);
+        const double m = size * singleMass;
+        V3 I = size * 2.0 / 5.0 * singleMass * radius * radius * V3(1.0, 1.0, 1.0);
+        for (int n = 0; n < size; ++n) {  // Steiner terms (vector.cpp transport: y^2+z^2, z^2+x^2, x^2+y^2)
+            const V3& pr = proto[size][n];
+            I += singleMass * radius * radius * V3(pr.y * pr.y + pr.z * pr.z, pr.z * pr.z + pr.x * pr.x,
+                                                   pr.x * pr.x + pr.y * pr.y);
+        }
         at(EF_M, e) = m;
         at(EF_RAD, e) = radius;
         set3(EF_I, e, I);
         set3(EF_FG, e, demF * m);
+        efirst.push_back((int)pelem.size());
+        for (int n = 0; n < size; ++n) {
+            pelem.push_back(e);
+            pproto.push_back(proto[size][n].x); pproto.push_back(proto[size][n].y); pproto.push_back(proto[size][n].z);
+        }
     }
+    efirst.push_back((int)pelem.size());
     // device buffers
     int rc;
     if ((rc = errword_alloc(d->err))) return rc;
@@ -674,11 +709,32 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
         (rc = ralloc(R, "ghostOf", &b.ghostOf, 8 * (size_t)std::max(P, 1))) ||
         (rc = ralloc(R, "side", &b.side, 3 * (size_t)std::max(P, 1))) || (rc = ralloc(R, "ghostB", &b.ghostB, p.Gcap)) ||
         (rc = ralloc(R, "posI", &b.posI, p.pairCap)) || (rc = ralloc(R, "posJ", &b.posJ, p.pairCap)) ||
-        (rc = dalloc(&b.prec, 2 * (size_t)p.pairCap)) || (rc = ralloc(R, "vmax", &b.vmax, (size_t)(std::max(P, 1) + 63) / 64)) || (rc = dalloc(&b.energy, 6)) ||
+        (rc = dalloc(&b.prec, 2 * (size_t)p.pairCap)) || (rc = ralloc(R, "vmax", &b.vmax, (size_t)(std::max(NE, 1) + 63) / 64)) || (rc = dalloc(&b.energy, 6)) ||
         (rc = dalloc(&b.wcf, (size_t)std::max(P, 1))))
         return rc;
     HIPCHK(hipMemset(b.wcf, 0, sizeof(ContactForce) * (size_t)std::max(P, 1)));
     if (p.ncyl && (rc = ralloc(R, "nearCyl", &b.nearCyl, (size_t)std::max(P, 1)))) return rc;
+    if (p.clumps) {
+        if ((rc = dalloc(&b.pelem, (size_t)P)) || (rc = dalloc(&b.efirst, (size_t)NE + 1)) ||
+            (rc = dalloc(&b.pproto, 3 * (size_t)P)) || (rc = ralloc(R, "pcor", &b.pcor, 9 * (size_t)P)) ||
+            (rc = ralloc(R, "ppred", &b.ppred, 9 * (size_t)P)) || (rc = dalloc(&b.pmn, 6 * (size_t)p.pairCap)))
+            return rc;
+        HIPCHK(hipMemcpy(b.pelem, pelem.data(), sizeof(int) * pelem.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b.efirst, efirst.data(), sizeof(int) * efirst.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b.pproto, pproto.data(), sizeof(double) * pproto.size(), hipMemcpyHostToDevice));
+    }
+    if (p.spring) {
+        const size_t Pm = (size_t)std::max(P, 1);
+        if ((rc = ralloc(R, "hist", &b.hist, 3 * (size_t)p.pairCap)) || (rc = ralloc(R, "whist", &b.whist, 4 * Pm)) ||
+            (rc = dalloc(&b.histOld, 3 * (size_t)p.pairCap)) || (rc = dalloc(&b.pairsOld, 2 * (size_t)p.pairCap)) ||
+            (rc = dalloc(&b.csrOffOld, Pm + 1)) || (rc = dalloc(&b.csrOld, 2 * (size_t)p.pairCap)) ||
+            (rc = dalloc(&b.ghostIndexOld, (size_t)p.Gcap)))
+            return rc;
+        HIPCHK(hipMemset(b.hist, 0, 3 * sizeof(double) * (size_t)p.pairCap));
+        std::vector<double> wh(4 * Pm, 0.0);
+        for (size_t e = 0; e < Pm; ++e) wh[4 * e + 3] = -1.0;  // no wall in contact
+        HIPCHK(hipMemcpy(b.whist, wh.data(), wh.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     if (p.nobj) {
         ObjDev* objs = nullptr;
         if ((rc = ralloc(R, "objs", &objs, (size_t)p.nobj)) ||
@@ -701,6 +757,7 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
     HIPCHK(hipMemsetAsync(b.csrOff, 0, sizeof(uint32_t) * ((size_t)P + 1), d->stream));
     HIPCHK(hipMemsetAsync(b.ghostOfCount, 0, sizeof(int) * std::max(P, 1), d->stream));
     if (P) {
+        launch_dem_particles(p, b, d->stream);   // generateParticles (elmt.cpp:122-137)
         launch_dem_rebuild(p, b, 1, d->stream);  // evalNeighborTable at init (DEM.cpp:258-262)
         launch_dem_predict(p, b, d->stream);     // the first sub-step's predictor
     }
@@ -781,6 +838,9 @@ int hyb_dem_destroy(hyb_dem* d) {
     dfree(b.cursor); dfree(b.spos); dfree(b.ghostOfCount); dfree(b.ghostOf); dfree(b.side); dfree(b.ghostB);
     dfree(b.posI); dfree(b.posJ); dfree(b.prec); dfree(b.vmax); dfree(b.energy); dfree(b.nearObj); dfree(b.nearObjCount);
     dfree(b.wcf); dfree(b.ocf); dfree(b.nearCyl);
+    dfree(b.hist); dfree(b.whist); dfree(b.histOld); dfree(b.pairsOld); dfree(b.csrOffOld); dfree(b.csrOld);
+    dfree(b.ghostIndexOld);
+    dfree(b.pelem); dfree(b.efirst); dfree(b.pproto); dfree(b.pcor); dfree(b.ppred); dfree(b.pmn);
     if (b.objs) (void)hipFree((void*)b.objs);
     save_free(d);
     errword_free(d->err);
@@ -796,6 +856,13 @@ int hyb_lb_create(const hyb_config* cfg, hyb_lb** out) {  // LB.cpp:85-188 + LB.
     const hyb_config& c = *cfg;
     if (!(c.unitLength > 0) || !(c.unitDensity > 0)) { delete l; return fail(HYB_E_ARG, "unit length/density must be > 0"); }
     l->nonNewtonian = c.nonNewtonian != 0;
+    if (c.rheology != HYB_RHEOLOGY_BINGHAM && c.rheology != HYB_RHEOLOGY_FRICTIONAL) {
+        delete l;
+        return fail(HYB_E_ARG, "unknown rheology");
+    }
+    if (!(c.frictionCoefFluid >= 0.0)) { delete l; return fail(HYB_E_ARG, "frictionCoefFluid must be >= 0"); }
+    l->frictional = c.rheology == HYB_RHEOLOGY_FRICTIONAL;
+    l->frictionFluid = c.frictionCoefFluid;
     l->turbulenceOn = c.turbulence != 0;
     l->freeSurface = c.freeSurface != 0;
     l->forceField = c.forceField != 0;
@@ -807,7 +874,7 @@ int hyb_lb_create(const hyb_config* cfg, hyb_lb** out) {  // LB.cpp:85-188 + LB.
     double unitTime = c.unitTime;
     if (unitTime == 0.0) {
         const double tauOpt = 1.0;
-        unitTime = l->nonNewtonian ? c.unitDensity * c.unitLength * c.unitLength / c.plasticVisc * (1.0 / 3.0) * (tauOpt - 0.5)
+        unitTime = (l->nonNewtonian || l->frictional) ? c.unitDensity * c.unitLength * c.unitLength / c.plasticVisc * (1.0 / 3.0) * (tauOpt - 0.5)
                                    : c.unitDensity * c.unitLength * c.unitLength / c.initVisc * (1.0 / 3.0) * (tauOpt - 0.5);
     }
     l->unit = make_units(c.unitLength, unitTime, c.unitDensity);
@@ -925,7 +992,7 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
     // plus margin for rows cut by overlapping slots
     if (d && d->P) {
         double rmax = 0.0;
-        for (int e = 0; e < d->P; ++e) rmax = std::max(rmax, d->rec[(size_t)e * HYB_ELEMENT_RECORD + 2]);
+        for (int e = 0; e < d->E; ++e) rmax = std::max(rmax, d->rec[(size_t)e * HYB_ELEMENT_RECORD + 2]);
         rmax *= d->cfg.scale / l->unit.Length;
         const double side = 2.0 * std::ceil(rmax) + 3.0;
         if (!(side < 4096.0)) return fail(HYB_E_ARG, "particle radius too large for the lattice");
@@ -1707,17 +1774,17 @@ int hyb_dem_get(hyb_dem* d, const char* cname, void* host, size_t cap, size_t* n
     if (!d->inited) return fail(HYB_E_STATE, "hyb_dem_get before init");
     HIPCHK(hipStreamSynchronize(d->stream));
     const std::string name(cname);
-    const int P = d->P;
+    const int P = d->P, NE = d->E;
     DEMState st;
     std::memset(&st, 0, sizeof(st));
     if (P) HIPCHK(hipMemcpy(&st, d->b.st, sizeof(st), hipMemcpyDeviceToHost));
     if (int rc = dem_pending(d)) return rc;
     std::vector<double> el;
     if (P) {
-        el.resize((size_t)EF_COUNT * P);
+        el.resize((size_t)EF_COUNT * NE);
         HIPCHK(hipMemcpy(el.data(), d->b.el, el.size() * 8, hipMemcpyDeviceToHost));
     }
-    auto at = [&](int f, int e) { return el[(size_t)f * P + e]; };
+    auto at = [&](int f, int e) { return el[(size_t)f * NE + e]; };
     std::vector<double> v;
     std::vector<uint32_t> u;
     const int G = P ? st.nGhost : 0;
@@ -1801,40 +1868,56 @@ int hyb_dem_get(hyb_dem* d, const char* cname, void* host, size_t cap, size_t* n
     if (name == "e_x" || name == "e_xp" || name == "e_w" || name == "e_wp") {
         // at step end the predicted copies equal the corrected ones (elmt.cpp:190-224)
         const int base = (name == "e_x" || name == "e_xp") ? EF_X : EF_W;
-        for (int e = 0; e < P; ++e)
+        for (int e = 0; e < NE; ++e)
             for (int k = 0; k < 18; ++k) v.push_back(at(base + k, e));
         return put(v.data(), v.size() * 8, host, cap, need);
     }
     if (name == "e_q" || name == "e_qp") {
-        for (int e = 0; e < P; ++e)
+        for (int e = 0; e < NE; ++e)
             for (int k = 0; k < 24; ++k) v.push_back(at(EF_Q + k, e));
         return put(v.data(), v.size() * 8, host, cap, need);
     }
     if (name == "e_forces") {
-        for (int e = 0; e < P; ++e)
+        for (int e = 0; e < NE; ++e)
             for (int k = 0; k < 21; ++k) v.push_back(at(EF_FH + k, e));
         return put(v.data(), v.size() * 8, host, cap, need);
     }
     if (name == "e_misc") {
-        for (int e = 0; e < P; ++e) {
+        for (int e = 0; e < NE; ++e) {
             v.push_back(at(EF_M, e)); v.push_back(at(EF_RAD, e));
             for (int k = 0; k < 3; ++k) v.push_back(at(EF_I + k, e));
             for (int k = 0; k < 12; ++k) v.push_back(at(EF_WG + k, e));
-            v.push_back(at(EF_FVOL, e)); v.push_back(1.0);
+            v.push_back(at(EF_FVOL, e)); v.push_back(d->rec[(size_t)e * HYB_ELEMENT_RECORD + 1]);
         }
         return put(v.data(), v.size() * 8, host, cap, need);
     }
     if (name == "p_data" || name == "p_idx") {
+        // per standard particle: position, velocity, radiusVec after the corrector, element, prototype index
+        std::vector<double> pc;
+        std::vector<int> pe, pf;
+        if (P && d->p.clumps) {
+            pc.resize(9 * (size_t)P);
+            pe.resize((size_t)P);
+            pf.resize((size_t)NE + 1);
+            HIPCHK(hipMemcpy(pc.data(), d->b.pcor, 8 * pc.size(), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(pe.data(), d->b.pelem, 4 * pe.size(), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(pf.data(), d->b.efirst, 4 * pf.size(), hipMemcpyDeviceToHost));
+        }
         for (int s = 0; s < P + G; ++s) {
-            const int o = s < P ? s : (int)gi[s - P];
+            const int o = s < P ? s : (int)gi[s - P];  // standard particle
+            const int e = d->p.clumps ? pe[o] : o;
             if (name == "p_data") {
-                // ghosts: origin position + period vector (elmt.cpp:292-298)
-                for (int k = 0; k < 3; ++k) v.push_back(s < P ? at(EF_X + k, o) : at(EF_X + k, o) + gv[3 * (s - P) + k]);
-                for (int k = 0; k < 3; ++k) v.push_back(at(EF_X + 3 + k, o));
-                for (int k = 0; k < 3; ++k) v.push_back(0.0);
-                v.push_back(at(EF_RAD, o));
+                // ghosts: origin position + period vector, the origin's velocity and radiusVec (elmt.cpp:292-298)
+                for (int k = 0; k < 3; ++k) {
+                    const double x = d->p.clumps ? pc[9 * (size_t)o + k] : at(EF_X + k, o);
+                    v.push_back(s < P ? x : x + gv[3 * (s - P) + k]);
+                }
+                for (int k = 0; k < 3; ++k) v.push_back(d->p.clumps ? pc[9 * (size_t)o + 3 + k] : at(EF_X + 3 + k, o));
+                for (int k = 0; k < 3; ++k) v.push_back(d->p.clumps ? pc[9 * (size_t)o + 6 + k] : 0.0);
+                v.push_back(at(EF_RAD, e));
             } else {
-                u.push_back((uint32_t)s); u.push_back((uint32_t)o); u.push_back(0u); u.push_back(0u);
+                u.push_back((uint32_t)s); u.push_back((uint32_t)e);
+                u.push_back(d->p.clumps ? (uint32_t)(o - pf[e]) : 0u); u.push_back(0u);
             }
         }
         if (name == "p_data") return put(v.data(), v.size() * 8, host, cap, need);
@@ -1970,7 +2053,7 @@ struct LbScalars {
 struct DemScalars {
     int32_t P, pairCap, Gcap, multiStep;
     double demTime;
-    uint32_t demTimeStep, pad;
+    uint32_t demTimeStep, E;
     int64_t substeps;
 };
 
@@ -2078,7 +2161,7 @@ int hyb_state_save(hyb_lb* l, hyb_dem* d, const char* path) {
     if (d) {
         DemScalars s;
         std::memset(&s, 0, sizeof(s));
-        s.P = d->P; s.pairCap = d->p.pairCap; s.Gcap = d->p.Gcap; s.multiStep = d->multiStep;
+        s.P = d->P; s.E = (uint32_t)d->E; s.pairCap = d->p.pairCap; s.Gcap = d->p.Gcap; s.multiStep = d->multiStep;
         s.demTime = d->demTime; s.demTimeStep = d->demTimeStep; s.substeps = d->substeps;
         if (!wr(fh.f, &s, sizeof(s))) return fail(HYB_E_ARG, "state file: write error");
         if ((rc = write_blocks(fh.f, d->saved))) return rc;
@@ -2136,7 +2219,8 @@ int hyb_state_load(hyb_lb* l, hyb_dem* d, const char* path) {
     if (d) {
         DemScalars s;
         if (!rd(fh.f, &s, sizeof(s))) return fail(HYB_E_ARG, "state file: truncated");
-        if (s.P != d->P || s.pairCap != d->p.pairCap || s.Gcap != d->p.Gcap || s.multiStep != d->multiStep)
+        if (s.P != d->P || s.E != (uint32_t)d->E || s.pairCap != d->p.pairCap || s.Gcap != d->p.Gcap ||
+            s.multiStep != d->multiStep)
             return fail(HYB_E_ARG, "state file: particle set or DEM configuration differs");
         if ((rc = read_blocks(fh.f, d->saved, [](const std::string&) { return (int)HYB_OK; }))) {
             d->inited = false;  // partly overwritten: unusable until destroyed

@@ -1,4 +1,4 @@
-// Discrete-element kernels for gfx950 (single-sphere elements).
+// Discrete-element kernels for gfx950.
 //
 // One DEM sub-step (DEM.cpp:331-376) is three launches on the handle's stream:
 //  1. k_dem_rebuild (one workgroup): evalMaxDisp and, when the displacement
@@ -8,16 +8,23 @@
 //     and unsigned stencil wrap, in the reference's pair order. It also builds a
 //     per-particle list of the pairs each particle takes part in.
 //  2. k_dem_pairs: one thread per pair evaluates the contact (DEM.cpp:1801-1900,
-//     2138-2222) and writes each element's signed increments into that
-//     element's slot of the pair (and, on saving sub-steps, its statistics).
+//     2138-2222) and writes each particle's signed increments into that
+//     particle's slot of the pair (and, on saving sub-steps, its statistics).
 //  3. k_dem_gather: per element, the increments in pair-list order (the same
 //     order as the reference's sequential loop, so no atomics and bit-identical
-//     sums), the wall contact, Newton-Euler (DEM.cpp:1149-1181), the Gear
-//     corrector (elmt.cpp:179-254) and the predictor (elmt.cpp:139-177) of the
-//     next sub-step. An initial predictor runs once after initialisation.
+//     sums; an element of several spheres merges its particles' lists by pair
+//     index), the wall, object and cylinder contacts of its particles,
+//     Newton-Euler (DEM.cpp:1149-1181), the Gear corrector (elmt.cpp:179-254),
+//     the particle update (elmt.cpp:261-290) and the predictor (elmt.cpp:139-177)
+//     of the next sub-step. An initial predictor runs once after initialisation.
 // The corrector keeps the predicted copies it has just used until the new
 // prediction overwrites them, so partner reads in k_dem_pairs always see the
 // predicted state of the current sub-step.
+//
+// Single-sphere elements are their own particle (particle i = element i, the
+// element arrays serve both). With multi-sphere elements (DEMParams::clumps)
+// particles have their own state arrays, refreshed from their element after the
+// corrector and after the predictor.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -86,15 +93,62 @@ __device__ __forceinline__ int linearize(const V3& x, const double cw[3], const 
                  (double)nC[0] * (floor(x.y / cw[1]) + 1 + ((double)nC[1] * (floor(x.z / cw[2])) + 1)));
 }
 
-__device__ __forceinline__ V3 slot_pos(const El& E, const DEMBuffers& b, int P, int s) {
-    if (s < P) return E.v3(EF_X, s);
-    const int g = s - P;
-    const int o = (int)b.ghostIndex[g];
-    return E.v3(EF_X, o) + V3(b.ghostVec[3 * g], b.ghostVec[3 * g + 1], b.ghostVec[3 * g + 2]);
+// Element (clusterIndex) of standard particle i.
+__device__ __forceinline__ int part_elem(const DEMParams& p, const DEMBuffers& b, int i) {
+    return p.clumps ? b.pelem[i] : i;
 }
 
-__device__ __forceinline__ int slot_cluster(const DEMBuffers& b, int P, int s) {
-    return s < P ? s : (int)b.ghostIndex[s - P];
+// Corrected position of standard particle i (particle::updateCorrected, elmt.cpp:277-290).
+__device__ __forceinline__ V3 part_x(const El& E, const DEMParams& p, const DEMBuffers& b, int i) {
+    if (p.clumps) {
+        const double* c = b.pcor + 9 * (size_t)i;
+        return V3(c[0], c[1], c[2]);
+    }
+    return E.v3(EF_X, i);
+}
+
+// Position of a particle slot: standard particles, then ghosts = origin + period vector (elmt.cpp:292-298).
+__device__ __forceinline__ V3 slot_pos(const El& E, const DEMParams& p, const DEMBuffers& b, int s) {
+    if (s < p.P) return part_x(E, p, b, s);
+    const int g = s - p.P;
+    const int o = (int)b.ghostIndex[g];
+    return part_x(E, p, b, o) + V3(b.ghostVec[3 * g], b.ghostVec[3 * g + 1], b.ghostVec[3 * g + 2]);
+}
+
+__device__ __forceinline__ int slot_cluster(const DEMParams& p, const DEMBuffers& b, int s) {
+    return part_elem(p, b, s < p.P ? s : (int)b.ghostIndex[s - p.P]);
+}
+
+// particle::updateCorrected / updatePredicted (elmt.cpp:261-290) for the
+// particles of element e, from the element's position x0, velocity x1,
+// orientation q and angular velocity w (the corrected or the predicted set):
+// position x0 + r project(prototype, q), radiusVec = position - x0, velocity
+// x1 + w x radiusVec; the particle of a size-1 element sits at its centre.
+// out: DEMBuffers::pcor or ppred.
+__device__ __forceinline__ void place_particles(const DEMBuffers& b, double* out, int e, double r, const V3& x0,
+                                                const V3& x1, const Q4& q, const V3& w) {
+#pragma clang fp contract(off)
+    const int lo = b.efirst[e], hi = b.efirst[e + 1];
+    for (int i = lo; i < hi; ++i) {
+        V3 px = x0, rv(0.0, 0.0, 0.0), pv = x1;
+        if (hi - lo > 1) {
+            const V3 proto(b.pproto[3 * (size_t)i], b.pproto[3 * (size_t)i + 1], b.pproto[3 * (size_t)i + 2]);
+            px = px + r * project(proto, q);
+            rv = px - x0;
+            pv = pv + w.cross(rv);
+        }
+        double* o = out + 9 * (size_t)i;
+        o[0] = px.x; o[1] = px.y; o[2] = px.z;
+        o[3] = pv.x; o[4] = pv.y; o[5] = pv.z;
+        o[6] = rv.x; o[7] = rv.y; o[8] = rv.z;
+    }
+}
+
+// Both particle states of element e from the element arrays.
+__device__ __forceinline__ void place_both(const El& E, const DEMBuffers& b, int e) {
+    const double r = E.at(EF_RAD, e);
+    place_particles(b, b.pcor, e, r, E.v3(EF_X, e), E.v3(EF_X + 3, e), E.q4(EF_Q, e), E.v3(EF_WG, e));
+    place_particles(b, b.ppred, e, r, E.v3(EF_XP, e), E.v3(EF_XP + 3, e), E.q4(EF_QP, e), E.v3(EF_WPG, e));
 }
 
 // Walks the chains of the cell pair (cell0, cell1) exactly as DEM.cpp:1455-1481.
@@ -165,7 +219,7 @@ __device__ int walk_pairs(const DEMParams& p, const DEMBuffers& b, const El& E, 
 
 __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, int force) {
 #pragma clang fp contract(off)
-    const El E{b.el, p.P};
+    const El E{b.el, p.E};
     const int t = threadIdx.x;
     const int P = p.P;
     __shared__ double red[RB];
@@ -179,7 +233,7 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
         // max |x1|^2 over elements, from the per-wavefront maxima the corrector
         // left; a NaN or infinite maximum stops the DEM (error 6)
         double m = 0.0;
-        const int nw = (P + 63) / 64;
+        const int nw = (p.E + 63) / 64;
         for (int k = t; k < nw; k += RB) {
             const double v = b.vmax[k];
             if (!(v <= 1.79769313486231570e308)) sh_err = DEM_ERR_NUMERIC;
@@ -204,13 +258,28 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
     __syncthreads();
     if (!doit || P == 0) return;
     if (t == 0) b.st->rebuilds += 1;
+    // tangential springs: keep the old pair list, springs, per-particle lists and
+    // ghosts; the new pairs take over the springs of the same element pairs below
+    __shared__ int oldT, oldG;
+    if (p.spring) {
+        if (t == 0) { oldT = b.st->nPairs; oldG = b.st->nGhost; }
+        __syncthreads();
+        for (int i = t; i < 2 * oldT; i += RB) b.pairsOld[i] = b.pairs[i];
+        for (int i = t; i < 3 * oldT; i += RB) b.histOld[i] = b.hist[i];
+        for (int i = t; i <= P; i += RB) b.csrOffOld[i] = b.csrOff[i];
+        const int nc = (int)b.csrOff[P];
+        for (int i = t; i < nc; i += RB) b.csrOld[i] = b.csr[i];
+        for (int i = t; i < oldG; i += RB) b.ghostIndexOld[i] = b.ghostIndex[i];
+        __syncthreads();
+    }
     int G = 0;
     if (p.npbc) {
         // ---- pbcShift (DEM.cpp:1554-1584)
         // The predictor of this sub-step already ran (fused into the previous
         // corrector), so a shifted element gets its predicted position
-        // recomputed from the shifted x0 exactly as the predictor would.
-        for (int e = t; e < P; e += RB) {
+        // recomputed from the shifted x0 exactly as the predictor would, and
+        // its particles are placed again (updateParticlesCorrected, DEM.cpp:1398).
+        for (int e = t; e < p.E; e += RB) {
             V3 x0 = E.v3(EF_X, e);
             bool moved = false;
             for (int k = 0; k < p.npbc; ++k) {
@@ -226,6 +295,7 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
                 const V3 xp0 = x0 + E.v3(EF_X + 3, e) * c2[0] + E.v3(EF_X + 6, e) * c2[1] + E.v3(EF_X + 9, e) * c2[2] +
                                E.v3(EF_X + 12, e) * c2[3] + E.v3(EF_X + 15, e) * c2[4];
                 E.set3(EF_XP, e, xp0);
+                if (p.clumps) place_both(E, b, e);
             }
         }
         __syncthreads();
@@ -233,8 +303,8 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
         int nSide = 0;
         for (int k = 0; k < p.npbc; ++k) {
             const PbcDev& pb = p.pbc[k];
-            for (int e = t; e < P; e += RB) {
-                const V3 x0 = E.v3(EF_X, e);
+            for (int e = t; e < P; e += RB) {  // e: standard particle
+                const V3 x0 = part_x(E, p, b, e);
                 const double left = pb.n1.dot(x0 - pb.p1);
                 const double right = pb.n2.dot(x0 - pb.p2);
                 const int flag = left < p.nebrRange ? 1 : (right < p.nebrRange ? 2 : 0);
@@ -303,7 +373,7 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
     for (int c = t; c <= nc; c += RB) b.cellCount[c] = 0;
     __syncthreads();
     for (int s = t; s < S; s += RB) {
-        const int c = linearize(slot_pos(E, b, P, s), p.cellWidth, p.nCells) + S;
+        const int c = linearize(slot_pos(E, p, b, s), p.cellWidth, p.nCells) + S;
         if (c < 0 || c >= S + nc) {
             sh_err = 2;
             b.cellOf[s] = 0;
@@ -336,12 +406,12 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
     __syncthreads();
     // ---- pair list in the reference's order (DEM.cpp:1425-1484): count, scan, fill
     for (int s = t; s < S; s += RB) {  // packed slot records: position, cluster
-        const V3 x = slot_pos(E, b, P, s);
+        const V3 x = slot_pos(E, p, b, s);
         double* r = b.spos + 4 * (size_t)s;
         r[0] = x.x;
         r[1] = x.y;
         r[2] = x.z;
-        r[3] = (double)slot_cluster(b, P, s);
+        r[3] = (double)slot_cluster(p, b, s);
     }
     __syncthreads();
     const int W = b.W;
@@ -356,10 +426,10 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
     if (T > p.pairCap) { if (t == 0) dem_error(p, b, 4); return; }
     for (int w = t; w < W; w += RB) walk_pairs(p, b, E, S, (unsigned)w, b.pairs + 2 * (size_t)b.work[w]);
     if (t == 0) b.st->nPairs = T;
-    // ---- near-wall list (DEM.cpp:1496-1513): the first wall within range
+    // ---- near-wall list (DEM.cpp:1496-1513): the first wall within range of each standard particle
     int nw = 0;
     for (int e = t; e < P; e += RB) {
-        const V3 x0 = E.v3(EF_X, e);
+        const V3 x0 = part_x(E, p, b, e);
         uint32_t wi = 0xFFFFFFFFu;
         for (int k = 0; k < p.nwalls; ++k)
             if (p.walls[k].dist(x0) < p.nebrRange) { wi = (uint32_t)k; break; }
@@ -378,7 +448,7 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
     if (p.nobj) {
         const double nebrRange2 = p.nebrRange * p.nebrRange;
         for (int e = t; e < P; e += RB) {
-            const V3 x0 = E.v3(EF_X, e);
+            const V3 x0 = part_x(E, p, b, e);
             int c = 0;
             for (int o = 0; o < p.nobj; ++o) {
                 const V3 x0ij = x0 - b.objs[o].x0;
@@ -459,6 +529,36 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
         __syncthreads();
     }
     if (t == 0) b.st->nNearWall = (int)red[0];
+    if (p.spring) {
+        // a new pair takes the spring of the old pair of the same two standard
+        // particles (ghosts count as their origin; found in the old lists of
+        // either particle), turned to the new pair's orientation; pairs without
+        // one start with a released spring
+        for (int k = t; k < T; k += RB) {
+            const int n0 = (int)b.pairs[2 * k], n1 = (int)b.pairs[2 * k + 1];
+            const int e0 = n0 < P ? n0 : (int)b.ghostIndex[n0 - P];
+            const int e1 = n1 < P ? n1 : (int)b.ghostIndex[n1 - P];
+            double xi[3] = {0.0, 0.0, 0.0};
+            bool found = false;
+            for (int side = 0; side < 2 && !found; ++side) {
+                const int e = side == 0 ? e0 : e1;
+                for (uint32_t i = b.csrOffOld[e]; i < b.csrOffOld[e + 1]; ++i) {
+                    const uint32_t kk = b.csrOld[i];
+                    const int m0 = (int)b.pairsOld[2 * kk], m1 = (int)b.pairsOld[2 * kk + 1];
+                    const int f0 = m0 < P ? m0 : (int)b.ghostIndexOld[m0 - P];
+                    const int f1 = m1 < P ? m1 : (int)b.ghostIndexOld[m1 - P];
+                    double sgn = 0.0;
+                    if (f0 == e0 && f1 == e1) sgn = 1.0;
+                    else if (f0 == e1 && f1 == e0) sgn = -1.0;
+                    if (sgn == 0.0) continue;
+                    for (int c = 0; c < 3; ++c) xi[c] = sgn * b.histOld[3 * (size_t)kk + c];
+                    found = true;
+                    break;
+                }
+            }
+            for (int c = 0; c < 3; ++c) b.hist[3 * (size_t)k + c] = xi[c];
+        }
+    }
 }
 
 // ------------------------------------------------------------------ contacts, Newton-Euler, Gear
@@ -481,29 +581,78 @@ __device__ __forceinline__ double normal_contact(const DEMParams& p, double over
     return (fn < 0.0) ? 0.0 : fn;
 }
 
-__device__ __forceinline__ double tangential_contact(const DEMParams& p, double vrelt, double fn, double effRad,
-                                                     double effMass, double friction) {  // DEM.cpp:2176-2222
-    double ks;
+// Tangential stiffness and damping coefficient of a contact (DEM.cpp:2176-2222).
+__device__ __forceinline__ void tangential_coeffs(const DEMParams& p, double fn, double effRad, double effMass,
+                                                  double& ks, double& gammas) {
     if (p.hertzian) ks = p.ksConst * sqrt(effRad) * pow(fabs(fn), 1.0 / 3.0);
     else ks = p.linearStiff;
+    gammas = 2.0 * p.viscTang * sqrt(effMass * ks);
+}
+
+__device__ __forceinline__ double tangential_contact(const DEMParams& p, double vrelt, double fn, double effRad,
+                                                     double effMass, double friction) {  // DEM.cpp:2176-2222
+    double ks, gammas;
+    tangential_coeffs(p, fn, effRad, effMass, ks, gammas);
     const double fsMax = friction * fn;
-    const double gammas = 2.0 * p.viscTang * sqrt(effMass * ks);
     const double dumpfs = gammas * vrelt;
     return (fsMax < dumpfs) ? fsMax : dumpfs;  // std::min(dumpfs, fsMax)
 }
 
-// Predicted position / velocity of a particle slot (updateParticlesPredicted, DEM.cpp:1184-1204).
-__device__ __forceinline__ void slot_pred(const El& E, const DEMBuffers& b, int P, int s, V3& x, V3& v, int& elem) {
-    if (s < P) {
-        x = E.v3(EF_XP, s);
-        v = E.v3(EF_XP + 3, s);
-        elem = s;
-    } else {
-        const int g = s - P;
-        elem = (int)b.ghostIndex[g];
-        x = E.v3(EF_XP, elem) + V3(b.ghostVec[3 * g], b.ghostVec[3 * g + 1], b.ghostVec[3 * g + 2]);
-        v = E.v3(EF_XP + 3, elem);
+// Tangential spring (extension, not in the reference): the spring xi of the
+// contact is turned onto the current tangent plane (normal en), advanced by the
+// tangential relative velocity vt over one sub-step, and gives F = k_s xi +
+// gamma_s vt with the viscous law's k_s and gamma_s; above friction x fn the
+// force is scaled to that limit and xi set to the spring length that gives it.
+// Returns F (the force on the contact's first body, along vt while sticking).
+__device__ __forceinline__ V3 spring_force(const DEMParams& p, double fn, double effRad, double effMass,
+                                           double friction, const V3& en, const V3& vt, double xi[3]) {
+#pragma clang fp contract(off)
+    double ks, gammas;
+    tangential_coeffs(p, fn, effRad, effMass, ks, gammas);
+    V3 s(xi[0], xi[1], xi[2]);
+    s = s - en * s.dot(en);
+    s = s + vt * p.deltat;
+    V3 F = s * ks + vt * gammas;
+    const double fsMax = friction * fn;
+    const double fm = F.norm();
+    if (fm > fsMax) {
+        F = fm > 0.0 ? F * (fsMax / fm) : V3(0.0, 0.0, 0.0);
+        s = ks > 0.0 ? (F - vt * gammas) / ks : V3(0.0, 0.0, 0.0);
     }
+    xi[0] = s.x; xi[1] = s.y; xi[2] = s.z;
+    return F;
+}
+
+// Predicted position, velocity and radiusVec of a particle slot and its element
+// (updateParticlesPredicted, DEM.cpp:1184-1204; a ghost is its origin moved by
+// the period vector, with the origin's velocity and radiusVec, elmt.cpp:292-298).
+template <bool CL>
+__device__ __forceinline__ void slot_pred(const El& E, const DEMParams& p, const DEMBuffers& b, int s, V3& x, V3& v,
+                                          V3& rv, int& elem) {
+    const int o = s < p.P ? s : (int)b.ghostIndex[s - p.P];
+    if constexpr (CL) {
+        const double* c = b.ppred + 9 * (size_t)o;
+        x = V3(c[0], c[1], c[2]);
+        v = V3(c[3], c[4], c[5]);
+        rv = V3(c[6], c[7], c[8]);
+        elem = b.pelem[o];
+    } else {
+        x = E.v3(EF_XP, o);
+        v = E.v3(EF_XP + 3, o);
+        rv = V3(0.0, 0.0, 0.0);
+        elem = o;
+    }
+    if (s >= p.P) {
+        const int g = s - p.P;
+        x = x + V3(b.ghostVec[3 * g], b.ghostVec[3 * g + 1], b.ghostVec[3 * g + 2]);
+    }
+}
+
+// Number of spheres of element e.
+template <bool CL>
+__device__ __forceinline__ int elem_size(const DEMBuffers& b, int e) {
+    if constexpr (CL) return b.efirst[e + 1] - b.efirst[e];
+    return 1;
 }
 
 // Contact statistics and wall/object force records written by the contact code.
@@ -525,17 +674,19 @@ __device__ __forceinline__ void set_force_t(ContactForce& c, const V3& ft) {
 }
 
 // The contact of neighbour pair k (DEM.cpp:1801-1900) at the predicted state:
-// the signed increments for the pair's first (ri) and second (rj) element, and
-// its statistics record st.
+// the signed increments for the pair's first (ri) and second (rj) particle's
+// element, the normal-force moments of elements of size > 1 (mnI, mnJ; flag
+// bit 2), and its statistics record st.
+// SP: tangential spring (xi: the pair's spring); CL: multi-sphere elements present.
+template <bool SP, bool CL>
 __device__ __forceinline__ void pair_contact(const DEMParams& p, const DEMBuffers& b, const El& E, int k, PairRec& ri,
-                                             PairRec& rj, StatRec& st) {
+                                             PairRec& rj, StatRec& st, double* xi, V3& mnI, V3& mnJ) {
 #pragma clang fp contract(off)
-    const int P = p.P;
     const int n0 = (int)b.pairs[2 * k], n1 = (int)b.pairs[2 * k + 1];
-    V3 xI, vI, xJ, vJ;
+    V3 xI, vI, rvI, xJ, vJ, rvJ;
     int eI, eJ;
-    slot_pred(E, b, P, n0, xI, vI, eI);
-    slot_pred(E, b, P, n1, xJ, vJ, eJ);
+    slot_pred<CL>(E, p, b, n0, xI, vI, rvI, eI);
+    slot_pred<CL>(E, p, b, n1, xJ, vJ, rvJ, eJ);
     const double rI = E.at(EF_RAD, eI), rJ = E.at(EF_RAD, eJ);
     const double sigij = rI + rJ;
     const double sigij2 = sigij * sigij;
@@ -543,7 +694,10 @@ __device__ __forceinline__ void pair_contact(const DEMParams& p, const DEMBuffer
     ri.flag = rj.flag = 0;
     ri.pad = rj.pad = 0;
     st = StatRec{(double)n0, (double)n1, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0};
-    if (!(vd.norm2() < sigij2)) return;
+    if (!(vd.norm2() < sigij2)) {
+        if constexpr (SP) xi[0] = xi[1] = xi[2] = 0.0;  // out of contact: the spring is released
+        return;
+    }
     const double distance = vd.norm();
     const double overlap = rI + rJ - distance;
     const V3 relVel = vJ - vI;
@@ -558,20 +712,41 @@ __device__ __forceinline__ void pair_contact(const DEMParams& p, const DEMBuffer
     const V3 vecRadI = rI * en;
     const V3 vecRadJ = -rJ * en;
     // radiusVec is zero for single-sphere elements
-    const V3 centerDistI = vecRadI + V3(0.0, 0.0, 0.0);
-    const V3 centerDistJ = vecRadJ + V3(0.0, 0.0, 0.0);
+    const V3 centerDistI = vecRadI + rvI;
+    const V3 centerDistJ = vecRadJ + rvJ;
     const V3 mn = -1.0 * normalForce;
     ri.fn[0] = mn.x; ri.fn[1] = mn.y; ri.fn[2] = mn.z;  // F - n == F + (-n)
     rj.fn[0] = normalForce.x; rj.fn[1] = normalForce.y; rj.fn[2] = normalForce.z;
     ri.flag = rj.flag = 1;
+    if constexpr (CL) {  // only elements of size > 1 take the normal force's moment (DEM.cpp:1862-1867)
+        if (elem_size<CL>(b, eI) > 1) {
+            mnI = -1.0 * centerDistI.cross(normalForce);  // M - c x F == M + (-(c x F))
+            ri.flag = 5;
+        }
+        if (elem_size<CL>(b, eJ) > 1) {
+            mnJ = centerDistJ.cross(normalForce);
+            rj.flag = 5;
+        }
+    }
     const V3 wI = E.v3(EF_WPG, eI), wJ = E.v3(EF_WPG, eJ);
     const V3 relVelContact = relVel - wI.cross(vecRadI) + wJ.cross(vecRadJ);
     const V3 tangRelVelContact = relVelContact - normalRelVel;
-    const double nt = tangRelVelContact.norm();
+    double nt, ft;
+    V3 et, tangForce;
+    if constexpr (SP) {
+        tangForce = spring_force(p, fn, effRad, effMass, p.fricPart, en, tangRelVelContact, xi);
+        nt = tangForce.norm();
+        ft = nt;
+        if (nt != 0.0) et = tangForce / nt;
+    } else {
+        nt = tangRelVelContact.norm();
+        if (nt != 0.0) {
+            ft = tangential_contact(p, nt, fn, effRad, effMass, p.fricPart);
+            et = tangRelVelContact / nt;
+            tangForce = ft * et;
+        }
+    }
     if (nt != 0.0) {
-        const double ft = tangential_contact(p, nt, fn, effRad, effMass, p.fricPart);
-        const V3 et = tangRelVelContact / nt;
-        const V3 tangForce = ft * et;
         const V3 mI3 = centerDistI.cross(tangForce);
         const V3 mJ3 = -1.0 * centerDistJ.cross(tangForce);
         const V3 mt = -1.0 * tangForce;
@@ -579,7 +754,8 @@ __device__ __forceinline__ void pair_contact(const DEMParams& p, const DEMBuffer
         ri.m[0] = mI3.x; ri.m[1] = mI3.y; ri.m[2] = mI3.z;
         rj.ft[0] = mt.x; rj.ft[1] = mt.y; rj.ft[2] = mt.z;
         rj.m[0] = mJ3.x; rj.m[1] = mJ3.y; rj.m[2] = mJ3.z;
-        ri.flag = rj.flag = 3;
+        ri.flag |= 2;
+        rj.flag |= 2;
         set_stat(st, overlap, fn, ft, en, et);
     }
 }
@@ -588,18 +764,35 @@ __device__ __forceinline__ void pair_contact(const DEMParams& p, const DEMBuffer
 // state; the signed increments for each of the two elements go into that
 // element's slot of the pair in its csr list, so that k_dem_gather adds them in
 // pair-list order.
+template <bool SP, bool CL>
 __global__ void __launch_bounds__(256) k_dem_pairs(DEMParams p, DEMBuffers b) {
-    const El E{b.el, p.P};
+    const El E{b.el, p.E};
     if (b.st->error) return;
     const int T = b.st->nPairs;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < T; k += gridDim.x * blockDim.x) {
         const uint32_t si = b.posI[k], sj = b.posJ[k];
         PairRec ri, rj;
         StatRec st;
-        pair_contact(p, b, E, k, ri, rj, st);
+        double xi[3];
+        if constexpr (SP) {
+            xi[0] = b.hist[3 * (size_t)k]; xi[1] = b.hist[3 * (size_t)k + 1]; xi[2] = b.hist[3 * (size_t)k + 2];
+        }
+        V3 mnI, mnJ;
+        pair_contact<SP, CL>(p, b, E, k, ri, rj, st, xi, mnI, mnJ);
+        if constexpr (SP) {
+            b.hist[3 * (size_t)k] = xi[0]; b.hist[3 * (size_t)k + 1] = xi[1]; b.hist[3 * (size_t)k + 2] = xi[2];
+        }
         if (b.pstat) b.pstat[k] = st;
         if (si != 0xFFFFFFFFu) b.prec[si] = ri;
         if (sj != 0xFFFFFFFFu) b.prec[sj] = rj;
+        if constexpr (CL) {
+            if (si != 0xFFFFFFFFu && (ri.flag & 4)) {
+                b.pmn[3 * (size_t)si] = mnI.x; b.pmn[3 * (size_t)si + 1] = mnI.y; b.pmn[3 * (size_t)si + 2] = mnI.z;
+            }
+            if (sj != 0xFFFFFFFFu && (rj.flag & 4)) {
+                b.pmn[3 * (size_t)sj] = mnJ.x; b.pmn[3 * (size_t)sj + 1] = mnJ.y; b.pmn[3 * (size_t)sj + 2] = mnJ.z;
+            }
+        }
     }
 }
 
@@ -613,8 +806,9 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // Gear predictor of one element from its corrected state (elmt.cpp:139-177).
-__device__ __forceinline__ void predict_elem(const El& E, const DEMParams& p, int e, const V3 x[6], const Q4 q[6],
-                                             const V3 w[6]) {
+template <bool CL>
+__device__ __forceinline__ void predict_elem(const El& E, const DEMParams& p, const DEMBuffers& b, int e, const V3 x[6],
+                                             const Q4 q[6], const V3 w[6]) {
     const double* c1 = p.c1;
     const double* c2 = p.c2;
     const V3 xp0 = x[0] + x[1] * c2[0] + x[2] * c2[1] + x[3] * c2[2] + x[4] * c2[3] + x[5] * c2[4];
@@ -644,18 +838,21 @@ __device__ __forceinline__ void predict_elem(const El& E, const DEMParams& p, in
     E.set3(EF_WP + 9, e, wp3); E.set3(EF_WP + 12, e, wp4); E.set3(EF_WP + 15, e, wp5);
     E.set3(EF_WPG, e, wp0);
     E.set3(EF_WPL, e, project(wp0, qp0.adjoint()));
+    // particles from the predicted element (updateParticlesPredicted, DEM.cpp:1184-1204)
+    if constexpr (CL) place_particles(b, b.ppred, e, E.at(EF_RAD, e), xp0, xp1, qp0, wp0);
 }
 
 __global__ void __launch_bounds__(64) k_dem_predict(DEMParams p, DEMBuffers b) {
 #pragma clang fp contract(off)
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    const El E{b.el, p.P};
+    const El E{b.el, p.E};
     double v2 = 0.0;
-    if (e < p.P) {
+    if (e < p.E) {
         V3 x[6], w[6];
         Q4 q[6];
         for (int k = 0; k < 6; ++k) { x[k] = E.v3(EF_X + 3 * k, e); q[k] = E.q4(EF_Q + 4 * k, e); w[k] = E.v3(EF_W + 3 * k, e); }
-        predict_elem(E, p, e, x, q, w);
+        if (p.clumps) predict_elem<true>(E, p, b, e, x, q, w);
+        else predict_elem<false>(E, p, b, e, x, q, w);
         v2 = x[1].norm2();
     }
     v2 = wave_max(v2);
@@ -665,25 +862,30 @@ __global__ void __launch_bounds__(64) k_dem_predict(DEMParams p, DEMBuffers b) {
 // Per element: particle-contact increments in pair-list order, the wall contact
 // (DEM.cpp:1692-1717, 1902-1982), Newton-Euler (DEM.cpp:1149-1181), the Gear
 // corrector (elmt.cpp:179-254) and the predictor of the next sub-step.
+template <bool SP, bool CL>
 __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffers& b, int e);
 
+template <bool SP, bool CL>
 __global__ void __launch_bounds__(64) k_dem_gather(DEMParams p, DEMBuffers b) {
     if (b.st->error) return;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    double v2 = e < p.P ? gather_elem(p, b, e) : 0.0;
+    double v2 = e < p.E ? gather_elem<SP, CL>(p, b, e) : 0.0;
     v2 = wave_max(v2);
     if (threadIdx.x == 0) b.vmax[blockIdx.x] = v2;
 }
 
 // Returns |x1|^2 of the corrected element for the next displacement check.
+// SP: tangential spring; CL: multi-sphere elements present.
+template <bool SP, bool CL>
 __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffers& b, int e) {
 #pragma clang fp contract(off)
-    const El E{b.el, p.P};
+    const El E{b.el, p.E};
     V3 FP(0.0, 0.0, 0.0), MP(0.0, 0.0, 0.0), FW(0.0, 0.0, 0.0), MW(0.0, 0.0, 0.0);
-    const uint32_t lo = b.csrOff[e], hi = b.csrOff[e + 1];
+    // the element's standard particles (consecutive; a single sphere is its own particle)
+    const int ilo = CL ? b.efirst[e] : e, ihi = CL ? b.efirst[e + 1] : e + 1;
+    const bool multi = CL && ihi - ilo > 1;
     // the element state does not depend on the contact records: its loads are
     // issued first, so that they overlap with the record loads below
-    const uint32_t wi = b.nearWall[e];
     const V3 xpe = E.v3(EF_XP, e);
     const V3 vpe = E.v3(EF_XP + 3, e);
     const double re = E.at(EF_RAD, e), me = E.at(EF_M, e);
@@ -693,10 +895,10 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
     const V3 xp[6] = {xpe, vpe, E.v3(EF_XP + 6, e), E.v3(EF_XP + 9, e), E.v3(EF_XP + 12, e), E.v3(EF_XP + 15, e)};
     const V3 wp[6] = {E.v3(EF_WP, e), E.v3(EF_WP + 3, e), E.v3(EF_WP + 6, e), E.v3(EF_WP + 9, e), E.v3(EF_WP + 12, e),
                       E.v3(EF_WP + 15, e)};
-    const Q4 q2 = E.q4(EF_Q + 8, e);  // single spheres: q2 is never set by the force routine
+    Q4 q2 = E.q4(EF_Q + 8, e);  // single spheres: q2 is never set by the force routine
     const Q4 qp[6] = {E.q4(EF_QP, e), E.q4(EF_QP + 4, e), E.q4(EF_QP + 8, e), E.q4(EF_QP + 12, e),
                       E.q4(EF_QP + 16, e), E.q4(EF_QP + 20, e)};
-    // contact increments in pair-list order; records are loaded GU at a time
+    // contact increments in pair-list order
     const auto add = [&](const PairRec& r) {
         const int flag = r.flag;
         if (flag & 1) FP = FP + V3(r.fn[0], r.fn[1], r.fn[2]);
@@ -705,35 +907,202 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
             FP = FP + V3(r.ft[0], r.ft[1], r.ft[2]);
         }
     };
-    constexpr uint32_t GU = 4;
-    uint32_t i = lo;
-    for (; i + GU <= hi; i += GU) {
-        PairRec r[GU];
+    if (!multi) {  // one list; records are loaded GU at a time
+        const uint32_t lo = b.csrOff[ilo], hi = b.csrOff[ilo + 1];
+        constexpr uint32_t GU = 4;
+        uint32_t i = lo;
+        for (; i + GU <= hi; i += GU) {
+            PairRec r[GU];
 #pragma unroll
-        for (uint32_t k = 0; k < GU; ++k) r[k] = b.prec[i + k];
+            for (uint32_t k = 0; k < GU; ++k) r[k] = b.prec[i + k];
 #pragma unroll
-        for (uint32_t k = 0; k < GU; ++k) add(r[k]);
+            for (uint32_t k = 0; k < GU; ++k) add(r[k]);
+        }
+        for (; i < hi; ++i) add(b.prec[i]);
+    } else {
+        // the lists of the element's particles merged by pair index: every pair
+        // of the element appears in exactly one of them (pairs within one
+        // element are never listed), so this is the reference's pair order
+        constexpr int MAXS = 4;
+        uint32_t cur[MAXS], end[MAXS];
+        const int nl = ihi - ilo;
+        for (int l = 0; l < MAXS; ++l) {
+            cur[l] = l < nl ? b.csrOff[ilo + l] : 0u;
+            end[l] = l < nl ? b.csrOff[ilo + l + 1] : 0u;
+        }
+        for (;;) {
+            int best = -1;
+            uint32_t bk = 0xFFFFFFFFu;
+            for (int l = 0; l < nl; ++l)
+                if (cur[l] < end[l]) {
+                    const uint32_t k = b.csr[cur[l]];
+                    if (k < bk) { bk = k; best = l; }
+                }
+            if (best < 0) break;
+            const uint32_t i = cur[best]++;
+            const PairRec r = b.prec[i];
+            if (r.flag & 1) FP = FP + V3(r.fn[0], r.fn[1], r.fn[2]);
+            if (r.flag & 4) MP = MP + V3(b.pmn[3 * (size_t)i], b.pmn[3 * (size_t)i + 1], b.pmn[3 * (size_t)i + 2]);
+            if (r.flag & 2) {
+                MP = MP + V3(r.m[0], r.m[1], r.m[2]);
+                FP = FP + V3(r.ft[0], r.ft[1], r.ft[2]);
+            }
+        }
     }
-    for (; i < hi; ++i) add(b.prec[i]);
-    StatRec wst{wi == 0xFFFFFFFFu ? -1.0 : (double)wi, (double)e, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0},
-                0.0};
-    ContactForce wcf{{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0, 0.0};
-    if (wi != 0xFFFFFFFFu) {
-        const WallDev& w = p.walls[wi];
-        const double distance = w.dist(xpe);
-        const double overlap = re - distance;
-        if (overlap > 0.0) {
-            const V3 en = w.n;
-            const V3 cpv = w.getSpeed(xpe);
-            const V3 relVel = vpe - cpv;
+    // predicted position and velocity of standard particle i of this element
+    const auto part_pred = [&](int i, V3& px, V3& pv) {
+        if constexpr (CL) {
+            const double* c = b.ppred + 9 * (size_t)i;
+            px = V3(c[0], c[1], c[2]);
+            pv = V3(c[3], c[4], c[5]);
+        } else {
+            px = xpe;
+            pv = vpe;
+        }
+    };
+    // wall contacts of the element's particles (DEM.cpp:1692-1717, 1902-1982)
+    for (int i = ilo; i < ihi; ++i) {
+        V3 px, pv;
+        part_pred(i, px, pv);
+        const uint32_t wi = b.nearWall[i];
+        StatRec wst{wi == 0xFFFFFFFFu ? -1.0 : (double)wi, (double)i, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0},
+                    {0.0, 0.0, 0.0}, 0.0};
+        ContactForce wcf{{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0, 0.0};
+        // spring against the near wall (spring model): kept while the same wall stays in contact
+        double wxi[3] = {0.0, 0.0, 0.0};
+        bool wcontact = false;
+        if (SP && wi != 0xFFFFFFFFu && b.whist[4 * (size_t)i + 3] == (double)wi) {
+            wxi[0] = b.whist[4 * (size_t)i]; wxi[1] = b.whist[4 * (size_t)i + 1]; wxi[2] = b.whist[4 * (size_t)i + 2];
+        }
+        if (wi != 0xFFFFFFFFu) {
+            const WallDev& w = p.walls[wi];
+            const double distance = w.dist(px);
+            const double overlap = re - distance;
+            if (overlap > 0.0) {
+                const V3 en = w.n;
+                const V3 cpv = w.getSpeed(px);
+                const V3 relVel = pv - cpv;
+                const double normRelVel = relVel.dot(en);
+                const V3 normalRelVel = en * normRelVel;
+                const double fn = normal_contact(p, 2.0 * overlap, normRelVel, re, me);
+                const V3 normalForce = en * fn;
+                const V3 vecRadJ = -re * en;
+                V3 centerDistJ = vecRadJ;
+                if (multi) centerDistJ = centerDistJ + (px - xpe);
+                FW = FW + normalForce;
+                if (multi) MW = MW + centerDistJ.cross(normalForce);
+                set_force(wcf, normalForce, 1.0);
+                const V3 relVelContact = relVel + wpg.cross(vecRadJ);
+                const V3 tangRelVelContact = relVelContact - normalRelVel;
+                if constexpr (SP) {
+                    wcontact = true;
+                    const V3 tangForce = spring_force(p, fn, re, me, p.fricWall, en, tangRelVelContact, wxi);
+                    const double ft = tangForce.norm();
+                    if (ft != 0.0) {
+                        MW = MW - centerDistJ.cross(tangForce);
+                        FW = FW - tangForce;
+                        set_force_t(wcf, tangForce);
+                        if (b.wstat) set_stat(wst, overlap, fn, ft, en, tangForce / ft);
+                    }
+                } else {
+                    const double nt = tangRelVelContact.norm();
+                    if (nt != 0.0) {
+                        const double ft = tangential_contact(p, nt, fn, re, me, p.fricWall);
+                        const V3 et = tangRelVelContact / tangRelVelContact.norm();
+                        const V3 tangForce = ft * et;
+                        MW = MW - centerDistJ.cross(tangForce);
+                        FW = FW - tangForce;
+                        set_force_t(wcf, tangForce);
+                        if (b.wstat) set_stat(wst, overlap, fn, ft, en, et);
+                    }
+                }
+            }
+            b.wcf[i] = wcf;
+        }
+        if constexpr (SP) {
+            double* wh = b.whist + 4 * (size_t)i;
+            wh[0] = wcontact ? wxi[0] : 0.0;
+            wh[1] = wcontact ? wxi[1] : 0.0;
+            wh[2] = wcontact ? wxi[2] : 0.0;
+            wh[3] = wcontact ? (double)wi : -1.0;
+        }
+        if (b.wstat) b.wstat[i] = wst;
+    }
+    // object contacts (DEM.cpp:1747-1770, 2060-2136), in list order after the walls
+    if (p.nobj) {
+        for (int i = ilo; i < ihi; ++i) {
+            V3 px, pv;
+            part_pred(i, px, pv);
+            const int no = b.nearObjCount[i];
+            if (b.ostat)
+                for (int k = no; k < NEAR_OBJ; ++k)
+                    b.ostat[(size_t)i * NEAR_OBJ + k] = StatRec{-1.0, (double)i, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0},
+                                                                {0.0, 0.0, 0.0}, 0.0};
+            for (int k = 0; k < no; ++k) {
+                const uint32_t oslot = b.nearObj[(size_t)i * NEAR_OBJ + k];
+                const ObjDev& ob = b.objs[oslot];
+                const V3 vectorDistance = px - ob.x0;
+                const double distance = vectorDistance.norm();
+                const double overlap = re + ob.r - distance;
+                StatRec ost{(double)oslot, (double)i, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0};
+                ContactForce ocf{{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0, 0.0};
+                if (!(overlap > 0.0)) {
+                    if (b.ostat) b.ostat[(size_t)i * NEAR_OBJ + k] = ost;
+                    b.ocf[(size_t)i * NEAR_OBJ + k] = ocf;
+                    continue;
+                }
+                const V3 en = (1.0 / distance) * vectorDistance;
+                const V3 relVel = pv - ob.x1;
+                const double normRelVel = relVel.dot(en);
+                const V3 normalRelVel = en * normRelVel;
+                const double fn = normal_contact(p, 2.0 * overlap, normRelVel, re, me);
+                const V3 normalForce = en * fn;
+                const V3 vecRadJ = -re * en;
+                const V3 centerDistJ = vecRadJ + (px - xpe);  // zero offset for a single sphere
+                FW = FW + normalForce;
+                if (multi) MW = MW + centerDistJ.cross(normalForce);
+                set_force(ocf, normalForce, 1.0);
+                const V3 relVelContact = relVel + wpg.cross(vecRadJ);
+                const V3 tangRelVelContact = relVelContact - normalRelVel;
+                const double nt = tangRelVelContact.norm();
+                if (nt != 0.0) {
+                    const double ft = tangential_contact(p, nt, fn, re, me, p.fricPart);
+                    const V3 et = tangRelVelContact / tangRelVelContact.norm();
+                    const V3 tangForce = ft * et;
+                    MW = MW - centerDistJ.cross(tangForce);
+                    FW = FW - tangForce;
+                    set_force_t(ocf, tangForce);
+                    if (b.ostat) set_stat(ost, overlap, fn, ft, en, et);
+                }
+                if (b.ostat) b.ostat[(size_t)i * NEAR_OBJ + k] = ost;
+                b.ocf[(size_t)i * NEAR_OBJ + k] = ocf;
+            }
+        }
+    }
+    // cylinder contacts (DEM.cpp:1719-1744, 1984-2058), after the objects
+    if (p.ncyl) {
+        for (int i = ilo; i < ihi; ++i) {
+            const uint32_t ci = b.nearCyl[i];
+            if (ci == 0xFFFFFFFFu) continue;
+            V3 px, pv;
+            part_pred(i, px, pv);
+            const CylDev& cyl = p.cyl[ci];
+            const double distance = cyl.dist(px);
+            const double overlap = re - distance;
+            if (!(overlap > 0.0)) continue;
+            const V3 vecDistance = cyl.vecDist(px);
+            const V3 contactPoint = px - vecDistance;
+            const V3 en = vecDistance / (re - overlap);
+            const V3 contactPointVelocity = cyl.getSpeed(contactPoint);
+            const V3 relVel = pv - contactPointVelocity;
             const double normRelVel = relVel.dot(en);
             const V3 normalRelVel = en * normRelVel;
             const double fn = normal_contact(p, 2.0 * overlap, normRelVel, re, me);
             const V3 normalForce = en * fn;
             const V3 vecRadJ = -re * en;
-            const V3 centerDistJ = vecRadJ;
+            const V3 centerDistJ = vecRadJ + (px - xpe);  // zero offset for a single sphere
             FW = FW + normalForce;
-            set_force(wcf, normalForce, 1.0);
+            if (multi) MW = MW + centerDistJ.cross(normalForce);
             const V3 relVelContact = relVel + wpg.cross(vecRadJ);
             const V3 tangRelVelContact = relVelContact - normalRelVel;
             const double nt = tangRelVelContact.norm();
@@ -743,89 +1112,6 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
                 const V3 tangForce = ft * et;
                 MW = MW - centerDistJ.cross(tangForce);
                 FW = FW - tangForce;
-                set_force_t(wcf, tangForce);
-                if (b.wstat) set_stat(wst, overlap, fn, ft, en, et);
-            }
-        }
-        b.wcf[e] = wcf;
-    }
-    if (b.wstat) b.wstat[e] = wst;
-    // object contacts (DEM.cpp:1747-1770, 2060-2136), in list order after the wall
-    if (p.nobj) {
-        const int no = b.nearObjCount[e];
-        if (b.ostat)
-            for (int k = no; k < NEAR_OBJ; ++k)
-                b.ostat[(size_t)e * NEAR_OBJ + k] = StatRec{-1.0, (double)e, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0},
-                                                            {0.0, 0.0, 0.0}, 0.0};
-        for (int k = 0; k < no; ++k) {
-            const uint32_t oslot = b.nearObj[(size_t)e * NEAR_OBJ + k];
-            const ObjDev& ob = b.objs[oslot];
-            const V3 vectorDistance = xpe - ob.x0;
-            const double distance = vectorDistance.norm();
-            const double overlap = re + ob.r - distance;
-            StatRec ost{(double)oslot, (double)e, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0};
-            ContactForce ocf{{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0, 0.0};
-            if (!(overlap > 0.0)) {
-                if (b.ostat) b.ostat[(size_t)e * NEAR_OBJ + k] = ost;
-                b.ocf[(size_t)e * NEAR_OBJ + k] = ocf;
-                continue;
-            }
-            const V3 en = (1.0 / distance) * vectorDistance;
-            const V3 relVel = vpe - ob.x1;
-            const double normRelVel = relVel.dot(en);
-            const V3 normalRelVel = en * normRelVel;
-            const double fn = normal_contact(p, 2.0 * overlap, normRelVel, re, me);
-            const V3 normalForce = en * fn;
-            const V3 vecRadJ = -re * en;
-            const V3 centerDistJ = vecRadJ + (xpe - xpe);  // particle = element centre (single spheres)
-            FW = FW + normalForce;
-            set_force(ocf, normalForce, 1.0);
-            const V3 relVelContact = relVel + wpg.cross(vecRadJ);
-            const V3 tangRelVelContact = relVelContact - normalRelVel;
-            const double nt = tangRelVelContact.norm();
-            if (nt != 0.0) {
-                const double ft = tangential_contact(p, nt, fn, re, me, p.fricPart);
-                const V3 et = tangRelVelContact / tangRelVelContact.norm();
-                const V3 tangForce = ft * et;
-                MW = MW - centerDistJ.cross(tangForce);
-                FW = FW - tangForce;
-                set_force_t(ocf, tangForce);
-                if (b.ostat) set_stat(ost, overlap, fn, ft, en, et);
-            }
-            if (b.ostat) b.ostat[(size_t)e * NEAR_OBJ + k] = ost;
-            b.ocf[(size_t)e * NEAR_OBJ + k] = ocf;
-        }
-    }
-    // cylinder contact (DEM.cpp:1719-1744, 1984-2058), after the objects
-    if (p.ncyl) {
-        const uint32_t ci = b.nearCyl[e];
-        if (ci != 0xFFFFFFFFu) {
-            const CylDev& cyl = p.cyl[ci];
-            const double distance = cyl.dist(xpe);
-            const double overlap = re - distance;
-            if (overlap > 0.0) {
-                const V3 vecDistance = cyl.vecDist(xpe);
-                const V3 contactPoint = xpe - vecDistance;
-                const V3 en = vecDistance / (re - overlap);
-                const V3 contactPointVelocity = cyl.getSpeed(contactPoint);
-                const V3 relVel = vpe - contactPointVelocity;
-                const double normRelVel = relVel.dot(en);
-                const V3 normalRelVel = en * normRelVel;
-                const double fn = normal_contact(p, 2.0 * overlap, normRelVel, re, me);
-                const V3 normalForce = en * fn;
-                const V3 vecRadJ = -re * en;
-                const V3 centerDistJ = vecRadJ + (xpe - xpe);  // particle = element centre (single spheres)
-                FW = FW + normalForce;
-                const V3 relVelContact = relVel + wpg.cross(vecRadJ);
-                const V3 tangRelVelContact = relVelContact - normalRelVel;
-                const double nt = tangRelVelContact.norm();
-                if (nt != 0.0) {
-                    const double ft = tangential_contact(p, nt, fn, re, me, p.fricWall);
-                    const V3 et = tangRelVelContact / tangRelVelContact.norm();
-                    const V3 tangForce = ft * et;
-                    MW = MW - centerDistJ.cross(tangForce);
-                    FW = FW - tangForce;
-                }
             }
         }
     }
@@ -841,6 +1127,10 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
     const V3 momentBf = project(moment, qp0.adjoint());
     const V3 waBf = newtonAcc(momentBf, Ie, wpl);
     const V3 w1 = project(waBf, qp0);
+    if (multi) {  // rotational acceleration as a quaternion (DEM.cpp:1176-1179)
+        const Q4 waQuat = quatAcc(waBf, qp[1]);
+        q2 = 0.5 * qp0.multiply(waQuat);
+    }
     E.set3(EF_FP, e, FP); E.set3(EF_MP, e, MP); E.set3(EF_FW, e, FW); E.set3(EF_MW, e, MW);
     // corrector
     const double* co1 = p.co1;
@@ -873,45 +1163,55 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
     for (int k = 0; k < 6; ++k) {
         E.set3(EF_X + 3 * k, e, x[k]);
         E.set3(EF_W + 3 * k, e, w[k]);
-        if (k != 2) E.set4(EF_Q + 4 * k, e, q[k]);
+        if (k != 2 || multi) E.set4(EF_Q + 4 * k, e, q[k]);
     }
     E.set3(EF_WG, e, w[0]);
     E.set3(EF_WL, e, project(w[0], q[0].adjoint()));
+    // particles from the corrected element (updateParticlesCorrected, DEM.cpp:1206-1227)
+    if constexpr (CL) place_particles(b, b.pcor, e, re, x[0], x[1], q[0], w[0]);
     // predictor of the next sub-step
-    predict_elem(E, p, e, x, q, w);
+    predict_elem<CL>(E, p, b, e, x, q, w);
     return x[1].norm2();
 }
 
 // Coupling records of every particle slot after the DEM step: corrected
-// position (ghosts: origin + period vector, elmt.cpp:292-298), element velocity
-// and angular velocity, radius.
+// position and radiusVec (ghosts: origin + period vector with the origin's
+// radiusVec, elmt.cpp:292-298), the element's velocity and angular velocity,
+// radius.
 __global__ void k_dem_couple_table(DEMParams p, DEMBuffers b, CoupleRec* __restrict__ cp, CoupleRecF* __restrict__ cpf,
                                    int* __restrict__ nslots, Units u) {
 #pragma clang fp contract(off)
-    const El E{b.el, p.P};
+    const El E{b.el, p.E};
     const int S = b.st->nSlots;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s == 0) *nslots = S;
     if (s >= S) return;
-    int o = s;
-    V3 x0 = E.v3(EF_X, s < p.P ? s : 0);
+    const int o = s < p.P ? s : (int)b.ghostIndex[s - p.P];  // standard particle
+    const int e = part_elem(p, b, o);
+    V3 x0, rv(0.0, 0.0, 0.0);
+    if (p.clumps) {
+        const double* c = b.pcor + 9 * (size_t)o;
+        x0 = V3(c[0], c[1], c[2]);
+        rv = V3(c[6], c[7], c[8]);
+    } else {
+        x0 = E.v3(EF_X, o);
+    }
     if (s >= p.P) {
         const int g = s - p.P;
-        o = (int)b.ghostIndex[g];
-        x0 = E.v3(EF_X, o) + V3(b.ghostVec[3 * g], b.ghostVec[3 * g + 1], b.ghostVec[3 * g + 2]);
+        x0 = x0 + V3(b.ghostVec[3 * g], b.ghostVec[3 * g + 1], b.ghostVec[3 * g + 2]);
     }
     CoupleRec r;
     r.x0[0] = x0.x; r.x0[1] = x0.y; r.x0[2] = x0.z;
-    r.rv[0] = r.rv[1] = r.rv[2] = 0.0;
-    const V3 v = E.v3(EF_X + 3, o), w = E.v3(EF_WG, o);
+    r.rv[0] = rv.x; r.rv[1] = rv.y; r.rv[2] = rv.z;
+    const V3 v = E.v3(EF_X + 3, e), w = E.v3(EF_WG, e);
     r.v[0] = v.x; r.v[1] = v.y; r.v[2] = v.z;
     r.w[0] = w.x; r.w[1] = w.y; r.w[2] = w.z;
-    r.r = E.at(EF_RAD, o);
-    r.elem = o;
+    r.r = E.at(EF_RAD, e);
+    r.elem = e;
     r.pad = 0;
     cp[s] = r;
     CoupleRecF q;
-    const V3 cc = x0 / u.Length;  // radiusVec is zero for single spheres
+    const V3 cc = x0 / u.Length - rv / u.Length;  // radius = node - cc (LB.cpp:1875)
     const V3 vs = v / u.Speed, ws = w / u.AngVel;
     q.c[0] = (float)cc.x; q.c[1] = (float)cc.y; q.c[2] = (float)cc.z;
     q.v[0] = (float)vs.x; q.v[1] = (float)vs.y; q.v[2] = (float)vs.z;
@@ -921,7 +1221,7 @@ __global__ void k_dem_couple_table(DEMParams p, DEMBuffers b, CoupleRec* __restr
 }
 
 // Element hydrodynamic force and torque (LB.cpp:1897-1918): the run sums the
-// collision appended for the element's own slot and for its ghosts' slots,
+// collision appended for the slots of the element's particles and their ghosts,
 // added exactly (fixed point, so the result does not depend on the order of the
 // appends), then scaled to physical units. One wavefront per element; lane l
 // accumulates component l % 7 of every ninth entry (l / 7 < 9), so that each
@@ -936,13 +1236,14 @@ __global__ void __launch_bounds__(COMBINE_BLOCK) k_dem_hydro_combine(DEMParams p
     const int lane = (int)(threadIdx.x & 63);
     const int w = (int)(threadIdx.x >> 6);
     const int e = (int)((blockIdx.x * (unsigned)COMBINE_BLOCK + threadIdx.x) >> 6);
-    if (e >= p.P) return;  // whole wavefronts
-    const El E{b.el, p.P};
-    const int ng = b.ghostOfCount[e] < 8 ? b.ghostOfCount[e] : 8;
+    if (e >= p.E) return;  // whole wavefronts
+    const El E{b.el, p.E};
+    const int ilo = p.clumps ? b.efirst[e] : e, ihi = p.clumps ? b.efirst[e + 1] : e + 1;
     const int q = lane % 7, j = lane / 7;
     long long acc[4] = {0, 0, 0, 0};
-    for (int k = -1; k < ng; ++k) {
-        const uint32_t slot = k < 0 ? (uint32_t)e : (uint32_t)b.ghostOf[e * 8 + k];
+    for (int i = ilo; i < ihi; ++i)
+    for (int k = -1, ng = b.ghostOfCount[i] < 8 ? b.ghostOfCount[i] : 8; k < ng; ++k) {
+        const uint32_t slot = k < 0 ? (uint32_t)i : (uint32_t)b.ghostOf[i * 8 + k];
         const uint32_t cnt = hs.count[slot];
         const uint32_t n = cnt < hs.cap ? cnt : hs.cap;
         const HydroSeg* L = hs.segs + (size_t)slot * hs.cap;
@@ -966,7 +1267,10 @@ __global__ void __launch_bounds__(COMBINE_BLOCK) k_dem_hydro_combine(DEMParams p
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     // every lane has read the counts above; empty the lists for the next step
-    if (lane <= ng) hs.count[lane == 0 ? (uint32_t)e : (uint32_t)b.ghostOf[e * 8 + lane - 1]] = 0u;
+    for (int i = ilo; i < ihi; ++i) {
+        const int ng = b.ghostOfCount[i] < 8 ? b.ghostOfCount[i] : 8;
+        if (lane <= ng) hs.count[lane == 0 ? (uint32_t)i : (uint32_t)b.ghostOf[i * 8 + lane - 1]] = 0u;
+    }
     if (lane < 7) {
         long long t[4] = {0, 0, 0, 0};
         for (int r = 0; r < COMBINE_ROWS; ++r)
@@ -998,7 +1302,7 @@ __global__ void __launch_bounds__(EN_BLOCK) k_dem_energy(DEMParams p, DEMBuffers
                                                          double* __restrict__ out) {
 #pragma clang fp contract(off)
     __shared__ double tk[EN_BLOCK], rk[EN_BLOCK], gp[EN_BLOCK];
-    const El E{b.el, p.P};
+    const El E{b.el, p.E};
     const V3 F(p.demF[0], p.demF[1], p.demF[2]);
     const double gn = sqrt(F.norm2());
     const bool grav = gn != 0.0;
@@ -1011,9 +1315,9 @@ __global__ void __launch_bounds__(EN_BLOCK) k_dem_energy(DEMParams p, DEMBuffers
         if (n.x * 0.0 + n.y * 0.0 + n.z * 1.0 < 0) c.z = c.z + sz;
     }
     double tKin = 0.0, rKin = 0.0, g = 0.0;
-    for (int base = 0; base < p.P; base += EN_BLOCK) {
+    for (int base = 0; base < p.E; base += EN_BLOCK) {
         const int e = base + threadIdx.x;
-        if (e < p.P) {
+        if (e < p.E) {
             const double m = E.at(EF_M, e);
             const V3 v = E.v3(EF_X + 3, e);
             tk[threadIdx.x] = 0.5 * m * v.norm2();
@@ -1027,7 +1331,7 @@ __global__ void __launch_bounds__(EN_BLOCK) k_dem_energy(DEMParams p, DEMBuffers
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            const int cnt = min(EN_BLOCK, p.P - base);
+            const int cnt = min(EN_BLOCK, p.E - base);
             for (int i = 0; i < cnt; ++i) {
                 tKin += tk[i];
                 rKin += rk[i];
@@ -1053,19 +1357,27 @@ void launch_dem_energy(const DEMParams& p, const DEMBuffers& b, const double dem
 
 namespace {
 // Particle state of a saving sub-step, after the predictor (DEM.cpp:349-359:
-// particles take the predicted element position and velocity; wGlobal is the
+// particles hold the predicted position and velocity; wGlobal is the element's
 // last corrected angular velocity).
 __global__ void k_dem_save_elements(DEMParams p, DEMBuffers b, double* __restrict__ out, double* __restrict__ info) {
-    const El E{b.el, p.P};
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e == 0) info[0] = (double)b.st->nPairs;
-    if (e >= p.P) return;
-    double* o = out + (size_t)e * 9;
+    const El E{b.el, p.E};
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) info[0] = (double)b.st->nPairs;
+    if (i >= p.P) return;
+    const int e = part_elem(p, b, i);
+    double* o = out + (size_t)i * 9;
     for (int k = 0; k < 3; ++k) {
-        o[k] = E.at(EF_XP + k, e);
-        o[3 + k] = E.at(EF_XP + 3 + k, e);
+        o[k] = p.clumps ? b.ppred[9 * (size_t)i + k] : E.at(EF_XP + k, e);
+        o[3 + k] = p.clumps ? b.ppred[9 * (size_t)i + 3 + k] : E.at(EF_XP + 3 + k, e);
         o[6 + k] = E.at(EF_WG + k, e);
     }
+}
+
+// Both particle states of every element (set-up and state load).
+__global__ void k_dem_particles(DEMParams p, DEMBuffers b) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p.E) return;
+    place_both(El{b.el, p.E}, b, e);
 }
 }  // namespace
 
@@ -1074,20 +1386,38 @@ void launch_dem_save_elements(const DEMParams& p, const DEMBuffers& b, double* o
 }
 
 void launch_dem_predict(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {
-    if (p.P <= 0) return;
-    k_dem_predict<<<(p.P + 63) / 64, 64, 0, s>>>(p, b);
+    if (p.E <= 0) return;
+    k_dem_predict<<<(p.E + 63) / 64, 64, 0, s>>>(p, b);
+}
+
+void launch_dem_particles(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {
+    if (p.E <= 0 || !p.clumps) return;
+    k_dem_particles<<<(p.E + 63) / 64, 64, 0, s>>>(p, b);
 }
 
 void launch_dem_pairs(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {
     if (p.P <= 0) return;
     int blocks = (p.pairCap + 255) / 256;
     if (blocks > 512) blocks = 512;
-    k_dem_pairs<<<blocks, 256, 0, s>>>(p, b);
+    if (p.spring) {
+        if (p.clumps) k_dem_pairs<true, true><<<blocks, 256, 0, s>>>(p, b);
+        else k_dem_pairs<true, false><<<blocks, 256, 0, s>>>(p, b);
+    } else {
+        if (p.clumps) k_dem_pairs<false, true><<<blocks, 256, 0, s>>>(p, b);
+        else k_dem_pairs<false, false><<<blocks, 256, 0, s>>>(p, b);
+    }
 }
 
 void launch_dem_gather(const DEMParams& p, const DEMBuffers& b, hipStream_t s) {
-    if (p.P <= 0) return;
-    k_dem_gather<<<(p.P + 63) / 64, 64, 0, s>>>(p, b);
+    if (p.E <= 0) return;
+    const int g = (p.E + 63) / 64;
+    if (p.spring) {
+        if (p.clumps) k_dem_gather<true, true><<<g, 64, 0, s>>>(p, b);
+        else k_dem_gather<true, false><<<g, 64, 0, s>>>(p, b);
+    } else {
+        if (p.clumps) k_dem_gather<false, true><<<g, 64, 0, s>>>(p, b);
+        else k_dem_gather<false, false><<<g, 64, 0, s>>>(p, b);
+    }
 }
 
 
@@ -1100,9 +1430,9 @@ void launch_dem_couple_table(const DEMParams& p, const DEMBuffers& b, CoupleRec*
 
 void launch_dem_hydro_combine(const DEMParams& p, const DEMBuffers& b, const HydroSink& hs, const Units& u,
                               hipStream_t s) {
-    if (p.P <= 0) return;
+    if (p.E <= 0) return;
     const int waves_per_block = COMBINE_BLOCK / 64;
-    k_dem_hydro_combine<<<(p.P + waves_per_block - 1) / waves_per_block, COMBINE_BLOCK, 0, s>>>(p, b, hs, u);
+    k_dem_hydro_combine<<<(p.E + waves_per_block - 1) / waves_per_block, COMBINE_BLOCK, 0, s>>>(p, b, hs, u);
 }
 
 }  // namespace hyb

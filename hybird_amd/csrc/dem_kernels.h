@@ -29,7 +29,9 @@ enum ElemField : int {
 };
 
 struct DEMParams {
-    int P;                 // standard particles (= elements; single-sphere elements)
+    int P;                 // standard particles: the spheres of all elements (elements with size > 1 hold several)
+    int E;                 // elements
+    int clumps;            // some element has size > 1: particles live in their own arrays (DEMBuffers::pelem ...)
     int Gcap;              // ghost capacity
     int pairCap;           // neighbour-pair capacity
     int npbc, nwalls;
@@ -46,6 +48,7 @@ struct DEMParams {
     double demF[3];
     int ncyl;              // cylinders (DEM.cpp:870-935)
     CylDev cyl[2];
+    int spring;            // tangential spring with history (HYB_TANGENTIAL_SPRING) instead of the viscous law
     int* herr;             // the handle's error word (mapped pinned host memory), nullptr: none
 };
 
@@ -70,9 +73,11 @@ struct DEMState {
     int rebuilds;
 };
 
-// Device buffers of the neighbour machinery.
+// Device buffers of the neighbour machinery. Per-particle arrays are indexed by
+// the standard particle (particles of one element are consecutive, in prototype
+// order); ghost slots follow them.
 struct DEMBuffers {
-    double* el;            // [EF_COUNT][P]
+    double* el;            // [EF_COUNT][E]
     DEMState* st;
     uint32_t* ghostIndex;  // [Gcap]
     double* ghostVec;      // [Gcap][3]
@@ -107,13 +112,36 @@ struct DEMBuffers {
     struct StatRec* pstat;  // [pairCap] per pair
     struct StatRec* wstat;  // [P] wall contact of each element
     struct StatRec* ostat;  // [P][NEAR_OBJ] object contacts of each element
+    // tangential springs (spring model only, else nullptr): per pair xi of the pair's second particle
+    // relative to its first [pairCap][3]; per element xi against its near wall and that wall's index [P][4];
+    // copies of the pair list, its springs, the per-particle lists and the ghosts taken at the start of a
+    // rebuild, from which the new pairs take over their springs
+    double* hist;
+    double* whist;
+    double* histOld;
+    uint32_t* pairsOld;
+    uint32_t* csrOffOld;
+    uint32_t* csrOld;
+    uint32_t* ghostIndexOld;
+    // multi-sphere elements (DEMParams::clumps only, else nullptr; with single spheres the particle is its
+    // element): element of each particle, first particle of each element, the particle's prototype vector
+    // (DEM.cpp:404-433), particle state after the corrector (position, velocity, radiusVec, elmt.cpp:277-290)
+    // and after the predictor (elmt.cpp:261-275), and per csr slot the moment of the normal force that
+    // elements of size > 1 take (DEM.cpp:1862-1867)
+    int* pelem;            // [P]
+    int* efirst;           // [E + 1]
+    double* pproto;        // [P][3]
+    double* pcor;          // [P][9]
+    double* ppred;         // [P][9]
+    double* pmn;           // [2 * pairCap][3]
     int scap;
     int ncells;
     int W;                 // ncells * 14
 };
 
-// Increments one contact adds to one element (size-1 elements): the normal force
-// (applied when flag & 1), then torque and tangential force (when flag & 2).
+// Increments one contact adds to one element: the normal force (applied when
+// flag & 1), the normal-force moment of an element of size > 1 (flag & 4, held
+// in DEMBuffers::pmn), then torque and tangential force (when flag & 2).
 struct PairRec {
     double fn[3];
     double ft[3];
@@ -143,7 +171,8 @@ struct StatRec {
 };
 
 // The particle state a saving sub-step exports (after the predictor): per
-// element the predicted position and velocity and wGlobal; info[0] = pairs.
+// particle the predicted position and velocity and its element's wGlobal;
+// info[0] = pairs.
 void launch_dem_save_elements(const DEMParams& p, const DEMBuffers& b, double* out, double* info, hipStream_t s);
 // force = 1: rebuild unconditionally (initialisation); 0: displacement check
 // (DEM.cpp:1314-1324, 342-347), rebuilding when it fires.
@@ -153,6 +182,9 @@ void launch_dem_rebuild(const DEMParams& p, const DEMBuffers& b, int force, hipS
 void launch_dem_pairs(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
 void launch_dem_gather(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
 void launch_dem_predict(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
+// Particle positions, velocities and radius vectors of every particle from its
+// element's corrected and predicted state (multi-sphere elements only).
+void launch_dem_particles(const DEMParams& p, const DEMBuffers& b, hipStream_t s);
 void launch_dem_energy(const DEMParams& p, const DEMBuffers& b, const double demSize[3], double* out, hipStream_t s);
 void launch_dem_couple_table(const DEMParams& p, const DEMBuffers& b, CoupleRec* cp, CoupleRecF* cpf, int* nslots,
                              const Units& u, hipStream_t s);

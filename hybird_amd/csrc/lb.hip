@@ -198,6 +198,12 @@ __device__ __forceinline__ double shear_visc64(const double f[19], const double 
     double nuTurb = 0.0;
     if (p.turbulence) nuTurb = p.turbConst * sr;
     double nuApp = p.nonNewtonian ? p.plasticVisc + p.yieldStress / (n * 2.0 * sr) : viscOld;
+    if (p.frictional) {
+        // Coulomb-like yield stress from the gauge pressure (n - rho0) c_s^2, none under tension
+        const double pg = (n - p.initDensity) / 3.0;
+        const double ty = p.frictionFluid * (pg > 0.0 ? pg : 0.0);
+        nuApp = ty > 0.0 ? p.plasticVisc + ty / (n * 2.0 * sr) : p.plasticVisc;
+    }
     const double a = nuApp + nuTurb;
     // std::min(maxVisc, a) then std::max(minVisc, .): NaN and +inf end at maxVisc
     const double lo = (a < maxVisc) ? a : maxVisc;
@@ -561,6 +567,10 @@ __device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uin
         const float sr = sqrtf(0.5f * (g00 * g00 + g11 * g11 + g22 * g22 + 2.0f * (g01 * g01 + g20 * g20 + g12 * g12)));
         float nuTurb = p.turbulence ? (float)p.turbConst * sr : 0.0f;
         float nuApp = p.nonNewtonian ? (float)p.plasticVisc + (float)p.yieldStress / (rho * 2.0f * sr) : (float)viscOld;
+        if (p.frictional) {
+            const float ty = (float)p.frictionFluid * (drho > 0.0f ? drho * (1.0f / 3.0f) : 0.0f);  // rho0 = 1
+            nuApp = ty > 0.0f ? (float)p.plasticVisc + ty / (rho * 2.0f * sr) : (float)p.plasticVisc;
+        }
         const float a = nuApp + nuTurb;
         const float maxV = (float)((1.8 - 0.5) / 3.0), minV = (float)((0.501 - 0.5) / 3.0);
         const float lo = (a < maxV) ? a : maxV;

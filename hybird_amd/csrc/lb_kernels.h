@@ -18,6 +18,8 @@ struct LBKParams {
     double visc0;        // initial kinematic viscosity (lattice units)
     double plasticVisc, yieldStress, turbConst;
     int nonNewtonian, turbulence;
+    int frictional;       // frictional apparent viscosity instead of Bingham (yield stress = frictionFluid x gauge pressure)
+    double frictionFluid;
     int collision;       // HYB_COLLISION_BGK / HYB_COLLISION_TRT
     double trtMagic;
     double unitLength, unitSpeed, unitAngVel;

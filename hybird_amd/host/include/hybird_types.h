@@ -120,12 +120,13 @@ public:
     tVect x0, x1, w0, FHydro, MHydro, FParticle, MParticle, FWall, MWall, FGrav;
     tQuat q0, q1;
     double fluidVolume = 0.0;
+    intList components;  // its standard particles (elmt.h)
 };
 class particle {
 public:
-    unsigned int particleIndex = 0, clusterIndex = 0;
+    unsigned int particleIndex = 0, clusterIndex = 0, protoIndex = 0;
     double r = 0.0;
-    tVect x0, x1;
+    tVect x0, x1, radiusVec;
     bool isGhost = false;
 };
 class wall {

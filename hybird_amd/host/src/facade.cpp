@@ -173,6 +173,11 @@ void LB::latticeBoltzmannGet(GetPot& lbmCfgFile, GetPot& command_line) {  // LB.
     const std::string prec = upper(extension_key<std::string>(lbmCfgFile, command_line, "precision", "FP64"));
     ASSERT(prec == "FP64" || prec == "FP32");
     cfg_.precision = prec == "FP32" ? HYB_FP32_SHIFTED : HYB_FP64;
+    const std::string rheo = upper(extension_key<std::string>(lbmCfgFile, command_line, "rheology", "BINGHAM"));
+    ASSERT(rheo == "BINGHAM" || rheo == "FRICTIONAL");
+    cfg_.rheology = rheo == "FRICTIONAL" ? HYB_RHEOLOGY_FRICTIONAL : HYB_RHEOLOGY_BINGHAM;
+    cfg_.frictionCoefFluid = extension_key<double>(lbmCfgFile, command_line, "frictionCoefFluid", 0.0);
+    ASSERT(cfg_.frictionCoefFluid >= 0.0);
     // checkpoint and restart (the reference cannot resume a run)
     restartFile_ = extension_key<std::string>(lbmCfgFile, command_line, "restartFile", "");
     // fluid files and the status line need u of every collision (kept on the device in
@@ -350,6 +355,11 @@ void DEM::discreteElementGet(GetPot& lbmCfgFile, GetPot& command_line) {  // DEM
     cfg_.viscTang = viscTang;
     cfg_.frictionCoefPart = frictionCoefPart;
     cfg_.frictionCoefWall = frictionCoefWall;
+    {  // extension: tangential contact law
+        const std::string tm = upper(extension_key<std::string>(lbmCfgFile, command_line, "tangentialModel", "VISCOUS"));
+        ASSERT(tm == "VISCOUS" || tm == "SPRING");
+        cfg_.tangentialModel = tm == "SPRING" ? HYB_TANGENTIAL_SPRING : HYB_TANGENTIAL_VISCOUS;
+    }
     // particle initial state (DEM.cpp:63-116)
     string particleFile;
     PARSE_CLASS_MEMBER(lbmCfgFile, particleFile, "particleFile", "particles.dat");
@@ -460,16 +470,25 @@ void DEM::discreteElementInit(unsIntList& boundary, unsIntList& lbSize, measureU
         b.p2 = b.p1 + b.v;
         pbcs.push_back(b);
     }
+    // the spheres of every element, consecutive and in prototype order
+    // (elmt::generateParticles, elmt.cpp:122-137); positions from the device
     particles.clear();
-    for (const elmt& e : elmts) {
-        particle p;
-        p.particleIndex = p.clusterIndex = e.index;
-        p.r = e.radius;
-        p.x0 = e.x0;
-        p.x1 = e.x1;
-        particles.push_back(p);
+    for (elmt& e : elmts) {
+        e.components.clear();
+        for (unsigned i = 0; i < e.size; ++i) {
+            particle p;
+            p.particleIndex = (unsigned)particles.size();
+            p.clusterIndex = e.index;
+            p.protoIndex = i;
+            p.r = e.radius;
+            p.x0 = e.x0;
+            p.x1 = e.x1;
+            e.components.push_back((int)p.particleIndex);
+            particles.push_back(p);
+        }
     }
     stdParticles = (unsigned)particles.size();
+    if (stdParticles != elmts.size()) syncHost();
 }
 
 void DEM::discreteElementStep(IO& io) {  // DEM.cpp:331-376
@@ -498,6 +517,8 @@ void DEM::exportSaved(IO& io) {
     if (n == 0) return;
     const std::vector<double> misc = field("e_misc");  // [E][19]: m, radius, I, ...
     const size_t E = elmts.size();
+    const size_t P = stdParticles;  // the captures hold standard particles
+    const auto cl = [&](size_t i) { return (size_t)particles[i].clusterIndex; };
     const double zero = 0.0;
     for (int32_t ev = 0; ev < n; ++ev) {
         const std::vector<double> info = saved_part(h_, ev, "info");
@@ -522,10 +543,10 @@ void DEM::exportSaved(IO& io) {
                 f << objects[o].r << " " << zero << " " << zero << " " << zero << " " << zero << " " << zero << " "
                   << zero << " " << zero << " " << "\n";
             }
-            for (size_t i = 0; i < E; ++i) {
+            for (size_t i = 0; i < P; ++i) {
                 put3(f, pos(i));
                 put3(f, vel(i));
-                f << elmts[i].radius << " " << zero << " " << zero << " " << zero << " ";
+                f << elmts[cl(i)].radius << " " << zero << " " << zero << " " << zero << " ";
                 put3(f, wg(i));
                 f << zero << " " << "\n";
             }
@@ -564,7 +585,7 @@ void DEM::exportSaved(IO& io) {
                   << 0 << " " << 0 << " " << "torque " << 0 << " " << 0 << " " << 0 << " " << "radius "
                   << objects[o].r << " " << "invMass " << zero << " " << "invInertia " << zero << " " << "\n";
             }
-            for (size_t i = 0; i < E; ++i) {
+            for (size_t i = 0; i < P; ++i) {
                 f << "BaseParticle id " << i + stdObjects << " " << "indSpecies " << 0 << " " << "position ";
                 put3(f, pos(i));
                 f << "orientation " << 0 << " " << 0 << " " << 0 << " " << 0 << " " << "velocity ";
@@ -572,8 +593,8 @@ void DEM::exportSaved(IO& io) {
                 f << "angularVelocity ";
                 put3(f, wg(i));
                 f << 0 << " " << "force " << 0 << " " << 0 << " " << 0 << " " << "torque " << 0 << " " << 0 << " " << 0
-                  << " " << "radius " << elmts[i].radius << " " << "invMass " << 1 / misc[19 * i] << " "
-                  << "invInertia " << 1 / misc[19 * i + 2] << " " << "\n";
+                  << " " << "radius " << elmts[cl(i)].radius << " " << "invMass " << 1 / misc[19 * cl(i)] << " "
+                  << "invInertia " << 1 / misc[19 * cl(i) + 2] << " " << "\n";
             }
             f << "Interactions " << 0;
             f << "\n";
@@ -590,7 +611,7 @@ void DEM::exportSaved(IO& io) {
             for (size_t k = 0; k + 12 <= pr.size(); k += 12) {
                 const double* r = &pr[k];
                 const size_t a = (size_t)r[0], b = (size_t)r[1];
-                if (r[11] == 0.0 || a >= E || b >= E) continue;
+                if (r[11] == 0.0 || a >= P || b >= P) continue;
                 stat_line(f, time, (long)(a + stdObjects), (long)(b + stdObjects), 0.5 * (pos(a) + pos(b)), r);
             }
             const std::vector<double> wl = saved_part(h_, ev, "walls");
@@ -599,7 +620,7 @@ void DEM::exportSaved(IO& io) {
                 if (r[11] == 0.0) continue;
                 const size_t e = (size_t)r[1];
                 const tVect en(r[5], r[6], r[7]);
-                const tVect centre = pos(e) - en * (elmts[e].radius - r[2]);
+                const tVect centre = pos(e) - en * (elmts[cl(e)].radius - r[2]);
                 stat_line(f, time, (long)(e + stdObjects), -(long)(r[0] + 1), centre, r);
             }
             const std::vector<double> ob = saved_part(h_, ev, "objects");
@@ -643,9 +664,20 @@ void DEM::syncHost() {
         el.FGrav = tVect(F[18], F[19], F[20]);
         el.m = misc[e * 19];
         el.fluidVolume = misc[e * 19 + 17];
-        if (e < particles.size()) {
+        if (particles.size() == elmts.size()) {  // single spheres: the particle is its element
             particles[e].x0 = el.x0;
             particles[e].x1 = el.x1;
+        }
+    }
+    if (particles.size() != elmts.size()) {  // spheres of multi-sphere elements: [S][10] x0, x1, radiusVec, r
+        std::vector<double> pd;
+        if (!view(h_, hyb_dem_get, "p_data", pd) || pd.size() < 10 * particles.size())
+            hybird_check(HYB_E_STATE, "DEM::syncHost", __FILE__, __LINE__);
+        for (size_t i = 0; i < particles.size(); ++i) {
+            const double* r = &pd[10 * i];
+            particles[i].x0 = tVect(r[0], r[1], r[2]);
+            particles[i].x1 = tVect(r[3], r[4], r[5]);
+            particles[i].radiusVec = tVect(r[6], r[7], r[8]);
         }
     }
 }

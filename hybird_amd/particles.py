@@ -149,8 +149,15 @@ def rsa_bed(seed: int, box_lo, box_hi, radius_fn, count: int, periodic=(False, F
     return Bed(np.array(xs, dtype=np.float64).reshape(-1, 3), np.array(rs, dtype=np.float64))
 
 
-def write_particle_file(path: str, bed: Bed, velocities: np.ndarray | None = None) -> None:
-    """Write ``bed`` in the reference particle-file format (DEM.cpp:78-127)."""
+def write_particle_file(path: str, bed: Bed, velocities: np.ndarray | None = None, sizes=None, omegas=None,
+                        quats=None) -> None:
+    """Write ``bed`` in the reference particle-file format (DEM.cpp:78-127).
+
+    Optional per element: ``sizes`` (spheres per element, 1-4, laid out by the
+    prototypes of DEM.cpp:404-433), angular velocities ``omegas`` and
+    orientation quaternions ``quats`` (q0, q1, q2, q3); defaults 1, zero and
+    the identity.
+    """
     with open(path, "w") as fh:
         fh.write(f"{bed.count}\n")
         for i in range(bed.count):
@@ -158,8 +165,11 @@ def write_particle_file(path: str, bed: Bed, velocities: np.ndarray | None = Non
             vx = vy = vz = 0.0
             if velocities is not None:
                 vx, vy, vz = velocities[i]
-            vals = [float(bed.r[i]), float(x), float(y), float(z), float(vx), float(vy), float(vz)]
-            fh.write(f"{i} 1 " + " ".join(repr(v) for v in vals) + " 0 0 0 1 0 0 0 0 0 0 0\n")
+            size = 1 if sizes is None else int(sizes[i])
+            w = (0.0, 0.0, 0.0) if omegas is None else tuple(float(c) for c in omegas[i])
+            q = (1.0, 0.0, 0.0, 0.0) if quats is None else tuple(float(c) for c in quats[i])
+            vals = [float(bed.r[i]), float(x), float(y), float(z), float(vx), float(vy), float(vz), *w, *q]
+            fh.write(f"{i} {size} " + " ".join(repr(v) for v in vals) + " 0 0 0 0\n")
 
 
 def read_particle_file(path: str):

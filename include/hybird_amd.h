@@ -75,6 +75,18 @@ enum hyb_problem {
 enum hyb_contact { HYB_CONTACT_LINEAR = 0, HYB_CONTACT_HERTZIAN = 1 }; /* elmt.h:141 */
 enum hyb_collision { HYB_COLLISION_BGK = 0, HYB_COLLISION_TRT = 1 };
 enum hyb_precision { HYB_FP64 = 0, HYB_FP32_SHIFTED = 1 };
+/* Apparent-viscosity law of non-Newtonian nodes (node.cpp:125-140 is the Bingham law). FRICTIONAL (not in
+ * the reference) replaces the Bingham yield stress by a Coulomb-like one proportional to the gauge pressure:
+ * nu = plasticVisc + frictionCoefFluid * max(n - rho0, 0) / 3 / (2 n shearRate), clamped like the Bingham
+ * viscosity; it needs no nonNewtonianSolve flag. */
+enum hyb_rheology { HYB_RHEOLOGY_BINGHAM = 0, HYB_RHEOLOGY_FRICTIONAL = 1 };
+/* Tangential contact law. VISCOUS is the reference's history-free law (DEM.cpp:2176-2222):
+ * F_t = min(2 viscTang sqrt(m_eff k_s) |v_t|, mu F_n) along v_t. SPRING (not in the reference) keeps a
+ * tangential spring xi per particle pair and per element's near wall: xi is projected on the current tangent
+ * plane, advanced by v_t dt, F_t = k_s xi + 2 viscTang sqrt(m_eff k_s) v_t with the same k_s, limited to
+ * mu F_n (slip resets xi to the limit); xi is carried across neighbour rebuilds and restarts. Objects and
+ * cylinders keep the viscous law. */
+enum hyb_tangential { HYB_TANGENTIAL_VISCOUS = 0, HYB_TANGENTIAL_SPRING = 1 };
 
 /* Configuration: the .cfg keys of the reference, in the reference's (physical)
  * units; the library converts to lattice units exactly as LB.cpp:85-188 and
@@ -111,6 +123,9 @@ typedef struct hyb_config {
     double drumSpeed;       /* HYB_PROBLEM_DRUM: drum angular velocity (rad/s, cfg key drumSpeed) */
     double fluidMass;       /* HYB_PROBLEM_DRUM: target liquid mass (physical units, cfg key fluidMass) */
     int32_t pairCapacity;   /* DEM neighbour-pair capacity (0: sized from the particle count) */
+    int32_t rheology;       /* HYB_RHEOLOGY_BINGHAM (reference) or HYB_RHEOLOGY_FRICTIONAL */
+    double frictionCoefFluid; /* HYB_RHEOLOGY_FRICTIONAL: yield stress per unit gauge pressure */
+    int32_t tangentialModel;  /* HYB_TANGENTIAL_VISCOUS (reference) or HYB_TANGENTIAL_SPRING */
     int32_t reserved[3];
 } hyb_config;
 

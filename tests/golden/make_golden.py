@@ -199,6 +199,33 @@ def _cases():
                                          boundary1=7, boundary2=7, boundary3=7, initVisc=50.0, lbFZ=-5.0e-4,
                                          youngMod=1.0e6, density=3000.0, drumSpeed=0.3, fluidMass=800.0),
                          bed=bed6, vel=vel6, cycles=[0, 1, 2, 5, 15, 30], keys=LATTICE_KEYS + DEM_KEYS)
+    # multi-sphere elements (DEM.cpp:404-433, 1149-1227, 1801-2136; elmt.cpp:17-83,
+    # 122-137, 261-299; LB.cpp:1862-1905): a tilted two-sphere dumbbell pressed
+    # into the floor, a spinning three-sphere element pressed across the periodic
+    # x plane into a four-sphere element that straddles it (contact through ghost
+    # particles), a single sphere on the floor pushed into the dumbbell (mixed
+    # sizes in one pair), all starting with small overlaps in the fluid between
+    # no-slip y and z walls
+    c30, s30 = np.cos(np.pi / 12.0), np.sin(np.pi / 12.0)  # 30 degrees about y
+    c50, s50 = np.cos(np.radians(25.0)), np.sin(np.radians(25.0))  # 50 degrees about (1, 1, 1) / sqrt(3)
+    k3 = s50 / np.sqrt(3.0)
+    bed7 = Bed(np.array([[7.0, 7.5, 2.98], [14.1, 8.6, 12.0], [1.6, 8.0, 12.5], [11.25, 6.8, 1.99]]),
+               np.array([2.0, 1.8, 1.6, 1.5]))
+    cases["clumps"] = dict(cfg=base_config(lbSizeX=20, lbSizeY=18, lbSizeZ=22, initVisc=0.05, lbFX=1.0e-4,
+                                           lbFZ=-4.0e-3, youngMod=2.0e2, density=3.0, boundary2=7, boundary3=7),
+                           bed=bed7, sizes=np.array([2, 3, 4, 1]),
+                           vel=np.array([[0.0, 0.0, -0.03], [0.05, -0.01, 0.0], [-0.01, 0.0, -0.01], [-0.03, 0.0, 0.0]]),
+                           omegas=np.array([[0.0, 0.01, 0.0], [0.0, 0.0, 0.04], [0.02, -0.01, 0.0], [0.0, 0.0, 0.0]]),
+                           quats=np.array([[c30, 0.0, s30, 0.0], [1.0, 0.0, 0.0, 0.0], [c50, k3, k3, k3],
+                                           [1.0, 0.0, 0.0, 0.0]]),
+                           cycles=[0, 1, 2, 5, 15, 30], keys=LATTICE_KEYS + DEM_KEYS)
+    # the same clumps with the reference's output files: MercuryDPM .data, .restart
+    # and .fstat every 3 sub-steps (per sphere, DEM.cpp:2280-2565), particle .vtu
+    # (per sphere, IO.cpp:250-420), recycle .dat (per element, IO.cpp:486-535) and
+    # the status line with its side files
+    cases["clumps_io"] = dict(cases["clumps"], cfg=dict(cases["clumps"]["cfg"], saveCount=3, partExpTime=4.0,
+                                                        recycleExpTime=4.0, screenExpTime=2.0),
+                              cycles=[0, 8], keys=["types"], files=True)
     return cases
 
 
@@ -207,7 +234,7 @@ def run_case(name: str, case: dict, outdir: str) -> None:
         cfg = dict(case["cfg"])
         pf = os.path.join(tmp, "particles.dat")
         of = os.path.join(tmp, "objects.dat")
-        write_particle_file(pf, case["bed"], case.get("vel"))
+        write_particle_file(pf, case["bed"], case.get("vel"), case.get("sizes"), case.get("omegas"), case.get("quats"))
         if case.get("objects") is not None:
             write_object_file(of, case["objects"])
         else:
@@ -252,6 +279,9 @@ def run_case(name: str, case: dict, outdir: str) -> None:
         arrays["bed_x"] = case["bed"].x
         arrays["bed_r"] = case["bed"].r
         arrays["bed_v"] = np.zeros_like(case["bed"].x) if case.get("vel") is None else np.asarray(case["vel"])
+        for key in ("sizes", "omegas", "quats"):
+            if case.get(key) is not None:
+                arrays[f"bed_{key}"] = np.asarray(case[key])
         arrays["objects"] = np.zeros((0, 7)) if case.get("objects") is None else np.asarray(case["objects"], float)
         np.savez_compressed(os.path.join(outdir, f"{name}.npz"), **arrays)
 

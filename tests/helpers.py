@@ -20,7 +20,8 @@ def load_case(name: str):
     cfg = {str(k): float(v) for k, v in zip(z["cfg_keys"], z["cfg_vals"])}
     cfg["problemName"] = str(z["cfg_problem"])
     cfg["contactModel"] = "HERTZIAN"
-    rec = element_records(z["bed_x"], z["bed_r"], z["bed_v"])
+    opt = {k: z[f"bed_{n}"] for k, n in (("size", "sizes"), ("w", "omegas"), ("q", "quats")) if f"bed_{n}" in z.files}
+    rec = element_records(z["bed_x"], z["bed_r"], z["bed_v"], **opt)
     return cfg, rec, z
 
 

@@ -18,11 +18,14 @@ pytestmark = pytest.mark.gpu
 BIN = os.path.join(ROOT, "hybird_amd", "host", "hybird_gpu")
 
 
-def _write_case(tmp, cfg, bed_x, bed_r, cycles, objects=None, bed_v=None):
+def _write_case(tmp, cfg, bed_x, bed_r, cycles, objects=None, bed_v=None, z=None):
+    """The cfg, particle and object files of a run; z: a fixture whose clump fields (sizes, spins,
+    orientations) go into the particle file."""
     from hybird_amd.cfg import write_cfg
     from hybird_amd.particles import Bed, write_empty_object_file, write_object_file, write_particle_file
     pf, of = os.path.join(tmp, "p.dat"), os.path.join(tmp, "o.dat")
-    write_particle_file(pf, Bed(np.asarray(bed_x), np.asarray(bed_r)), bed_v)
+    opt = {k: z[f"bed_{k}"] for k in ("sizes", "omegas", "quats") if z is not None and f"bed_{k}" in z.files}
+    write_particle_file(pf, Bed(np.asarray(bed_x), np.asarray(bed_r)), bed_v, **opt)
     if objects is not None and len(objects):
         write_object_file(of, objects)
     else:
@@ -142,18 +145,8 @@ def test_dropin_output_files_match_reference(tmp_path, name):
     assert {"export.dat", "energy.dat", "maxVel.dat", "force.dat", "plasticity.dat"} <= set(ref)
     assert sorted(mine) == sorted(ref), (sorted(set(mine) - set(ref)), sorted(set(ref) - set(mine)))
     for fn, text in ref.items():
-        if fn in ("export.dat", "energy.dat", "maxVel.dat", "force.dat", "plasticity.dat"):
-            # words equal, numbers to the six printed digits ("MaxFSpeed= 0.02(2%)" splits at the brackets)
-            split = lambda t: [w for line in t.splitlines() for w in re.split(r"[\s()%;]+", line) if w] + ["|"]
-            wr, wm = split(text), split(mine[fn])
-            assert len(text.splitlines()) == len(mine[fn].splitlines()) and len(wr) == len(wm), fn
-            for x, y in zip(wr, wm):
-                try:
-                    fx, fy = float(x), float(y)
-                except ValueError:
-                    assert x == y, (fn, x, y)
-                    continue
-                assert abs(fx - fy) <= 1e-5 * max(abs(fx), abs(fy)) + 1e-12, (fn, x, y)
+        if fn in STATUS_FILES:
+            _status_close(text, mine[fn], fn)
             continue
         if fn.endswith(".dat"):
             a = np.array([float(v) for v in text.split()])
@@ -173,6 +166,23 @@ def test_dropin_output_files_match_reference(tmp_path, name):
                 assert np.array_equal(ar[key], am[key]), (fn, key)
             else:
                 _close(ar[key], am[key], f"{fn}:{key}")
+
+
+STATUS_FILES = ("export.dat", "energy.dat", "maxVel.dat", "force.dat", "plasticity.dat")
+
+
+def _status_close(text, mine, fn):
+    """Words equal, numbers to the six printed digits ("MaxFSpeed= 0.02(2%)" splits at the brackets)."""
+    split = lambda t: [w for line in t.splitlines() for w in re.split(r"[\s()%;]+", line) if w] + ["|"]
+    wr, wm = split(text), split(mine)
+    assert len(text.splitlines()) == len(mine.splitlines()) and len(wr) == len(wm), fn
+    for x, y in zip(wr, wm):
+        try:
+            fx, fy = float(x), float(y)
+        except ValueError:
+            assert x == y, (fn, x, y)
+            continue
+        assert abs(fx - fy) <= 1e-5 * max(abs(fx), abs(fy)) + 1e-12, (fn, x, y)
 
 
 def _tokens_close(ref, mine, what, rtol=1e-9):
@@ -221,6 +231,63 @@ def test_dropin_mercury_files_match_reference(tmp_path):
     assert sorted(mine) == sorted(ref)
     for fn, text in ref.items():
         _tokens_close(text, mine[fn], fn, rtol=2e-5 if fn.endswith(".vtu") else 1e-9)
+
+
+@pytest.mark.skipif(not os.path.exists(BIN), reason="hybird_gpu not built (needs the reference's hybird.cpp)")
+def test_dropin_clumps_match_oracle_and_reference_files(tmp_path):
+    """Multi-sphere elements through the reference's main: the state after 15
+    cycles against the oracle, and the output files of an 8-cycle run
+    (MercuryDPM .data, .restart, .fstat per sphere every 3 sub-steps, particle
+    .vtu per sphere, recycle .dat per element, the status line and its side
+    files) against the files the reference wrote for it."""
+    cfg, rec, z = load_case("clumps")
+    tmp = str(tmp_path)
+    d = os.path.join(tmp, "state")
+    os.makedirs(d)
+    path = _write_case(d, cfg, z["bed_x"], z["bed_r"], 15, bed_v=z["bed_v"], z=z)
+    env = dict(os.environ, HYBIRD_AMD_DUMP_DIR=d)
+    print("[gpu dropin] hybird_gpu clumps 15 cycles", flush=True)
+    r = subprocess.run([BIN, "-c", path, "-d", os.path.join(d, "out")], env=env, capture_output=True, text=True,
+                       timeout=300, cwd=d)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    orc = OracleSim(fixture_config(cfg), rec)
+    orc.cycle(15)
+    t = orc.get("types")
+    assert np.array_equal(_load(d, "lb_types", np.uint8), t)
+    act = (t == 0) | (t == 3)
+    assert rel_l2(_load(d, "lb_f").reshape(-1, 19)[act], orc.get("f")[act]) <= 1e-12
+    assert rel_l2(_load(d, "dem_e_x").reshape(-1, 6, 3), orc.get("e_x")) <= 1e-11
+    assert np.array_equal(_load(d, "dem_neighborTable", np.uint32), orc.get("neighborTable"))
+
+    cfg, rec, z = load_case("clumps_io")
+    d = os.path.join(tmp, "files")
+    os.makedirs(os.path.join(d, "out"))
+    path = _write_case(d, cfg, z["bed_x"], z["bed_r"], int(max(z["cycles"])), bed_v=z["bed_v"], z=z)
+    print("[gpu dropin] hybird_gpu clumps output files", flush=True)
+    r = subprocess.run([BIN, "-c", path, "-d", os.path.join(d, "out"), "-n", "run"], capture_output=True, text=True,
+                       timeout=300, cwd=d)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    mine = {}
+    for dirpath, _, names in os.walk(os.path.join(d, "out")):
+        for fn in names:
+            mine[fn] = open(os.path.join(dirpath, fn)).read()
+    ref = {k.split("/", 1)[1]: bytes(z[k]).decode() for k in z.files if k.startswith("file/")}
+    assert {"NONE.data", "NONE.fstat", "NONE.restart", "part0000000004.vtu", "recycle_4.dat"} <= set(ref)
+    for fn, text in ref.items():
+        assert fn in mine, fn
+        if fn in STATUS_FILES:
+            _status_close(text, mine[fn], fn)
+        elif fn.endswith((".data", ".restart", ".fstat")):
+            _tokens_close(text, mine[fn], fn, rtol=1e-9)
+        elif fn.endswith(".vtu"):
+            ar, am = _arrays(text), _arrays(mine[fn])
+            assert sorted(ar) == sorted(am), fn
+            for key in ar:
+                _close(ar[key], am[key], f"{fn}:{key}")
+        else:  # recycle files: six printed digits
+            a = np.array([float(v) for v in text.split()])
+            b = np.array([float(v) for v in mine[fn].split()])
+            _close(a, b, fn)
 
 
 @pytest.mark.skipif(not os.path.exists(BIN), reason="hybird_gpu not built (needs the reference's hybird.cpp)")

@@ -11,7 +11,7 @@ import pytest
 from helpers import fixture_config, load_case, objects_of
 from oracle.oracle import OracleSim
 
-CASES = ["lb16", "chan", "bingham", "turb", "turb_bingham", "couple2", "dense", "dem60", "fs", "walls", "walls_fs", "io", "io_fs", "objects"]
+CASES = ["lb16", "chan", "bingham", "turb", "turb_bingham", "couple2", "dense", "dem60", "fs", "walls", "walls_fs", "io", "io_fs", "objects", "clumps"]
 
 
 @pytest.mark.parametrize("name", CASES)
