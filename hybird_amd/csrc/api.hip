@@ -248,9 +248,17 @@ struct hyb_lb {
     int64_t fsSteps = 0;
     bool capture = false;  // keep n, u, hydroForce of every collision for the host views
     float* ffield = nullptr;  // optional per-node body force [3][N] (lattice units)
-    // adjoint: checkpoints of the post-collision state every adjEvery steps
+    // adjoint: checkpoints of the post-collision state every adjEvery steps, the
+    // reverse pass's buffers (states of one interval, two adjoint population
+    // buffers, dL/dF, tau partials) and the device time of both loops
     int adjSteps = -1, adjEvery = 0;
     std::vector<void*> adjCk;
+    std::vector<void*> adjSeg;
+    double* adjLam[2] = {nullptr, nullptr};
+    double* adjGF = nullptr;
+    double* adjTP = nullptr;
+    double adjFwdMs = 0.0, adjBwdMs = 0.0;
+    hipEvent_t adjEv[2] = {nullptr, nullptr};
     bool profiling = false;
     std::vector<hipEvent_t> evs;
     size_t evUsed = 0;
@@ -1269,29 +1277,69 @@ static int adj_check(hyb_lb* l) {
     return HYB_OK;
 }
 
+// Releases the checkpoints and the reverse pass's buffers.
+static void adj_free(hyb_lb* l) {
+    for (void* q : l->adjCk) (void)hipFree(q);
+    for (void* q : l->adjSeg) (void)hipFree(q);
+    l->adjCk.clear();
+    l->adjSeg.clear();
+    dfree(l->adjLam[0]); dfree(l->adjLam[1]); dfree(l->adjGF); dfree(l->adjTP);
+    l->adjSteps = -1;
+}
+
+// Elapsed device time between the handle's two adjoint events.
+static int adj_elapsed(hyb_lb* l, double* ms) {
+    float f = 0.0f;
+    HIPCHK(hipEventSynchronize(l->adjEv[1]));
+    HIPCHK(hipEventElapsedTime(&f, l->adjEv[0], l->adjEv[1]));
+    *ms = (double)f;
+    return HYB_OK;
+}
+
 int hyb_adj_forward(hyb_lb* l, hyb_dem* d, int32_t steps, int32_t every) {
     if (!l || !l->inited) return fail(HYB_E_STATE, "hyb_adj_forward before init");
     if (steps < 1 || every < 1) return fail(HYB_E_ARG, "steps and checkpoint interval must be >= 1");
     if (l->steps != 0) return fail(HYB_E_STATE, "hyb_adj_forward starts from the initial state (before any step)");
     int rc = adj_check(l);
     if (rc) return rc;
-    for (void* ck : l->adjCk) (void)hipFree(ck);
-    l->adjCk.clear();
+    adj_free(l);
     const size_t pb = l->popBytes();
     const int M = (steps + every - 1) / every;  // checkpoints of s_0, s_k, ..., s_{(M-1)k}
-    for (int m = 0; m < M; ++m) {
-        void* ck = nullptr;
-        HIPCHK(hipMalloc(&ck, pb));
-        l->adjCk.push_back(ck);
+    const int K = std::min(every, steps);        // states of one interval in the reverse pass
+    auto grab = [&](std::vector<void*>& v, int n) -> int {
+        for (int m = 0; m < n; ++m) {
+            void* q = nullptr;
+            if (hipMalloc(&q, pb) != hipSuccess) return fail(HYB_E_HIP, "adjoint buffer allocation failed");
+            v.push_back(q);
+        }
+        return HYB_OK;
+    };
+    if ((rc = grab(l->adjCk, M)) || (rc = grab(l->adjSeg, K)) || (rc = dalloc(&l->adjLam[0], pb / 8)) ||
+        (rc = dalloc(&l->adjLam[1], pb / 8)) || (rc = dalloc(&l->adjGF, 3 * l->N)) ||
+        (rc = dalloc(&l->adjTP, adj_tau_slots(l->params())))) {
+        adj_free(l);
+        return rc;
     }
+    for (hipEvent_t& ev : l->adjEv)
+        if (!ev) HIPCHK(hipEventCreate(&ev));
     HIPCHK(hipMemcpyAsync(l->adjCk[0], l->f[l->cur], pb, hipMemcpyDeviceToDevice, l->stream));
+    HIPCHK(hipEventRecord(l->adjEv[0], l->stream));
     for (int t = 1; t <= steps; ++t) {
         if ((rc = hyb_lb_step(l, d))) return rc;
         if (t % every == 0 && t / every < M)
             HIPCHK(hipMemcpyAsync(l->adjCk[t / every], l->f[l->cur], pb, hipMemcpyDeviceToDevice, l->stream));
     }
+    HIPCHK(hipEventRecord(l->adjEv[1], l->stream));
+    if ((rc = adj_elapsed(l, &l->adjFwdMs))) return rc;
     l->adjSteps = steps;
     l->adjEvery = every;
+    return HYB_OK;
+}
+
+int hyb_adj_device_time(hyb_lb* l, double* forward_ms, double* backward_ms) {
+    if (!l) return fail(HYB_E_ARG, "hyb_adj_device_time: null handle");
+    if (forward_ms) *forward_ms = l->adjFwdMs;
+    if (backward_ms) *backward_ms = l->adjBwdMs;
     return HYB_OK;
 }
 
@@ -1306,24 +1354,14 @@ int hyb_adj_backward(hyb_lb* l, hyb_dem* d, const double* seed, double* grad_for
     const bool withParticles = d && d->P && l->slotCap > 0;
     const uint32_t* solid = withParticles ? l->solid : nullptr;
     hipStream_t s = l->stream;
-    // device buffers: segment of K states, two adjoint buffers, gradients
-    std::vector<void*> seg;
-    double *lam[2] = {nullptr, nullptr}, *gF = nullptr, *tp = nullptr;
+    // the buffers hyb_adj_forward allocated: segment of K states, two adjoint buffers, gradients
+    std::vector<void*>& seg = l->adjSeg;
+    double* lam[2] = {l->adjLam[0], l->adjLam[1]};
+    double* gF = l->adjGF;
+    double* tp = l->adjTP;
     const size_t slots = adj_tau_slots(p);
-    auto cleanup = [&]() {
-        for (void* q : seg) (void)hipFree(q);
-        (void)hipFree(lam[0]); (void)hipFree(lam[1]); (void)hipFree(gF); (void)hipFree(tp);
-    };
-    for (int i = 0; i < K; ++i) {
-        void* q = nullptr;
-        if (hipMalloc(&q, pb) != hipSuccess) { cleanup(); return fail(HYB_E_HIP, "adjoint segment allocation failed"); }
-        seg.push_back(q);
-    }
-    if (hipMalloc((void**)&lam[0], pb) != hipSuccess || hipMalloc((void**)&lam[1], pb) != hipSuccess ||
-        hipMalloc((void**)&gF, 24 * N) != hipSuccess || hipMalloc((void**)&tp, 8 * slots) != hipSuccess) {
-        cleanup();
-        return fail(HYB_E_HIP, "adjoint buffer allocation failed");
-    }
+    if ((int)seg.size() < std::min(K, T) || !lam[0] || !lam[1] || !gF || !tp)
+        return fail(HYB_E_STATE, "hyb_adj_backward: reverse-pass buffers missing");
     // the seed goes through lam[1] (node-major) into lam[0] (planes)
     HIPCHK(hipMemsetAsync(lam[0], 0, pb, s));
     HIPCHK(hipMemcpyAsync(lam[1], seed, 19 * 8 * N, hipMemcpyHostToDevice, s));
@@ -1333,6 +1371,7 @@ int hyb_adj_backward(hyb_lb* l, hyb_dem* d, const double* seed, double* grad_for
     HIPCHK(hipMemsetAsync(tp, 0, 8 * slots, s));
     int cur = 0;
     const int M = (int)l->adjCk.size();
+    HIPCHK(hipEventRecord(l->adjEv[0], s));
     for (int m = M - 1; m >= 0; --m) {
         const int t0 = m * K;
         const int len = std::min(K, T - t0);
@@ -1346,8 +1385,9 @@ int hyb_adj_backward(hyb_lb* l, hyb_dem* d, const double* seed, double* grad_for
             cur ^= 1;
         }
     }
+    HIPCHK(hipEventRecord(l->adjEv[1], s));
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(s));
+    if (int rc2 = adj_elapsed(l, &l->adjBwdMs)) return rc2;
     if (grad_f0) {  // node-major copy in the free adjoint buffer
         launch_planes_to_nodes(lam[cur], lam[cur ^ 1], l->link, N, PS, 19, s);
         HIPCHK(hipMemcpyAsync(grad_f0, lam[cur ^ 1], 19 * 8 * N, hipMemcpyDeviceToHost, s));
@@ -1368,7 +1408,6 @@ int hyb_adj_backward(hyb_lb* l, hyb_dem* d, const double* seed, double* grad_for
         }
         *grad_tau = sum + comp;
     }
-    cleanup();
     return HYB_OK;
 }
 
@@ -1383,7 +1422,9 @@ int hyb_lb_destroy(hyb_lb* l) {
     dfree(l->curvw); dfree(l->macBuf[0]); dfree(l->macBuf[1]);
     dfree(l->dcount); dfree(l->cp); dfree(l->cpf); dfree(l->boxes); dfree(l->nslots); dfree(l->segs);
     dfree(l->segCount);
-    for (void* ck : l->adjCk) (void)hipFree(ck);
+    adj_free(l);
+    for (hipEvent_t ev : l->adjEv)
+        if (ev) (void)hipEventDestroy(ev);
     if (l->ffield) (void)hipFree(l->ffield);
     errword_free(l->err);
     FSBuffers& b = l->fsb;
@@ -2212,9 +2253,7 @@ int hyb_state_load(hyb_lb* l, hyb_dem* d, const char* path) {
         l->capture = s.capture != 0; l->macValid = s.macValid != 0; l->havePart = s.havePart != 0;
         // per-step scratch: the run-sum lists start empty
         if (l->segCount) HIPCHK(hipMemset(l->segCount, 0, sizeof(uint32_t) * (size_t)l->slotCap));
-        for (void* ck : l->adjCk) (void)hipFree(ck);
-        l->adjCk.clear();
-        l->adjSteps = -1;
+        adj_free(l);
     }
     if (d) {
         DemScalars s;

@@ -47,6 +47,7 @@ constexpr int BY = HYB_LB_BY;
 #define HYB_LB_NTLOAD 1
 #endif
 
+
 // Population stores and loads: optionally non-temporal (every population is
 // written once and read once per step, so neither benefits from caching).
 template <typename T>
@@ -90,6 +91,10 @@ __device__ __forceinline__ bool active_at_stream(const LBKParams& p, size_t n) {
 // walls bounce back the node's own f_opp(i) instead; they only occur next to
 // walls, so that fix-up runs as a separate, rarely taken branch and the common
 // path issues its 19 loads back to back.
+template <typename T, bool CURVED>
+__device__ __forceinline__ void wall_fixup(const T* __restrict__ fin, size_t PS, const LBKParams& p, const Coord& c,
+                                           uint32_t lw, T f[19]);
+
 template <typename T, bool CURVED = false>
 __device__ __forceinline__ void pull19(const T* __restrict__ fin, size_t PS, const LBKParams& p, const Coord& c,
                                        uint32_t lw, T f[19]) {
@@ -105,6 +110,16 @@ __device__ __forceinline__ void pull19(const T* __restrict__ fin, size_t PS, con
         // a wall neighbour's index is valid memory; its value is replaced below
         f[i] = ld_pop(&fin[(size_t)i * PS + (i == 0 ? self : nb)]);
     }
+    wall_fixup<T, CURVED>(fin, PS, p, c, lw, f);
+}
+
+// The links of node c that end on a wall: bounce-back of the node's own
+// f_opp(i), the slip reflection or the curved-wall rule, in place of the pulled
+// value.
+template <typename T, bool CURVED>
+__device__ __forceinline__ void wall_fixup(const T* __restrict__ fin, size_t PS, const LBKParams& p, const Coord& c,
+                                           uint32_t lw, T f[19]) {
+    const uint32_t self = (uint32_t)c.idx;
     if (lw & LINK_WALLS) {
         const uint32_t sw = (lw & LINK_SLIP) ? p.slipw[self] : 0u;
         const uint32_t cw = (CURVED && (lw & LINK_CURVED)) ? p.curvw[self] : 0u;

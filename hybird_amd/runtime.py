@@ -40,7 +40,7 @@ EXPORTS = ["hyb_config_defaults", "hyb_last_error", "hyb_device_count", "hyb_syn
            "hyb_lb_kernel_time", "hyb_lb_active_nodes", "hyb_lb_set_capture", "hyb_dem_init_lattice",
            "hyb_lb_set_force_field", "hyb_lb_set_populations", "hyb_adj_forward", "hyb_adj_backward",
            "hyb_state_save", "hyb_state_load", "hyb_dem_set_objects", "hyb_dem_set_save_count", "hyb_dem_saved",
-           "hyb_dem_saved_get", "hyb_copy_bandwidth", "hyb_lb_wall_fhydro_into"]
+           "hyb_dem_saved_get", "hyb_copy_bandwidth", "hyb_lb_wall_fhydro_into", "hyb_adj_device_time"]
 
 
 class HybError(RuntimeError):
@@ -82,6 +82,7 @@ def load():
                        ("hyb_dem_saved", [vp, C.POINTER(i32)]),
                        ("hyb_dem_saved_get", [vp, i32, C.c_char_p, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
                        ("hyb_copy_bandwidth", [C.c_size_t, i32, i32, C.POINTER(C.c_double)]),
+                       ("hyb_adj_device_time", [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
                        ("hyb_lb_wall_fhydro_into", [vp, vp, i32])]:
         fn = getattr(L, name)
         fn.argtypes = args
@@ -187,6 +188,12 @@ class Simulation:
         _check(load().hyb_adj_backward(self._lb, self._dem, sd.ctypes.data, gF.ctypes.data, gT.ctypes.data,
                                        g0.ctypes.data), "hyb_adj_backward")
         return gF, float(gT[0]), g0
+
+    def adj_device_time(self):
+        """Device milliseconds of the last adjoint forward loop and reverse loop (HIP events)."""
+        f, b = C.c_double(0.0), C.c_double(0.0)
+        _check(load().hyb_adj_device_time(self._lb, C.byref(f), C.byref(b)), "hyb_adj_device_time")
+        return f.value, b.value
 
     def step_lb(self, n: int = 1) -> None:
         """LB steps only (no DEM sub-steps, no coupling): particles stay frozen."""

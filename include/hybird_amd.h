@@ -264,7 +264,10 @@ int hyb_lb_set_force_field(hyb_lb* lb, const float* F);
 int hyb_lb_set_populations(hyb_lb* lb, const double* f);
 /* Runs `steps` LB steps from the initial state (particles frozen: no DEM or
  * coupling steps) and keeps the state every `every` steps for the reverse pass
- * (FP64, no free surface, constant viscosity). */
+ * (FP64, no free surface, constant viscosity). It also allocates the buffers
+ * of the reverse pass (the states of one checkpoint interval, two adjoint
+ * population buffers, the gradient accumulators), kept until the next
+ * hyb_adj_forward or hyb_lb_destroy. */
 int hyb_adj_forward(hyb_lb* lb, hyb_dem* dem, int32_t steps, int32_t every);
 /* Gradients of a loss L(f_T) of the final post-stream populations, given
  * seed = dL/df_T [N][19]: grad_force [N][3] (per-node body force), *grad_tau
@@ -272,6 +275,11 @@ int hyb_adj_forward(hyb_lb* lb, hyb_dem* dem, int32_t steps, int32_t every);
  * Any output may be NULL. Recomputes the states between checkpoints. */
 int hyb_adj_backward(hyb_lb* lb, hyb_dem* dem, const double* seed, double* grad_force, double* grad_tau,
                      double* grad_f0);
+/* Device time (HIP events on the handle's stream) of the step loop of the last
+ * hyb_adj_forward (LB steps and checkpoint copies) and of the reverse loop of the
+ * last hyb_adj_backward (segment replays and adjoint steps), in milliseconds;
+ * neither includes allocation or host transfers. */
+int hyb_adj_device_time(hyb_lb* lb, double* forward_ms, double* backward_ms);
 
 /* ---- checkpoint and restart (no counterpart in the reference, which cannot
  * resume a run) ----
