@@ -944,7 +944,7 @@ static int lb_alloc(hyb_lb* l, int slotCap) {
     R.push_back(Blk{"f1", l->f[1], pb});
     if ((rc = ralloc(R, "type", &l->type, N)) || (rc = ralloc(R, "link", &l->link, N)) ||
         (rc = ralloc(R, "solid", &l->solid, N)) ||
-        (rc = dalloc(&l->maxXYZ, 3 * (size_t)l->Z)) || (rc = dalloc(&l->dcount, (size_t)l->Z)) ||
+        (rc = dalloc(&l->maxXYZ, 3 * (size_t)l->Z)) || (rc = dalloc(&l->dcount, (size_t)l->Z + 2)) ||
         (rc = dalloc(&l->nslots, 2)))
         return rc;
     if (l->varVisc() && (rc = ralloc(R, "visc", &l->visc, N))) return rc;
@@ -1092,29 +1092,23 @@ int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
                         l->freeSurface ? l->fsb.rho[0] : nullptr, l->fsb.mass, l->fsb.uS, p, s);
     launch_fill_u32(l->solid, 0u, l->N, s);
     HIPCHK(hipMemsetAsync(l->nslots, 0, 2 * sizeof(int), s));
-    HIPCHK(hipMemsetAsync(l->dcount, 0, sizeof(unsigned long long) * l->Z, s));
+    HIPCHK(hipMemsetAsync(l->dcount, 0, sizeof(unsigned long long) * (l->Z + 2), s));
     launch_count_active(l->link, l->dcount, p, s);
     // particle nodes at init (LB.cpp:475-495)
     if (d && d->P) lb_couple_launch(l, d);
+    launch_count_mass(l->link, d && d->P && l->slotCap > 0 ? l->solid : nullptr, l->dcount + l->Z, l->params(), s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
     {
-        std::vector<unsigned long long> cz(l->Z);
+        std::vector<unsigned long long> cz(l->Z + 2);
         HIPCHK(hipMemcpy(cz.data(), l->dcount, cz.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
         l->activeCount = 0;
-        for (unsigned long long v : cz) l->activeCount += v;
-    }
-    // total mass of active nodes outside particles (LB.cpp:209-216): fluid nodes carry mass 1
-    {
-        std::vector<uint32_t> lk(l->N), so(l->N);
-        HIPCHK(hipMemcpy(lk.data(), l->link, l->N * 4, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(so.data(), l->solid, l->N * 4, hipMemcpyDeviceToHost));
-        double tm = 0.0;
-        for (size_t i = 0; i < l->N; ++i) {
-            const bool owned = (so[i] >> SOLID_SLOT_BITS) == l->ctag && (so[i] & SOLID_SLOT_MASK);
-            if ((lk[i] & LINK_ACTIVE) && !owned) tm += (lk[i] & LINK_INTERFACE) ? 0.5 * l->initDensity : l->initDensity;
-        }
-        // DRUM: the target liquid mass instead (LB.cpp:206-208)
+        for (int z = 0; z < l->Z; ++z) l->activeCount += cz[z];
+        // total mass of active nodes outside particles (LB.cpp:209-216): fluid nodes carry mass rho0,
+        // interface nodes half of it; with rho0 = 1 every partial sum of the reference's node loop is
+        // a multiple of 1/2, so the counts give its sum exactly; DRUM takes the target liquid mass
+        // instead (LB.cpp:206-208)
+        const double tm = (double)cz[l->Z] * l->initDensity + (double)cz[l->Z + 1] * (0.5 * l->initDensity);
         l->totalMass = l->cfg.problem == HYB_PROBLEM_DRUM ? l->cfg.fluidMass / l->unit.Mass : tm;
     }
     l->steps = 0;

@@ -66,6 +66,7 @@ __global__ void __launch_bounds__(FS_BLOCK)
         ctr->nA = 0;
         ctr->nB = 0;
         for (int i = 0; i < 4; ++i) ctr->acc[i] = 0ull;
+        for (int i = 0; i < 4; ++i) ctr->tot[i] = 0ull;
     }
     const double rho0 = p.initDensity;
     FS_FOR(pos, nI) {
@@ -476,21 +477,18 @@ __global__ void __launch_bounds__(FS_BLOCK) k_fs_total_mass(const uint8_t* __res
 }
 
 // redistributeMass(-0.01 * (mass - totalMass)) over the new interface list
-// (LB.cpp:1815-1821, 1796-1804). One workgroup; the sum is cleared for the next pass.
+// (LB.cpp:1815-1821, 1796-1804), grid-stride; the next pass's first kernel
+// clears the sum.
 __global__ void __launch_bounds__(FS_BLOCK) k_fs_mass_fix(const uint32_t* __restrict__ next,
-                                                          FSCounters* __restrict__ ctr, int cur,
+                                                          const FSCounters* __restrict__ ctr, int cur,
                                                           double* __restrict__ mass, double totalMass) {
 #pragma clang fp contract(off)
     const uint32_t n = ctr->n[cur ^ 1];
+    if (!n) return;
     const double massDeficit = exact_value(ctr->tot) - totalMass;
     const double massSurplus = -0.01 * massDeficit;
-    if (n) {
-        const double add = massSurplus / (double)n;
-        for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) mass[next[q]] += add;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int i = 0; i < 4; ++i) ctr->tot[i] = 0ull;
+    const double add = massSurplus / (double)n;
+    FS_FOR(q, n) mass[next[q]] += add;
 }
 
 void launch_fs_mass_fix(Prec prec, const uint8_t* type, const uint32_t* solid, uint32_t ctag, FSBuffers& b, int cur,
@@ -504,7 +502,7 @@ void launch_fs_mass_fix(Prec prec, const uint8_t* type, const uint32_t* solid, u
     else
         k_fs_total_mass<float><<<(unsigned)g, FS_BLOCK, 0, s>>>(type, solid, ctag, b.mass, (const float*)b.rho[rcur],
                                                                 b.ctr, N);
-    k_fs_mass_fix<<<1, FS_BLOCK, 0, s>>>(b.list[cur ^ 1], b.ctr, cur, b.mass, totalMass);
+    k_fs_mass_fix<<<FS_GRID, FS_BLOCK, 0, s>>>(b.list[cur ^ 1], b.ctr, cur, b.mass, totalMass);
 }
 
 void launch_fs_curved_extra(Prec prec, const void* fpost, const uint32_t* link, FSBuffers& b, int cur,

@@ -858,6 +858,25 @@ __global__ void k_count_active(const uint32_t* __restrict__ link, unsigned long 
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(&cnt[blockIdx.z], (unsigned long long)__popcll(b));
 }
 
+// Active nodes outside particles, fluid (cnt[0]) and interface (cnt[1]): the
+// initial liquid mass of LB.cpp:209-216 in node units.
+__global__ void k_count_mass(const uint32_t* __restrict__ link, const uint32_t* __restrict__ solid,
+                             unsigned long long* __restrict__ cnt, LBKParams p) {
+    Coord c;
+    bool fl = false, in = false;
+    if (node_coord(p, c)) {
+        const uint32_t lw = link[c.idx];
+        const bool owned = solid && owner_slot(solid[c.idx], p.ctag) != 0u;
+        fl = (lw & LINK_ACTIVE) && !(lw & LINK_INTERFACE) && !owned;
+        in = (lw & LINK_ACTIVE) && (lw & LINK_INTERFACE) && !owned;
+    }
+    const unsigned long long bf = __ballot(fl), bi = __ballot(in);
+    if ((threadIdx.x & 63) == 0) {
+        if (bf) atomicAdd(&cnt[0], (unsigned long long)__popcll(bf));
+        if (bi) atomicAdd(&cnt[1], (unsigned long long)__popcll(bi));
+    }
+}
+
 // ------------------------------------------------------------------ particle classification
 // Each particle slot (real particles, then ghosts) marks the active nodes
 // strictly inside it (vector.cpp:153-158) with the current coupling-step tag.
@@ -1251,6 +1270,12 @@ void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, 
 void launch_fill_u32(uint32_t* a, uint32_t v, size_t n, hipStream_t s) {
     if (!n) return;
     k_fill_u32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a, v, n);
+}
+
+void launch_count_mass(const uint32_t* link, const uint32_t* solid, unsigned long long* count, const LBKParams& p,
+                       hipStream_t s) {
+    const dim3 g = lattice_grid(p), b(BX, BY, 1);
+    k_count_mass<<<g, b, 0, s>>>(link, solid, count, p);
 }
 
 void launch_count_active(const uint32_t* link, unsigned long long* count, const LBKParams& p, hipStream_t s) {

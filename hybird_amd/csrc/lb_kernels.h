@@ -160,5 +160,8 @@ void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, 
 bool launch_copy_probe(int width, const void* src, void* dst, size_t bytes, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_count_active(const uint32_t* link, unsigned long long* count, const LBKParams& p, hipStream_t s);
+// count[0], count[1] += active fluid and interface nodes outside particles (solid: ownership words or nullptr).
+void launch_count_mass(const uint32_t* link, const uint32_t* solid, unsigned long long* count, const LBKParams& p,
+                       hipStream_t s);
 
 }  // namespace hyb
