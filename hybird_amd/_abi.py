@@ -23,6 +23,8 @@ HYB_RHEOLOGY_BINGHAM = 0
 HYB_RHEOLOGY_FRICTIONAL = 1
 HYB_TANGENTIAL_VISCOUS = 0
 HYB_TANGENTIAL_SPRING = 1
+HYB_WALL_CONTACTS_FIRST = 0
+HYB_WALL_CONTACTS_ALL = 1
 
 ELEMENT_RECORD = 20
 
@@ -77,7 +79,8 @@ class HybConfig(C.Structure):
         ("rheology", C.c_int32),
         ("frictionCoefFluid", C.c_double),
         ("tangentialModel", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("wallContacts", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -134,6 +137,8 @@ def config_from_cfg(cfg: dict, collision: str = "BGK", precision: str = "FP64", 
     c.frictionCoefFluid = float(g("frictionCoefFluid", 0.0))
     c.tangentialModel = (HYB_TANGENTIAL_SPRING if str(g("tangentialModel", "VISCOUS")).upper() == "SPRING"
                          else HYB_TANGENTIAL_VISCOUS)
+    c.wallContacts = (HYB_WALL_CONTACTS_ALL if str(g("wallContacts", "FIRST")).upper() == "ALL"
+                      else HYB_WALL_CONTACTS_FIRST)
     return c
 
 

@@ -605,6 +605,11 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
     if (c.tangentialModel != HYB_TANGENTIAL_VISCOUS && c.tangentialModel != HYB_TANGENTIAL_SPRING)
         return fail(HYB_E_ARG, "unknown tangential contact model");
     p.spring = c.tangentialModel == HYB_TANGENTIAL_SPRING;
+    if (c.wallContacts != HYB_WALL_CONTACTS_FIRST && c.wallContacts != HYB_WALL_CONTACTS_ALL)
+        return fail(HYB_E_ARG, "unknown wall-contact mode");
+    p.allWalls = c.wallContacts == HYB_WALL_CONTACTS_ALL;
+    if (p.allWalls && p.spring)
+        return fail(HYB_E_UNSUPPORTED, "wallContacts = ALL with the tangential spring is not supported");
     p.numVisc = c.numVisc;
     p.demF[0] = demF.x; p.demF[1] = demF.y; p.demF[2] = demF.z;
     // objects: periodic copies of those within 2 nebrRange of a periodic plane,
@@ -718,9 +723,13 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
         (rc = ralloc(R, "side", &b.side, 3 * (size_t)std::max(P, 1))) || (rc = ralloc(R, "ghostB", &b.ghostB, p.Gcap)) ||
         (rc = ralloc(R, "posI", &b.posI, p.pairCap)) || (rc = ralloc(R, "posJ", &b.posJ, p.pairCap)) ||
         (rc = dalloc(&b.prec, 2 * (size_t)p.pairCap)) || (rc = ralloc(R, "vmax", &b.vmax, (size_t)(std::max(NE, 1) + 63) / 64)) || (rc = dalloc(&b.energy, 6)) ||
-        (rc = dalloc(&b.wcf, (size_t)std::max(P, 1))))
+        (rc = dalloc(&b.wcf, (size_t)std::max(P, 1) * wall_slots(p))))
         return rc;
-    HIPCHK(hipMemset(b.wcf, 0, sizeof(ContactForce) * (size_t)std::max(P, 1)));
+    HIPCHK(hipMemset(b.wcf, 0, sizeof(ContactForce) * (size_t)std::max(P, 1) * wall_slots(p)));
+    if (p.allWalls) {
+        if ((rc = ralloc(R, "nearWallMask", &b.nearWallMask, (size_t)std::max(P, 1)))) return rc;
+        HIPCHK(hipMemset(b.nearWallMask, 0, sizeof(uint32_t) * (size_t)std::max(P, 1)));
+    }
     if (p.ncyl && (rc = ralloc(R, "nearCyl", &b.nearCyl, (size_t)std::max(P, 1)))) return rc;
     if (p.clumps) {
         if ((rc = dalloc(&b.pelem, (size_t)P)) || (rc = dalloc(&b.efirst, (size_t)NE + 1)) ||
@@ -795,7 +804,7 @@ int hyb_dem_set_save_count(hyb_dem* d, int32_t saveCount) {
     const size_t P = (size_t)std::max(d->P, 1);
     int rc;
     if ((rc = dalloc(&d->saveElems, (size_t)slots * P * 9)) || (rc = dalloc(&d->saveInfo, (size_t)slots)) ||
-        (rc = dalloc(&d->savePairs, (size_t)slots * d->p.pairCap)) || (rc = dalloc(&d->saveWalls, (size_t)slots * P)) ||
+        (rc = dalloc(&d->savePairs, (size_t)slots * d->p.pairCap)) || (rc = dalloc(&d->saveWalls, (size_t)slots * P * wall_slots(d->p))) ||
         (d->p.nobj && (rc = dalloc(&d->saveObjs, (size_t)slots * P * NEAR_OBJ))))
         return rc;
     d->saveSlots = slots;
@@ -821,7 +830,7 @@ int hyb_dem_step(hyb_dem* d) {  // DEM.cpp:331-376
                 const size_t P = (size_t)d->P;
                 DEMBuffers bs = d->b;
                 bs.pstat = d->savePairs + (size_t)slot * d->p.pairCap;
-                bs.wstat = d->saveWalls + (size_t)slot * P;
+                bs.wstat = d->saveWalls + (size_t)slot * P * wall_slots(d->p);
                 bs.ostat = d->saveObjs ? d->saveObjs + (size_t)slot * P * NEAR_OBJ : nullptr;
                 launch_dem_save_elements(d->p, d->b, d->saveElems + (size_t)slot * P * 9, d->saveInfo + slot, d->stream);
                 launch_dem_pairs(d->p, bs, d->stream);
@@ -841,7 +850,7 @@ int hyb_dem_destroy(hyb_dem* d) {
     if (!d) return HYB_OK;
     if (d->stream) (void)hipStreamSynchronize(d->stream);
     DEMBuffers& b = d->b;
-    dfree(b.el); dfree(b.st); dfree(b.ghostIndex); dfree(b.ghostVec); dfree(b.pairs); dfree(b.nearWall);
+    dfree(b.el); dfree(b.st); dfree(b.ghostIndex); dfree(b.ghostVec); dfree(b.pairs); dfree(b.nearWall); dfree(b.nearWallMask);
     dfree(b.cellTable); dfree(b.cellOf); dfree(b.cellCount); dfree(b.work); dfree(b.csrOff); dfree(b.csr);
     dfree(b.cursor); dfree(b.spos); dfree(b.ghostOfCount); dfree(b.ghostOf); dfree(b.side); dfree(b.ghostB);
     dfree(b.posI); dfree(b.posJ); dfree(b.prec); dfree(b.vmax); dfree(b.energy); dfree(b.nearObj); dfree(b.nearObjCount);
@@ -1797,7 +1806,10 @@ int hyb_dem_saved_get(hyb_dem* d, int32_t event, const char* cpart, void* host, 
     if (part == "elements") return rows(d->saveElems + (size_t)event * P * 9, P * 9 * sizeof(double));
     if (part == "pairs")
         return rows(d->savePairs + (size_t)event * d->p.pairCap, (size_t)nPairs * sizeof(StatRec));
-    if (part == "walls") return rows(d->saveWalls + (size_t)event * P, P * sizeof(StatRec));
+    if (part == "walls") {
+        const size_t ws = (size_t)wall_slots(d->p);
+        return rows(d->saveWalls + (size_t)event * P * ws, P * ws * sizeof(StatRec));
+    }
     if (part == "objects")
         return rows(d->saveObjs ? (const void*)(d->saveObjs + (size_t)event * P * NEAR_OBJ) : nullptr,
                     d->saveObjs ? P * NEAR_OBJ * sizeof(StatRec) : 0);
@@ -1856,13 +1868,20 @@ int hyb_dem_get(hyb_dem* d, const char* cname, void* host, size_t cap, size_t* n
         const size_t rows = wallView ? d->walls.size() : d->objects.size();
         std::vector<V3> F(rows, V3(0.0, 0.0, 0.0));
         if (P && d->substeps > 0) {
-            const size_t per = wallView ? 1 : NEAR_OBJ;
+            const size_t per = wallView ? (size_t)wall_slots(d->p) : NEAR_OBJ;
             std::vector<ContactForce> cf((size_t)P * per);
             std::vector<uint32_t> idx((size_t)P * per);
-            std::vector<int> cnt((size_t)P, 1);
+            std::vector<int> cnt((size_t)P, (int)per);
             if (wallView) {
                 HIPCHK(hipMemcpy(cf.data(), d->b.wcf, sizeof(ContactForce) * cf.size(), hipMemcpyDeviceToHost));
-                HIPCHK(hipMemcpy(idx.data(), d->b.nearWall, 4 * idx.size(), hipMemcpyDeviceToHost));
+                if (d->p.allWalls) {  // record w of a particle is its contact with wall w, if that wall is near
+                    std::vector<uint32_t> mask((size_t)P);
+                    HIPCHK(hipMemcpy(mask.data(), d->b.nearWallMask, 4 * mask.size(), hipMemcpyDeviceToHost));
+                    for (size_t e = 0; e < (size_t)P; ++e)
+                        for (size_t w = 0; w < per; ++w) idx[e * per + w] = (mask[e] >> w & 1u) ? (uint32_t)w : 0xFFFFFFFFu;
+                } else {
+                    HIPCHK(hipMemcpy(idx.data(), d->b.nearWall, 4 * idx.size(), hipMemcpyDeviceToHost));
+                }
             } else if (d->p.nobj) {
                 HIPCHK(hipMemcpy(cf.data(), d->b.ocf, sizeof(ContactForce) * cf.size(), hipMemcpyDeviceToHost));
                 HIPCHK(hipMemcpy(idx.data(), d->b.nearObj, 4 * idx.size(), hipMemcpyDeviceToHost));
@@ -1965,9 +1984,16 @@ int hyb_dem_get(hyb_dem* d, const char* cname, void* host, size_t cap, size_t* n
     }
     if (name == "nearWallTable") {
         std::vector<uint32_t> nw(P);
-        if (P) HIPCHK(hipMemcpy(nw.data(), d->b.nearWall, 4 * (size_t)P, hipMemcpyDeviceToHost));
-        for (int e = 0; e < P; ++e)
-            if (nw[e] != 0xFFFFFFFFu) { u.push_back((uint32_t)e); u.push_back(nw[e]); }
+        if (P) HIPCHK(hipMemcpy(nw.data(), d->p.allWalls ? d->b.nearWallMask : d->b.nearWall, 4 * (size_t)P,
+                                hipMemcpyDeviceToHost));
+        for (int e = 0; e < P; ++e) {
+            if (d->p.allWalls) {  // every near wall, in wall order
+                for (int w = 0; w < d->p.nwalls; ++w)
+                    if (nw[e] >> w & 1u) { u.push_back((uint32_t)e); u.push_back((uint32_t)w); }
+            } else if (nw[e] != 0xFFFFFFFFu) {
+                u.push_back((uint32_t)e); u.push_back(nw[e]);
+            }
+        }
         return put(u.data(), u.size() * 4, host, cap, need);
     }
     if (name == "ghostIndex") return put(gi.data(), gi.size() * 4, host, cap, need);

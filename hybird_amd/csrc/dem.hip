@@ -427,13 +427,19 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
     for (int w = t; w < W; w += RB) walk_pairs(p, b, E, S, (unsigned)w, b.pairs + 2 * (size_t)b.work[w]);
     if (t == 0) b.st->nPairs = T;
     // ---- near-wall list (DEM.cpp:1496-1513): the first wall within range of each standard particle
+    // (all walls within range as a bit mask with DEMParams::allWalls)
     int nw = 0;
     for (int e = t; e < P; e += RB) {
         const V3 x0 = part_x(E, p, b, e);
-        uint32_t wi = 0xFFFFFFFFu;
+        uint32_t wi = 0xFFFFFFFFu, mask = 0u;
         for (int k = 0; k < p.nwalls; ++k)
-            if (p.walls[k].dist(x0) < p.nebrRange) { wi = (uint32_t)k; break; }
+            if (p.walls[k].dist(x0) < p.nebrRange) {
+                if (wi == 0xFFFFFFFFu) wi = (uint32_t)k;
+                mask |= 1u << k;
+                if (!p.allWalls) break;
+            }
         b.nearWall[e] = wi;
+        if (p.allWalls) b.nearWallMask[e] = mask;
         nw += wi != 0xFFFFFFFFu;
         // near-cylinder list (DEM.cpp:1535-1550): the first cylinder within range
         if (p.ncyl) {
@@ -960,10 +966,73 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
             pv = vpe;
         }
     };
-    // wall contacts of the element's particles (DEM.cpp:1692-1717, 1902-1982)
+    // the contact of standard particle i (predicted px, pv) with wall wi (DEM.cpp:1902-1982): adds to FW, MW
+    // and fills its force record and statistics; wxi: the spring (spring model)
+    const auto wall_contact = [&](int i, uint32_t wi, const V3& px, const V3& pv, ContactForce& wcf, StatRec& wst,
+                                  double wxi[3]) -> bool {
+        const WallDev& w = p.walls[wi];
+        const double distance = w.dist(px);
+        const double overlap = re - distance;
+        if (!(overlap > 0.0)) return false;
+        const V3 en = w.n;
+        const V3 cpv = w.getSpeed(px);
+        const V3 relVel = pv - cpv;
+        const double normRelVel = relVel.dot(en);
+        const V3 normalRelVel = en * normRelVel;
+        const double fn = normal_contact(p, 2.0 * overlap, normRelVel, re, me);
+        const V3 normalForce = en * fn;
+        const V3 vecRadJ = -re * en;
+        V3 centerDistJ = vecRadJ;
+        if (multi) centerDistJ = centerDistJ + (px - xpe);
+        FW = FW + normalForce;
+        if (multi) MW = MW + centerDistJ.cross(normalForce);
+        set_force(wcf, normalForce, 1.0);
+        const V3 relVelContact = relVel + wpg.cross(vecRadJ);
+        const V3 tangRelVelContact = relVelContact - normalRelVel;
+        if constexpr (SP) {
+            const V3 tangForce = spring_force(p, fn, re, me, p.fricWall, en, tangRelVelContact, wxi);
+            const double ft = tangForce.norm();
+            if (ft != 0.0) {
+                MW = MW - centerDistJ.cross(tangForce);
+                FW = FW - tangForce;
+                set_force_t(wcf, tangForce);
+                if (b.wstat) set_stat(wst, overlap, fn, ft, en, tangForce / ft);
+            }
+        } else {
+            const double nt = tangRelVelContact.norm();
+            if (nt != 0.0) {
+                const double ft = tangential_contact(p, nt, fn, re, me, p.fricWall);
+                const V3 et = tangRelVelContact / tangRelVelContact.norm();
+                const V3 tangForce = ft * et;
+                MW = MW - centerDistJ.cross(tangForce);
+                FW = FW - tangForce;
+                set_force_t(wcf, tangForce);
+                if (b.wstat) set_stat(wst, overlap, fn, ft, en, et);
+            }
+        }
+        return true;
+    };
+    // wall contacts of the element's particles (DEM.cpp:1692-1717): the first wall within range of each
+    // particle, or every wall within range in wall order (DEMParams::allWalls; record i * nwalls + w)
     for (int i = ilo; i < ihi; ++i) {
         V3 px, pv;
         part_pred(i, px, pv);
+        if (!SP && p.allWalls) {
+            const uint32_t mask = b.nearWallMask[i];
+            const int ws = p.nwalls;
+            for (int wi = 0; wi < ws; ++wi) {
+                StatRec wst{(mask >> wi & 1u) ? (double)wi : -1.0, (double)i, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0},
+                            {0.0, 0.0, 0.0}, 0.0};
+                ContactForce wcf{{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0, 0.0};
+                if (mask >> wi & 1u) {
+                    double unused[3] = {0.0, 0.0, 0.0};
+                    wall_contact(i, (uint32_t)wi, px, pv, wcf, wst, unused);
+                }
+                b.wcf[(size_t)i * ws + wi] = wcf;
+                if (b.wstat) b.wstat[(size_t)i * ws + wi] = wst;
+            }
+            continue;
+        }
         const uint32_t wi = b.nearWall[i];
         StatRec wst{wi == 0xFFFFFFFFu ? -1.0 : (double)wi, (double)i, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0},
                     {0.0, 0.0, 0.0}, 0.0};
@@ -975,48 +1044,7 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
             wxi[0] = b.whist[4 * (size_t)i]; wxi[1] = b.whist[4 * (size_t)i + 1]; wxi[2] = b.whist[4 * (size_t)i + 2];
         }
         if (wi != 0xFFFFFFFFu) {
-            const WallDev& w = p.walls[wi];
-            const double distance = w.dist(px);
-            const double overlap = re - distance;
-            if (overlap > 0.0) {
-                const V3 en = w.n;
-                const V3 cpv = w.getSpeed(px);
-                const V3 relVel = pv - cpv;
-                const double normRelVel = relVel.dot(en);
-                const V3 normalRelVel = en * normRelVel;
-                const double fn = normal_contact(p, 2.0 * overlap, normRelVel, re, me);
-                const V3 normalForce = en * fn;
-                const V3 vecRadJ = -re * en;
-                V3 centerDistJ = vecRadJ;
-                if (multi) centerDistJ = centerDistJ + (px - xpe);
-                FW = FW + normalForce;
-                if (multi) MW = MW + centerDistJ.cross(normalForce);
-                set_force(wcf, normalForce, 1.0);
-                const V3 relVelContact = relVel + wpg.cross(vecRadJ);
-                const V3 tangRelVelContact = relVelContact - normalRelVel;
-                if constexpr (SP) {
-                    wcontact = true;
-                    const V3 tangForce = spring_force(p, fn, re, me, p.fricWall, en, tangRelVelContact, wxi);
-                    const double ft = tangForce.norm();
-                    if (ft != 0.0) {
-                        MW = MW - centerDistJ.cross(tangForce);
-                        FW = FW - tangForce;
-                        set_force_t(wcf, tangForce);
-                        if (b.wstat) set_stat(wst, overlap, fn, ft, en, tangForce / ft);
-                    }
-                } else {
-                    const double nt = tangRelVelContact.norm();
-                    if (nt != 0.0) {
-                        const double ft = tangential_contact(p, nt, fn, re, me, p.fricWall);
-                        const V3 et = tangRelVelContact / tangRelVelContact.norm();
-                        const V3 tangForce = ft * et;
-                        MW = MW - centerDistJ.cross(tangForce);
-                        FW = FW - tangForce;
-                        set_force_t(wcf, tangForce);
-                        if (b.wstat) set_stat(wst, overlap, fn, ft, en, et);
-                    }
-                }
-            }
+            wcontact = wall_contact(i, wi, px, pv, wcf, wst, wxi);
             b.wcf[i] = wcf;
         }
         if constexpr (SP) {

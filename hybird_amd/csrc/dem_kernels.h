@@ -49,8 +49,12 @@ struct DEMParams {
     int ncyl;              // cylinders (DEM.cpp:870-935)
     CylDev cyl[2];
     int spring;            // tangential spring with history (HYB_TANGENTIAL_SPRING) instead of the viscous law
+    int allWalls;          // every wall within nebrRange is a contact partner (HYB_WALL_CONTACTS_ALL), not only the first
     int* herr;             // the handle's error word (mapped pinned host memory), nullptr: none
 };
+
+// Wall-contact records per particle: one per wall with DEMParams::allWalls, else one.
+__host__ __device__ inline int wall_slots(const DEMParams& p) { return p.allWalls && p.nwalls > 0 ? p.nwalls : 1; }
 
 // DEMState::error codes: 1 ghost capacity, 2 particle outside the cell grid,
 // 3 stencil outside the cell table, 4 pair capacity, 5 more than NEAR_OBJ objects
@@ -82,7 +86,8 @@ struct DEMBuffers {
     uint32_t* ghostIndex;  // [Gcap]
     double* ghostVec;      // [Gcap][3]
     uint32_t* pairs;       // [pairCap][2]
-    uint32_t* nearWall;    // [P] wall index or 0xFFFFFFFF
+    uint32_t* nearWall;    // [P] wall index or 0xFFFFFFFF (the first wall within range)
+    uint32_t* nearWallMask;  // [P] bit w: wall w within range (DEMParams::allWalls only, else nullptr)
     int* cellTable;        // [Scap + ncells]
     int* cellOf;           // [Scap]
     int* cellCount;        // [ncells + 1]
@@ -106,11 +111,11 @@ struct DEMBuffers {
     uint32_t* nearCyl;     // [P] first cylinder within nebrRange or 0xFFFFFFFF
     // forces of the last sub-step's wall and object contacts (for walls[].FParticle
     // and objects[].FParticle, summed on request in the reference's order)
-    struct ContactForce* wcf;  // [P] contact of each element with its near wall
+    struct ContactForce* wcf;  // [P][WS] contact of each particle with its near wall(s); WS = allWalls ? nwalls : 1
     struct ContactForce* ocf;  // [P][NEAR_OBJ] contacts with its near objects
     // contact statistics of a saving sub-step (nullptr on the others)
     struct StatRec* pstat;  // [pairCap] per pair
-    struct StatRec* wstat;  // [P] wall contact of each element
+    struct StatRec* wstat;  // [P][WS] wall contacts of each particle
     struct StatRec* ostat;  // [P][NEAR_OBJ] object contacts of each element
     // tangential springs (spring model only, else nullptr): per pair xi of the pair's second particle
     // relative to its first [pairCap][3]; per element xi against its near wall and that wall's index [P][4];

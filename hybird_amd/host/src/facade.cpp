@@ -360,6 +360,11 @@ void DEM::discreteElementGet(GetPot& lbmCfgFile, GetPot& command_line) {  // DEM
         ASSERT(tm == "VISCOUS" || tm == "SPRING");
         cfg_.tangentialModel = tm == "SPRING" ? HYB_TANGENTIAL_SPRING : HYB_TANGENTIAL_VISCOUS;
     }
+    {  // extension: contacts with every wall in range instead of the first one
+        const std::string wc = upper(extension_key<std::string>(lbmCfgFile, command_line, "wallContacts", "FIRST"));
+        ASSERT(wc == "FIRST" || wc == "ALL");
+        cfg_.wallContacts = wc == "ALL" ? HYB_WALL_CONTACTS_ALL : HYB_WALL_CONTACTS_FIRST;
+    }
     // particle initial state (DEM.cpp:63-116)
     string particleFile;
     PARSE_CLASS_MEMBER(lbmCfgFile, particleFile, "particleFile", "particles.dat");

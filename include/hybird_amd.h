@@ -87,6 +87,11 @@ enum hyb_rheology { HYB_RHEOLOGY_BINGHAM = 0, HYB_RHEOLOGY_FRICTIONAL = 1 };
  * mu F_n (slip resets xi to the limit); xi is carried across neighbour rebuilds and restarts. Objects and
  * cylinders keep the viscous law. */
 enum hyb_tangential { HYB_TANGENTIAL_VISCOUS = 0, HYB_TANGENTIAL_SPRING = 1 };
+/* Walls a particle can touch. FIRST is the reference's near-wall list (DEM.cpp:1496-1513): only the first
+ * wall (in wall order) within nebrRange of a particle is listed, so a particle in the corner of a side wall
+ * and the floor has no floor contact and sinks through it under gravity. ALL (not in the reference) lists
+ * every wall within nebrRange and applies their contacts in wall order; not combined with SPRING. */
+enum hyb_wall_contacts { HYB_WALL_CONTACTS_FIRST = 0, HYB_WALL_CONTACTS_ALL = 1 };
 
 /* Configuration: the .cfg keys of the reference, in the reference's (physical)
  * units; the library converts to lattice units exactly as LB.cpp:85-188 and
@@ -126,7 +131,8 @@ typedef struct hyb_config {
     int32_t rheology;       /* HYB_RHEOLOGY_BINGHAM (reference) or HYB_RHEOLOGY_FRICTIONAL */
     double frictionCoefFluid; /* HYB_RHEOLOGY_FRICTIONAL: yield stress per unit gauge pressure */
     int32_t tangentialModel;  /* HYB_TANGENTIAL_VISCOUS (reference) or HYB_TANGENTIAL_SPRING */
-    int32_t reserved[3];
+    int32_t wallContacts;     /* HYB_WALL_CONTACTS_FIRST (reference) or HYB_WALL_CONTACTS_ALL */
+    int32_t reserved[2];
 } hyb_config;
 
 /* Fills *cfg with the reference's parse defaults (LB.cpp:93-185, DEM.cpp:15-181)

@@ -190,6 +190,7 @@ struct DEM {
     uint32_t stdParticles = 0;
     std::vector<Ghost> ghosts;
     bool newNeighborList = false;
+    bool allWalls = false;  // HYB_WALL_CONTACTS_ALL (not in the reference): no `break` after the first near wall
     // particleEnergy (utils.h:165-190): kin, rotKin, trKin, grav, elastic, total
     double energy[6] = {0, 0, 0, 0, 0, 0};
     // values the reference freezes in function-local statics at the first call

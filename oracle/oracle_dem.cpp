@@ -25,6 +25,7 @@ void Pbc::setPlanes() {  // utils.cpp:240-245
 
 // ---------------------------------------------------------------- parsing / init
 void DEM::get(const hyb_config& c, int n, const double* rec) {  // DEM.cpp:13-183
+    allWalls = c.wallContacts == HYB_WALL_CONTACTS_ALL;  // extension: every wall in range, in wall order
     mat.density = c.density;
     mat.contactModel = c.contactModel;
     mat.youngMod = c.youngMod;
@@ -374,7 +375,7 @@ void DEM::evalNearWallTable() {  // DEM.cpp:1496-1513
             if (walls[w].dist(particles[n].x0) < nebrRange) {
                 nearWallTable.push_back(n);
                 nearWallTable.push_back((uint32_t)w);
-                break;
+                if (!allWalls) break;
             }
         }
     }
