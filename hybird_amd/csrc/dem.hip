@@ -230,28 +230,23 @@ __global__ void __launch_bounds__(RB) k_dem_rebuild(DEMParams p, DEMBuffers b, i
     __syncthreads();
     // ---- evalMaxDisp (DEM.cpp:1314-1324) and the trigger (DEM.cpp:342-347)
     if (!force) {
-        // max |x1|^2 over elements, from the per-wavefront maxima the corrector
-        // left; a NaN or infinite maximum stops the DEM (error 6)
-        double m = 0.0;
-        const int nw = (p.E + 63) / 64;
-        for (int k = t; k < nw; k += RB) {
-            const double v = b.vmax[k];
-            if (!(v <= 1.79769313486231570e308)) sh_err = DEM_ERR_NUMERIC;
-            if (v > m) m = v;
+        // max |x1|^2 over elements, left by the corrector (and cleared here for the
+        // next one); a NaN or infinite maximum stops the DEM (error 6)
+        if (t == 0) {
+            unsigned long long* vb = reinterpret_cast<unsigned long long*>(b.vmax);
+            const double m = __longlong_as_double((long long)*vb);
+            *vb = 0ull;
+            if (!(m <= 1.79769313486231570e308)) {
+                sh_err = DEM_ERR_NUMERIC;
+            } else {
+                const double maxVel = sqrt(m);
+                b.st->maxDisp += maxVel * p.deltat;
+                doit = b.st->maxDisp > p.trigger;
+                if (doit) b.st->maxDisp = 0.0;
+            }
         }
-        red[t] = m;
         __syncthreads();
         if (sh_err) { if (t == 0) dem_error(p, b, DEM_ERR_NUMERIC); return; }
-        for (int w = RB / 2; w > 0; w >>= 1) {
-            if (t < w && red[t + w] > red[t]) red[t] = red[t + w];
-            __syncthreads();
-        }
-        if (t == 0) {
-            const double maxVel = sqrt(red[0]);
-            b.st->maxDisp += maxVel * p.deltat;
-            doit = b.st->maxDisp > p.trigger;
-            if (doit) b.st->maxDisp = 0.0;
-        }
     } else if (t == 0) {
         doit = 1;
     }
@@ -811,41 +806,45 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// Folds a wavefront's max |x1|^2 (>= 0, or NaN) into DEMBuffers::vmax: the bit
+// patterns of non-negative doubles order like the values, and NaN patterns lie
+// above every finite value, so an integer atomic max keeps the maximum (order-free).
+__device__ __forceinline__ void vmax_add(const DEMBuffers& b, double v2) {
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(reinterpret_cast<unsigned long long*>(b.vmax), (unsigned long long)__double_as_longlong(v2));
+}
+
+// Gear predictor of one quantity's six derivatives (elmt.cpp:139-177), with the
+// position (c2) or angular-velocity (c1) coefficients.
+template <class T>
+__device__ __forceinline__ void gear_predict(const T x[6], const double* c, T o[6]) {
+    o[0] = x[0] + x[1] * c[0] + x[2] * c[1] + x[3] * c[2] + x[4] * c[3] + x[5] * c[4];
+    o[1] = x[1] + x[2] * c[0] + x[3] * c[1] + x[4] * c[2] + x[5] * c[3];
+    o[2] = x[2] + x[3] * c[0] + x[4] * c[1] + x[5] * c[2];
+    o[3] = x[3] + x[4] * c[0] + x[5] * c[1];
+    o[4] = x[4] + x[5] * c[0];
+    o[5] = x[5];
+}
+
 // Gear predictor of one element from its corrected state (elmt.cpp:139-177).
 template <bool CL>
 __device__ __forceinline__ void predict_elem(const El& E, const DEMParams& p, const DEMBuffers& b, int e, const V3 x[6],
                                              const Q4 q[6], const V3 w[6]) {
-    const double* c1 = p.c1;
-    const double* c2 = p.c2;
-    const V3 xp0 = x[0] + x[1] * c2[0] + x[2] * c2[1] + x[3] * c2[2] + x[4] * c2[3] + x[5] * c2[4];
-    const V3 xp1 = x[1] + x[2] * c2[0] + x[3] * c2[1] + x[4] * c2[2] + x[5] * c2[3];
-    const V3 xp2 = x[2] + x[3] * c2[0] + x[4] * c2[1] + x[5] * c2[2];
-    const V3 xp3 = x[3] + x[4] * c2[0] + x[5] * c2[1];
-    const V3 xp4 = x[4] + x[5] * c2[0];
-    const V3 xp5 = x[5];
-    Q4 qp0 = q[0] + q[1] * c2[0] + q[2] * c2[1] + q[3] * c2[2] + q[4] * c2[3] + q[5] * c2[4];
-    const Q4 qp1 = q[1] + q[2] * c2[0] + q[3] * c2[1] + q[4] * c2[2] + q[5] * c2[3];
-    const Q4 qp2 = q[2] + q[3] * c2[0] + q[4] * c2[1] + q[5] * c2[2];
-    const Q4 qp3 = q[3] + q[4] * c2[0] + q[5] * c2[1];
-    const Q4 qp4 = q[4] + q[5] * c2[0];
-    const Q4 qp5 = q[5];
-    qp0.normalize();
-    const V3 wp0 = w[0] + w[1] * c1[0] + w[2] * c1[1] + w[3] * c1[2] + w[4] * c1[3] + w[5] * c1[4];
-    const V3 wp1 = w[1] + w[2] * c1[0] + w[3] * c1[1] + w[4] * c1[2] + w[5] * c1[3];
-    const V3 wp2 = w[2] + w[3] * c1[0] + w[4] * c1[1] + w[5] * c1[2];
-    const V3 wp3 = w[3] + w[4] * c1[0] + w[5] * c1[1];
-    const V3 wp4 = w[4] + w[5] * c1[0];
-    const V3 wp5 = w[5];
-    E.set3(EF_XP, e, xp0); E.set3(EF_XP + 3, e, xp1); E.set3(EF_XP + 6, e, xp2);
-    E.set3(EF_XP + 9, e, xp3); E.set3(EF_XP + 12, e, xp4); E.set3(EF_XP + 15, e, xp5);
-    E.set4(EF_QP, e, qp0); E.set4(EF_QP + 4, e, qp1); E.set4(EF_QP + 8, e, qp2);
-    E.set4(EF_QP + 12, e, qp3); E.set4(EF_QP + 16, e, qp4); E.set4(EF_QP + 20, e, qp5);
-    E.set3(EF_WP, e, wp0); E.set3(EF_WP + 3, e, wp1); E.set3(EF_WP + 6, e, wp2);
-    E.set3(EF_WP + 9, e, wp3); E.set3(EF_WP + 12, e, wp4); E.set3(EF_WP + 15, e, wp5);
-    E.set3(EF_WPG, e, wp0);
-    E.set3(EF_WPL, e, project(wp0, qp0.adjoint()));
+    V3 xp[6], wp[6];
+    Q4 qp[6];
+    gear_predict(x, p.c2, xp);
+    gear_predict(q, p.c2, qp);
+    qp[0].normalize();
+    gear_predict(w, p.c1, wp);
+    for (int k = 0; k < 6; ++k) {
+        E.set3(EF_XP + 3 * k, e, xp[k]);
+        E.set4(EF_QP + 4 * k, e, qp[k]);
+        E.set3(EF_WP + 3 * k, e, wp[k]);
+    }
+    E.set3(EF_WPG, e, wp[0]);
+    E.set3(EF_WPL, e, project(wp[0], qp[0].adjoint()));
     // particles from the predicted element (updateParticlesPredicted, DEM.cpp:1184-1204)
-    if constexpr (CL) place_particles(b, b.ppred, e, E.at(EF_RAD, e), xp0, xp1, qp0, wp0);
+    if constexpr (CL) place_particles(b, b.ppred, e, E.at(EF_RAD, e), xp[0], xp[1], qp[0], wp[0]);
 }
 
 __global__ void __launch_bounds__(64) k_dem_predict(DEMParams p, DEMBuffers b) {
@@ -861,8 +860,7 @@ __global__ void __launch_bounds__(64) k_dem_predict(DEMParams p, DEMBuffers b) {
         else predict_elem<false>(E, p, b, e, x, q, w);
         v2 = x[1].norm2();
     }
-    v2 = wave_max(v2);
-    if (threadIdx.x == 0) b.vmax[blockIdx.x] = v2;
+    vmax_add(b, wave_max(v2));
 }
 
 // Per element: particle-contact increments in pair-list order, the wall contact
@@ -875,9 +873,8 @@ template <bool SP, bool CL>
 __global__ void __launch_bounds__(64) k_dem_gather(DEMParams p, DEMBuffers b) {
     if (b.st->error) return;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    double v2 = e < p.E ? gather_elem<SP, CL>(p, b, e) : 0.0;
-    v2 = wave_max(v2);
-    if (threadIdx.x == 0) b.vmax[blockIdx.x] = v2;
+    const double v2 = e < p.E ? gather_elem<SP, CL>(p, b, e) : 0.0;
+    vmax_add(b, wave_max(v2));
 }
 
 // Returns |x1|^2 of the corrected element for the next displacement check.
@@ -890,20 +887,17 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
     // the element's standard particles (consecutive; a single sphere is its own particle)
     const int ilo = CL ? b.efirst[e] : e, ihi = CL ? b.efirst[e + 1] : e + 1;
     const bool multi = CL && ihi - ilo > 1;
-    // the element state does not depend on the contact records: its loads are
-    // issued first, so that they overlap with the record loads below
+    // the state the contacts and Newton-Euler need; it does not depend on the
+    // contact records, so its loads are issued first (the rest of the Gear state is
+    // loaded group by group in the corrector, which keeps the registers in use low)
     const V3 xpe = E.v3(EF_XP, e);
     const V3 vpe = E.v3(EF_XP + 3, e);
     const double re = E.at(EF_RAD, e), me = E.at(EF_M, e);
     const V3 wpg = E.v3(EF_WPG, e);
     const V3 FH = E.v3(EF_FH, e), MH = E.v3(EF_MH, e);
     const V3 Ie = E.v3(EF_I, e), wpl = E.v3(EF_WPL, e);
-    const V3 xp[6] = {xpe, vpe, E.v3(EF_XP + 6, e), E.v3(EF_XP + 9, e), E.v3(EF_XP + 12, e), E.v3(EF_XP + 15, e)};
-    const V3 wp[6] = {E.v3(EF_WP, e), E.v3(EF_WP + 3, e), E.v3(EF_WP + 6, e), E.v3(EF_WP + 9, e), E.v3(EF_WP + 12, e),
-                      E.v3(EF_WP + 15, e)};
     Q4 q2 = E.q4(EF_Q + 8, e);  // single spheres: q2 is never set by the force routine
-    const Q4 qp[6] = {E.q4(EF_QP, e), E.q4(EF_QP + 4, e), E.q4(EF_QP + 8, e), E.q4(EF_QP + 12, e),
-                      E.q4(EF_QP + 16, e), E.q4(EF_QP + 20, e)};
+    const Q4 qp0 = E.q4(EF_QP, e), qp1 = E.q4(EF_QP + 4, e);
     // contact increments in pair-list order
     const auto add = [&](const PairRec& r) {
         const int flag = r.flag;
@@ -1151,55 +1145,88 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
     const V3 demF(p.demF[0], p.demF[1], p.demF[2]);
     const V3 x2 = (FVisc + FH + FP + FW) / me + demF;
     const V3 moment = MVisc + MH + MP + MW;
-    const Q4 qp0 = qp[0];
     const V3 momentBf = project(moment, qp0.adjoint());
     const V3 waBf = newtonAcc(momentBf, Ie, wpl);
     const V3 w1 = project(waBf, qp0);
     if (multi) {  // rotational acceleration as a quaternion (DEM.cpp:1176-1179)
-        const Q4 waQuat = quatAcc(waBf, qp[1]);
+        const Q4 waQuat = quatAcc(waBf, qp1);
         q2 = 0.5 * qp0.multiply(waQuat);
     }
     E.set3(EF_FP, e, FP); E.set3(EF_MP, e, MP); E.set3(EF_FW, e, FW); E.set3(EF_MW, e, MW);
-    // corrector
+    // corrector (elmt.cpp:179-254) and the next sub-step's predictor, one quantity at a time
     const double* co1 = p.co1;
     const double* co2 = p.co2;
-    const V3 x2Corr = x2 - xp[2];
-    V3 x[6];
-    x[0] = xp[0] + x2Corr * co2[0];
-    x[1] = xp[1] + x2Corr * co2[1];
-    x[2] = x2;
-    x[3] = xp[3] + x2Corr * co2[3];
-    x[4] = xp[4] + x2Corr * co2[4];
-    x[5] = xp[5] + x2Corr * co2[5];
-    const V3 w1Corr = w1 - wp[1];
-    V3 w[6];
-    w[0] = wp[0] + w1Corr * co1[0];
-    w[1] = w1;
-    w[2] = wp[2] + w1Corr * co1[2];
-    w[3] = wp[3] + w1Corr * co1[3];
-    w[4] = wp[4] + w1Corr * co1[4];
-    w[5] = wp[5] + w1Corr * co1[5];
-    const Q4 q2Corr = q2 - qp[2];
-    Q4 q[6];
-    q[0] = qp[0] + q2Corr * co2[0];
-    q[1] = qp[1] + q2Corr * co2[1];
-    q[2] = q2;
-    q[3] = qp[3] + q2Corr * co2[3];
-    q[4] = qp[4] + q2Corr * co2[4];
-    q[5] = qp[5] + q2Corr * co2[5];
-    q[0].normalize();
-    for (int k = 0; k < 6; ++k) {
-        E.set3(EF_X + 3 * k, e, x[k]);
-        E.set3(EF_W + 3 * k, e, w[k]);
-        if (k != 2 || multi) E.set4(EF_Q + 4 * k, e, q[k]);
+    V3 xc[2], xn[2];  // corrected and predicted position and velocity
+    {
+        const V3 xp[6] = {xpe, vpe, E.v3(EF_XP + 6, e), E.v3(EF_XP + 9, e), E.v3(EF_XP + 12, e), E.v3(EF_XP + 15, e)};
+        const V3 x2Corr = x2 - xp[2];
+        V3 x[6], o[6];
+        x[0] = xp[0] + x2Corr * co2[0];
+        x[1] = xp[1] + x2Corr * co2[1];
+        x[2] = x2;
+        x[3] = xp[3] + x2Corr * co2[3];
+        x[4] = xp[4] + x2Corr * co2[4];
+        x[5] = xp[5] + x2Corr * co2[5];
+        gear_predict(x, p.c2, o);
+        for (int k = 0; k < 6; ++k) {
+            E.set3(EF_X + 3 * k, e, x[k]);
+            E.set3(EF_XP + 3 * k, e, o[k]);
+        }
+        xc[0] = x[0]; xc[1] = x[1];
+        xn[0] = o[0]; xn[1] = o[1];
     }
-    E.set3(EF_WG, e, w[0]);
-    E.set3(EF_WL, e, project(w[0], q[0].adjoint()));
-    // particles from the corrected element (updateParticlesCorrected, DEM.cpp:1206-1227)
-    if constexpr (CL) place_particles(b, b.pcor, e, re, x[0], x[1], q[0], w[0]);
-    // predictor of the next sub-step
-    predict_elem<CL>(E, p, b, e, x, q, w);
-    return x[1].norm2();
+    Q4 qc0, qn0;  // corrected and predicted orientation
+    {
+        const Q4 qp[6] = {qp0, qp1, E.q4(EF_QP + 8, e), E.q4(EF_QP + 12, e), E.q4(EF_QP + 16, e), E.q4(EF_QP + 20, e)};
+        const Q4 q2Corr = q2 - qp[2];
+        Q4 q[6], o[6];
+        q[0] = qp[0] + q2Corr * co2[0];
+        q[1] = qp[1] + q2Corr * co2[1];
+        q[2] = q2;
+        q[3] = qp[3] + q2Corr * co2[3];
+        q[4] = qp[4] + q2Corr * co2[4];
+        q[5] = qp[5] + q2Corr * co2[5];
+        q[0].normalize();
+        gear_predict(q, p.c2, o);
+        o[0].normalize();
+        for (int k = 0; k < 6; ++k) {
+            if (k != 2 || multi) E.set4(EF_Q + 4 * k, e, q[k]);
+            E.set4(EF_QP + 4 * k, e, o[k]);
+        }
+        qc0 = q[0];
+        qn0 = o[0];
+    }
+    V3 wc0, wn0;  // corrected and predicted angular velocity
+    {
+        const V3 wp[6] = {E.v3(EF_WP, e), E.v3(EF_WP + 3, e), E.v3(EF_WP + 6, e), E.v3(EF_WP + 9, e),
+                          E.v3(EF_WP + 12, e), E.v3(EF_WP + 15, e)};
+        const V3 w1Corr = w1 - wp[1];
+        V3 w[6], o[6];
+        w[0] = wp[0] + w1Corr * co1[0];
+        w[1] = w1;
+        w[2] = wp[2] + w1Corr * co1[2];
+        w[3] = wp[3] + w1Corr * co1[3];
+        w[4] = wp[4] + w1Corr * co1[4];
+        w[5] = wp[5] + w1Corr * co1[5];
+        gear_predict(w, p.c1, o);
+        for (int k = 0; k < 6; ++k) {
+            E.set3(EF_W + 3 * k, e, w[k]);
+            E.set3(EF_WP + 3 * k, e, o[k]);
+        }
+        E.set3(EF_WG, e, w[0]);
+        E.set3(EF_WL, e, project(w[0], qc0.adjoint()));
+        E.set3(EF_WPG, e, o[0]);
+        E.set3(EF_WPL, e, project(o[0], qn0.adjoint()));
+        wc0 = w[0];
+        wn0 = o[0];
+    }
+    // particles from the corrected and from the predicted element (updateParticlesCorrected and
+    // updateParticlesPredicted, DEM.cpp:1184-1227)
+    if constexpr (CL) {
+        place_particles(b, b.pcor, e, re, xc[0], xc[1], qc0, wc0);
+        place_particles(b, b.ppred, e, re, xn[0], xn[1], qn0, wn0);
+    }
+    return xc[1].norm2();
 }
 
 // Coupling records of every particle slot after the DEM step: corrected

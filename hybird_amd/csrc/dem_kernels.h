@@ -103,7 +103,8 @@ struct DEMBuffers {
     uint32_t* posI;        // [pairCap] csr slot of a pair in its first particle's list (0xFFFFFFFF: ghost)
     uint32_t* posJ;        // [pairCap] csr slot in its second particle's list
     struct PairRec* prec;  // [2 * pairCap] per csr slot: signed force/torque increments of one pair
-    double* vmax;          // [(P + 63) / 64] per-wavefront max |x1|^2 written by the corrector
+    double* vmax;          // [1] max |x1|^2 over the elements after the corrector, kept as the bit pattern of a
+                           // non-negative double (integer atomic max); the displacement check reads and clears it
     double* energy;        // [6] particleEnergy: kin, rotKin, trKin, grav, elastic, total
     const ObjDev* objs;    // [nobj]
     uint32_t* nearObj;     // [P][NEAR_OBJ] objects within nebrRange of each particle, ascending

@@ -479,15 +479,18 @@ __device__ __forceinline__ float cdotf(int j, float ux, float uy, float uz) {
 }
 
 // Collision of one node in the FP32 deviation form: moments, penalty force,
-// Guo shift, [Bingham/Smagorinsky], BGK/TRT with Guo forcing. Writes the side
+// Guo shift, [Bingham/Smagorinsky], BGK/TRT with Guo forcing. sid: the node's
+// particle slot + 1 (0: none) with that slot's record r. Writes the side
 // outputs (node force, macroscopic fields, viscosity, free-surface state) and,
-// when out is given, the post-collision populations into out[]; returns whether
-// out was written.
+// when doOut, the post-collision populations into out[] (which may be g);
+// returns whether out was written. bad is set for a non-finite density (the
+// caller reports it), so that no global store sits between the loads of the
+// nodes of one lane.
 template <bool VARVISC, int COLL, bool FS, bool CURVED = false>
-__device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uint32_t lw, size_t N,
-                                          const uint32_t* __restrict__ solid, const CoupleRecF* __restrict__ cp,
-                                          const RunSlot<float>& rs, double* __restrict__ visc, double* __restrict__ mac,
-                                          const LBKParams& p, const FSView& fsv, bool doOut, float out[19]) {
+__device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uint32_t lw, size_t N, uint32_t sid,
+                                          const CoupleRecF& r, const RunSlot<float>& rs, double* __restrict__ visc,
+                                          double* __restrict__ mac, const LBKParams& p, const FSView& fsv, bool doOut,
+                                          float out[19], bool& bad) {
 #pragma clang fp contract(fast)
     float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
 #pragma unroll
@@ -501,7 +504,7 @@ __device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uin
         if (cz(j) < 0) jz -= g[j];
     }
     const float rho = 1.0f + drho;
-    if (p.herr && !isfinite(rho)) *p.herr = LB_ERR_NUMERIC;
+    bad = !isfinite(rho);
     const float irho = 1.0f / rho;
     float ux = jx * irho, uy = jy * irho, uz = jz * irho;
     float Tx = (float)p.F[0], Ty = (float)p.F[1], Tz = (float)p.F[2];
@@ -511,10 +514,7 @@ __device__ __forceinline__ bool collide32(const float g[19], const Coord& c, uin
         Tz += p.ffield[2 * N + c.idx];
     }
     float hx = 0.0f, hy = 0.0f, hz = 0.0f;
-    const uint32_t sw = solid ? solid[c.idx] : 0u;
-    const uint32_t sid = owner_slot(sw, p.ctag);
     if (sid) {
-        const CoupleRecF& r = cp[sid - 1];
         const double massD = node_mass<FS, float>(lw, c.idx, fsv, p);
         const float mass = (float)massD;
         // rigid-body velocity at the node: v + w x (node - c)
@@ -642,9 +642,15 @@ __device__ __forceinline__ void fused32_node(const float* __restrict__ fin, floa
 #pragma unroll
         for (int j = 0; j < 19; ++j) g[j] = ld_pop(&fin[(size_t)j * p.PS + c.idx]);
     }
+    const uint32_t sid = owner_slot(solid ? solid[c.idx] : 0u, p.ctag);
+    CoupleRecF r{};
+    if (sid) r = cp[sid - 1];
     float out[19];
-    if (!collide32<VARVISC, COLL, FS, CURVED>(g, c, lw, N, solid, cp, rs, visc, mac, p, fsv, fout != nullptr, out))
-        return;
+    bool bad;
+    const bool wrote = collide32<VARVISC, COLL, FS, CURVED>(g, c, lw, N, sid, r, rs, visc, mac, p, fsv,
+                                                           fout != nullptr, out, bad);
+    if (bad && p.herr) *p.herr = LB_ERR_NUMERIC;
+    if (!wrote) return;
 #pragma unroll
     for (int j = 0; j < 19; ++j) st_pop(&fout[(size_t)j * p.PS + c.idx], out[j]);
 }
@@ -1128,7 +1134,6 @@ void launch_lb_fused(Prec prec, bool pull, bool varVisc, int collision, const vo
     const dim3 g = lattice_grid(p), b(BX, BY, 1);
     const FSView none{};
     const HydroSink sink = (hs && solid) ? *hs : HydroSink{nullptr, nullptr, 0u};
-
     if (p.curvw) {
         HYB_CURVED_CASE(true, false, HYB_COLLISION_BGK)
         HYB_CURVED_CASE(true, false, HYB_COLLISION_TRT)
