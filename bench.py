@@ -192,7 +192,8 @@ def horizon(sim32, sim64, done: int, steps: int, every: int):
         f32, f64 = sim32.get("e_forces")[:, 0], sim64.get("e_forces")[:, 0]
         x32, x64 = sim32.get("e_x")[:, 0], sim64.get("e_x")[:, 0]
         rows.append({"step": done, "velocity_rel_l2": _rel(u32, u64), "drag_rel_l2": _rel(f32, f64),
-                     "position_max_abs": float(np.max(np.abs(x32 - x64)))})
+                     "position_max_abs": float(np.max(np.abs(x32 - x64))),
+                     "neighbour_rebuilds_fp32": int(float(sim32.get("rebuilds")[0]))})
         print(f"[bench] horizon step {done}: velocity {rows[-1]['velocity_rel_l2']:.3e}", file=sys.stderr,
               flush=True)
     return rows
@@ -247,7 +248,8 @@ def dam_break(steps: int = 10000, every: int = 2000, phi: float = 0.25, seed: in
         rows.append({"step": done, "mass_drift_fp64": m["FP64"] / m0["FP64"] - 1.0,
                      "mass_drift_fp32": m["FP32"] / m0["FP32"] - 1.0,
                      "mass_rel_fp32_vs_fp64": m["FP32"] / m["FP64"] - 1.0,
-                     "front_x_fp64": front(sims["FP64"]), "front_x_fp32": front(sims["FP32"])})
+                     "front_x_fp64": front(sims["FP64"]), "front_x_fp32": front(sims["FP32"]),
+                     "neighbour_rebuilds_fp32": int(float(sims["FP32"].get("rebuilds")[0]))})
         print(f"[bench] dam break step {done}: front {rows[-1]['front_x_fp64']} / {rows[-1]['front_x_fp32']}",
               file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
@@ -315,6 +317,7 @@ def collapse(steps: int, warmup: int, err_steps: int = 2000, phi: float = 0.2, s
         out["node_types_differing"] = int(np.sum(ta != tb))
         out["mass_rel_fp32_vs_fp64"] = mb / ma - 1.0
         out["position_max_abs_fp32_vs_fp64"] = float(np.max(np.abs(b.get("e_x")[:, 0] - a.get("e_x")[:, 0])))
+        out["neighbour_rebuilds_fp32"] = int(float(b.get("rebuilds")[0]))
     for s in sims.values():
         s.close()
     return out

@@ -994,7 +994,10 @@ static void lb_couple_launch(hyb_lb* l, hyb_dem* d) {
     }
     const LBKParams p = l->params();
     launch_dem_couple_table(d->p, d->b, l->cp, l->cpf, l->nslots, l->unit, l->stream);
-    launch_classify_set(l->nslots, l->slotCap, l->cp, l->boxes, l->nslots + 1, l->solid, l->link, p, l->stream);
+    // without free surface no inactive node becomes active and the ownership of an inactive node is never
+    // read (views mask it), so every node inside a sphere is marked without reading its link word
+    launch_classify_set(l->nslots, l->slotCap, l->cp, l->boxes, l->nslots + 1, l->solid,
+                        l->freeSurface ? l->link : nullptr, p, l->stream);
 }
 
 int hyb_lb_init(hyb_lb* l, hyb_dem* d) {  // LB.cpp:190-219
@@ -1582,10 +1585,12 @@ int hyb_lb_get(hyb_lb* l, const char* cname, void* host, size_t cap, size_t* nee
         HIPCHK(hipMemcpy(t.data(), l->type, N, hipMemcpyDeviceToHost));
         return put(t.data(), N, host, cap, need);
     }
-    if (name == "pflag" || name == "solidIndex") {
-        std::vector<uint32_t> so(N);
+    if (name == "pflag" || name == "solidIndex") {  // particle nodes are active nodes (LB.cpp:475-495)
+        std::vector<uint32_t> so(N), lw(N);
         HIPCHK(hipMemcpy(so.data(), l->solid, N * 4, hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < N; ++i) so[i] = (so[i] >> SOLID_SLOT_BITS) == l->ctag ? (so[i] & SOLID_SLOT_MASK) : 0u;
+        HIPCHK(hipMemcpy(lw.data(), l->link, N * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; ++i)
+            so[i] = ((lw[i] & LINK_ACTIVE) && (so[i] >> SOLID_SLOT_BITS) == l->ctag) ? (so[i] & SOLID_SLOT_MASK) : 0u;
         if (name == "pflag") {
             std::vector<uint8_t> pf(N);
             for (size_t i = 0; i < N; ++i) pf[i] = so[i] ? 1 : 0;

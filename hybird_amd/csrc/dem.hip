@@ -784,8 +784,15 @@ __global__ void __launch_bounds__(256) k_dem_pairs(DEMParams p, DEMBuffers b) {
             b.hist[3 * (size_t)k] = xi[0]; b.hist[3 * (size_t)k + 1] = xi[1]; b.hist[3 * (size_t)k + 2] = xi[2];
         }
         if (b.pstat) b.pstat[k] = st;
-        if (si != 0xFFFFFFFFu) b.prec[si] = ri;
-        if (sj != 0xFFFFFFFFu) b.prec[sj] = rj;
+        // a record without contact is read only for its flag: the other fields are not rewritten
+        if (si != 0xFFFFFFFFu) {
+            if (ri.flag) b.prec[si] = ri;
+            else b.prec[si].flag = 0;
+        }
+        if (sj != 0xFFFFFFFFu) {
+            if (rj.flag) b.prec[sj] = rj;
+            else b.prec[sj].flag = 0;
+        }
         if constexpr (CL) {
             if (si != 0xFFFFFFFFu && (ri.flag & 4)) {
                 b.pmn[3 * (size_t)si] = mnI.x; b.pmn[3 * (size_t)si + 1] = mnI.y; b.pmn[3 * (size_t)si + 2] = mnI.z;

@@ -885,7 +885,8 @@ __global__ void k_count_mass(const uint32_t* __restrict__ link, const uint32_t* 
 
 // ------------------------------------------------------------------ particle classification
 // Each particle slot (real particles, then ghosts) marks the active nodes
-// strictly inside it (vector.cpp:153-158) with the current coupling-step tag.
+// strictly inside it (vector.cpp:153-158) with the current coupling-step tag
+// (link == nullptr: every node inside, active or not).
 // Where spheres overlap the highest slot index wins, as in the full re-test of
 // LB.cpp:475-495.
 __device__ __forceinline__ NodeBox slot_box(const CoupleRec& r, const LBKParams& p) {
@@ -912,6 +913,8 @@ constexpr int SLOT_WAVES = 4;
 #define HYB_SLOT_BATCH 4
 #endif
 constexpr int CH = HYB_SLOT_BATCH;
+// wavefronts per particle slot in the classification (a sphere of radius 6 spans 196 rows: four batches)
+constexpr int CLASSIFY_PARTS = 4;
 
 struct SlotRows {  // LDS of one wavefront: inclusive node counts and first x of 64 rows
     int scan[64];
@@ -924,18 +927,20 @@ struct SlotRows {  // LDS of one wavefront: inclusive node counts and first x of
 // them. Rows are taken 64 at a time; each row contributes the x interval that
 // can hold inside nodes (from the circle radius with one node of margin on each
 // side, every candidate is then tested exactly), and a wavefront prefix sum of
-// the interval lengths spreads the candidates evenly over the lanes. A row whose
-// (y, z) distance already reaches the radius holds no inside node: rounded sums
-// of non-negative squares are monotone.
+// the interval lengths spreads the candidates evenly over the lanes, so that the
+// lanes of one access hold consecutive nodes of few rows (few cache lines per
+// instruction). A row whose (y, z) distance already reaches the radius holds no
+// inside node: rounded sums of non-negative squares are monotone.
+// part / parts: this wavefront takes the 64-row batches part, part + parts, ...
 template <class F>
 __device__ __forceinline__ void for_slot_batches(const NodeBox& b, int lane, SlotRows& sh, const V3& cen, double rad,
-                                                 const LBKParams& p, F&& f) {
+                                                 const LBKParams& p, int part, int parts, F&& f) {
 #pragma clang fp contract(off)
     const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
     if (nx <= 0 || ny <= 0 || nz <= 0) return;
     const int rows = ny * nz;
     const double r2 = rad * rad;
-    for (int base = 0; base < rows; base += 64) {
+    for (int base = 64 * part; base < rows; base += 64 * parts) {
         const int r = base + lane;
         int xa = 0, cnt = 0;
         if (r < rows) {
@@ -1006,17 +1011,21 @@ __global__ void __launch_bounds__(256) k_classify_set(const int* __restrict__ ns
     const int unit = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int nunits = (int)((gridDim.x * blockDim.x) >> 6);
     SlotRows& sh = shr[threadIdx.x >> 6];
-    for (int s = unit; s < ns; s += nunits) {
+    // CLASSIFY_PARTS wavefronts per slot, each on its own 64-row batches of the slot's box, so that the
+    // longest chain of one wavefront is one batch (the marks are an order-free maximum)
+    for (int us = unit; us < ns * CLASSIFY_PARTS; us += nunits) {
+        const int s = us / CLASSIFY_PARTS, part = us % CLASSIFY_PARTS;
         const CoupleRec r = cp[s];
         const NodeBox b = slot_box(r, p);
-        if (lane == 0) boxes[s] = b;
+        if (lane == 0 && part == 0) boxes[s] = b;
         const V3 cen(r.x0[0] / p.unitLength, r.x0[1] / p.unitLength, r.x0[2] / p.unitLength);
         const double rad = r.r / p.unitLength;
         const uint32_t val = (p.ctag << SOLID_SLOT_BITS) | (uint32_t)(s + 1);
-        for_slot_batches(b, lane, sh, cen, rad, p, [&](const size_t* idx, const int*, const int*, const int*, const bool* ok) {
+        for_slot_batches(b, lane, sh, cen, rad, p, part, CLASSIFY_PARTS,
+                         [&](const size_t* idx, const int*, const int*, const int*, const bool* ok) {
             uint32_t lk[CH];
 #pragma unroll
-            for (int k = 0; k < CH; ++k) lk[k] = ok[k] ? link[idx[k]] : 0u;
+            for (int k = 0; k < CH; ++k) lk[k] = ok[k] ? (link ? link[idx[k]] : LINK_ACTIVE) : 0u;
 #pragma unroll
             for (int k = 0; k < CH; ++k)
                 if (ok[k] && (lk[k] & LINK_ACTIVE)) atomicMax(&solid[idx[k]], val);
@@ -1266,7 +1275,7 @@ void launch_lb_init_pops(Prec prec, void* f, const uint32_t* link, const int* mx
 void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, NodeBox* boxes, int* nboxes,
                          uint32_t* solid, const uint32_t* link, const LBKParams& p, hipStream_t s) {
     if (max_slots <= 0) return;
-    int grid = (max_slots + SLOT_WAVES - 1) / SLOT_WAVES;  // one wavefront per slot
+    int grid = (max_slots * CLASSIFY_PARTS + SLOT_WAVES - 1) / SLOT_WAVES;  // CLASSIFY_PARTS wavefronts per slot
     if (grid > 16384) grid = 16384;
     k_classify_set<<<grid, 64 * SLOT_WAVES, 0, s>>>(nslots, cp, boxes, nboxes, solid, link, p);
 }

@@ -153,6 +153,8 @@ void launch_lb_max_active(const uint32_t* link, int* maxXYZ, const LBKParams& p,
 // With free surface, also the initial density (rho), interface mass and interface velocity.
 void launch_lb_init_pops(Prec prec, void* f, const uint32_t* link, const int* maxXYZ, double* visc, const double* u0,
                          const double* Finit, void* rho, double* mass, double* uS, const LBKParams& p, hipStream_t s);
+// link == nullptr: mark every node inside a sphere, active or not (for lattices whose
+// inactive nodes stay inactive and are never read as particle nodes)
 void launch_classify_set(const int* nslots, int max_slots, const CoupleRec* cp, NodeBox* boxes, int* nboxes,
                          uint32_t* solid, const uint32_t* link, const LBKParams& p, hipStream_t s);
 // Non-temporal device copy of `bytes` with 4-, 8- or 16-byte accesses per lane
