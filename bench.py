@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_LUP = {"fp32": 2 * 19 * 4, "fp64": 2 * 19 * 8}
 # HBM bytes per launch of k_fused32 on the 256^3 workload: FETCH_SIZE + WRITE_SIZE from rocprofv3 --pmc,
 # scaled by the same counters on a plain copy (profiles/hbm_traffic_fused_dense_suspension_256.md)
-TRAFFIC_FUSED32_256 = 1.527e9 + 1.212e9
+TRAFFIC_FUSED32_256 = 1.529e9 + 1.212e9
 
 
 def workload(size: int, seed: int = 1, phi: float = 0.2):

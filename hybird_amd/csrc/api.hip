@@ -2282,13 +2282,21 @@ int hyb_state_load(hyb_lb* l, hyb_dem* d, const char* path) {
         adj_free(l);
     }
     if (d) {
+        // the lattice (if any) already holds the file's state: a DEM section that fails leaves the pair
+        // inconsistent, so both handles are then unusable until destroyed
         DemScalars s;
-        if (!rd(fh.f, &s, sizeof(s))) return fail(HYB_E_ARG, "state file: truncated");
+        if (!rd(fh.f, &s, sizeof(s))) {
+            if (l) l->inited = false;
+            return fail(HYB_E_ARG, "state file: truncated");
+        }
         if (s.P != d->P || s.E != (uint32_t)d->E || s.pairCap != d->p.pairCap || s.Gcap != d->p.Gcap ||
-            s.multiStep != d->multiStep)
+            s.multiStep != d->multiStep) {
+            if (l) l->inited = false;
             return fail(HYB_E_ARG, "state file: particle set or DEM configuration differs");
+        }
         if ((rc = read_blocks(fh.f, d->saved, [](const std::string&) { return (int)HYB_OK; }))) {
-            d->inited = false;  // partly overwritten: unusable until destroyed
+            if (l) l->inited = false;
+            d->inited = false;
             return rc;
         }
         errword_clear(d->err);

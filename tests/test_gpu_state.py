@@ -79,3 +79,24 @@ def test_restart_rejects_other_configurations(tmp_path):
         a.load_state(bad)
     for s in (a, b, c):
         s.close()
+
+
+def test_restart_with_other_particles_leaves_both_handles_unusable(tmp_path):
+    # same lattice, one more particle: the lattice section loads, the DEM section is rejected, and the pair
+    # (lattice with the file's state, DEM without) refuses to step instead of running inconsistently
+    rt = _runtime()
+    cfg, rec, _ = load_case("couple2")
+    a = rt.Simulation(fixture_config(cfg), rec)
+    a.cycle(2)
+    path = tmp_path / "couple2.state"
+    a.save_state(path)
+    extra = rec[:1].copy()
+    extra[0, 0] = rec.shape[0]
+    extra[0, 3:6] = [12.0, 5.0, 18.0]  # clear of the other two spheres and of the walls
+    b = rt.Simulation(fixture_config(cfg), np.concatenate([rec, extra]))
+    with pytest.raises(rt.HybError, match="particle set"):
+        b.load_state(path)
+    with pytest.raises(rt.HybError):
+        b.cycle(1)
+    a.close()
+    b.close()
