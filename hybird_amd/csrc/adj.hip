@@ -32,6 +32,12 @@ constexpr int ABX = 256;
 #ifndef ADJ_MIN_BLOCKS
 #define ADJ_MIN_BLOCKS 2
 #endif
+// issue the adjoint populations' loads together with the state's (more bytes in
+// flight per wavefront) instead of after the forward intermediates: measured on
+// the 128^3 frozen bed, backward/forward 2.80 against 2.93
+#ifndef ADJ_HOIST
+#define ADJ_HOIST 1
+#endif
 // non-temporal loads of the state and adjoint populations and stores of the
 // new adjoint populations (each is touched once per step)
 #ifndef ADJ_NT
@@ -95,6 +101,20 @@ __global__ void __launch_bounds__(ABX, ADJ_MIN_BLOCKS)
 #pragma unroll
             for (int i = 0; i < 19; ++i) g[i] = sPrev[(size_t)i * p.PS + self];
         }
+        // mu_t = P^T lambda_{t+1}: plane j from the neighbour in direction j
+        const auto load_mu = [&]() {
+#pragma unroll
+            for (int j = 0; j < 19; ++j)
+                m[j] = ld_nt(&lamNext[(size_t)j * p.PS + (j == 0 ? self : nb32(nx, ny, nz, sX, sXY, j))]);
+            if (lw & LINK_WALLS) {
+#pragma unroll
+                for (int j = 1; j < 19; ++j)
+                    if (lw & (1u << j)) m[j] = lamNext[(size_t)opp(j) * p.PS + self];
+            }
+        };
+        if (ADJ_HOIST) load_mu();
+        // the force gradient accumulated so far, read here so that its load overlaps the populations' loads
+        const double gF0 = gradF[self], gF1 = gradF[N + self], gF2 = gradF[2 * N + self];
         // ---- forward intermediates of C (the fused kernel's formulas)
         double n = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
 #pragma unroll
@@ -147,15 +167,7 @@ __global__ void __launch_bounds__(ABX, ADJ_MIN_BLOCKS)
         // and c.T per pair instead of keeping the values of the loop above alive.
         double usx = us.x, usy = us.y, usz = us.z, Tx = T.x, Ty = T.y, Tz = T.z;
         asm volatile("" : "+v"(usx), "+v"(usy), "+v"(usz), "+v"(Tx), "+v"(Ty), "+v"(Tz));
-        // mu_t = P^T lambda_{t+1}: plane j from the neighbour in direction j
-#pragma unroll
-        for (int j = 0; j < 19; ++j)
-            m[j] = ld_nt(&lamNext[(size_t)j * p.PS + (j == 0 ? self : nb32(nx, ny, nz, sX, sXY, j))]);
-        if (lw & LINK_WALLS) {
-#pragma unroll
-            for (int j = 1; j < 19; ++j)
-                if (lw & (1u << j)) m[j] = lamNext[(size_t)opp(j) * p.PS + self];
-        }
+        if (!ADJ_HOIST) load_mu();
         // ---- reverse
         double nbar = 0.0, qbar = 0.0, rbar = 0.0, omPbar = 0.0, omMbar = 0.0;
         V3 usbar, Tbar;
@@ -209,9 +221,9 @@ __global__ void __launch_bounds__(ABX, ADJ_MIN_BLOCKS)
         Tbar += usbar / (2.0 * n);
         nbar += -usbar.dot(T) / (2.0 * n * n);
         // T = F + hF
-        gradF[self] += Tbar.x;
-        gradF[N + self] += Tbar.y;
-        gradF[2 * N + self] += Tbar.z;
+        gradF[self] = gF0 + Tbar.x;
+        gradF[N + self] = gF1 + Tbar.y;
+        gradF[2 * N + self] = gF2 + Tbar.z;
         if (sid) {  // hF = -(m / n)(u - u_p)
             ubar += (-pm / n) * Tbar;
             nbar += (pm / (n * n)) * Tbar.dot(dv);
