@@ -203,6 +203,21 @@ __device__ int walk_pairs(const DEMParams& p, const DEMBuffers& b, const El& E, 
             }
         }
     };
+    // the same visit on a record already loaded (s1: position and cluster of n1)
+    const auto visit_rec = [&](int n0, const V3& x0, int c0, int n1, const double s1[4]) {
+        if (cell0 != cell1 || n1 < n0) {
+            if (c0 != (int)s1[3]) {
+                const V3 r10 = V3(s1[0], s1[1], s1[2]) - x0;
+                if (r10.norm2() < r2) {
+                    if (out) {
+                        out[2 * cnt] = (uint32_t)n0;
+                        out[2 * cnt + 1] = (uint32_t)n1;
+                    }
+                    ++cnt;
+                }
+            }
+        }
+    };
     for (int i0 = a0; i0 < e0; ++i0) {
         const int n0 = b.cursor[i0];
         const double* s0 = b.spos + 4 * (size_t)n0;
@@ -211,7 +226,23 @@ __device__ int walk_pairs(const DEMParams& p, const DEMBuffers& b, const El& E, 
         if (linked) {
             for (int n1 = b.cellTable[cell1]; n1 > -1; n1 = b.cellTable[n1]) visit(n0, x0, c0, n1);
         } else {
-            for (int i1 = a1; i1 < e1; ++i1) visit(n0, x0, c0, b.cursor[i1]);
+            // cell1's slots WB at a time: their slot numbers, then their records, are loaded together
+            constexpr int WB = 8;
+            for (int i1 = a1; i1 < e1; i1 += WB) {
+                int n1[WB];
+                double s1[WB][4];
+#pragma unroll
+                for (int k = 0; k < WB; ++k) n1[k] = i1 + k < e1 ? b.cursor[i1 + k] : -1;
+#pragma unroll
+                for (int k = 0; k < WB; ++k)
+                    if (n1[k] >= 0) {
+                        const double* q = b.spos + 4 * (size_t)n1[k];
+                        s1[k][0] = q[0]; s1[k][1] = q[1]; s1[k][2] = q[2]; s1[k][3] = q[3];
+                    }
+#pragma unroll
+                for (int k = 0; k < WB; ++k)
+                    if (n1[k] >= 0) visit_rec(n0, x0, c0, n1[k], s1[k]);
+            }
         }
     }
     return cnt;
