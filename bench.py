@@ -185,7 +185,11 @@ def horizon(sim32, sim64, done: int, steps: int, every: int):
     rows = []
     while done < steps:
         n = min(every - done % every, steps - done)
+        sim32.sync()
+        t0 = time.perf_counter()
         sim32.cycle(n)
+        sim32.sync()
+        ms32 = (time.perf_counter() - t0) / n * 1e3  # neighbour rebuilds of these cycles included
         sim64.cycle(n)
         done += n
         u32, u64 = sim32.get("u")[act], sim64.get("u")[act]
@@ -193,7 +197,8 @@ def horizon(sim32, sim64, done: int, steps: int, every: int):
         x32, x64 = sim32.get("e_x")[:, 0], sim64.get("e_x")[:, 0]
         rows.append({"step": done, "velocity_rel_l2": _rel(u32, u64), "drag_rel_l2": _rel(f32, f64),
                      "position_max_abs": float(np.max(np.abs(x32 - x64))),
-                     "neighbour_rebuilds_fp32": int(float(sim32.get("rebuilds")[0]))})
+                     "neighbour_rebuilds_fp32": int(float(sim32.get("rebuilds")[0])),
+                     "fp32_ms_per_step": round(ms32, 4)})
         print(f"[bench] horizon step {done}: velocity {rows[-1]['velocity_rel_l2']:.3e}", file=sys.stderr,
               flush=True)
     return rows
@@ -241,15 +246,21 @@ def dam_break(steps: int = 10000, every: int = 2000, phi: float = 0.25, seed: in
     t0 = time.perf_counter()
     while done < steps:
         n = min(every, steps - done)
-        for s in sims.values():
+        ms = {}
+        for k, s in sims.items():
+            s.sync()
+            tc = time.perf_counter()
             s.cycle(n)
+            s.sync()
+            ms[k] = (time.perf_counter() - tc) / n * 1e3
         done += n
         m = {k: float(s.get("fluidMass")[0]) for k, s in sims.items()}
         rows.append({"step": done, "mass_drift_fp64": m["FP64"] / m0["FP64"] - 1.0,
                      "mass_drift_fp32": m["FP32"] / m0["FP32"] - 1.0,
                      "mass_rel_fp32_vs_fp64": m["FP32"] / m["FP64"] - 1.0,
                      "front_x_fp64": front(sims["FP64"]), "front_x_fp32": front(sims["FP32"]),
-                     "neighbour_rebuilds_fp32": int(float(sims["FP32"].get("rebuilds")[0]))})
+                     "neighbour_rebuilds_fp32": int(float(sims["FP32"].get("rebuilds")[0])),
+                     "fp32_ms_per_step": round(ms["FP32"], 4), "fp64_ms_per_step": round(ms["FP64"], 4)})
         print(f"[bench] dam break step {done}: front {rows[-1]['front_x_fp64']} / {rows[-1]['front_x_fp32']}",
               file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
@@ -403,6 +414,8 @@ def main() -> int:
     mlups64 = n_act * K / sec64 / 1e6
     psteps = bed.count * 10 * K / sec32
     achieved = BYTES_PER_LUP["fp32"] * n_act / (kms32 * 1e-3) / 1e9
+    # the algorithmic LB bytes of one step over the whole coupled step (DEM, coupling and sums included)
+    whole_step_gbs = BYTES_PER_LUP["fp32"] * n_act / (sec32 / K) / 1e9
     line = {
         "metric": METRIC,
         "value": round(mlups32, 1),
@@ -425,6 +438,7 @@ def main() -> int:
         "fp64_mlups": round(mlups64, 1),
         "fp64_ms_per_step": round(sec64 / K * 1e3, 4),
         "fp32_over_fp64": round(mlups32 / mlups64, 3),
+        "whole_step_hbm_frac": round(whole_step_gbs / HBM_PEAK_GBS, 4),
         "velocity_rel_l2_fp32_vs_fp64": vel_err,
         "drag_rel_l2_fp32_vs_fp64": drag_err,
         "lb_kernel_ms": {"fp32": round(kms32, 4), "fp64": round(kms64, 4)},
