@@ -12,7 +12,9 @@ per-particle hydrodynamic reduction.
 
 The headline value is the MLUPS of the FP32 deviation-stored path; the plain
 FP64 path runs the same workload beside it, and the velocity-field error of
-FP32 against FP64 is reported with them. Prints one JSON line.
+FP32 against FP64 is reported with them; the FP32 path is timed once more with
+the tangential contact spring (history per contact) in place of the viscous law.
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -383,6 +385,12 @@ def main() -> int:
     sim64 = make_sim(cfg, bed, "FP64")
     sec64, kms64, _ = timed(sim64, args.steps, args.warmup)
     print(f"[bench] fp64 done: {sec64:.3f}s", file=sys.stderr, flush=True)
+    # the same workload with the tangential contact spring (history per pair and wall contact) in place of
+    # the reference's viscous, history-free tangential law
+    sim_sp = make_sim(dict(cfg, tangentialModel="SPRING"), bed, "FP32")
+    sec_sp, _, reb_sp = timed(sim_sp, args.steps, args.warmup)
+    sim_sp.close()
+    print(f"[bench] fp32 spring law done: {sec_sp:.3f}s", file=sys.stderr, flush=True)
     # accuracy of the FP32 form against FP64 after the same number of cycles
     t_o = sim64.get("types")
     act = (t_o == 0) | (t_o == 3)
@@ -438,6 +446,9 @@ def main() -> int:
         "fp64_mlups": round(mlups64, 1),
         "fp64_ms_per_step": round(sec64 / K * 1e3, 4),
         "fp32_over_fp64": round(mlups32 / mlups64, 3),
+        "spring_law": {"mlups": round(n_act * K / sec_sp / 1e6, 1), "ms_per_step": round(sec_sp / K * 1e3, 4),
+                       "particle_steps_per_s": round(bed.count * 10 * K / sec_sp, 1),
+                       "dem_rebuilds_in_timed_steps": reb_sp},
         "whole_step_hbm_frac": round(whole_step_gbs / HBM_PEAK_GBS, 4),
         "velocity_rel_l2_fp32_vs_fp64": vel_err,
         "drag_rel_l2_fp32_vs_fp64": drag_err,

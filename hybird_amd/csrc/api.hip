@@ -610,8 +610,6 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
     if (c.wallContacts != HYB_WALL_CONTACTS_FIRST && c.wallContacts != HYB_WALL_CONTACTS_ALL)
         return fail(HYB_E_ARG, "unknown wall-contact mode");
     p.allWalls = c.wallContacts == HYB_WALL_CONTACTS_ALL;
-    if (p.allWalls && p.spring)
-        return fail(HYB_E_UNSUPPORTED, "wallContacts = ALL with the tangential spring is not supported");
     p.numVisc = c.numVisc;
     p.demF[0] = demF.x; p.demF[1] = demF.y; p.demF[2] = demF.z;
     // objects: periodic copies of those within 2 nebrRange of a periodic plane,
@@ -743,15 +741,15 @@ static int dem_init(hyb_dem* d, const LatticeInfo& lbi) {
         HIPCHK(hipMemcpy(b.pproto, pproto.data(), sizeof(double) * pproto.size(), hipMemcpyHostToDevice));
     }
     if (p.spring) {
-        const size_t Pm = (size_t)std::max(P, 1);
-        if ((rc = ralloc(R, "hist", &b.hist, 3 * (size_t)p.pairCap)) || (rc = ralloc(R, "whist", &b.whist, 4 * Pm)) ||
+        const size_t Pm = (size_t)std::max(P, 1), Wm = Pm * (size_t)wall_slots(p);  // a spring per wall slot
+        if ((rc = ralloc(R, "hist", &b.hist, 3 * (size_t)p.pairCap)) || (rc = ralloc(R, "whist", &b.whist, 4 * Wm)) ||
             (rc = dalloc(&b.histOld, 3 * (size_t)p.pairCap)) || (rc = dalloc(&b.pairsOld, 2 * (size_t)p.pairCap)) ||
             (rc = dalloc(&b.csrOffOld, Pm + 1)) || (rc = dalloc(&b.csrOld, 2 * (size_t)p.pairCap)) ||
             (rc = dalloc(&b.ghostIndexOld, (size_t)p.Gcap)))
             return rc;
         HIPCHK(hipMemset(b.hist, 0, 3 * sizeof(double) * (size_t)p.pairCap));
-        std::vector<double> wh(4 * Pm, 0.0);
-        for (size_t e = 0; e < Pm; ++e) wh[4 * e + 3] = -1.0;  // no wall in contact
+        std::vector<double> wh(4 * Wm, 0.0);
+        for (size_t e = 0; e < Wm; ++e) wh[4 * e + 3] = -1.0;  // no wall in contact
         HIPCHK(hipMemcpy(b.whist, wh.data(), wh.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     if (p.nobj) {

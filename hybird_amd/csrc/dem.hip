@@ -1049,19 +1049,32 @@ __device__ __forceinline__ double gather_elem(const DEMParams& p, const DEMBuffe
     for (int i = ilo; i < ihi; ++i) {
         V3 px, pv;
         part_pred(i, px, pv);
-        if (!SP && p.allWalls) {
+        if (p.allWalls) {
             const uint32_t mask = b.nearWallMask[i];
             const int ws = p.nwalls;
             for (int wi = 0; wi < ws; ++wi) {
+                const size_t slot = (size_t)i * ws + wi;
                 StatRec wst{(mask >> wi & 1u) ? (double)wi : -1.0, (double)i, 0.0, 0.0, 0.0, {0.0, 0.0, 0.0},
                             {0.0, 0.0, 0.0}, 0.0};
                 ContactForce wcf{{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0, 0.0};
+                // spring model: one spring per (particle, wall), kept while that wall stays in contact
+                double wxi[3] = {0.0, 0.0, 0.0};
+                bool wcontact = false;
                 if (mask >> wi & 1u) {
-                    double unused[3] = {0.0, 0.0, 0.0};
-                    wall_contact(i, (uint32_t)wi, px, pv, wcf, wst, unused);
+                    if (SP && b.whist[4 * slot + 3] == (double)wi) {
+                        wxi[0] = b.whist[4 * slot]; wxi[1] = b.whist[4 * slot + 1]; wxi[2] = b.whist[4 * slot + 2];
+                    }
+                    wcontact = wall_contact(i, (uint32_t)wi, px, pv, wcf, wst, wxi);
                 }
-                b.wcf[(size_t)i * ws + wi] = wcf;
-                if (b.wstat) b.wstat[(size_t)i * ws + wi] = wst;
+                b.wcf[slot] = wcf;
+                if (b.wstat) b.wstat[slot] = wst;
+                if constexpr (SP) {
+                    double* wh = b.whist + 4 * slot;
+                    wh[0] = wcontact ? wxi[0] : 0.0;
+                    wh[1] = wcontact ? wxi[1] : 0.0;
+                    wh[2] = wcontact ? wxi[2] : 0.0;
+                    wh[3] = wcontact ? (double)wi : -1.0;
+                }
             }
             continue;
         }

@@ -119,7 +119,8 @@ struct DEMBuffers {
     struct StatRec* wstat;  // [P][WS] wall contacts of each particle
     struct StatRec* ostat;  // [P][NEAR_OBJ] object contacts of each element
     // tangential springs (spring model only, else nullptr): per pair xi of the pair's second particle
-    // relative to its first [pairCap][3]; per element xi against its near wall and that wall's index [P][4];
+    // relative to its first [pairCap][3]; per particle and wall slot (wall_slots) xi against the wall and that
+    // wall's index, -1 without contact [P][WS][4];
     // copies of the pair list, its springs, the per-particle lists and the ghosts taken at the start of a
     // rebuild, from which the new pairs take over their springs
     double* hist;

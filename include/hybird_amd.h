@@ -82,7 +82,7 @@ enum hyb_precision { HYB_FP64 = 0, HYB_FP32_SHIFTED = 1 };
 enum hyb_rheology { HYB_RHEOLOGY_BINGHAM = 0, HYB_RHEOLOGY_FRICTIONAL = 1 };
 /* Tangential contact law. VISCOUS is the reference's history-free law (DEM.cpp:2176-2222):
  * F_t = min(2 viscTang sqrt(m_eff k_s) |v_t|, mu F_n) along v_t. SPRING (not in the reference) keeps a
- * tangential spring xi per particle pair and per element's near wall: xi is projected on the current tangent
+ * tangential spring xi per particle pair and per particle's wall contact: xi is projected on the current tangent
  * plane, advanced by v_t dt, F_t = k_s xi + 2 viscTang sqrt(m_eff k_s) v_t with the same k_s, limited to
  * mu F_n (slip resets xi to the limit); xi is carried across neighbour rebuilds and restarts. Objects and
  * cylinders keep the viscous law. */
@@ -90,7 +90,8 @@ enum hyb_tangential { HYB_TANGENTIAL_VISCOUS = 0, HYB_TANGENTIAL_SPRING = 1 };
 /* Walls a particle can touch. FIRST is the reference's near-wall list (DEM.cpp:1496-1513): only the first
  * wall (in wall order) within nebrRange of a particle is listed, so a particle in the corner of a side wall
  * and the floor has no floor contact and sinks through it under gravity. ALL (not in the reference) lists
- * every wall within nebrRange and applies their contacts in wall order; not combined with SPRING. */
+ * every wall within nebrRange and applies their contacts in wall order (with SPRING, one spring per particle
+ * and wall). */
 enum hyb_wall_contacts { HYB_WALL_CONTACTS_FIRST = 0, HYB_WALL_CONTACTS_ALL = 1 };
 
 /* Configuration: the .cfg keys of the reference, in the reference's (physical)

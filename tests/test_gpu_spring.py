@@ -23,7 +23,14 @@ spring would break:
   loss stays small;
 * the spring survives a neighbour rebuild: the same collision, started so that
   a rebuild falls inside the contact, ends in the same state (up to rounding)
-  as when it does not (a reset spring would lose its stored force).
+  as when it does not (a reset spring would lose its stored force);
+* with every wall in range as a contact partner (wallContacts = ALL, one spring
+  per particle and wall): a sphere touching one wall moves exactly as with the
+  first wall only, and a sphere resting in the corner of the floor and a side
+  wall under equal normal loads, given a speed along the corner line, sticks
+  on both walls: the two springs xi_f, xi_s obey
+  m xi_f'' = -k ((1 + m r^2 / I) xi_f + xi_s) and the mirror equation, so the
+  symmetric mode it starts in has w^2 = 4.5 k_s / m (3.5 with one contact).
 
 The DEM runs alone here (lbSolve = 0).
 """
@@ -55,10 +62,11 @@ def _sim(cfg, x, r, v=None, w=None, model="SPRING"):
     return runtime.Simulation(fixture_config(cfg), rec)
 
 
-def _floor_case(v0, model="SPRING", mu=0.5):
+def _floor_case(v0, model="SPRING", mu=0.5, walls="FIRST"):
     """One sphere at the static overlap on the floor (wall at z = 0.5) with tangential speed v0 along x."""
     cfg = base_config(lbSolve=0, lbSizeX=24, lbSizeY=24, lbSizeZ=20, lbFZ=-G, density=RHO, youngMod=E, poisson=NU,
-                      restitution=0.88, viscTang=0.0, frictionCoefWall=mu, frictionCoefPart=mu, multiStep=10)
+                      restitution=0.88, viscTang=0.0, frictionCoefWall=mu, frictionCoefPart=mu, multiStep=10,
+                      wallContacts=walls)
     m = 4.0 / 3.0 * math.pi * R ** 3 * RHO
     kn = 2.0 / 3.0 * E / (1.0 - NU * NU)
     # wall contact: F_n = kn sqrt(r) (2 delta)^(3/2) = m g
@@ -192,3 +200,51 @@ def test_spring_oblique_collision_invariants_and_rebuild():
     assert reb_b, "no neighbour rebuild happened during the contact"
     assert np.allclose(v_b, v_a, rtol=0, atol=1e-9 * np.abs(v_a).max())
     assert np.allclose(w_b, w_a, rtol=0, atol=1e-9 * np.abs(w_a).max())
+
+
+def test_spring_every_wall_single_contact_equals_first_wall():
+    print("[gpu spring] floor slide with every wall in range against the first wall only", flush=True)
+    out = {}
+    for walls in ("FIRST", "ALL"):
+        sim, _, _ = _floor_case(1.0e-2, walls=walls)
+        sim.cycle(60)
+        out[walls] = sim.get("e_x").copy(), sim.get("e_w").copy(), sim.get("walls")[:, 6:9].copy()
+        sim.close()
+    for a, b in zip(out["FIRST"], out["ALL"]):
+        assert np.array_equal(a, b)
+
+
+def test_spring_every_wall_corner_stick_oscillation():
+    """Sphere in the corner of the x = 0.5 wall and the floor, gravity (-g, 0, -g), speed v0 along y."""
+    v0 = 1.0e-4
+    cfg = base_config(lbSolve=0, lbSizeX=24, lbSizeY=24, lbSizeZ=20, lbFX=-G, lbFZ=-G, boundary0=7, boundary1=7,
+                      density=RHO, youngMod=E, poisson=NU, restitution=0.88, viscTang=0.0, frictionCoefWall=0.5,
+                      frictionCoefPart=0.5, multiStep=10, wallContacts="ALL")
+    m = 4.0 / 3.0 * math.pi * R ** 3 * RHO
+    kn = 2.0 / 3.0 * E / (1.0 - NU * NU)
+    two_delta = (m * G / (kn * math.sqrt(R))) ** (2.0 / 3.0)  # static overlap of each contact
+    c0 = 0.5 + R - 0.5 * two_delta
+    sim = _sim(cfg, [[c0, 12.0, c0]], [R], [[0.0, v0, 0.0]])
+    assert sim.get("nearWallTable").tolist() == [0, 0, 0, 2]  # walls x = 0.5, x = 22.5, floor, ceiling
+    ks = 2.0 * E / ((2.0 - NU) * (1.0 + NU)) * math.sqrt(R) * (m * G) ** (1.0 / 3.0)
+    print("[gpu spring] stick oscillation in a corner, two wall springs", flush=True)
+    t, vf, vsd = [0.0], [v0], [v0]
+    for k in range(1, 121):
+        sim.cycle(1)
+        x, w = sim.get("e_x")[0], sim.get("e_w")[0]
+        t.append(float(k))
+        vf.append(x[1, 1] + R * w[0, 0])   # slip along y at the floor contact (0, 0, -R)
+        vsd.append(x[1, 1] - R * w[0, 2])  # at the side contact (-R, 0, 0)
+    pos = sim.get("e_x")[0][0]
+    sim.close()
+    # the sphere stays at its static position: the oscillation is purely tangential
+    assert abs(pos[0] - c0) <= 1e-9 * c0 and abs(pos[2] - c0) <= 1e-9 * c0
+    # symmetric mode: both contacts slip alike
+    assert np.max(np.abs(np.array(vf) - np.array(vsd))) <= 1e-6 * v0
+    z = _zero_crossings(np.array(t), np.array(vf))
+    assert len(z) >= 3, z
+    w = math.sqrt(4.5 * ks / m)
+    half = (z[2] - z[0]) / 2.0
+    assert abs(half - math.pi / w) <= 5e-3 * math.pi / w, (half, math.pi / w, math.pi / math.sqrt(3.5 * ks / m))
+    assert abs(z[0] - 0.5 * math.pi / w) <= 1e-2 * math.pi / w, (z[0], 0.5 * math.pi / w)
+    assert max(abs(v) for v in vf[30:]) == pytest.approx(v0, rel=2e-2)

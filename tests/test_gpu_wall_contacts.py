@@ -70,10 +70,3 @@ def test_gpu_all_walls_corner_sphere_rests():
     floor = gpu.get("walls")[4, 6:9]
     assert abs(floor[2] + m * abs(G)) <= 1e-12 * m * abs(G)
     gpu.close()
-
-
-def test_gpu_all_walls_rejects_spring():
-    rt = _runtime()
-    cfg, rec = _box_case()
-    with pytest.raises(rt.HybError, match="tangential spring"):
-        rt.Simulation(config_from_cfg(dict(cfg, tangentialModel="SPRING")), rec)
