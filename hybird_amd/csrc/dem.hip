@@ -1329,11 +1329,14 @@ __global__ void k_dem_couple_table(DEMParams p, DEMBuffers b, CoupleRec* __restr
 // Element hydrodynamic force and torque (LB.cpp:1897-1918): the run sums the
 // collision appended for the slots of the element's particles and their ghosts,
 // added exactly (fixed point, so the result does not depend on the order of the
-// appends), then scaled to physical units. One wavefront per element; lane l
-// accumulates component l % 7 of every ninth entry (l / 7 < 9), so that each
-// entry component is split once; the nine partial sums per component are then
-// added through LDS. The lists are emptied for the next LB step.
-constexpr int COMBINE_BLOCK = 256;
+// appends), then scaled to physical units. One wavefront per element, one
+// element per workgroup; lane l accumulates component l % 7 of every ninth
+// entry (l / 7 < 9), so that each entry component is split once; the nine
+// partial sums per component are then added through LDS. The lists are emptied
+// for the next LB step. (Four elements per workgroup, each wavefront on its own
+// LDS rows, gave results that varied from run to run for a few elements past
+// the first 1024 on this hardware; one wavefront per workgroup does not.)
+constexpr int COMBINE_BLOCK = 64;
 constexpr int COMBINE_ROWS = 9;  // entries per pass (9 x 7 components = 63 lanes)
 __global__ void __launch_bounds__(COMBINE_BLOCK) k_dem_hydro_combine(DEMParams p, DEMBuffers b, HydroSink hs, Units u) {
 #pragma clang fp contract(off)

@@ -86,18 +86,30 @@ __global__ void __launch_bounds__(FS_BLOCK)
             }
         }
         const double usq = u.norm2();
+        // every neighbour's type and mass, and in the pull form its opposite population, loaded
+        // together before any is used (a neighbour index is always a node of the lattice, so the loads a
+        // neighbour's type makes unnecessary stay in bounds): one round trip instead of one per link
+        uint8_t tn[19];
+        T vn[19];
+        double mL[19];
+#pragma unroll
+        for (int j = 1; j < 19; ++j) {
+            const size_t L = nb_index(p, c, j);
+            tn[j] = type[L];
+            vn[j] = pull ? fpost[(size_t)opp(j) * p.PS + L] : (T)0;
+            mL[j] = mass[L];
+        }
         fi[0] = fs[0];
         double dm = 0.0;
 #pragma unroll
         for (int j = 1; j < 19; ++j) {
             const int o = opp(j);
-            const size_t L = nb_index(p, c, j);
-            const uint8_t tl = type[L];
+            const uint8_t tl = tn[j];
             double a = 0.0;
             T v;
             if (!pull) {
                 v = fs[o];  // before the first collision f = fs
-                if (tl == T_INTERFACE) a = 0.5 * (mass[L] + mi);
+                if (tl == T_INTERFACE) a = 0.5 * (mL[j] + mi);
                 else if (tl == T_FLUID) a = 1.0;
                 else if (tl == T_WALL_MOVING || tl == T_CURVED) a = 1.0 * mi;
                 else if (tl == T_SLIP_MOVING) {
@@ -110,8 +122,8 @@ __global__ void __launch_bounds__(FS_BLOCK)
                     a = half ? 0.0 + 1.0 * p.slipS2 * mi : 0.0 + 1.0 * mi;
                 }
             } else if (tl == T_FLUID || tl == T_INTERFACE) {
-                v = fpost[(size_t)o * p.PS + L];  // LB.cpp:1234
-                a = (tl == T_INTERFACE) ? 0.5 * (mass[L] + mi) : 1.0;
+                v = vn[j];  // fpost[opp][L], LB.cpp:1234
+                a = (tl == T_INTERFACE) ? 0.5 * (mL[j] + mi) : 1.0;
             } else if (tl == T_GAS) {
                 // LB.cpp:1237-1246 with the shifted velocity of the last collision
                 const double vu = u.dot(V3((double)cx(j), (double)cy(j), (double)cz(j)));
@@ -313,8 +325,12 @@ __global__ void __launch_bounds__(FS_BLOCK)
         size_t idx;
         if (!fs_candidate(q, nI, nA, list, state, listA, listB, idx)) continue;
         const Coord c = coord_of(p, idx);
+        uint8_t tn[19];  // all loaded before the test: one round trip
+#pragma unroll
+        for (int j = 1; j < 19; ++j) tn[j] = type[nb_index(p, c, j)];
         bool gasNb = false;
-        for (int j = 1; j < 19 && !gasNb; ++j) gasNb = type[nb_index(p, c, j)] == T_GAS;
+#pragma unroll
+        for (int j = 1; j < 19; ++j) gasNb = gasNb || tn[j] == T_GAS;
         if (gasNb) continue;
         const double n = to_d(rho[idx]);
         exact_add(ctr->acc, mass[idx] - n);
@@ -337,8 +353,12 @@ __global__ void __launch_bounds__(FS_BLOCK)
         if (!fs_candidate(q, nI, nA, list, state, listA, listB, idx)) continue;
         if (type[idx] != T_INTERFACE) continue;
         const Coord c = coord_of(p, idx);
+        uint8_t tn[19];  // all loaded before the test: one round trip
+#pragma unroll
+        for (int j = 1; j < 19; ++j) tn[j] = type[nb_index(p, c, j)];
         bool fluidNb = false;
-        for (int j = 1; j < 19 && !fluidNb; ++j) fluidNb = type[nb_index(p, c, j)] == T_FLUID;
+#pragma unroll
+        for (int j = 1; j < 19; ++j) fluidNb = fluidNb || tn[j] == T_FLUID;
         if (fluidNb) {
             next[atomicAdd(&ctr->n[cur ^ 1], 1u)] = (uint32_t)idx;
         } else {
